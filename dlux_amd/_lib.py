@@ -1,0 +1,89 @@
+"""ctypes binding of ``libdlux_amd.so`` (the C ABI declared in ``include/dlux_amd.h``).
+
+The library is built in-tree by ``dlux_amd/build.py``.  There is no fallback:
+if the library is missing every compute entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_size_t, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_native" / "libdlux_amd.so"
+
+DLX_OK = 0
+DLX_PREC_FP32 = 0
+DLX_PREC_BF16X3 = 1
+
+_dp = POINTER(c_double)
+_fp = c_void_p  # float* / void* device pointers are passed as integers
+
+
+class MftGeom(ctypes.Structure):
+    """``dlx_mft_geom``."""
+
+    _fields_ = [
+        ("n_in", c_int32),
+        ("n_out", c_int32),
+        ("inverse", c_int32),
+        ("has_focal", c_int32),
+        ("focal_length", c_double),
+        ("focal_shift", c_double),
+    ]
+
+
+# name -> (restype, argtypes); must match include/dlux_amd.h
+SIGNATURES = {
+    "dlx_version": (c_char_p, []),
+    "dlx_last_error": (c_char_p, []),
+    "dlx_transfer_matrix": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, _dp, _dp, c_void_p,
+                                      c_void_p]),
+    "dlx_mft_workspace_size": (c_size_t, [POINTER(MftGeom), c_int32, c_int32]),
+    "dlx_mft": (c_int32, [POINTER(MftGeom), c_int32, c_void_p, c_void_p, _dp, _dp, _dp, _dp,
+                          _dp, _dp, c_int32, c_void_p, c_size_t, c_void_p]),
+    "dlx_pupil_field": (c_int32, [c_int32, c_int32, _dp, c_double, c_double, c_double,
+                                  c_double, _fp, _fp, _fp, c_double, c_void_p, c_void_p,
+                                  c_void_p]),
+    "dlx_eval_basis": (c_int32, [c_int32, c_int64, c_int32, _fp, _fp, _fp, _fp, c_void_p]),
+    "dlx_power": (c_int32, [c_int64, c_int32, c_void_p, _fp, c_void_p]),
+    "dlx_intensity": (c_int32, [c_int64, c_int32, c_void_p, c_int32, _fp, c_void_p]),
+    "dlx_psf_poly_workspace_size": (c_size_t, [POINTER(MftGeom), c_int32, c_int32]),
+    "dlx_psf_poly": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, c_double, c_double,
+                               c_double, c_double, c_double, _fp, _fp, _fp, c_double, _fp,
+                               c_int32, c_void_p, c_size_t, c_void_p]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the native library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"dlux_amd native library not found at {LIB_PATH}; build it with "
+            "`python -m dlux_amd.build` (hipcc, gfx950).  There is no CPU fallback."
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int, name: str) -> None:
+    """Raise RuntimeError carrying dlx_last_error() for a non-zero status."""
+    if status != DLX_OK:
+        msg = load().dlx_last_error().decode(errors="replace")
+        raise RuntimeError(f"{name} failed (status {status}): {msg}")
+
+
+def dptr(values) -> ctypes.Array:
+    """A host double array for the C ABI."""
+    vals = [float(v) for v in values]
+    return (c_double * len(vals))(*vals)

@@ -1,0 +1,293 @@
+// extern "C" entry points of libdlux_amd (declared in include/dlux_amd.h):
+// argument validation, workspace carving and kernel orchestration.
+#include <cstdarg>
+#include <cmath>
+#include <vector>
+
+#include "dlx_internal.h"
+
+namespace dlx {
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+void clear_error() { g_err.clear(); }
+
+namespace {
+
+// Instances per fused launch of the polychromatic PSF (bounds the workspace).
+constexpr int kWlChunk = 64;
+
+int check_geom(const dlx_mft_geom* g) {
+  DLX_CHECK_ARG(g != nullptr, "geom is NULL");
+  DLX_CHECK_ARG(g->n_in > 0 && g->n_in <= 16384, "n_in must be in [1, 16384], got %d", g->n_in);
+  DLX_CHECK_ARG(g->n_out > 0 && g->n_out <= 16384, "n_out must be in [1, 16384], got %d",
+                g->n_out);
+  if (g->has_focal)
+    DLX_CHECK_ARG(std::isfinite(g->focal_length + g->focal_shift) &&
+                      (g->focal_length + g->focal_shift) != 0.0,
+                  "focal_length + focal_shift must be finite and non-zero");
+  return DLX_OK;
+}
+
+// Generate `nb` transfer matrices with per-instance (lambda, dx, p_out, shift,
+// amplitude) into out[b] (stride N_out*N_in), in launches of DftChunk::kMax.
+int gen_dft(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
+            const double* pout, const double* shift, const double* amp, float2* out,
+            hipStream_t s) {
+  const long stride = (long)g.n_out * g.n_in;
+  for (int b0 = 0; b0 < nb; b0 += DftChunk::kMax) {
+    const int n = nb - b0 < DftChunk::kMax ? nb - b0 : DftChunk::kMax;
+    DftChunk c;
+    for (int i = 0; i < n; ++i) {
+      const int b = b0 + i;
+      DLX_CHECK_ARG(std::isfinite(wl[b]) && wl[b] > 0, "wavelength[%d] must be > 0", b);
+      DLX_CHECK_ARG(std::isfinite(dx[b]) && dx[b] != 0, "pixel_scale_in[%d] must be non-zero", b);
+      DLX_CHECK_ARG(std::isfinite(pout[b]), "pixel_scale_out[%d] must be finite", b);
+      const MftInst p = make_inst(g, wl[b], dx[b], pout[b], shift ? shift[b] : 0.0);
+      c.a_scale[i] = p.a_scale;
+      c.a_off[i] = p.a_off;
+      c.b_scale[i] = p.b_scale;
+      c.b_off[i] = p.b_off;
+      c.amp[i] = amp ? amp[b] : 1.0;
+    }
+    DLX_HIP(launch_dft_chunk(c, n, g.n_in, g.n_out, g.inverse, out + (long)b0 * stride, stride,
+                             s));
+  }
+  return DLX_OK;
+}
+
+}  // namespace
+}  // namespace dlx
+
+using namespace dlx;
+
+extern "C" {
+
+const char* dlx_version(void) { return "dlux_amd 0.1.0 gfx950"; }
+
+const char* dlx_last_error(void) { return dlx::g_err.c_str(); }
+
+int dlx_transfer_matrix(const dlx_mft_geom* geom, int32_t batch, const double* wavelength,
+                        const double* pixel_scale_in, const double* pixel_scale_out,
+                        const double* shift, void* out, void* stream) {
+  clear_error();
+  if (int st = check_geom(geom)) return st;
+  DLX_CHECK_ARG(batch >= 0, "batch must be >= 0");
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(wavelength && pixel_scale_in && pixel_scale_out && out,
+                "null argument to dlx_transfer_matrix");
+  return gen_dft(*geom, batch, wavelength, pixel_scale_in, pixel_scale_out, shift, nullptr,
+                 (float2*)out, (hipStream_t)stream);
+}
+
+size_t dlx_mft_workspace_size(const dlx_mft_geom* geom, int32_t batch, int32_t precision) {
+  (void)precision;
+  if (!geom || batch <= 0) return 0;
+  const size_t mat = align_up((size_t)batch * geom->n_out * geom->n_in * sizeof(float2));
+  return 3 * mat;  // W_x, W_y, U^T
+}
+
+int dlx_mft(const dlx_mft_geom* geom, int32_t batch, const void* in, void* out,
+            const double* wavelength, const double* pixel_scale_in,
+            const double* pixel_scale_out, const double* shift_x, const double* shift_y,
+            const double* field_scale, int32_t precision, void* workspace,
+            size_t workspace_bytes, void* stream) {
+  clear_error();
+  if (int st = check_geom(geom)) return st;
+  DLX_CHECK_ARG(batch >= 0, "batch must be >= 0");
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(in && out && wavelength && pixel_scale_in && pixel_scale_out,
+                "null argument to dlx_mft");
+  DLX_CHECK_ARG(precision == DLX_PREC_FP32 || precision == DLX_PREC_BF16X3,
+                "unknown precision %d", precision);
+  const size_t need = dlx_mft_workspace_size(geom, batch, precision);
+  if (!workspace || workspace_bytes < need) {
+    set_error("dlx_mft workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  const dlx_mft_geom& g = *geom;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t mat = align_up((size_t)batch * g.n_out * g.n_in * sizeof(float2));
+  float2* Wx = (float2*)workspace;
+  float2* Wy = (float2*)((char*)workspace + mat);
+  float2* Ut = (float2*)((char*)workspace + 2 * mat);
+
+  // Fold c_b * field_scale_b into the matrices: sqrt on each side when the two
+  // axes share a matrix, the whole factor on W_y otherwise.
+  bool same = true;
+  for (int b = 0; b < batch && same; ++b) {
+    const double sx = shift_x ? shift_x[b] : 0.0, sy = shift_y ? shift_y[b] : 0.0;
+    same = (sx == sy);
+  }
+  std::vector<double> ax(batch), ay(batch);
+  for (int b = 0; b < batch; ++b) {
+    const double c = mft_norm(g, wavelength[b], pixel_scale_in[b], pixel_scale_out[b]) *
+                     (field_scale ? field_scale[b] : 1.0);
+    if (same) {
+      ax[b] = std::sqrt(std::fabs(c));
+      ay[b] = c < 0 ? -ax[b] : ax[b];
+    } else {
+      ax[b] = 1.0;
+      ay[b] = c;
+    }
+  }
+  if (int st = gen_dft(g, batch, wavelength, pixel_scale_in, pixel_scale_out, shift_x,
+                       ax.data(), Wx, s))
+    return st;
+  if (same && std::equal(ax.begin(), ax.end(), ay.begin())) {
+    Wy = Wx;
+  } else {
+    if (int st = gen_dft(g, batch, wavelength, pixel_scale_in, pixel_scale_out, shift_y,
+                         ay.data(), Wy, s))
+      return st;
+  }
+  const long mstride = (long)g.n_out * g.n_in;
+  GemmBatch g1{(const float2*)in, g.n_in, (long)g.n_in * g.n_in, Wx, g.n_in, mstride,
+               Ut, g.n_in, mstride, g.n_in, g.n_out, g.n_in, batch, nullptr, EPI_STORE_T,
+               precision};
+  DLX_HIP(launch_cgemm_nt(g1, s));
+  GemmBatch g2{Wy, g.n_in, mstride, Ut, g.n_in, mstride, out, g.n_out,
+               (long)g.n_out * g.n_out, g.n_out, g.n_out, g.n_in, batch, nullptr, EPI_STORE,
+               precision};
+  DLX_HIP(launch_cgemm_nt(g2, s));
+  return DLX_OK;
+}
+
+int dlx_pupil_field(int32_t n, int32_t batch, const double* wavelength, double pixel_scale,
+                    double center, double tilt_x, double tilt_y, const float* amp,
+                    const float* opd, const float* phase, double scale, const void* in,
+                    void* out, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(n > 0, "n must be > 0");
+  DLX_CHECK_ARG(batch >= 0, "batch must be >= 0");
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(wavelength && out, "null argument to dlx_pupil_field");
+  DLX_CHECK_ARG(std::isfinite(pixel_scale) && std::isfinite(center) && std::isfinite(tilt_x) &&
+                    std::isfinite(tilt_y) && std::isfinite(scale),
+                "non-finite scalar argument to dlx_pupil_field");
+  hipStream_t s = (hipStream_t)stream;
+  const long npix = (long)n * n;
+  const double x0 = center - (n - 1) / 2.0 * pixel_scale;
+  for (int b0 = 0; b0 < batch; b0 += PupilChunk::kMax) {
+    const int nb = batch - b0 < PupilChunk::kMax ? batch - b0 : PupilChunk::kMax;
+    PupilChunk c;
+    for (int i = 0; i < nb; ++i) {
+      DLX_CHECK_ARG(std::isfinite(wavelength[b0 + i]) && wavelength[b0 + i] > 0,
+                    "wavelength[%d] must be > 0", b0 + i);
+      c.inv_wl[i] = 1.0 / wavelength[b0 + i];
+    }
+    DLX_HIP(launch_pupil_chunk(c, nb, n, pixel_scale, x0, tilt_x, tilt_y, amp, opd, phase,
+                               (float)scale, in ? (const float2*)in + b0 * npix : nullptr,
+                               (float2*)out + b0 * npix, s));
+  }
+  return DLX_OK;
+}
+
+int dlx_eval_basis(int32_t nbasis, int64_t npix, int32_t nsets, const float* basis,
+                   const float* coeffs, const float* base, float* out, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(nbasis >= 0 && npix >= 0 && nsets >= 0, "negative size to dlx_eval_basis");
+  if (npix == 0 || nsets == 0) return DLX_OK;
+  DLX_CHECK_ARG(out && (nbasis == 0 || (basis && coeffs)), "null argument to dlx_eval_basis");
+  DLX_HIP(launch_eval_basis(nbasis, npix, nsets, basis, coeffs, base, out,
+                            (hipStream_t)stream));
+  return DLX_OK;
+}
+
+int dlx_power(int64_t npix, int32_t batch, const void* field, float* out, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(npix >= 0 && batch >= 0, "negative size to dlx_power");
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(field && out, "null argument to dlx_power");
+  DLX_HIP(launch_power(npix, batch, (const float2*)field, out, (hipStream_t)stream));
+  return DLX_OK;
+}
+
+int dlx_intensity(int64_t npix, int32_t batch, const void* field, int32_t sum_over_batch,
+                  float* out, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(npix >= 0 && batch >= 0, "negative size to dlx_intensity");
+  if (batch == 0 || npix == 0) return DLX_OK;
+  DLX_CHECK_ARG(field && out, "null argument to dlx_intensity");
+  DLX_HIP(launch_intensity(npix, batch, (const float2*)field, sum_over_batch, out,
+                           (hipStream_t)stream));
+  return DLX_OK;
+}
+
+size_t dlx_psf_poly_workspace_size(const dlx_mft_geom* geom, int32_t nwl, int32_t precision) {
+  (void)precision;
+  if (!geom || nwl <= 0) return 0;
+  const size_t c = (size_t)(nwl < kWlChunk ? nwl : kWlChunk);
+  const size_t P = align_up(c * geom->n_in * geom->n_in * sizeof(float2));
+  const size_t W = align_up(c * geom->n_out * geom->n_in * sizeof(float2));
+  const size_t I = align_up(c * geom->n_out * geom->n_out * sizeof(float));
+  return P + 2 * W + I;
+}
+
+int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                 const double* weight, double pixel_scale_in, double pixel_scale_out,
+                 double center, double tilt_x, double tilt_y, const float* amp,
+                 const float* opd, const float* phase, double scale, float* psf,
+                 int32_t precision, void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  if (int st = check_geom(geom)) return st;
+  DLX_CHECK_ARG(nwl > 0, "nwl must be > 0, got %d", nwl);
+  DLX_CHECK_ARG(wavelength && weight && psf, "null argument to dlx_psf_poly");
+  DLX_CHECK_ARG(precision == DLX_PREC_FP32 || precision == DLX_PREC_BF16X3,
+                "unknown precision %d", precision);
+  const size_t need = dlx_psf_poly_workspace_size(geom, nwl, precision);
+  if (!workspace || workspace_bytes < need) {
+    set_error("dlx_psf_poly workspace too small: need %zu bytes, got %zu", need,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  for (int l = 0; l < nwl; ++l)
+    DLX_CHECK_ARG(std::isfinite(weight[l]) && weight[l] >= 0, "weight[%d] must be >= 0", l);
+  const dlx_mft_geom& g = *geom;
+  hipStream_t s = (hipStream_t)stream;
+  const int c = nwl < kWlChunk ? nwl : kWlChunk;
+  const size_t Pb = align_up((size_t)c * g.n_in * g.n_in * sizeof(float2));
+  const size_t Wb = align_up((size_t)c * g.n_out * g.n_in * sizeof(float2));
+  float2* P = (float2*)workspace;
+  float2* W = (float2*)((char*)workspace + Pb);
+  float2* Ut = (float2*)((char*)workspace + Pb + Wb);
+  float* I = (float*)((char*)workspace + Pb + 2 * Wb);
+  const long mstride = (long)g.n_out * g.n_in;
+  const long nout2 = (long)g.n_out * g.n_out;
+
+  std::vector<double> dx(c, pixel_scale_in), pout(c, pixel_scale_out), amp_w(c);
+  for (int l0 = 0; l0 < nwl; l0 += c) {
+    const int nb = nwl - l0 < c ? nwl - l0 : c;
+    if (int st = dlx_pupil_field(g.n_in, nb, wavelength + l0, pixel_scale_in, center, tilt_x,
+                                 tilt_y, amp, opd, phase, scale, nullptr, P, stream))
+      return st;
+    // |E|^2 = w * c^2 * |Y^T P X|^2: fold sqrt(c * sqrt(w)) into each matrix side.
+    for (int i = 0; i < nb; ++i) {
+      const double cn = mft_norm(g, wavelength[l0 + i], pixel_scale_in, pixel_scale_out);
+      amp_w[i] = std::sqrt(std::fabs(cn) * std::sqrt(weight[l0 + i]));
+    }
+    if (int st = gen_dft(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr,
+                         amp_w.data(), W, s))
+      return st;
+    GemmBatch g1{P, g.n_in, (long)g.n_in * g.n_in, W, g.n_in, mstride, Ut, g.n_in, mstride,
+                 g.n_in, g.n_out, g.n_in, nb, nullptr, EPI_STORE_T, precision};
+    DLX_HIP(launch_cgemm_nt(g1, s));
+    GemmBatch g2{W, g.n_in, mstride, Ut, g.n_in, mstride, I, g.n_out, nout2, g.n_out, g.n_out,
+                 g.n_in, nb, nullptr, EPI_INTENSITY, precision};
+    DLX_HIP(launch_cgemm_nt(g2, s));
+    DLX_HIP(launch_sum_planes(nout2, nb, I, l0 > 0, psf, s));
+  }
+  return DLX_OK;
+}
+
+}  // extern "C"

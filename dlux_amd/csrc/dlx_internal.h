@@ -1,0 +1,131 @@
+// Internal helpers shared by the dlux_amd HIP translation units (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <cstdio>
+#include <string>
+
+#include "../../include/dlux_amd.h"
+
+namespace dlx {
+
+// Per-thread error message returned by dlx_last_error().
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define DLX_CHECK_ARG(cond, ...)          \
+  do {                                    \
+    if (!(cond)) {                        \
+      ::dlx::set_error(__VA_ARGS__);      \
+      return DLX_ERR_INVALID;             \
+    }                                     \
+  } while (0)
+
+#define DLX_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t _e = (call);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      ::dlx::set_error("%s failed: %s", #call, hipGetErrorString(_e));       \
+      return DLX_ERR_HIP;                                                    \
+    }                                                                        \
+  } while (0)
+
+// Status of the last kernel launch on this thread.
+inline int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("launch of %s failed: %s", what, hipGetErrorString(e));
+    return DLX_ERR_HIP;
+  }
+  return DLX_OK;
+}
+
+inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
+
+// Device-side per-instance MFT parameters (one per wavelength / source).
+struct MftInst {
+  double a_scale;   // 1 / N_in
+  double a_off;     // -(N_in-1)/2 / N_in - shift_a / N_in    (a_k = k*a_scale + a_off)
+  double b_scale;   // p_out / fringe [/(f+df)]
+  double b_off;     // (-(N_out-1)/2 - shift_b) * b_scale      (b_l = l*b_scale + b_off)
+};
+
+// Host-side builder: the per-axis coordinate vectors of transfer_matrix
+// (utils/propagation.py:109-121), in FP64.
+inline MftInst make_inst(const dlx_mft_geom& g, double wavelength, double dx_in,
+                         double p_out, double shift) {
+  MftInst p;
+  const double n_in = g.n_in, n_out = g.n_out;
+  const double fringe = wavelength / (dx_in * n_in);
+  double so = p_out / fringe;
+  if (g.has_focal) so /= (g.focal_length + g.focal_shift);
+  p.a_scale = 1.0 / n_in;
+  p.a_off = -(n_in - 1.0) / 2.0 / n_in - shift / n_in;
+  p.b_scale = so;
+  p.b_off = (-(n_out - 1.0) / 2.0 - shift) * so;
+  return p;
+}
+
+// nfringes / (N_in N_out)  (utils/propagation.py:130-175, :246-254).
+inline double mft_norm(const dlx_mft_geom& g, double wavelength, double dx_in,
+                       double p_out) {
+  const double diameter = g.n_in * dx_in;
+  const double fringe = wavelength / diameter;
+  double out_size = g.n_out * p_out;
+  if (g.has_focal) out_size /= (g.focal_length + g.focal_shift);
+  const double nfr = out_size / fringe;
+  return nfr / ((double)g.n_in * (double)g.n_out);
+}
+
+// ---------------------------------------------------------------------------
+// kernels implemented in the .hip files, launched from the host orchestration
+// ---------------------------------------------------------------------------
+
+// Complex NT GEMM C[b](m,n) = sum_k A[b](m,k) B[b](n,k), then an epilogue.
+enum Epilogue {
+  EPI_STORE = 0,        // C = alpha * acc (complex64), ldc
+  EPI_STORE_T = 1,      // C^T = alpha * acc: written at C[n*ldc + m]
+  EPI_INTENSITY = 2,    // I = |alpha * acc|^2 (float32), ldc
+};
+
+struct GemmBatch {
+  const float2* A; long lda; long sA;
+  const float2* B; long ldb; long sB;
+  void* C; long ldc; long sC;
+  int M, N, K, batch;
+  const float* alpha;   // device, per batch (NULL => 1)
+  int epilogue;
+  int precision;        // DLX_PREC_*
+};
+
+hipError_t launch_cgemm_nt(const GemmBatch& g, hipStream_t s);
+
+// Per-launch instance parameters passed by value (kernel arguments), so no
+// device-side parameter buffer is needed.
+struct DftChunk {
+  static constexpr int kMax = 48;
+  double a_scale[kMax], a_off[kMax], b_scale[kMax], b_off[kMax], amp[kMax];
+};
+struct PupilChunk {
+  static constexpr int kMax = 128;
+  double inv_wl[kMax];
+};
+
+hipError_t launch_dft_chunk(const DftChunk& c, int nb, int n_in, int n_out, int inverse,
+                            float2* out, long bstride, hipStream_t s);
+hipError_t launch_pupil_chunk(const PupilChunk& c, int nb, int n, double dx, double x0,
+                              double tx, double ty, const float* amp, const float* opd,
+                              const float* phase, float scale, const float2* in,
+                              float2* out, hipStream_t s);
+hipError_t launch_eval_basis(int nbasis, long npix, int nsets, const float* basis,
+                             const float* coeffs, const float* base, float* out,
+                             hipStream_t s);
+hipError_t launch_power(long npix, int batch, const float2* f, float* out, hipStream_t s);
+hipError_t launch_intensity(long npix, int batch, const float2* f, int sum_batch, float* out,
+                            hipStream_t s);
+hipError_t launch_sum_planes(long npix, int nplanes, const float* planes, int accumulate,
+                             float* psf, hipStream_t s);
+
+}  // namespace dlx

@@ -1,0 +1,169 @@
+/*
+ * dlux_amd.h -- C ABI of the MI355X (gfx950) diffraction hot path.
+ *
+ * The reference (dLux 0.16.0) is a pure-Python JAX library with no FFI; each
+ * entry point below stands in for the reference routine cited beside it, and
+ * is what a jax.ffi handler, a torch binding or a ctypes stub binds.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (DLX_OK == 0).  No exceptions
+ *     cross this boundary; dlx_last_error() returns a per-thread message.
+ *   - All array pointers are DEVICE pointers owned by the caller.  Complex
+ *     arrays are interleaved complex64 (re, im float pairs), row-major, the
+ *     layout of jax/torch complex64.  Real arrays are float32 unless noted.
+ *   - Per-instance physical parameters are passed as HOST double arrays.
+ *   - Work is enqueued on `stream` (a hipStream_t, NULL = default stream) and
+ *     is stream-ordered; entry points do not synchronise.
+ *   - Entry points that need scratch take (workspace, workspace_bytes); the
+ *     matching *_workspace_size() query returns the bytes needed.  The library
+ *     never frees caller memory.
+ *   - Pupil grids are square N_in x N_in, axis 0 = y, axis 1 = x (the
+ *     reference's 'xy' meshgrid order).
+ */
+#ifndef DLUX_AMD_H
+#define DLUX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum dlx_status {
+  DLX_OK = 0,
+  DLX_ERR_INVALID = 1,     /* bad argument (shape, size, null pointer)        */
+  DLX_ERR_HIP = 2,         /* a HIP runtime call failed                       */
+  DLX_ERR_WORKSPACE = 3,   /* workspace too small                             */
+  DLX_ERR_UNSUPPORTED = 4  /* valid request the library does not implement    */
+};
+
+enum dlx_precision {
+  DLX_PREC_FP32 = 0,   /* complex64 operands, FP32 MFMA, FP32 accumulate         */
+  DLX_PREC_BF16X3 = 1  /* split-BF16 3-term (hi*hi + hi*lo + lo*hi) MFMA, FP32 acc */
+};
+
+/* Library version string ("dlux_amd x.y.z gfx950"). */
+const char* dlx_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* dlx_last_error(void);
+
+/*
+ * MFT geometry shared by a batch of instances (one per wavelength).
+ * Mirrors the arguments of dLux's transfer_matrix / calc_nfringes / MFT:
+ *   utils/propagation.py:67-77, :130-138, :178-188.
+ */
+typedef struct dlx_mft_geom {
+  int32_t n_in;            /* pupil side length N_in                          */
+  int32_t n_out;           /* output side length N_out                        */
+  int32_t inverse;         /* 0: exp(-2 pi i ...), 1: exp(+2 pi i ...)         */
+  int32_t has_focal;       /* 0: angular (focal_length None), 1: Cartesian     */
+  double focal_length;     /* metres, used when has_focal                      */
+  double focal_shift;      /* metres, added to focal_length (reference: 0)     */
+} dlx_mft_geom;
+
+/*
+ * Transfer (DFT) matrices for `batch` instances, written TRANSPOSED relative
+ * to the reference: out[b] is N_out x N_in with out[b][l][k] = T_b[k][l],
+ *   T_b = exp(-/+ 2 pi i * outer(a, b)),  a_k = (k - (N_in-1)/2 - s_b)/N_in,
+ *   b_l = (l - (N_out-1)/2 - s_b) * p_out_b * N_in * dx_b / lambda_b [/(f+df)].
+ * The phase argument a_k*b_l is formed in FP64 and reduced to [-1/2, 1/2)
+ * turns before an FP32 sincos.
+ * Replaces: transfer_matrix, utils/propagation.py:67-127.
+ */
+int dlx_transfer_matrix(const dlx_mft_geom* geom, int32_t batch,
+                        const double* wavelength, const double* pixel_scale_in,
+                        const double* pixel_scale_out, const double* shift,
+                        void* out, void* stream);
+
+/*
+ * Batched two-sided MFT of `batch` complex64 fields (N_in x N_in each) to
+ * N_out x N_out: out[b] = c_b * (Y_b^T in[b]) X_b, with X_b = T(shift_x[b]),
+ * Y_b = T(shift_y[b]) and c_b = nfringes_b / (N_in N_out).  Shifts are in
+ * output pixels (the reference's pixel=True form; the host layer converts
+ * pixel=False shifts).  shift_x / shift_y may be NULL (zero).
+ * field_scale (host, may be NULL) multiplies out[b] by a real factor
+ * (used for sqrt(weight) in polychromatic propagation).
+ * Replaces: MFT, utils/propagation.py:178-256 (vmapped over instances as in
+ * optical_systems.py:213-216).
+ */
+size_t dlx_mft_workspace_size(const dlx_mft_geom* geom, int32_t batch,
+                              int32_t precision);
+int dlx_mft(const dlx_mft_geom* geom, int32_t batch, const void* in, void* out,
+            const double* wavelength, const double* pixel_scale_in,
+            const double* pixel_scale_out, const double* shift_x,
+            const double* shift_y, const double* field_scale, int32_t precision,
+            void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Pupil field synthesis for a batch of wavelengths, in one pass:
+ *   out[b](y,x) = s * amp(y,x) * exp(i*(2 pi/lambda_b * (opd(y,x) + tx*X + ty*Y)
+ *                                         + phase(y,x)))
+ * with X, Y the pixel-centre coordinates c + (j - (N-1)/2) * dx (metres).
+ * amp / opd / phase are float32 N x N and may each be NULL (1 / 0 / 0).
+ * If `in` is non-NULL the result multiplies in[b] (complex64, batch x N x N);
+ * otherwise it multiplies the reference's initial phasor 1/N^2.
+ * Replaces: Wavefront.__init__ phasor (wavefronts.py:111-113), tilt
+ * (:370-395), add_opd / add_phase (:332-368), TransmissiveLayer
+ * (layers/optical_layers.py:181-186).
+ */
+int dlx_pupil_field(int32_t n, int32_t batch, const double* wavelength,
+                    double pixel_scale, double center, double tilt_x,
+                    double tilt_y, const float* amp, const float* opd,
+                    const float* phase, double scale, const void* in, void* out,
+                    void* stream);
+
+/*
+ * OPD = sum_j coeffs[j] * basis[j] (+ base if non-NULL), float32 N x N, for
+ * `nsets` coefficient sets at once (coeffs is nsets x nbasis, device f32;
+ * out is nsets x npix).  Reads the basis once for all sets.
+ * Replaces: eval_basis, utils/math.py:177-196 (BasisLayer.eval_basis,
+ * layers/optical_layers.py:304-313).
+ */
+int dlx_eval_basis(int32_t nbasis, int64_t npix, int32_t nsets,
+                   const float* basis, const float* coeffs, const float* base,
+                   float* out, void* stream);
+
+/*
+ * Sum over pixels of |E|^2 per instance (out: batch floats, device).
+ * Replaces: Wavefront.power, wavefronts.py:320-330 (and normalise's
+ * reduction, :397-424).
+ */
+int dlx_power(int64_t npix, int32_t batch, const void* field, float* out,
+              void* stream);
+
+/*
+ * Intensity: out[b] = |field[b]|^2 (sum_over_batch == 0) or
+ * out = sum_b |field[b]|^2 (sum_over_batch == 1), float32.
+ * Replaces: Wavefront.psf (wavefronts.py:268-279) and the wavelength sum of
+ * OpticalSystem.propagate (optical_systems.py:222-223).
+ */
+int dlx_intensity(int64_t npix, int32_t batch, const void* field,
+                  int32_t sum_over_batch, float* out, void* stream);
+
+/*
+ * Fused polychromatic PSF: for each of `nwl` wavelengths,
+ *   P_l = amp * exp(i*(2 pi/lambda_l*(opd + tilt) + phase)) * s
+ *   E_l = sqrt(w_l) * c_l * Y_l^T P_l X_l
+ * and psf = sum_l |E_l|^2 (float32 N_out x N_out).  amp/opd/phase as in
+ * dlx_pupil_field (NULL allowed); `scale` is the lambda-independent pupil
+ * scale s (initial 1/N^2 times any power normalisation).
+ * Replaces the reference chain OpticalSystem.propagate -> propagate_mono ->
+ * layers -> to_focus -> MFT -> psf.sum(0): optical_systems.py:147-223,
+ * :391-425, :560-594, :659-680.
+ */
+size_t dlx_psf_poly_workspace_size(const dlx_mft_geom* geom, int32_t nwl,
+                                   int32_t precision);
+int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                 const double* weight, double pixel_scale_in,
+                 double pixel_scale_out, double center, double tilt_x,
+                 double tilt_y, const float* amp, const float* opd,
+                 const float* phase, double scale, float* psf, int32_t precision,
+                 void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DLUX_AMD_H */

@@ -1,0 +1,492 @@
+"""FP64 NumPy oracle for dLux's diffraction hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker the HIP path is compared
+against.  Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
+leg of ``bench.py`` may import it; the product package (``dlux_amd``) never
+does, and nothing here is ever the thing that is measured or shipped.
+
+Every function restates, operation for operation, the dLux 0.16.0 routine it
+cites (paths relative to the reference's repository root), but computes in
+float64/complex128 -- dLux's own code under ``jax_enable_x64``.  dLux itself
+cannot be imported here (jax, equinox, zodiax, abcdLux and interpax are absent),
+so the restatement is pinned instead by the reference tests' known answers,
+closed forms and invariants (``tests/test_oracle.py``).
+
+Parity status per function (all pinned unless marked):
+  * nd_coords / pixel_coords / cart2polar / rotate_coords        pinned
+  * transfer_matrix / calc_nfringes / mft                        pinned
+  * fft / fft_spec / fft_phase_ramp / propagate_fft              pinned
+  * pad_to / crop_to / resize / downsample                       pinned
+  * noll_indices / zernike_factors / zernike / zernike_basis     pinned
+  * circle / spider / combine / circular_aperture                pinned
+  * eval_basis, wavefront chain, poly_psf, point_sources_psf     pinned
+  * poly_psf_grad (adjoint gradient)                             pinned (FD)
+  * asm_prop (angular spectrum)                                  parity unpinned
+      -- its conventions live in abcdLux (absent); only textbook invariants
+         (z=0 identity, unitarity, z1+z2 composition) are checked.
+"""
+
+from __future__ import annotations
+
+from math import lgamma, exp
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# coordinates  (src/dLux/utils/coordinates.py, src/dLux/coordinates.py)
+# ---------------------------------------------------------------------------
+
+
+def nd_coords_1d(n: int, scale: float = 1.0, offset: float = 0.0) -> np.ndarray:
+    """One axis of ``nd_coords``: linspace(-(n-1)/2*d - o, (n-1)/2*d - o, n).
+
+    utils/coordinates.py:320-324 (``pixel_fn``).
+    """
+    start = -(n - 1) / 2 * scale - offset
+    end = (n - 1) / 2 * scale - offset
+    return np.linspace(start, end, n)
+
+
+def pixel_coords(npixels: int, diameter: float, fft_style: bool = False) -> np.ndarray:
+    """(x, y) meshgrid of pixel centres, 'xy' indexing.  utils/coordinates.py:208-266."""
+    d = diameter / npixels
+    off = d / 2.0 if (fft_style and npixels % 2 == 0) else 0.0
+    xs = nd_coords_1d(npixels, d, off)
+    return np.array(np.meshgrid(xs, xs, indexing="xy"))
+
+
+def cart2polar(coords: np.ndarray) -> np.ndarray:
+    """utils/coordinates.py:166-184."""
+    x, y = coords
+    return np.array([np.hypot(x, y), np.arctan2(y, x)])
+
+
+def rotate_coords(coords: np.ndarray, rotation: float) -> np.ndarray:
+    """utils/coordinates.py:81-101 (rotation in radians)."""
+    x, y = coords
+    new_x = np.cos(-rotation) * x + np.sin(-rotation) * y
+    new_y = -np.sin(-rotation) * x + np.cos(-rotation) * y
+    return np.array([new_x, new_y])
+
+
+def spec_xs(n: int, d: float, c: float = 0.0) -> np.ndarray:
+    """``CoordSpec.xs`` = c + (arange(n) - (n-1)/2) * d.  coordinates.py:117-129."""
+    return c + (np.arange(n) - (n - 1) / 2) * d
+
+
+# ---------------------------------------------------------------------------
+# MFT  (src/dLux/utils/propagation.py)
+# ---------------------------------------------------------------------------
+
+
+def transfer_matrix(
+    wavelength,
+    npixels_in,
+    pixel_scale_in,
+    npixels_out,
+    pixel_scale_out,
+    shift=0.0,
+    focal_length=None,
+    focal_shift=0.0,
+    inverse=False,
+):
+    """N_in x N_out DFT matrix exp(-/+ 2 pi i a (x) b).  utils/propagation.py:67-127."""
+    fringe_size = wavelength / (pixel_scale_in * npixels_in)
+    scale_in = 1.0 / npixels_in
+    in_vec = nd_coords_1d(npixels_in, scale_in, shift * scale_in)
+    scale_out = pixel_scale_out / fringe_size
+    if focal_length is not None:
+        scale_out /= focal_length + focal_shift
+    out_vec = nd_coords_1d(npixels_out, scale_out, shift * scale_out)
+    matrix = -2j * np.pi * np.outer(in_vec, out_vec)
+    if inverse:
+        matrix *= -1
+    return np.exp(matrix)
+
+
+def calc_nfringes(
+    wavelength, npixels_in, pixel_scale_in, npixels_out, pixel_scale_out,
+    focal_length=None, focal_shift=0.0,
+):
+    """utils/propagation.py:130-175."""
+    diameter = npixels_in * pixel_scale_in
+    fringe_size = wavelength / diameter
+    output_size = npixels_out * pixel_scale_out
+    if focal_length is not None:
+        output_size /= focal_length + focal_shift
+    return output_size / fringe_size
+
+
+def mft(
+    phasor,
+    wavelength,
+    pixel_scale_in,
+    npixels_out,
+    pixel_scale_out,
+    focal_length=None,
+    shift=(0.0, 0.0),
+    pixel=True,
+    inverse=False,
+):
+    """Two-sided MFT (Y^T E) X times nfringes/(N_in N_out).  utils/propagation.py:178-256."""
+    phasor = np.asarray(phasor, np.complex128)
+    npixels_in = phasor.shape[-1]
+    shift = np.asarray(shift, np.float64).copy()
+    if not pixel:
+        shift = shift / pixel_scale_out
+    x_mat = transfer_matrix(wavelength, npixels_in, pixel_scale_in, npixels_out,
+                            pixel_scale_out, shift[0], focal_length, 0.0, inverse)
+    y_mat = transfer_matrix(wavelength, npixels_in, pixel_scale_in, npixels_out,
+                            pixel_scale_out, shift[1], focal_length, 0.0, inverse)
+    out = (y_mat.T @ phasor) @ x_mat
+    nfringes = calc_nfringes(wavelength, npixels_in, pixel_scale_in, npixels_out,
+                             pixel_scale_out, focal_length)
+    return out * np.exp(np.log(nfringes) - (np.log(npixels_in) + np.log(npixels_out)))
+
+
+# ---------------------------------------------------------------------------
+# FFT  (src/dLux/utils/propagation.py:8-64, src/dLux/utils/fourier.py:14-78)
+# ---------------------------------------------------------------------------
+
+
+def fft(phasor, wavelength, pixel_scale, focal_length=None, pad=2, inverse=False):
+    """pad -> ifftshift -> fft2 -> fftshift, / N_pad (inverse: ifft2, * N_pad)."""
+    phasor = np.asarray(phasor, np.complex128)
+    npixels = phasor.shape[-1]
+    fringe_size = wavelength / (pixel_scale * npixels)
+    new_pixel_scale = fringe_size / pad
+    if focal_length is not None:
+        new_pixel_scale *= focal_length
+    npad = (npixels * (pad - 1)) // 2
+    phasor = np.pad(phasor, npad)
+    if inverse:
+        phasor = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(phasor)))
+        phasor = phasor * phasor.shape[-1]
+    else:
+        phasor = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(phasor)))
+        phasor = phasor / phasor.shape[-1]
+    return phasor, new_pixel_scale
+
+
+def fft_spec(npixels_in, pixel_scale_in, wavelength, focal_length=None):
+    """utils/fourier.py:14-45."""
+    d_out = wavelength / (npixels_in * pixel_scale_in)
+    if focal_length is not None:
+        d_out = d_out * focal_length
+    if npixels_in % 2 != 0:
+        return d_out, 0.0
+    return d_out, -0.5 * d_out
+
+
+def fft_phase_ramp(xs, wavelength, shift, focal_length=None, inverse=False):
+    """utils/fourier.py:48-78."""
+    sign = -1 if inverse else 1
+    alpha = wavelength if focal_length is None else wavelength * focal_length
+    ramp = np.exp(sign * 2j * np.pi * np.asarray(xs) * shift / alpha)
+    return ramp[None, :] * ramp[:, None]
+
+
+# ---------------------------------------------------------------------------
+# array ops  (src/dLux/utils/array_ops.py)
+# ---------------------------------------------------------------------------
+
+
+def pad_to(array, npixels, fill=0.0):
+    """utils/array_ops.py:12-50 (parity-preserving centred pad)."""
+    n_in = array.shape[-1]
+    if n_in % 2 != npixels % 2:
+        raise ValueError("parity")
+    if npixels < n_in:
+        raise ValueError("smaller")
+    p = (npixels - n_in) // 2
+    return np.pad(array, p, constant_values=fill)
+
+
+def crop_to(array, npixels):
+    """utils/array_ops.py:53-90."""
+    n_in = array.shape[-1]
+    if n_in % 2 != npixels % 2:
+        raise ValueError("parity")
+    if npixels > n_in:
+        raise ValueError("larger")
+    start, stop = (n_in - npixels) // 2, (n_in + npixels) // 2
+    return array[start:stop, start:stop]
+
+
+def resize(array, npixels, fill=0.0):
+    """utils/array_ops.py:93-121."""
+    n_in = array.shape[-1]
+    if npixels == n_in:
+        return array
+    if npixels < n_in:
+        return crop_to(array, npixels)
+    return pad_to(array, npixels, fill)
+
+
+def downsample(array, n, mean=True):
+    """utils/array_ops.py:124-162 (two reshape-reduce passes)."""
+    method = np.mean if mean else np.sum
+    size_in = array.shape[0]
+    size_out = size_in // n
+    array = method(array.reshape((size_in * size_out, n)), 1)
+    array = array.reshape(size_in, size_out).T
+    array = method(array.reshape((size_out * size_out, n)), 1)
+    return array.reshape(size_out, size_out).T
+
+
+# ---------------------------------------------------------------------------
+# Zernikes  (src/dLux/utils/zernikes.py, src/dLux/utils/math.py)
+# ---------------------------------------------------------------------------
+
+
+def noll_indices(j: int):
+    """(n, m) from the Noll index.  utils/zernikes.py:99-119 (integer formula)."""
+    n = int(np.ceil(-1 / 2 + np.sqrt(1 + 8 * j) / 2) - 1)
+    smallest_j_in_row = n * (n + 1) / 2 + 1
+    number_of_shifts = (j - smallest_j_in_row + ~(n & 1) + 2) // 2
+    sign_of_shift = -(j & 1) + ~(j & 1) + 2
+    base_case = n & 1
+    m = int(sign_of_shift * (base_case + number_of_shifts * 2))
+    return n, m
+
+
+def factorial(x: float) -> float:
+    """exp(lgamma(x + 1)), 1 at 0; inf at negative integers.  utils/math.py:137-157."""
+    if x == 0:
+        return 1.0
+    if x < 0 and float(x).is_integer():
+        return float("inf")
+    return exp(lgamma(x + 1.0))
+
+
+def zernike_factors(j: int):
+    """Radial coefficients c_k and powers k, using signed m.  utils/zernikes.py:122-150."""
+    if j < 1:
+        raise ValueError("The Zernike index must be greater than 0.")
+    n, m = noll_indices(j)
+    k = np.arange(((n - m) // 2) + 1, dtype=float)
+    c = np.array([
+        (-1.0) ** kk * factorial(abs(n - kk)) / factorial(kk)
+        / factorial(((n + m) // 2) - kk) / factorial(((n - m) // 2) - kk)
+        for kk in k
+    ])
+    return c, k
+
+
+def zernike(j: int, coordinates, diameter: float = 2.0):
+    """Noll Zernike on the disk of given diameter.  utils/zernikes.py:227-257."""
+    coords = np.asarray(coordinates, np.float64) / (diameter / 2)
+    rho, theta = cart2polar(coords)
+    aperture = rho <= 1.0
+    n, m = noll_indices(j)
+    c, k = zernike_factors(j)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rads = rho[:, :, None] ** (abs(n) - 2 * k)[None, None, :]
+        radial = (c * rads).sum(axis=2)
+    norm = np.sqrt(n + 1)
+    if m != 0:
+        norm *= np.sqrt(2)
+    trig = np.cos if m >= 0 else np.sin
+    return aperture * radial * (norm * trig(abs(m) * theta))
+
+
+def zernike_basis(js, coordinates, diameter: float = 2.0):
+    """utils/zernikes.py:297-315."""
+    return np.array([zernike(int(j), coordinates, diameter) for j in js])
+
+
+def eval_basis(basis, coefficients):
+    """tensordot over the leading coefficient dims.  utils/math.py:177-196."""
+    coefficients = np.asarray(coefficients, np.float64)
+    axes = tuple(range(coefficients.ndim))
+    return np.tensordot(np.asarray(basis, np.float64), coefficients, axes=(axes, axes))
+
+
+# ---------------------------------------------------------------------------
+# hard masks  (src/dLux/utils/geometry.py, src/dLux/utils/apertures.py)
+# ---------------------------------------------------------------------------
+
+
+def circle(coords, radius, invert=False):
+    """Hard circle r < R (or r > R).  utils/geometry.py:95-117, :405-418."""
+    dist = cart2polar(coords)[0] - radius
+    return (dist > 0).astype(float) if invert else (dist < 0).astype(float)
+
+
+def spider(coords, width, angles):
+    """One-sided vanes: blocked where max(|x'|-w/2, y') < 0.  geometry.py:195-218, :463-485."""
+    blocked = np.zeros(coords.shape[1:], bool)
+    for angle in np.atleast_1d(angles):
+        x, y = rotate_coords(coords, np.deg2rad(angle))
+        blocked |= np.maximum(np.abs(x) - width / 2, y) < 0
+    return (~blocked).astype(float)
+
+
+def combine(arrays, oversample=1, use_sum=False):
+    """Product (or sum) then downsample.  utils/geometry.py:20-46."""
+    arr = np.array(arrays)
+    out = arr.sum(0) if use_sum else arr.prod(0)
+    return out if oversample == 1 else downsample(out, oversample)
+
+
+def circular_aperture(
+    npixels, diameter, oversample=5, secondary_diameter=None, spider_width=None,
+    spider_angles=None, zernike_nolls=None, zernike_oversize=0.01,
+):
+    """Static circular aperture (+ basis masked by the primary).  utils/apertures.py:153-242."""
+    coords = pixel_coords(npixels * oversample, diameter)
+    layers = [circle(coords, diameter / 2)]
+    if secondary_diameter is not None and secondary_diameter > 0:
+        layers.append(circle(coords, secondary_diameter / 2, invert=True))
+    if spider_width is not None:
+        layers.append(spider(coords, spider_width, spider_angles))
+    transmission = combine(layers, oversample)
+    if zernike_nolls is None:
+        return transmission
+    coords = pixel_coords(npixels, diameter)
+    basis = zernike_basis(zernike_nolls, coords, diameter * (1.0 + zernike_oversize))
+    support = downsample(layers[0], oversample) > 0
+    return transmission, basis * support[None]
+
+
+# ---------------------------------------------------------------------------
+# Wavefront chain  (src/dLux/wavefronts.py, src/dLux/optical_systems.py)
+# ---------------------------------------------------------------------------
+
+
+def wavefront_init(npixels: int):
+    """ones/N^2 phasor.  wavefronts.py:111-113."""
+    return np.ones((npixels, npixels), np.complex128) / npixels ** 2
+
+
+def tilt_opd(npixels, pixel_scale, angles, center=0.0):
+    """theta_x X + theta_y Y on meshgrid(xs, xs).  wavefronts.py:370-395, :584-609."""
+    xs = spec_xs(npixels, pixel_scale, center)
+    X, Y = np.meshgrid(xs, xs)
+    return angles[0] * X + angles[1] * Y
+
+
+def add_opd(phasor, wavelength, opd):
+    """phasor * exp(1j * 2 pi / lambda * opd).  wavefronts.py:332-368."""
+    return phasor * np.exp(1j * (2 * np.pi / wavelength) * opd)
+
+
+def normalise(phasor, mode="power", value=1.0):
+    """wavefronts.py:397-424."""
+    if mode == "power":
+        return phasor * np.sqrt(value / np.sum(np.abs(phasor) ** 2))
+    return phasor * np.sqrt(value / np.max(np.abs(phasor) ** 2))
+
+
+def pupil_field(wavelength, mask, basis, coeffs, diameter, offset=(0.0, 0.0),
+                normalise_power=True):
+    """Initial wavefront -> tilt -> TransmissiveLayer(mask, normalise) -> BasisLayer(opd).
+
+    optical_systems.py:363-389 + layers/optical_layers.py:181-186, :319-327.
+    """
+    n = mask.shape[-1]
+    d = diameter / n
+    E = wavefront_init(n)
+    E = add_opd(E, wavelength, tilt_opd(n, d, np.asarray(offset, np.float64)))
+    E = E * mask
+    if normalise_power:
+        E = normalise(E)
+    if basis is not None:
+        E = add_opd(E, wavelength, eval_basis(basis, coeffs))
+    return E
+
+
+def poly_psf(mask, basis, coeffs, wavelengths, weights, diameter, npixels_out,
+             pixel_scale_out, offset=(0.0, 0.0), normalise_power=True,
+             return_fields=False):
+    """Sum_l w_l |MFT(P_l)|^2 -- OpticalSystem.propagate.  optical_systems.py:147-223, :659-680."""
+    wavelengths = np.atleast_1d(np.asarray(wavelengths, np.float64))
+    weights = (np.ones_like(wavelengths) / len(wavelengths) if weights is None
+               else np.atleast_1d(np.asarray(weights, np.float64)))
+    n = mask.shape[-1]
+    d = diameter / n
+    psf = np.zeros((npixels_out, npixels_out))
+    fields = []
+    for wl, w in zip(wavelengths, weights):
+        P = pupil_field(wl, mask, basis, coeffs, diameter, offset, normalise_power)
+        E = mft(P, wl, d, npixels_out, pixel_scale_out) * w ** 0.5
+        fields.append(E)
+        psf += np.abs(E) ** 2
+    return (psf, fields) if return_fields else psf
+
+
+def point_sources_psf(mask, basis, coeffs, wavelengths, spectrum_weights, positions,
+                      fluxes, diameter, npixels_out, pixel_scale_out):
+    """PointSources.model: weights normalised, x flux, sum over sources.  sources.py:392-411."""
+    w = np.asarray(spectrum_weights, np.float64)
+    w = w / w.sum()
+    out = 0.0
+    for pos, flux in zip(positions, fluxes):
+        out = out + poly_psf(mask, basis, coeffs, wavelengths, w * flux, diameter,
+                             npixels_out, pixel_scale_out, offset=pos)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Gradient of the phase-retrieval loss (JAX AD of the chain above, written out)
+# ---------------------------------------------------------------------------
+
+
+def gaussian_loss(psf, data, flux):
+    """0.5 * sum((flux * psf - data)^2 / max(data, 1))."""
+    var = np.maximum(data, 1.0)
+    return 0.5 * np.sum((flux * psf - data) ** 2 / var)
+
+
+def poly_psf_grad(mask, basis, coeffs, wavelengths, weights, diameter, npixels_out,
+                  pixel_scale_out, data, flux):
+    """(loss, dloss/dcoeffs) by the adjoint of the MFT chain, in FP64.
+
+    With P_l = A exp(i k_l OPD) and E_l = sqrt(w_l) s Y^T P_l X:
+      Ebar_l = 2 G E_l,  G = dL/dPSF
+      Pbar_l = sqrt(w_l) s conj(Y) Ebar_l X^H
+      dL/dOPD = sum_l k_l Im(conj(P_l) Pbar_l)   (real-cotangent convention)
+      dL/dc_j = sum_px Z_j dL/dOPD
+    """
+    wavelengths = np.atleast_1d(np.asarray(wavelengths, np.float64))
+    weights = np.atleast_1d(np.asarray(weights, np.float64))
+    n = mask.shape[-1]
+    d = diameter / n
+    psf, fields = poly_psf(mask, basis, coeffs, wavelengths, weights, diameter,
+                           npixels_out, pixel_scale_out, return_fields=True)
+    var = np.maximum(data, 1.0)
+    loss = 0.5 * np.sum((flux * psf - data) ** 2 / var)
+    G = flux * (flux * psf - data) / var
+    g_opd = np.zeros((n, n))
+    for wl, w, E in zip(wavelengths, weights, fields):
+        P = pupil_field(wl, mask, basis, coeffs, diameter)
+        Y = transfer_matrix(wl, n, d, npixels_out, pixel_scale_out)
+        s = np.exp(np.log(calc_nfringes(wl, n, d, npixels_out, pixel_scale_out))
+                   - (np.log(n) + np.log(npixels_out)))
+        Ebar = 2.0 * G * E
+        Pbar = (w ** 0.5) * s * (np.conj(Y) @ Ebar @ np.conj(Y).T)
+        k = 2 * np.pi / wl
+        g_opd += k * np.imag(np.conj(P) * Pbar)
+    grad = np.tensordot(basis, g_opd, axes=([1, 2], [0, 1]))
+    return loss, grad
+
+
+# ---------------------------------------------------------------------------
+# Angular spectrum (textbook; abcdLux.asm.asm_prop is absent -> parity unpinned)
+# ---------------------------------------------------------------------------
+
+
+def asm_prop(u_in, dx, wavelength, z, npad):
+    """U_z = IFFT{ FFT{pad(U)} * exp(i 2 pi z sqrt(1/l^2 - fx^2 - fy^2)) }, evanescent -> 0.
+
+    Replaces layers/abcd_propagators.py:444-451 (abcdLux.asm.asm_prop, crop=False).
+    Centred pad (utils/array_ops.py:12-50 semantics), centred FFT (ifftshift before,
+    fftshift after), output keeps the input sampling dx and has npad pixels.
+    """
+    u = pad_to(np.asarray(u_in, np.complex128), npad)
+    U = np.fft.fft2(np.fft.ifftshift(u))
+    f = np.fft.fftfreq(npad, d=dx)
+    FX, FY = np.meshgrid(f, f)
+    arg = 1.0 / wavelength ** 2 - FX ** 2 - FY ** 2
+    H = np.where(arg > 0, np.exp(2j * np.pi * z * np.sqrt(np.maximum(arg, 0.0))), 0.0)
+    return np.fft.fftshift(np.fft.ifft2(U * H))

@@ -1,0 +1,152 @@
+"""GPU parity of the C-ABI kernels against the FP64 oracle.
+
+Error metric: peak-normalised max |x - x_ref| / max |x_ref| (fields and PSFs).
+Tolerances: FP32 form 2e-6 on fields / 1e-5 on PSFs; split-BF16x3 form 3e-5 on
+fields / 1e-5 on PSFs (the PSF bound is the one the hot path is held to).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import dlux_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _peak_err(x, ref):
+    x = np.asarray(x)
+    ref = np.asarray(ref)
+    return float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
+
+
+@pytest.fixture(scope="module")
+def rt():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dlux_amd import runtime
+    return runtime
+
+
+FIELD_TOL = {"fp32": 2e-6, "bf16x3": 3e-5}
+
+
+def _rand_field(rng, n):
+    return (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / n
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("n_in,n_out", [(32, 16), (16, 8), (33, 17), (256, 128), (96, 200)])
+def test_mft_random_field(rt, prec, n_in, n_out):
+    print(f"[mft] prec={prec} n_in={n_in} n_out={n_out}", flush=True)
+    rng = np.random.default_rng(n_in * 1000 + n_out)
+    E = _rand_field(rng, n_in)
+    wl, dx, p = 1e-6, 1.0 / n_in, 0.25e-6
+    out = rt.mft(E, wl, dx, n_out, p, precision=prec).cpu().numpy()
+    ref = orc.mft(E, wl, dx, n_out, p)
+    assert out.shape == (n_out, n_out)
+    assert _peak_err(out, ref) < FIELD_TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("focal,inverse,shift", [
+    (2.0, True, (1.0, -2.0)), (2.0, False, (1.0, -2.0)),
+    (None, True, (0.5, 0.5)), (None, False, (3.0, -1.5)),
+])
+def test_mft_reference_fixture_modes(rt, prec, focal, inverse, shift):
+    """The reference's MFT test fixture (32 -> 16, wl 1, dx 0.1, p 0.05), every mode."""
+    print(f"[mft-modes] prec={prec} focal={focal} inverse={inverse} shift={shift}", flush=True)
+    rng = np.random.default_rng(7)
+    E = _rand_field(rng, 32)
+    out = rt.mft(E, 1.0, 0.1, 16, 0.05, focal_length=focal, shift_x=shift[0],
+                 shift_y=shift[1], inverse=inverse, precision=prec).cpu().numpy()
+    ref = orc.mft(E, 1.0, 0.1, 16, 0.05, focal_length=focal, shift=shift, inverse=inverse)
+    assert np.all(np.isfinite(out))
+    assert _peak_err(out, ref) < FIELD_TOL[prec]
+
+
+def test_mft_batched_per_instance_params(rt):
+    print("[mft-batched]", flush=True)
+    rng = np.random.default_rng(3)
+    B, n_in, n_out = 5, 64, 48
+    E = np.stack([_rand_field(rng, n_in) for _ in range(B)])
+    wls = np.linspace(0.8e-6, 1.2e-6, B)
+    dxs = np.linspace(0.9, 1.1, B) / n_in
+    ps = np.linspace(2e-7, 3e-7, B)
+    fs = np.linspace(0.5, 1.5, B)
+    out = rt.mft(E, wls, dxs, n_out, ps, field_scale=fs).cpu().numpy()
+    for b in range(B):
+        ref = orc.mft(E[b], wls[b], dxs[b], n_out, ps[b]) * fs[b]
+        assert _peak_err(out[b], ref) < FIELD_TOL["fp32"]
+
+
+def test_transfer_matrix(rt):
+    print("[transfer-matrix]", flush=True)
+    wls = np.array([0.8e-6, 1.0e-6, 1.2e-6])
+    W = rt.transfer_matrix_T(wls, 64, 1 / 64, 32, 2.5e-7, shifts=0.75).cpu().numpy()
+    for b, wl in enumerate(wls):
+        ref = orc.transfer_matrix(wl, 64, 1 / 64, 32, 2.5e-7, shift=0.75)
+        assert np.max(np.abs(W[b].T - ref)) < 1e-6
+
+
+def test_pupil_field(rt):
+    print("[pupil-field]", flush=True)
+    rng = np.random.default_rng(11)
+    n, D = 64, 1.0
+    amp = (rng.random((n, n)) > 0.3).astype(np.float32)
+    opd = (rng.standard_normal((n, n)) * 50e-9).astype(np.float32)
+    phase = (rng.standard_normal((n, n)) * 0.3).astype(np.float32)
+    wls = np.array([0.9e-6, 1.1e-6])
+    tilt = (1e-6, -2e-6)
+    out = rt.pupil_field(n, wls, D / n, tilt=tilt, amp=amp, opd=opd, phase=phase,
+                         scale=1.0 / n ** 2).cpu().numpy()
+    for b, wl in enumerate(wls):
+        E = orc.wavefront_init(n)
+        E = orc.add_opd(E, wl, orc.tilt_opd(n, D / n, tilt))
+        E = E * amp
+        E = orc.add_opd(E, wl, opd.astype(np.float64))
+        E = E * np.exp(1j * phase.astype(np.float64))
+        assert _peak_err(out[b], E) < 1e-6
+
+
+def test_eval_basis_power_intensity(rt):
+    print("[basis/power/intensity]", flush=True)
+    rng = np.random.default_rng(5)
+    basis = rng.standard_normal((10, 37, 37)).astype(np.float32)
+    c = rng.standard_normal(10).astype(np.float32)
+    out = rt.eval_basis(basis, c).cpu().numpy()
+    ref = np.tensordot(basis.astype(np.float64), c.astype(np.float64), axes=(0, 0))
+    assert np.max(np.abs(out - ref)) < 1e-5 * np.max(np.abs(ref))
+    F = (rng.standard_normal((3, 40, 40)) + 1j * rng.standard_normal((3, 40, 40))).astype(
+        np.complex64)
+    Ft = torch.as_tensor(F).cuda()
+    pw = rt.power(Ft).cpu().numpy()
+    assert np.allclose(pw, np.sum(np.abs(F.astype(np.complex128)) ** 2, axis=(1, 2)), rtol=1e-5)
+    I = rt.intensity(Ft).cpu().numpy()
+    assert np.allclose(I, np.abs(F) ** 2, rtol=1e-6)
+    Is = rt.intensity(Ft, sum_batch=True).cpu().numpy()
+    assert np.allclose(Is, np.sum(np.abs(F) ** 2, 0), rtol=1e-5)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_psf_poly_small(rt, prec):
+    """Fused chain vs oracle.poly_psf: aperture + spiders + Zernikes, 8 wavelengths."""
+    print(f"[psf-poly] prec={prec}", flush=True)
+    n, D, n_out = 128, 1.0, 64
+    mask, basis = orc.circular_aperture(n, D, oversample=3, spider_width=0.02,
+                                        spider_angles=[0, 120, 240],
+                                        zernike_nolls=range(2, 12))
+    rng = np.random.default_rng(1)
+    coeffs = rng.normal(0, 30e-9, basis.shape[0])
+    wls = np.linspace(0.8e-6, 1.2e-6, 8)
+    w = np.ones(8) / 8
+    p_out = 1e-6 / (4 * D)
+    ref = orc.poly_psf(mask, basis, coeffs, wls, w, D, n_out, p_out)
+    opd = np.tensordot(basis, coeffs, axes=(0, 0))
+    norm = 1.0 / np.sqrt(np.sum((mask / n ** 2) ** 2))
+    psf = rt.psf_poly(n, n_out, wls, w, D / n, p_out, amp=mask, opd=opd,
+                      scale=norm / n ** 2, precision=prec).cpu().numpy()
+    err = _peak_err(psf, ref)
+    print(f"  peak-normalised error {err:.3e}")
+    assert err < 1e-5
