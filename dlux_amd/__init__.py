@@ -2,10 +2,25 @@
 
 Hand-written HIP kernels (``dlux_amd/csrc``) behind a C ABI
 (``include/dlux_amd.h``), driven from a Python layer that keeps dLux's
-optical-system / layer / propagator API.
+optical-system / layer / propagator API:
+
+    import dlux_amd as dl
+    import dlux_amd.utils as dlu
 """
 
-from . import runtime
+from . import runtime, utils
 from .runtime import set_precision, get_precision, precision
+from .base import Base
+from .coordinates import Spec, PadSpec, CoordSpec
+from .psfs import PSF
+from .wavefronts import Wavefront
+from . import layers
+from .layers import (OpticalLayer, TransmissiveLayer, AberratedLayer, BasisLayer, Tilt,
+                     Normalise, Optic, BasisOptic, Propagator, FFT, MFT, ASMPropagator)
+from .optical_systems import (BaseOpticalSystem, OpticalSystem, ParametricOpticalSystem,
+                              LayeredOpticalSystem, AngularOpticalSystem,
+                              CartesianOpticalSystem)
+from .spectra import Spectrum
+from .sources import PointSource, PointSources
 
 __version__ = "0.1.0"

@@ -52,6 +52,19 @@ SIGNATURES = {
     "dlx_psf_poly": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, c_double, c_double,
                                c_double, c_double, c_double, _fp, _fp, _fp, c_double, _fp,
                                c_int32, c_void_p, c_size_t, c_void_p]),
+    "dlx_noll_indices": (c_int32, [c_int32, POINTER(c_int32), POINTER(c_int32)]),
+    "dlx_zernike_basis": (c_int32, [c_int32, POINTER(c_int32), c_int64, _fp, _fp, c_double,
+                                    _fp, c_void_p]),
+    "dlx_circular_aperture": (c_int32, [c_int32, c_double, c_int32, c_double, c_double,
+                                        c_double, c_int32, _dp, _fp, _fp, c_void_p]),
+    "dlx_fft2_workspace_size": (c_size_t, [c_int32, c_int32, c_int32]),
+    "dlx_fft2": (c_int32, [c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p,
+                           c_size_t, c_void_p]),
+    "dlx_asm_workspace_size": (c_size_t, [c_int32, c_int32, c_int32]),
+    "dlx_asm_prop": (c_int32, [c_int32, c_int32, c_int32, _dp, c_double, c_double, c_void_p,
+                               c_void_p, c_void_p, c_size_t, c_void_p]),
+    "dlx_profile_enable": (c_int32, [c_int32]),
+    "dlx_profile_read": (c_int32, [c_int32, POINTER(c_double), POINTER(c_int32)]),
 }
 
 _lib = None

@@ -27,6 +27,40 @@ namespace {
 // Instances per fused launch of the polychromatic PSF (bounds the workspace).
 constexpr int kWlChunk = 64;
 
+// Optional live timing of the two MFT contraction launches of dlx_psf_poly:
+// hipEvents recorded on the caller's stream around each launch.
+struct ProfSlot {
+  hipEvent_t beg, end;
+  int stage;
+};
+constexpr int kProfSlots = 8192;
+struct Profiler {
+  int enabled = 0;
+  int n = 0;
+  std::vector<ProfSlot> slots;
+} g_prof;
+
+int prof_begin(int stage, hipStream_t s, int* slot) {
+  *slot = -1;
+  if (!g_prof.enabled || g_prof.n >= kProfSlots) return DLX_OK;
+  if ((int)g_prof.slots.size() <= g_prof.n) {
+    ProfSlot ps{};
+    DLX_HIP(hipEventCreate(&ps.beg));
+    DLX_HIP(hipEventCreate(&ps.end));
+    g_prof.slots.push_back(ps);
+  }
+  *slot = g_prof.n++;
+  g_prof.slots[*slot].stage = stage;
+  DLX_HIP(hipEventRecord(g_prof.slots[*slot].beg, s));
+  return DLX_OK;
+}
+
+int prof_end(int slot, hipStream_t s) {
+  if (slot < 0) return DLX_OK;
+  DLX_HIP(hipEventRecord(g_prof.slots[slot].end, s));
+  return DLX_OK;
+}
+
 int check_geom(const dlx_mft_geom* g) {
   DLX_CHECK_ARG(g != nullptr, "geom is NULL");
   DLX_CHECK_ARG(g->n_in > 0 && g->n_in <= 16384, "n_in must be in [1, 16384], got %d", g->n_in);
@@ -279,14 +313,44 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
     if (int st = gen_dft(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr,
                          amp_w.data(), W, s))
       return st;
+    int slot;
     GemmBatch g1{P, g.n_in, (long)g.n_in * g.n_in, W, g.n_in, mstride, Ut, g.n_in, mstride,
                  g.n_in, g.n_out, g.n_in, nb, nullptr, EPI_STORE_T, precision};
+    if (int st = prof_begin(1, s, &slot)) return st;
     DLX_HIP(launch_cgemm_nt(g1, s));
+    if (int st = prof_end(slot, s)) return st;
     GemmBatch g2{W, g.n_in, mstride, Ut, g.n_in, mstride, I, g.n_out, nout2, g.n_out, g.n_out,
                  g.n_in, nb, nullptr, EPI_INTENSITY, precision};
+    if (int st = prof_begin(2, s, &slot)) return st;
     DLX_HIP(launch_cgemm_nt(g2, s));
+    if (int st = prof_end(slot, s)) return st;
     DLX_HIP(launch_sum_planes(nout2, nb, I, l0 > 0, psf, s));
   }
+  return DLX_OK;
+}
+
+int dlx_profile_enable(int32_t enable) {
+  clear_error();
+  g_prof.enabled = enable ? 1 : 0;
+  g_prof.n = 0;
+  return DLX_OK;
+}
+
+int dlx_profile_read(int32_t stage, double* total_ms, int32_t* count) {
+  clear_error();
+  DLX_CHECK_ARG(total_ms && count, "null argument to dlx_profile_read");
+  double t = 0.0;
+  int c = 0;
+  for (int i = 0; i < g_prof.n; ++i) {
+    if (g_prof.slots[i].stage != stage) continue;
+    DLX_HIP(hipEventSynchronize(g_prof.slots[i].end));
+    float ms = 0.f;
+    DLX_HIP(hipEventElapsedTime(&ms, g_prof.slots[i].beg, g_prof.slots[i].end));
+    t += ms;
+    ++c;
+  }
+  *total_ms = t;
+  *count = c;
   return DLX_OK;
 }
 

@@ -1,0 +1,6 @@
+"""Optical layers (dLux.layers subset on the hot path)."""
+
+from .optical_layers import (OpticalLayer, TransmissiveLayer, AberratedLayer, BasisLayer,
+                             Tilt, Normalise)
+from .optics import Optic, BasisOptic
+from .propagators import Propagator, FFT, MFT, ASMPropagator

@@ -269,3 +269,71 @@ def psf_poly(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: float,
                           out.data_ptr(), prec, ws.data_ptr(), ws.numel(), stream_handle())
     _lib.check(st, "dlx_psf_poly")
     return out
+
+
+def _pow2(m: int) -> bool:
+    return m >= 2 and (m & (m - 1)) == 0 and m <= 8192
+
+
+def fft2(fields, m: int, inverse: bool = False) -> torch.Tensor:
+    """Centred zero-padded 2-D FFT (dLux normalisation) of (B, N, N) -> (B, m, m).
+
+    Power-of-two m runs the LDS FFT kernel (dlx_fft2).  Any other size runs
+    the same transform exactly as a matrix DFT through dlx_mft: with
+    wavelength 1, dx = 1/N, p_out = N/m and a half-pixel shift for even N, the
+    MFT's transfer matrices are the centred DFT matrices and its nfringes
+    normalisation is 1/m.
+    """
+    f = as_complex(fields)
+    squeeze = f.ndim == 2
+    if squeeze:
+        f = f[None]
+    b, n, _ = f.shape
+    m = int(m)
+    if (m - n) % 2 or m < n:
+        raise ValueError(f"padded size {m} must be >= {n} with even difference")
+    if _pow2(m):
+        lib = _lib.load()
+        ws = workspace(lib.dlx_fft2_workspace_size(n, m, b), f.device)
+        out = torch.empty((b, m, m), dtype=torch.complex64, device=f.device)
+        st = lib.dlx_fft2(n, m, b, int(bool(inverse)), f.data_ptr(), out.data_ptr(),
+                          ws.data_ptr(), ws.numel(), stream_handle())
+        _lib.check(st, "dlx_fft2")
+    else:
+        s = 0.5 if n % 2 == 0 else 0.0
+        out = mft(f, 1.0, 1.0 / n, m, n / m, shift_x=s, shift_y=s, inverse=inverse,
+                  precision="fp32")
+    return out[0] if squeeze else out
+
+
+def asm_prop(fields, wavelengths, pixel_scale: float, distance: float, m: int) -> torch.Tensor:
+    """Angular-spectrum propagation of (B, N, N) fields padded to (B, m, m)."""
+    f = as_complex(fields)
+    squeeze = f.ndim == 2
+    if squeeze:
+        f = f[None]
+    b, n, _ = f.shape
+    wl = _bcast(wavelengths, b)
+    lib = _lib.load()
+    ws = workspace(lib.dlx_asm_workspace_size(n, int(m), b), f.device)
+    out = torch.empty((b, int(m), int(m)), dtype=torch.complex64, device=f.device)
+    st = lib.dlx_asm_prop(n, int(m), b, _lib.dptr(wl), float(pixel_scale), float(distance),
+                          f.data_ptr(), out.data_ptr(), ws.data_ptr(), ws.numel(),
+                          stream_handle())
+    _lib.check(st, "dlx_asm_prop")
+    return out[0] if squeeze else out
+
+
+def profile_enable(enable: bool = True) -> None:
+    """Start (and reset) or stop live event timing of the MFT contraction launches."""
+    _lib.check(_lib.load().dlx_profile_enable(int(bool(enable))), "dlx_profile_enable")
+
+
+def profile_read(stage: int) -> tuple[float, int]:
+    """(total ms, launch count) of contraction stage 1 or 2 since profile_enable."""
+    import ctypes
+    t = ctypes.c_double()
+    c = ctypes.c_int32()
+    _lib.check(_lib.load().dlx_profile_read(int(stage), ctypes.byref(t), ctypes.byref(c)),
+               "dlx_profile_read")
+    return float(t.value), int(c.value)

@@ -162,6 +162,76 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
                  const float* phase, double scale, float* psf, int32_t precision,
                  void* workspace, size_t workspace_bytes, void* stream);
 
+/*
+ * Noll index j -> radial order n and signed azimuthal order m.
+ * Replaces: noll_indices, utils/zernikes.py:99-119.
+ */
+int dlx_noll_indices(int32_t j, int32_t* n, int32_t* m);
+
+/*
+ * Zernike basis on arbitrary coordinates: out[q][p] = Z_{js[q]}(x[p], y[p])
+ * on the disk of the given diameter (js: HOST int array; x, y, out: device
+ * float32, npix points each; out is nmodes x npix).
+ * Replaces: zernike_basis / zernike, utils/zernikes.py:227-315.
+ */
+int dlx_zernike_basis(int32_t nmodes, const int32_t* js, int64_t npix, const float* x,
+                      const float* y, double diameter, float* out, void* stream);
+
+/*
+ * Static circular aperture, npix x npix float32: a hard circle of the given
+ * diameter, an optional secondary obscuration (secondary_diameter <= 0: none)
+ * and n_spiders one-sided vanes of width spider_width at the given angles
+ * (degrees, HOST array), evaluated on an oversample x oversample sub-grid of
+ * pixel_coords(npix*oversample, array_diameter) and box-averaged.
+ * support (may be NULL) receives 1 where any sub-pixel is inside the primary.
+ * array_diameter <= 0 means `diameter`.
+ * Replaces: circular_aperture (transmission and support), utils/apertures.py:153-242.
+ */
+int dlx_circular_aperture(int32_t npix, double diameter, int32_t oversample,
+                          double secondary_diameter, double array_diameter,
+                          double spider_width, int32_t n_spiders,
+                          const double* spider_angles_deg, float* transmission,
+                          float* support, void* stream);
+
+/*
+ * Centred, zero-padded 2-D FFT of `batch` N x N complex64 fields to M x M
+ * (M - N even, M a power of two up to 8192):
+ *   forward: out = fftshift(fft2(ifftshift(pad_M(in)))) / M
+ *   inverse: out = fftshift(ifft2(ifftshift(pad_M(in)))) * M
+ * Replaces: FFT, utils/propagation.py:8-64 (Wavefront.propagate_FFT,
+ * wavefronts.py:654-727).  Returns DLX_ERR_UNSUPPORTED for other M.
+ */
+size_t dlx_fft2_workspace_size(int32_t n, int32_t m, int32_t batch);
+int dlx_fft2(int32_t n, int32_t m, int32_t batch, int32_t inverse, const void* in,
+             void* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Angular-spectrum free-space propagation of `batch` N x N complex64 fields,
+ * zero-padded to M x M (same M rules as dlx_fft2), over `distance` metres:
+ *   out = IFFT{ FFT{pad_M(in)} * H },  H = exp(2 pi i z sqrt(1/l^2 - fx^2 - fy^2))
+ * on propagating frequencies and 0 on evanescent ones; fx, fy = (i - M/2)/(M dx).
+ * Output keeps the input sampling dx and has M x M pixels (no crop).
+ * Replaces: abcdLux.asm.asm_prop as called by ASMPropagator
+ * (layers/abcd_propagators.py:444-451).  abcdLux is not available, so this
+ * follows the textbook transfer-function method.
+ */
+size_t dlx_asm_workspace_size(int32_t n, int32_t m, int32_t batch);
+int dlx_asm_prop(int32_t n, int32_t m, int32_t batch, const double* wavelength,
+                 double pixel_scale, double distance, const void* in, void* out,
+                 void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Diagnostics (no counterpart in the reference): live timing of the MFT
+ * contraction launches inside dlx_psf_poly.  While enabled, each stage-1
+ * (pupil x transfer matrix) and stage-2 (transfer matrix x intermediate, with
+ * the |E|^2 epilogue) launch is bracketed by hipEvents on the caller's stream
+ * (at most 8192 launches are recorded).  enable resets the record;
+ * dlx_profile_read waits for the recorded events and returns the summed
+ * milliseconds and the launch count of one stage (1 or 2).  Not thread-safe.
+ */
+int dlx_profile_enable(int32_t enable);
+int dlx_profile_read(int32_t stage, double* total_ms, int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

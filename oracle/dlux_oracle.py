@@ -490,3 +490,26 @@ def asm_prop(u_in, dx, wavelength, z, npad):
     arg = 1.0 / wavelength ** 2 - FX ** 2 - FY ** 2
     H = np.where(arg > 0, np.exp(2j * np.pi * z * np.sqrt(np.maximum(arg, 0.0))), 0.0)
     return np.fft.fftshift(np.fft.ifft2(U * H))
+
+
+def propagate_fft(phasor, wavelength, pixel_scale, center=0.0, pad=2, focal_length=None,
+                  spec_c=None, inverse=False):
+    """Wavefront.propagate_FFT with an optional output centre.  wavefronts.py:654-727.
+
+    Returns (phasor, pixel_scale, center).
+    """
+    n_in = phasor.shape[-1]
+    n_out = n_in * pad
+    d_fft, c_fft = fft_spec(n_out, pixel_scale, wavelength, focal_length)
+    if spec_c is not None:
+        in_ramp = fft_phase_ramp(spec_xs(n_in, pixel_scale, center), wavelength,
+                                 c_fft - spec_c, focal_length, inverse)
+        shift = fft_spec(n_out, d_fft, wavelength, focal_length)[1]
+        out_ramp = fft_phase_ramp(spec_xs(n_out, d_fft, spec_c), wavelength, shift,
+                                  focal_length, inverse)
+        c_out = spec_c
+    else:
+        in_ramp, out_ramp, c_out = 1.0, 1.0, c_fft
+    out, ps = fft(np.asarray(phasor) * in_ramp, wavelength, pixel_scale, focal_length, pad,
+                  inverse)
+    return out * out_ramp, ps, c_out

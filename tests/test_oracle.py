@@ -1,0 +1,303 @@
+"""Pins of the FP64 oracle (CPU only).
+
+dLux cannot be imported here, so every oracle function is pinned by something
+independent of its own formula: the reference tests' known answers, closed
+forms (Dirichlet kernel, textbook Zernikes), brute-force physical sums on tiny
+grids, and invariants (adjoint identity, Parseval, shift equivalences, finite
+differences).  Each check is chosen so that a plausible slip (a sign, a
+transposed operand, a half-pixel offset, a dropped normalisation) fails it.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import dlux_oracle as orc
+
+
+# ---------------------------------------------------------------- Zernikes
+def test_noll_known_indices():
+    """tests/utils/test_zernike.py:41-52."""
+    expected = [(0, 0), (1, 1), (1, -1), (2, 0), (2, -2), (2, 2), (3, -1), (3, 1), (3, -3),
+                (3, 3)]
+    assert [orc.noll_indices(j) for j in range(1, 11)] == expected
+
+
+def test_noll_rows_cover_each_order_once():
+    for n in range(0, 9):
+        js = range(n * (n + 1) // 2 + 1, (n + 1) * (n + 2) // 2 + 1)
+        ms = sorted(orc.noll_indices(j)[1] for j in js)
+        assert all(orc.noll_indices(j)[0] == n for j in js)
+        assert ms == sorted(range(-n, n + 1, 2))
+
+
+def test_textbook_zernikes_on_unit_disk():
+    coords = orc.pixel_coords(64, 2.0)          # unit disk: diameter 2
+    x, y = coords
+    r2 = x ** 2 + y ** 2
+    inside = np.sqrt(r2) <= 1.0
+    textbook = {
+        2: 2 * x, 3: 2 * y, 4: np.sqrt(3) * (2 * r2 - 1), 5: 2 * np.sqrt(6) * x * y,
+        6: np.sqrt(6) * (x ** 2 - y ** 2), 11: np.sqrt(5) * (6 * r2 ** 2 - 6 * r2 + 1),
+        7: np.sqrt(8) * (3 * r2 - 2) * y, 8: np.sqrt(8) * (3 * r2 - 2) * x,
+    }
+    for j, z in textbook.items():
+        got = orc.zernike(j, coords, 2.0)
+        assert np.allclose(got, z * inside, atol=1e-12), j
+
+
+def test_zernike_discrete_orthonormality():
+    n = 512
+    coords = orc.pixel_coords(n, 2.0)
+    Z = orc.zernike_basis(range(1, 16), coords, 2.0).reshape(15, -1)
+    inside = (np.hypot(*coords) <= 1).reshape(-1)
+    G = (Z[:, inside] @ Z[:, inside].T) / inside.sum()
+    assert np.allclose(G, np.eye(15), atol=2e-2)
+
+
+def test_zernike_nan_at_origin_for_negative_m_on_odd_grid():
+    """Signed-m extra terms: 0 * rho^(-|m|) is NaN at rho == 0 (utils/zernikes.py:122-175)."""
+    coords = orc.pixel_coords(5, 2.0)
+    z3 = orc.zernike(3, coords, 2.0)
+    assert np.isnan(z3[2, 2]) and np.isfinite(np.delete(z3.ravel(), 12)).all()
+    assert np.isfinite(orc.zernike(2, coords, 2.0)).all()
+
+
+def test_eval_basis_is_tensordot():
+    """tests/utils/test_math.py:80-135."""
+    rng = np.random.default_rng(0)
+    b = rng.normal(size=(3, 4, 6, 6))
+    c = rng.normal(size=(3, 4))
+    assert np.allclose(orc.eval_basis(b, c), np.tensordot(b, c, axes=((0, 1), (0, 1))))
+
+
+# --------------------------------------------------------------------- MFT
+def _dirichlet(b, n):
+    with np.errstate(invalid="ignore", divide="ignore"):
+        d = np.sin(np.pi * b) / np.sin(np.pi * b / n)
+    return np.where(np.abs(np.sin(np.pi * b / n)) < 1e-15, float(n), d)
+
+
+@pytest.mark.parametrize("n_in,n_out,p", [(32, 16, 0.03), (64, 33, 0.21), (17, 40, 0.5)])
+def test_mft_uniform_field_is_dirichlet(n_in, n_out, p):
+    """Closed form: a uniform field gives c D(b_y) D(b_x), D(b) = sin(pi b)/sin(pi b/N)."""
+    wl, dx = 1.0, 1.0 / n_in
+    E = np.ones((n_in, n_in))
+    out = orc.mft(E, wl, dx, n_out, p)
+    so = p / (wl / (dx * n_in))
+    b = (np.arange(n_out) - (n_out - 1) / 2) * so
+    c = orc.calc_nfringes(wl, n_in, dx, n_out, p) / (n_in * n_out)
+    ref = c * np.outer(_dirichlet(b, n_in), _dirichlet(b, n_in))
+    assert np.allclose(out, ref, atol=1e-12 * np.abs(ref).max())
+
+
+def test_mft_brute_force_fraunhofer_sum():
+    """Direct physical sum E(u, v) = c sum E(y, x) exp(-2 pi i (x u + y v) / lambda)."""
+    rng = np.random.default_rng(1)
+    n, n_out, wl, dx, p = 6, 5, 0.8e-6, 0.01, 3e-5
+    E = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    xs = (np.arange(n) - (n - 1) / 2) * dx
+    us = (np.arange(n_out) - (n_out - 1) / 2) * p
+    c = orc.calc_nfringes(wl, n, dx, n_out, p) / (n * n_out)
+    ref = np.zeros((n_out, n_out), complex)
+    for iv, v in enumerate(us):
+        for iu, u in enumerate(us):
+            for iy, yy in enumerate(xs):
+                for ix, xx in enumerate(xs):
+                    ref[iv, iu] += E[iy, ix] * np.exp(-2j * np.pi * (xx * u + yy * v) / wl)
+    assert np.allclose(orc.mft(E, wl, dx, n_out, p), c * ref, atol=1e-12)
+
+
+def test_mft_focal_length_and_inverse_brute_force():
+    rng = np.random.default_rng(2)
+    n, n_out, wl, dx, p, f = 5, 4, 1e-6, 0.02, 2e-6, 3.0
+    E = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    xs = (np.arange(n) - (n - 1) / 2) * dx
+    us = (np.arange(n_out) - (n_out - 1) / 2) * p / f
+    c = orc.calc_nfringes(wl, n, dx, n_out, p, f) / (n * n_out)
+    ref = np.zeros((n_out, n_out), complex)
+    for iv, v in enumerate(us):
+        for iu, u in enumerate(us):
+            ph = np.exp(+2j * np.pi * (xs[None, :] * u + xs[:, None] * v) / wl)
+            ref[iv, iu] = np.sum(E * ph)
+    out = orc.mft(E, wl, dx, n_out, p, focal_length=f, inverse=True)
+    assert np.allclose(out, c * ref, atol=1e-12)
+
+
+def test_mft_shift_units_are_consistent():
+    """tests/utils/test_propagation.py:141-176 (pixel vs pixel-scale shifts)."""
+    E = np.ones((32, 32), complex)
+    s = np.array([1.0, -2.0])
+    a = orc.mft(E, 1.0, 0.1, 16, 0.05, 2.0, s, pixel=True)
+    b = orc.mft(E, 1.0, 0.1, 16, 0.05, 2.0, s * 0.05, pixel=False)
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-12)
+
+
+def test_mft_integer_shift_moves_intensity_window():
+    rng = np.random.default_rng(3)
+    E = rng.normal(size=(24, 24)) + 1j * rng.normal(size=(24, 24))
+    big = np.abs(orc.mft(E, 1.0, 1 / 24, 40, 0.3)) ** 2
+    small = np.abs(orc.mft(E, 1.0, 1 / 24, 20, 0.3, shift=(3, -2))) ** 2
+    # sample l of the shifted grid sits at (l - (N-1)/2 - s) px, i.e. big index
+    # l + (40 - 20)/2 - s: the PSF moves by +s pixels (x: s = 3, y: s = -2)
+    assert np.allclose(small, big[10 + 2:30 + 2, 10 - 3:30 - 3], rtol=1e-10, atol=1e-14)
+
+
+def test_mft_equals_centred_fft_at_fft_sampling():
+    """With N_out = pad N, p_out = lambda/(pad N dx) and a half-pixel shift the MFT is the
+    centred FFT, including its 1/M normalisation."""
+    rng = np.random.default_rng(4)
+    n, pad, wl, dx = 16, 2, 1e-6, 1e-3
+    E = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    F, _ = orc.fft(E, wl, dx, pad=pad)
+    M = orc.mft(E, wl, dx, n * pad, wl / (pad * n * dx), shift=(0.5, 0.5))
+    assert np.allclose(np.abs(M) ** 2, np.abs(F) ** 2, atol=1e-12)
+
+
+def test_mft_adjoint_identity():
+    """<MFT x, y> = <x, MFT^H y> with MFT^H y = c conj(Y) y conj(X)^T (the gradient's adjoint)."""
+    rng = np.random.default_rng(5)
+    n, n_out, wl, dx, p = 20, 12, 0.9e-6, 1 / 20, 2e-7
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    y = rng.normal(size=(n_out, n_out)) + 1j * rng.normal(size=(n_out, n_out))
+    T = orc.transfer_matrix(wl, n, dx, n_out, p)
+    c = orc.calc_nfringes(wl, n, dx, n_out, p) / (n * n_out)
+    lhs = np.vdot(orc.mft(x, wl, dx, n_out, p), y)
+    rhs = np.vdot(x, c * np.conj(T) @ y @ np.conj(T).T)
+    assert np.isclose(lhs, rhs, rtol=1e-12)
+
+
+# --------------------------------------------------------------------- FFT
+def test_fft_round_trip_reference_fixture():
+    """tests/utils/test_propagation.py:74-92."""
+    ph = np.ones((32, 32), complex)
+    f, _ = orc.fft(ph, 1.0, 0.1, None, pad=1)
+    b, _ = orc.fft(f, 1.0, 0.1, None, pad=1, inverse=True)
+    assert np.allclose(b, ph, rtol=1e-6, atol=1e-6)
+
+
+def test_fft_output_shape_and_scale():
+    """tests/utils/test_propagation.py:59-72 and fft_spec focal scaling."""
+    out, ps = orc.fft(np.ones((32, 32)), 1.0, 0.1, 2.0, pad=2)
+    assert out.shape == (64, 64)
+    assert np.isclose(ps, 1.0 / (0.1 * 32) / 2 * 2.0)
+
+
+@pytest.mark.parametrize("n,pad", [(6, 2), (5, 3), (7, 2)])
+def test_fft_brute_force_and_parseval(n, pad):
+    rng = np.random.default_rng(n)
+    E = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    out, _ = orc.fft(E, 1.0, 1.0, pad=pad)
+    M = out.shape[0]
+    p = (M - n) // 2
+    h = M // 2
+    ref = np.zeros((M, M), complex)
+    idx = np.arange(n) + p - h                       # centred input indices
+    k = np.arange(M) - h                             # centred output indices
+    W = np.exp(-2j * np.pi * np.outer(k, idx) / M)
+    ref = W @ E @ W.T / M
+    assert np.allclose(out, ref, atol=1e-12)
+    assert np.isclose(np.sum(np.abs(out) ** 2), np.sum(np.abs(E) ** 2))
+
+
+def test_fft_spec_and_ramps():
+    """tests/utils/test_fourier.py:58-105."""
+    d, c = orc.fft_spec(16, 0.1, 1.0)
+    assert np.isfinite(d) and c != 0.0
+    assert orc.fft_spec(15, 0.1, 1.0)[1] == 0.0
+    assert np.isclose(orc.fft_spec(16, 0.1, 1.0, 2.0)[0], 2.0 * d)
+    xs = np.linspace(-0.5, 0.5, 16) * 0.1
+    s = np.array([1.0, -2.0])
+    fwd = orc.fft_phase_ramp(xs, 1.0, s[0])
+    inv = orc.fft_phase_ramp(xs, 1.0, s[0], inverse=True)
+    assert fwd.shape == (16, 16) and np.allclose(fwd, np.conj(inv))
+    assert not np.allclose(fwd, orc.fft_phase_ramp(xs, 1.0, s[0], focal_length=2.0))
+
+
+def test_propagate_fft_centred_spec_matches_mft():
+    """propagate_FFT with spec c = 0 samples the field on a grid centred on 0, which is the
+    MFT's centred grid at the same sampling: complex fields agree."""
+    rng = np.random.default_rng(6)
+    n, wl = 16, 1e-6
+    dx = 1.0 / n
+    E = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    F, ps, c = orc.propagate_fft(E, wl, dx, pad=2, spec_c=0.0)
+    M = orc.mft(E, wl, dx, 2 * n, ps)
+    assert np.allclose(np.abs(F), np.abs(M), atol=1e-12)
+
+
+# ------------------------------------------------------------------ masks
+def test_circle_area_and_spider_one_sided():
+    t = orc.circular_aperture(256, 1.0, oversample=4)
+    assert abs(t.sum() * (1 / 256) ** 2 - np.pi / 4) < 2e-3
+    coords = np.array([[[0.0, 0.0, 0.3]], [[-0.3, 0.3, 0.0]]])  # (x, y) points
+    s = orc.spider(coords, 0.1, [0.0])
+    # the vane at 0 deg occupies |x| < w/2, y < 0 only
+    assert s.tolist() == [[0.0, 1.0, 1.0]]
+
+
+def test_downsample_and_pad_crop():
+    a = np.arange(16.0).reshape(4, 4)
+    assert np.allclose(orc.downsample(a, 2), [[2.5, 4.5], [10.5, 12.5]])
+    assert np.allclose(orc.crop_to(orc.pad_to(a, 8), 4), a)
+    with pytest.raises(ValueError):
+        orc.pad_to(a, 7)
+
+
+# ------------------------------------------------------------------ chain
+def test_normalise_gives_unit_power():
+    """tests/layers/test_optics.py:47-56."""
+    E = orc.normalise(orc.wavefront_init(16) * orc.circular_aperture(16, 1.0))
+    assert np.isclose(np.sum(np.abs(E) ** 2), 1.0)
+
+
+def test_poly_psf_equal_wavelengths_and_weight_sum():
+    mask, basis = orc.circular_aperture(32, 1.0, oversample=1, zernike_nolls=range(2, 8))
+    c = np.random.default_rng(7).normal(0, 50e-9, basis.shape[0])
+    mono = orc.poly_psf(mask, basis, c, [1e-6], [1.0], 1.0, 16, 3e-7)
+    poly = orc.poly_psf(mask, basis, c, [1e-6] * 3, [0.2, 0.3, 0.1], 1.0, 16, 3e-7)
+    assert np.allclose(poly, 0.6 * mono, rtol=1e-12)
+
+
+def test_tilt_by_whole_pixels_shifts_psf():
+    mask = orc.circular_aperture(32, 1.0, oversample=1)
+    p = 2.5e-7
+    a = orc.poly_psf(mask, None, None, [1e-6], [1.0], 1.0, 24, p)
+    b = orc.poly_psf(mask, None, None, [1e-6], [1.0], 1.0, 24, p, offset=(2 * p, -3 * p))
+    # +x tilt moves the PSF to larger x index; +y to larger y index
+    assert np.allclose(b[:-3 or None, 2:][0:21, 0:22], a[3:, :-2][0:21, 0:22], atol=1e-15)
+
+
+def test_gradient_matches_central_differences():
+    rng = np.random.default_rng(8)
+    mask, basis = orc.circular_aperture(32, 1.0, oversample=1, zernike_nolls=range(2, 7))
+    c = rng.normal(0, 60e-9, basis.shape[0])
+    wls, w = np.array([0.9e-6, 1.1e-6]), np.array([0.4, 0.6])
+    data = rng.poisson(1e4 * orc.poly_psf(mask, basis, c * 0.5, wls, w, 1.0, 16, 3e-7))
+    loss, grad = orc.poly_psf_grad(mask, basis, c, wls, w, 1.0, 16, 3e-7, data, 1e4)
+    h = 1e-3 * 1e-9
+    fd = []
+    for j in range(len(c)):
+        cp, cm = c.copy(), c.copy()
+        cp[j] += h
+        cm[j] -= h
+        lp = orc.gaussian_loss(orc.poly_psf(mask, basis, cp, wls, w, 1.0, 16, 3e-7), data, 1e4)
+        lm = orc.gaussian_loss(orc.poly_psf(mask, basis, cm, wls, w, 1.0, 16, 3e-7), data, 1e4)
+        fd.append((lp - lm) / (2 * h))
+    assert np.allclose(grad, fd, rtol=1e-5)
+
+
+# -------------------------------------------------------------------- ASM
+def test_asm_invariants():
+    """Textbook pins only (abcdLux absent): z = 0 identity, unitarity on a band-limited
+    field, and z1 + z2 composition."""
+    rng = np.random.default_rng(9)
+    n, dx, wl = 32, 2e-6, 1e-6
+    x = (np.arange(n) - (n - 1) / 2) * dx
+    X, Y = np.meshgrid(x, x)
+    u = np.exp(-(X ** 2 + Y ** 2) / (2 * (8 * dx) ** 2)) * np.exp(1j * rng.normal(size=(n, n)))
+    assert np.allclose(orc.asm_prop(u, dx, wl, 0.0, 64), orc.pad_to(u, 64), atol=1e-12)
+    # every frequency propagates when dx > lambda/sqrt(2)... keep band-limited: dx = 2 lambda
+    a = orc.asm_prop(u, dx, wl, 1e-4, 32)
+    assert np.isclose(np.sum(np.abs(a) ** 2), np.sum(np.abs(u) ** 2))
+    b = orc.asm_prop(orc.asm_prop(u, dx, wl, 3e-5, 32), dx, wl, 7e-5, 32)
+    assert np.allclose(a, b, atol=1e-12)
