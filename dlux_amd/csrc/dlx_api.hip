@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "dlx_internal.h"
+#include "dlx_split.h"
 
 namespace dlx {
 
@@ -73,12 +74,20 @@ int check_geom(const dlx_mft_geom* g) {
   return DLX_OK;
 }
 
+}  // namespace
+
+// Bytes of one batch instance of an R x K complex operand.
+size_t mat_bytes(long rows, long k, bool split) {
+  return split ? (size_t)rows * split4_row_bytes(k) : (size_t)rows * k * sizeof(float2);
+}
+
 // Generate `nb` transfer matrices with per-instance (lambda, dx, p_out, shift,
-// amplitude) into out[b] (stride N_out*N_in), in launches of DftChunk::kMax.
+// amplitude) into out[b] -- complex64 or split4 -- in launches of DftChunk::kMax.
 int gen_dft(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
-            const double* pout, const double* shift, const double* amp, float2* out,
+            const double* pout, const double* shift, const double* amp, void* out, bool split,
             hipStream_t s) {
   const long stride = (long)g.n_out * g.n_in;
+  const long bbytes = (long)mat_bytes(g.n_out, g.n_in, true);
   for (int b0 = 0; b0 < nb; b0 += DftChunk::kMax) {
     const int n = nb - b0 < DftChunk::kMax ? nb - b0 : DftChunk::kMax;
     DftChunk c;
@@ -94,8 +103,104 @@ int gen_dft(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
       c.b_off[i] = p.b_off;
       c.amp[i] = amp ? amp[b] : 1.0;
     }
-    DLX_HIP(launch_dft_chunk(c, n, g.n_in, g.n_out, g.inverse, out + (long)b0 * stride, stride,
-                             s));
+    if (split) {
+      DLX_HIP(launch_dft_split4(c, n, g.n_in, g.n_out, g.inverse,
+                                (uint8_t*)out + (long)b0 * bbytes, bbytes, s));
+    } else {
+      DLX_HIP(launch_dft_chunk(c, n, g.n_in, g.n_out, g.inverse,
+                               (float2*)out + (long)b0 * stride, stride, s));
+    }
+  }
+  return DLX_OK;
+}
+
+namespace {
+
+// Transfer matrices for `nb` instances: from the device cache, or generated
+// into `fallback` when the cache is bypassed (graph capture, no memory).
+int get_matrices(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
+                 const double* pout, const double* shift, bool split, void* fallback,
+                 hipStream_t s, void** out, void** handle) {
+  if (int st = cache_acquire(g, nb, wl, dx, pout, shift, split, s, out, handle)) return st;
+  if (*out) return DLX_OK;
+  *out = fallback;
+  return gen_dft(g, nb, wl, dx, pout, shift, nullptr, fallback, split, s);
+}
+
+// FP32 two-stage MFT of `batch` complex64 fields P (N_in x N_in) against
+// complex64 transfer matrices Wx / Wy (N_out x N_in):
+//   stage 1: U^T = (P Wx^T)^T (complex64, N_out x N_in)
+//   stage 2: E = alpha_b Wy U, stored complex64 (EPI_STORE) or |E|^2 (EPI_INTENSITY).
+int mft_fp32(const dlx_mft_geom& g, int batch, const float2* P, const float2* Wx,
+             const float2* Wy, float2* U, const double* alpha, void* out, int epi,
+             hipStream_t s, bool prof) {
+  const long pstride = (long)g.n_in * g.n_in;
+  const long mstride = (long)g.n_out * g.n_in;
+  const long nout2 = (long)g.n_out * g.n_out;
+  for (int b0 = 0; b0 < batch; b0 += GemmBatch::kMaxBatch) {
+    const int nb = batch - b0 < GemmBatch::kMaxBatch ? batch - b0 : GemmBatch::kMaxBatch;
+    GemmBatch g1{};
+    g1.A = P + (long)b0 * pstride; g1.lda = g.n_in; g1.sA = pstride;
+    g1.B = Wx + (long)b0 * mstride; g1.ldb = g.n_in; g1.sB = mstride;
+    g1.C = U + (long)b0 * mstride; g1.ldc = g.n_in; g1.sC = mstride;
+    g1.M = g.n_in; g1.N = g.n_out; g1.K = g.n_in; g1.batch = nb;
+    g1.epilogue = EPI_STORE_T; g1.precision = DLX_PREC_FP32;
+    for (int i = 0; i < nb; ++i) g1.alpha[i] = 1.0f;
+    int slot = -1;
+    if (prof) { if (int st = prof_begin(1, s, &slot)) return st; }
+    DLX_HIP(launch_cgemm_nt(g1, s));
+    if (prof) { if (int st = prof_end(slot, s)) return st; }
+    GemmBatch g2{};
+    g2.A = Wy + (long)b0 * mstride; g2.lda = g.n_in; g2.sA = mstride;
+    g2.B = U + (long)b0 * mstride; g2.ldb = g.n_in; g2.sB = mstride;
+    g2.C = epi == EPI_STORE ? (void*)((float2*)out + (long)b0 * nout2)
+                            : (void*)((float*)out + (long)b0 * nout2);
+    g2.ldc = g.n_out; g2.sC = nout2;
+    g2.M = g.n_out; g2.N = g.n_out; g2.K = g.n_in; g2.batch = nb;
+    g2.epilogue = epi; g2.precision = DLX_PREC_FP32;
+    for (int i = 0; i < nb; ++i) g2.alpha[i] = (float)alpha[b0 + i];
+    if (prof) { if (int st = prof_begin(2, s, &slot)) return st; }
+    DLX_HIP(launch_cgemm_nt(g2, s));
+    if (prof) { if (int st = prof_end(slot, s)) return st; }
+  }
+  return DLX_OK;
+}
+
+// Split-BF16x3 two-stage MFT of `batch` split4 pupils P (N_in rows, k = N_in)
+// against split4 transfer matrices Wx / Wy (N_out rows, k = N_in):
+//   stage 1: U^T = (P Wx^T)^T written split4 (N_out rows, k = N_in)
+//   stage 2: E = alpha_b Wy U, stored complex64 (EPI_STORE) or |E|^2 (EPI_INTENSITY).
+int mft_split4(const dlx_mft_geom& g, int batch, const uint8_t* P, const uint8_t* Wx,
+               const uint8_t* Wy, uint8_t* U, const double* alpha, void* out, int epi,
+               hipStream_t s, int (*pb)(int, hipStream_t, int*), int (*pe)(int, hipStream_t)) {
+  const long pbytes = (long)mat_bytes(g.n_in, g.n_in, true);
+  const long wbytes = (long)mat_bytes(g.n_out, g.n_in, true);
+  const long rb = split4_row_bytes(g.n_in);
+  const long nout2 = (long)g.n_out * g.n_out;
+  for (int b0 = 0; b0 < batch; b0 += GemmSplit::kMaxBatch) {
+    const int nb = batch - b0 < GemmSplit::kMaxBatch ? batch - b0 : GemmSplit::kMaxBatch;
+    GemmSplit g1{};
+    g1.A = P + (long)b0 * pbytes; g1.a_row_bytes = rb; g1.a_batch_bytes = pbytes;
+    g1.B = Wx + (long)b0 * wbytes; g1.b_row_bytes = rb; g1.b_batch_bytes = wbytes;
+    g1.C = U + (long)b0 * wbytes; g1.ldc = rb; g1.c_batch = wbytes;
+    g1.M = g.n_in; g1.N = g.n_out; g1.K = g.n_in; g1.batch = nb; g1.epilogue = EPI_SPLIT_T;
+    for (int i = 0; i < nb; ++i) g1.alpha[i] = 1.0f;
+    int slot = -1;
+    if (pb) { if (int st = pb(1, s, &slot)) return st; }
+    DLX_HIP(launch_cgemm_split4(g1, s));
+    if (pe) { if (int st = pe(slot, s)) return st; }
+    GemmSplit g2{};
+    g2.A = Wy + (long)b0 * wbytes; g2.a_row_bytes = rb; g2.a_batch_bytes = wbytes;
+    g2.B = U + (long)b0 * wbytes; g2.b_row_bytes = rb; g2.b_batch_bytes = wbytes;
+    g2.ldc = g.n_out;
+    g2.c_batch = nout2;
+    g2.C = epi == EPI_STORE ? (void*)((float2*)out + (long)b0 * nout2)
+                            : (void*)((float*)out + (long)b0 * nout2);
+    g2.M = g.n_out; g2.N = g.n_out; g2.K = g.n_in; g2.batch = nb; g2.epilogue = epi;
+    for (int i = 0; i < nb; ++i) g2.alpha[i] = (float)alpha[b0 + i];
+    if (pb) { if (int st = pb(2, s, &slot)) return st; }
+    DLX_HIP(launch_cgemm_split4(g2, s));
+    if (pe) { if (int st = pe(slot, s)) return st; }
   }
   return DLX_OK;
 }
@@ -121,14 +226,16 @@ int dlx_transfer_matrix(const dlx_mft_geom* geom, int32_t batch, const double* w
   DLX_CHECK_ARG(wavelength && pixel_scale_in && pixel_scale_out && out,
                 "null argument to dlx_transfer_matrix");
   return gen_dft(*geom, batch, wavelength, pixel_scale_in, pixel_scale_out, shift, nullptr,
-                 (float2*)out, (hipStream_t)stream);
+                 out, false, (hipStream_t)stream);
 }
 
 size_t dlx_mft_workspace_size(const dlx_mft_geom* geom, int32_t batch, int32_t precision) {
-  (void)precision;
   if (!geom || batch <= 0) return 0;
-  const size_t mat = align_up((size_t)batch * geom->n_out * geom->n_in * sizeof(float2));
-  return 3 * mat;  // W_x, W_y, U^T
+  const bool split = precision == DLX_PREC_BF16X3;
+  const size_t mat = align_up((size_t)batch * mat_bytes(geom->n_out, geom->n_in, split));
+  size_t total = 3 * mat;  // W_x, W_y, U^T
+  if (split) total += align_up((size_t)batch * mat_bytes(geom->n_in, geom->n_in, true));
+  return total;
 }
 
 int dlx_mft(const dlx_mft_geom* geom, int32_t batch, const void* in, void* out,
@@ -151,50 +258,48 @@ int dlx_mft(const dlx_mft_geom* geom, int32_t batch, const void* in, void* out,
   }
   const dlx_mft_geom& g = *geom;
   hipStream_t s = (hipStream_t)stream;
-  const size_t mat = align_up((size_t)batch * g.n_out * g.n_in * sizeof(float2));
-  float2* Wx = (float2*)workspace;
-  float2* Wy = (float2*)((char*)workspace + mat);
-  float2* Ut = (float2*)((char*)workspace + 2 * mat);
+  const bool split = precision == DLX_PREC_BF16X3;
+  const size_t mat = align_up((size_t)batch * mat_bytes(g.n_out, g.n_in, split));
+  uint8_t* Wx_ws = (uint8_t*)workspace;
+  uint8_t* Wy_ws = Wx_ws + mat;
+  uint8_t* Ut = Wx_ws + 2 * mat;
 
-  // Fold c_b * field_scale_b into the matrices: sqrt on each side when the two
-  // axes share a matrix, the whole factor on W_y otherwise.
   bool same = true;
   for (int b = 0; b < batch && same; ++b) {
     const double sx = shift_x ? shift_x[b] : 0.0, sy = shift_y ? shift_y[b] : 0.0;
     same = (sx == sy);
   }
-  std::vector<double> ax(batch), ay(batch);
-  for (int b = 0; b < batch; ++b) {
-    const double c = mft_norm(g, wavelength[b], pixel_scale_in[b], pixel_scale_out[b]) *
-                     (field_scale ? field_scale[b] : 1.0);
-    if (same) {
-      ax[b] = std::sqrt(std::fabs(c));
-      ay[b] = c < 0 ? -ax[b] : ax[b];
-    } else {
-      ax[b] = 1.0;
-      ay[b] = c;
-    }
-  }
-  if (int st = gen_dft(g, batch, wavelength, pixel_scale_in, pixel_scale_out, shift_x,
-                       ax.data(), Wx, s))
+  // c_b * field_scale_b is applied in the stage-2 epilogue: the matrices are
+  // unscaled and therefore cacheable.
+  std::vector<double> c(batch);
+  for (int b = 0; b < batch; ++b)
+    c[b] = mft_norm(g, wavelength[b], pixel_scale_in[b], pixel_scale_out[b]) *
+           (field_scale ? field_scale[b] : 1.0);
+
+  void *Wx = nullptr, *Wy = nullptr, *hx = nullptr, *hy = nullptr;
+  if (int st = get_matrices(g, batch, wavelength, pixel_scale_in, pixel_scale_out, shift_x,
+                            split, Wx_ws, s, &Wx, &hx))
     return st;
-  if (same && std::equal(ax.begin(), ax.end(), ay.begin())) {
+  if (same) {
     Wy = Wx;
-  } else {
-    if (int st = gen_dft(g, batch, wavelength, pixel_scale_in, pixel_scale_out, shift_y,
-                         ay.data(), Wy, s))
-      return st;
+  } else if (int st = get_matrices(g, batch, wavelength, pixel_scale_in, pixel_scale_out,
+                                   shift_y, split, Wy_ws, s, &Wy, &hy)) {
+    return st;
   }
-  const long mstride = (long)g.n_out * g.n_in;
-  GemmBatch g1{(const float2*)in, g.n_in, (long)g.n_in * g.n_in, Wx, g.n_in, mstride,
-               Ut, g.n_in, mstride, g.n_in, g.n_out, g.n_in, batch, nullptr, EPI_STORE_T,
-               precision};
-  DLX_HIP(launch_cgemm_nt(g1, s));
-  GemmBatch g2{Wy, g.n_in, mstride, Ut, g.n_in, mstride, out, g.n_out,
-               (long)g.n_out * g.n_out, g.n_out, g.n_out, g.n_in, batch, nullptr, EPI_STORE,
-               precision};
-  DLX_HIP(launch_cgemm_nt(g2, s));
-  return DLX_OK;
+  int st;
+  if (split) {
+    uint8_t* X = Wx_ws + 3 * mat;
+    DLX_HIP(launch_to_split4((const float2*)in, batch, g.n_in, g.n_in, (long)g.n_in * g.n_in,
+                             X, (long)mat_bytes(g.n_in, g.n_in, true), s));
+    st = mft_split4(g, batch, X, (const uint8_t*)Wx, (const uint8_t*)Wy, Ut, c.data(), out,
+                    EPI_STORE, s, nullptr, nullptr);
+  } else {
+    st = mft_fp32(g, batch, (const float2*)in, (const float2*)Wx, (const float2*)Wy,
+                  (float2*)Ut, c.data(), out, EPI_STORE, s, false);
+  }
+  if (st) return st;
+  if (int r = cache_release(hx, s)) return r;
+  return cache_release(hy, s);
 }
 
 int dlx_pupil_field(int32_t n, int32_t batch, const double* wavelength, double pixel_scale,
@@ -259,11 +364,11 @@ int dlx_intensity(int64_t npix, int32_t batch, const void* field, int32_t sum_ov
 }
 
 size_t dlx_psf_poly_workspace_size(const dlx_mft_geom* geom, int32_t nwl, int32_t precision) {
-  (void)precision;
   if (!geom || nwl <= 0) return 0;
+  const bool split = precision == DLX_PREC_BF16X3;
   const size_t c = (size_t)(nwl < kWlChunk ? nwl : kWlChunk);
-  const size_t P = align_up(c * geom->n_in * geom->n_in * sizeof(float2));
-  const size_t W = align_up(c * geom->n_out * geom->n_in * sizeof(float2));
+  const size_t P = align_up(c * mat_bytes(geom->n_in, geom->n_in, split));
+  const size_t W = align_up(c * mat_bytes(geom->n_out, geom->n_in, split));
   const size_t I = align_up(c * geom->n_out * geom->n_out * sizeof(float));
   return P + 2 * W + I;
 }
@@ -279,51 +384,64 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
   DLX_CHECK_ARG(wavelength && weight && psf, "null argument to dlx_psf_poly");
   DLX_CHECK_ARG(precision == DLX_PREC_FP32 || precision == DLX_PREC_BF16X3,
                 "unknown precision %d", precision);
+  DLX_CHECK_ARG(std::isfinite(pixel_scale_in) && pixel_scale_in != 0.0 &&
+                    std::isfinite(pixel_scale_out) && std::isfinite(center) &&
+                    std::isfinite(tilt_x) && std::isfinite(tilt_y) && std::isfinite(scale),
+                "non-finite scalar argument to dlx_psf_poly");
   const size_t need = dlx_psf_poly_workspace_size(geom, nwl, precision);
   if (!workspace || workspace_bytes < need) {
     set_error("dlx_psf_poly workspace too small: need %zu bytes, got %zu", need,
               workspace_bytes);
     return DLX_ERR_WORKSPACE;
   }
-  for (int l = 0; l < nwl; ++l)
+  for (int l = 0; l < nwl; ++l) {
     DLX_CHECK_ARG(std::isfinite(weight[l]) && weight[l] >= 0, "weight[%d] must be >= 0", l);
+    DLX_CHECK_ARG(std::isfinite(wavelength[l]) && wavelength[l] > 0,
+                  "wavelength[%d] must be > 0", l);
+  }
   const dlx_mft_geom& g = *geom;
   hipStream_t s = (hipStream_t)stream;
+  const bool split = precision == DLX_PREC_BF16X3;
   const int c = nwl < kWlChunk ? nwl : kWlChunk;
-  const size_t Pb = align_up((size_t)c * g.n_in * g.n_in * sizeof(float2));
-  const size_t Wb = align_up((size_t)c * g.n_out * g.n_in * sizeof(float2));
-  float2* P = (float2*)workspace;
-  float2* W = (float2*)((char*)workspace + Pb);
-  float2* Ut = (float2*)((char*)workspace + Pb + Wb);
-  float* I = (float*)((char*)workspace + Pb + 2 * Wb);
-  const long mstride = (long)g.n_out * g.n_in;
+  const size_t Pb = align_up((size_t)c * mat_bytes(g.n_in, g.n_in, split));
+  const size_t Wb = align_up((size_t)c * mat_bytes(g.n_out, g.n_in, split));
+  uint8_t* P = (uint8_t*)workspace;
+  uint8_t* W_ws = P + Pb;
+  uint8_t* Ut = P + Pb + Wb;
+  float* I = (float*)(P + Pb + 2 * Wb);
   const long nout2 = (long)g.n_out * g.n_out;
+  const double x0 = center - (g.n_in - 1) / 2.0 * pixel_scale_in;
 
-  std::vector<double> dx(c, pixel_scale_in), pout(c, pixel_scale_out), amp_w(c);
+  std::vector<double> dx(c, pixel_scale_in), pout(c, pixel_scale_out), alpha(c);
   for (int l0 = 0; l0 < nwl; l0 += c) {
     const int nb = nwl - l0 < c ? nwl - l0 : c;
-    if (int st = dlx_pupil_field(g.n_in, nb, wavelength + l0, pixel_scale_in, center, tilt_x,
-                                 tilt_y, amp, opd, phase, scale, nullptr, P, stream))
+    // |E_l|^2 = (c_l sqrt(w_l))^2 |Y^T P X|^2: the scale is applied in the
+    // stage-2 epilogue, so the matrices depend on the geometry only.
+    for (int i = 0; i < nb; ++i)
+      alpha[i] = mft_norm(g, wavelength[l0 + i], pixel_scale_in, pixel_scale_out) *
+                 std::sqrt(weight[l0 + i]);
+    void *W = nullptr, *h = nullptr;
+    if (int st = get_matrices(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr, split,
+                              W_ws, s, &W, &h))
       return st;
-    // |E|^2 = w * c^2 * |Y^T P X|^2: fold sqrt(c * sqrt(w)) into each matrix side.
-    for (int i = 0; i < nb; ++i) {
-      const double cn = mft_norm(g, wavelength[l0 + i], pixel_scale_in, pixel_scale_out);
-      amp_w[i] = std::sqrt(std::fabs(cn) * std::sqrt(weight[l0 + i]));
+    if (split) {
+      PupilChunk pc;
+      for (int i = 0; i < nb; ++i) pc.inv_wl[i] = 1.0 / wavelength[l0 + i];
+      DLX_HIP(launch_pupil_split4(pc, nb, g.n_in, pixel_scale_in, x0, tilt_x, tilt_y, amp, opd,
+                                  phase, (float)scale, P, (long)mat_bytes(g.n_in, g.n_in, true),
+                                  s));
+      if (int st = mft_split4(g, nb, P, (const uint8_t*)W, (const uint8_t*)W, Ut, alpha.data(),
+                              I, EPI_INTENSITY, s, prof_begin, prof_end))
+        return st;
+    } else {
+      if (int st = dlx_pupil_field(g.n_in, nb, wavelength + l0, pixel_scale_in, center, tilt_x,
+                                   tilt_y, amp, opd, phase, scale, nullptr, P, stream))
+        return st;
+      if (int st = mft_fp32(g, nb, (const float2*)P, (const float2*)W, (const float2*)W,
+                            (float2*)Ut, alpha.data(), I, EPI_INTENSITY, s, true))
+        return st;
     }
-    if (int st = gen_dft(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr,
-                         amp_w.data(), W, s))
-      return st;
-    int slot;
-    GemmBatch g1{P, g.n_in, (long)g.n_in * g.n_in, W, g.n_in, mstride, Ut, g.n_in, mstride,
-                 g.n_in, g.n_out, g.n_in, nb, nullptr, EPI_STORE_T, precision};
-    if (int st = prof_begin(1, s, &slot)) return st;
-    DLX_HIP(launch_cgemm_nt(g1, s));
-    if (int st = prof_end(slot, s)) return st;
-    GemmBatch g2{W, g.n_in, mstride, Ut, g.n_in, mstride, I, g.n_out, nout2, g.n_out, g.n_out,
-                 g.n_in, nb, nullptr, EPI_INTENSITY, precision};
-    if (int st = prof_begin(2, s, &slot)) return st;
-    DLX_HIP(launch_cgemm_nt(g2, s));
-    if (int st = prof_end(slot, s)) return st;
+    if (int st = cache_release(h, s)) return st;
     DLX_HIP(launch_sum_planes(nout2, nb, I, l0 > 0, psf, s));
   }
   return DLX_OK;

@@ -1,0 +1,193 @@
+// Device-resident cache of transfer matrices.
+//
+// The transfer matrices of an MFT depend only on geometry and per-instance
+// (wavelength, pixel scales, shift) -- not on the field or the weights -- so
+// repeated propagations (phase-retrieval iterations, a serving loop over the
+// same wavelength grid) reuse them instead of regenerating N_out x N_in
+// complex exponentials per call.  An entry holds the matrices of one batch of
+// instances, contiguous, in complex64 or split4 form.
+//
+// Stream safety: an entry records an event after its generation; a user on any
+// stream waits on it (hipStreamWaitEvent) before the matrices are read, and
+// records a "last use" event that eviction synchronises on before freeing.
+// The cache is bypassed while the stream is being captured into a graph
+// (allocation and event records are not capturable); callers then generate the
+// matrices into their workspace.
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "dlx_internal.h"
+#include "dlx_split.h"
+
+namespace dlx {
+namespace {
+
+struct MatEntry {
+  std::vector<double> key;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipEvent_t ready = nullptr;
+  hipEvent_t last_use = nullptr;
+  bool used = false;
+  uint64_t tick = 0;
+};
+
+struct MatCache {
+  std::mutex mu;
+  std::vector<MatEntry*> entries;
+  size_t budget = (size_t)4 << 30;   // 4 GiB by default (DLUX_AMD_CACHE_MB)
+  size_t used = 0;
+  uint64_t tick = 0, hits = 0, misses = 0;
+  bool init = false;
+};
+
+MatCache& cache() {
+  static MatCache c;
+  if (!c.init) {
+    c.init = true;
+    if (const char* v = getenv("DLUX_AMD_CACHE_MB")) c.budget = (size_t)atoll(v) << 20;
+  }
+  return c;
+}
+
+void free_entry(MatEntry* e) {
+  if (e->used) (void)hipEventSynchronize(e->last_use);
+  (void)hipEventSynchronize(e->ready);
+  (void)hipFree(e->ptr);
+  (void)hipEventDestroy(e->ready);
+  (void)hipEventDestroy(e->last_use);
+  delete e;
+}
+
+// Evict least-recently-used entries until `need` more bytes fit the budget.
+void make_room(MatCache& c, size_t need) {
+  while (!c.entries.empty() && c.used + need > c.budget) {
+    size_t lru = 0;
+    for (size_t i = 1; i < c.entries.size(); ++i)
+      if (c.entries[i]->tick < c.entries[lru]->tick) lru = i;
+    MatEntry* e = c.entries[lru];
+    c.used -= e->bytes;
+    c.entries.erase(c.entries.begin() + lru);
+    free_entry(e);
+  }
+}
+
+}  // namespace
+
+int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
+                  const double* pout, const double* shift, bool split, hipStream_t s,
+                  void** ptr, void** handle) {
+  *ptr = nullptr;
+  *handle = nullptr;
+  MatCache& c = cache();
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return DLX_OK;
+  }
+  if (cs != hipStreamCaptureStatusNone || c.budget == 0) return DLX_OK;
+  int dev = 0;
+  DLX_HIP(hipGetDevice(&dev));
+
+  std::vector<double> key;
+  key.reserve(9 + 4 * (size_t)nb);
+  key.insert(key.end(), {(double)dev, (double)g.n_in, (double)g.n_out, (double)g.inverse,
+                         (double)g.has_focal, g.has_focal ? g.focal_length : 0.0,
+                         g.has_focal ? g.focal_shift : 0.0, split ? 1.0 : 0.0, (double)nb});
+  key.insert(key.end(), wl, wl + nb);
+  key.insert(key.end(), dx, dx + nb);
+  key.insert(key.end(), pout, pout + nb);
+  for (int i = 0; i < nb; ++i) key.push_back(shift ? shift[i] : 0.0);
+
+  std::lock_guard<std::mutex> lock(c.mu);
+  for (MatEntry* e : c.entries) {
+    if (e->key.size() == key.size() &&
+        std::memcmp(e->key.data(), key.data(), key.size() * sizeof(double)) == 0) {
+      e->tick = ++c.tick;
+      ++c.hits;
+      DLX_HIP(hipStreamWaitEvent(s, e->ready, 0));
+      *ptr = e->ptr;
+      *handle = e;
+      return DLX_OK;
+    }
+  }
+  ++c.misses;
+  const size_t per = split ? (size_t)g.n_out * split4_row_bytes(g.n_in)
+                           : (size_t)g.n_out * g.n_in * sizeof(float2);
+  const size_t bytes = align_up(per * (size_t)nb);
+  if (bytes > c.budget) return DLX_OK;  // larger than the whole cache: caller generates
+  make_room(c, bytes);
+  void* mem = nullptr;
+  if (hipMalloc(&mem, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return DLX_OK;  // no memory for the cache: caller generates into its workspace
+  }
+  MatEntry* e = new MatEntry();
+  e->key = std::move(key);
+  e->ptr = mem;
+  e->bytes = bytes;
+  e->tick = ++c.tick;
+  if (hipEventCreateWithFlags(&e->ready, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&e->last_use, hipEventDisableTiming) != hipSuccess) {
+    (void)hipFree(mem);
+    delete e;
+    set_error("hipEventCreate failed in the matrix cache");
+    return DLX_ERR_HIP;
+  }
+  if (int st = gen_dft(g, nb, wl, dx, pout, shift, nullptr, mem, split, s)) {
+    free_entry(e);
+    return st;
+  }
+  DLX_HIP(hipEventRecord(e->ready, s));
+  c.entries.push_back(e);
+  c.used += bytes;
+  *ptr = mem;
+  *handle = e;
+  return DLX_OK;
+}
+
+int cache_release(void* handle, hipStream_t s) {
+  if (!handle) return DLX_OK;
+  MatCache& c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  MatEntry* e = (MatEntry*)handle;
+  DLX_HIP(hipEventRecord(e->last_use, s));
+  e->used = true;
+  return DLX_OK;
+}
+
+}  // namespace dlx
+
+using namespace dlx;
+
+extern "C" int dlx_cache_clear(void) {
+  clear_error();
+  MatCache& c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  for (MatEntry* e : c.entries) free_entry(e);
+  c.entries.clear();
+  c.used = 0;
+  return DLX_OK;
+}
+
+extern "C" int dlx_cache_set_budget(size_t bytes) {
+  clear_error();
+  MatCache& c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  c.budget = bytes;
+  make_room(c, 0);
+  return DLX_OK;
+}
+
+extern "C" int dlx_cache_stats(uint64_t* hits, uint64_t* misses, size_t* bytes,
+                               int32_t* entries) {
+  clear_error();
+  MatCache& c = cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (hits) *hits = c.hits;
+  if (misses) *misses = c.misses;
+  if (bytes) *bytes = c.used;
+  if (entries) *entries = (int32_t)c.entries.size();
+  return DLX_OK;
+}

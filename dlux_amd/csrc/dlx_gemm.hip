@@ -8,12 +8,9 @@
 //   stage 2:  E   =  W_y . U         A = W_y (N_out x N_in), B = U^T (N_out x N_in)
 // (utils/propagation.py:243 computes the same product as (Y^T E) X.)
 //
-// Two arithmetic forms:
-//   DLX_PREC_FP32    v_mfma_f32_32x32x2_f32, 4 real products per complex MAC.
-//   DLX_PREC_BF16X3  each fp32 operand is split at LDS-staging time into
-//                    hi = bf16(x), lo = bf16(x - hi); a real product is
-//                    hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_bf16 with
-//                    FP32 accumulation (12 MFMAs per complex 32x32x16 step).
+// This is the complex64 / FP32 form (v_mfma_f32_32x32x2_f32, four real
+// products per complex MAC, FP32 accumulation).  The split-BF16x3 form lives
+// in dlx_gemm_bf16.hip.
 //
 // Workgroup = 4 waves (2 x 2), each wave owns TM x TN blocks of 32 x 32
 // outputs, so the workgroup tile is (64 TM) x (64 TN).  One LDS stage holds a
@@ -25,33 +22,9 @@
 namespace dlx {
 namespace {
 
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
-typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4_t;
 
 constexpr int kThreads = 256;
-
-// Round-to-nearest-even float -> bf16 bits (inputs are finite).
-__device__ __forceinline__ uint32_t bf16_bits(float x) {
-  uint32_t u = __float_as_uint(x);
-  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
-
-// x = hi + lo + O(2^-17 |x|); returns (hi bits, lo bits).
-__device__ __forceinline__ void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
-  uint32_t h = bf16_bits(x);
-  float hf = __uint_as_float(h << 16);
-  hi = (uint16_t)h;
-  lo = (uint16_t)bf16_bits(x - hf);
-}
-
-__device__ __forceinline__ bf16x8_t as_bf16x8(u32x4_t v) {
-  return __builtin_bit_cast(bf16x8_t, v);
-}
-
-__device__ __forceinline__ u32x4_t neg_bf16x8(u32x4_t v) {
-  return v ^ (u32x4_t){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-}
 
 // ---------------------------------------------------------------------------
 // Operand staging: a ROWS x BK tile of a row-major complex64 matrix.
@@ -146,132 +119,7 @@ cgemm_nt_fp32(GemmBatch g) {
 
   // Epilogue: 32x32 MFMA output layout -- lane holds column (lane % 32),
   // register r holds row 8*(r/4) + 4*(lane/32) + r%4.
-  const float alpha = g.alpha ? g.alpha[b] : 1.f;
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int m = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * lk + (r & 3);
-        const int n = n0 + wn + 32 * j + lr;
-        if (m < g.M && n < g.N) {
-          const float re = alpha * accR[i][j][r], im = alpha * accI[i][j][r];
-          if (EPI == EPI_STORE) {
-            ((float2*)g.C)[(long)b * g.sC + (long)m * g.ldc + n] = make_float2(re, im);
-          } else if (EPI == EPI_STORE_T) {
-            ((float2*)g.C)[(long)b * g.sC + (long)n * g.ldc + m] = make_float2(re, im);
-          } else {
-            ((float*)g.C)[(long)b * g.sC + (long)m * g.ldc + n] = re * re + im * im;
-          }
-        }
-      }
-}
-
-// ---------------------------------------------------------------------------
-// split-BF16 x3 form
-// ---------------------------------------------------------------------------
-template <int TM, int TN, int EPI>
-__global__ void __launch_bounds__(kThreads)
-cgemm_nt_bf16x3(GemmBatch g) {
-  constexpr int BK = 32;
-  constexpr int BM = 64 * TM, BN = 64 * TN;
-  constexpr int LD = BK + 8;  // uint16 stride (80 B): conflict-free 16 B frags
-  // planes: 0 = re hi, 1 = re lo, 2 = im hi, 3 = im lo
-  __shared__ __attribute__((aligned(16))) uint16_t As[4 * BM * LD];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[4 * BN * LD];
-
-  const int b = blockIdx.z;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  const float2* __restrict__ A = g.A + (long)b * g.sA;
-  const float2* __restrict__ B = g.B + (long)b * g.sB;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
-  const int lr = lane & 31, lk = lane >> 5;
-
-  f32x16_t accR[TM][TN], accI[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) { accR[i][j][r] = 0.f; accI[i][j][r] = 0.f; }
-
-  Stage<BM, BK> sa;
-  Stage<BN, BK> sb;
-  sa.load(A, g.lda, m0, g.M, 0, g.K);
-  sb.load(B, g.ldb, n0, g.N, 0, g.K);
-
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    __syncthreads();
-    {
-      const int kk = threadIdx.x % BK, r0 = threadIdx.x / BK;
-#pragma unroll
-      for (int i = 0; i < Stage<BM, BK>::kPer; ++i) {
-        const int o = (r0 + i * Stage<BM, BK>::kRowStep) * LD + kk;
-        uint16_t h, l;
-        split_bf16(sa.v[i].x, h, l); As[0 * BM * LD + o] = h; As[1 * BM * LD + o] = l;
-        split_bf16(sa.v[i].y, h, l); As[2 * BM * LD + o] = h; As[3 * BM * LD + o] = l;
-      }
-#pragma unroll
-      for (int i = 0; i < Stage<BN, BK>::kPer; ++i) {
-        const int o = (r0 + i * Stage<BN, BK>::kRowStep) * LD + kk;
-        uint16_t h, l;
-        split_bf16(sb.v[i].x, h, l); Bs[0 * BN * LD + o] = h; Bs[1 * BN * LD + o] = l;
-        split_bf16(sb.v[i].y, h, l); Bs[2 * BN * LD + o] = h; Bs[3 * BN * LD + o] = l;
-      }
-    }
-    __syncthreads();
-    if (k0 + BK < g.K) {
-      sa.load(A, g.lda, m0, g.M, k0 + BK, g.K);
-      sb.load(B, g.ldb, n0, g.N, k0 + BK, g.K);
-    }
-#pragma unroll
-    for (int ks = 0; ks < BK; ks += 16) {
-      u32x4_t fa[TM][4], fb[TN][4];
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-          fa[i][p] = *(const u32x4_t*)&As[p * BM * LD + (wm + 32 * i + lr) * LD + ks + 8 * lk];
-#pragma unroll
-      for (int j = 0; j < TN; ++j)
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-          fb[j][p] = *(const u32x4_t*)&Bs[p * BN * LD + (wn + 32 * j + lr) * LD + ks + 8 * lk];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const bf16x8_t arh = as_bf16x8(fa[i][0]), arl = as_bf16x8(fa[i][1]);
-        const bf16x8_t aih = as_bf16x8(fa[i][2]), ail = as_bf16x8(fa[i][3]);
-        const bf16x8_t naih = as_bf16x8(neg_bf16x8(fa[i][2]));
-        const bf16x8_t nail = as_bf16x8(neg_bf16x8(fa[i][3]));
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          const bf16x8_t brh = as_bf16x8(fb[j][0]), brl = as_bf16x8(fb[j][1]);
-          const bf16x8_t bih = as_bf16x8(fb[j][2]), bil = as_bf16x8(fb[j][3]);
-          f32x16_t r = accR[i][j], im = accI[i][j];
-          // re += ar*br - ai*bi  (3 terms each)
-          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, brh, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brl, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nail, bih, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bil, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brh, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bih, r, 0, 0, 0);
-          // im += ar*bi + ai*br
-          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, bih, im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bil, im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ail, brh, im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brl, im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bih, im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brh, im, 0, 0, 0);
-          accR[i][j] = r;
-          accI[i][j] = im;
-        }
-      }
-    }
-  }
-
-  const float alpha = g.alpha ? g.alpha[b] : 1.f;
+  const float alpha = g.alpha[b];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -297,18 +145,10 @@ template <int TM, int TN>
 hipError_t dispatch(const GemmBatch& g, hipStream_t s) {
   dim3 grid((g.N + 64 * TN - 1) / (64 * TN), (g.M + 64 * TM - 1) / (64 * TM), g.batch);
   dim3 block(kThreads);
-  if (g.precision == DLX_PREC_BF16X3) {
-    switch (g.epilogue) {
-      case EPI_STORE: hipLaunchKernelGGL((cgemm_nt_bf16x3<TM, TN, EPI_STORE>), grid, block, 0, s, g); break;
-      case EPI_STORE_T: hipLaunchKernelGGL((cgemm_nt_bf16x3<TM, TN, EPI_STORE_T>), grid, block, 0, s, g); break;
-      default: hipLaunchKernelGGL((cgemm_nt_bf16x3<TM, TN, EPI_INTENSITY>), grid, block, 0, s, g); break;
-    }
-  } else {
-    switch (g.epilogue) {
-      case EPI_STORE: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_STORE>), grid, block, 0, s, g); break;
-      case EPI_STORE_T: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_STORE_T>), grid, block, 0, s, g); break;
-      default: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_INTENSITY>), grid, block, 0, s, g); break;
-    }
+  switch (g.epilogue) {
+    case EPI_STORE: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_STORE>), grid, block, 0, s, g); break;
+    case EPI_STORE_T: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_STORE_T>), grid, block, 0, s, g); break;
+    default: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_INTENSITY>), grid, block, 0, s, g); break;
   }
   return hipGetLastError();
 }
@@ -317,6 +157,7 @@ hipError_t dispatch(const GemmBatch& g, hipStream_t s) {
 
 hipError_t launch_cgemm_nt(const GemmBatch& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.batch <= 0) return hipSuccess;
+  if (g.batch > GemmBatch::kMaxBatch) return hipErrorInvalidValue;
   // Small problems: 64x64 workgroup tiles keep enough workgroups in flight.
   const long tiles128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch;
   if (g.M <= 64 || g.N <= 64 || tiles128 < 512) return dispatch<1, 1>(g, s);

@@ -1,0 +1,564 @@
+// Split-BF16x3 complex NT GEMM on split4 operands (gfx950 matrix cores).
+//
+//   C[b](m, n) = alpha_b * sum_k A[b](m, k) * B[b](n, k)
+//
+// A and B are split4 matrices (dlx_split.h): each real product is
+// hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_bf16 with FP32 accumulation,
+// 12 MFMAs per complex 32x32x16 block step.
+//
+// Workgroup = 4 waves (2 x 2), each wave owns TM x TN blocks of 32 x 32
+// complex outputs.  A K-stage is 32 k (4 chunks): every row of a tile is 16
+// pieces of 16 bytes, staged global -> registers -> LDS with 16-byte accesses.
+// LDS is double-buffered so the next stage's LDS writes follow the current
+// stage's MFMAs without a barrier in between; there is one barrier per stage.
+// Workgroups are numbered so that all tiles of one instance run on the same
+// XCD and share its L2 (the hardware deals workgroups round-robin over 8 XCDs).
+#include "dlx_internal.h"
+#include "dlx_split.h"
+
+#include <cstdlib>
+#include <string>
+
+namespace dlx {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4_t;
+
+constexpr int kThreads = 256;
+constexpr int kStageChunks = 4;                        // 32 k per stage
+constexpr int kRowLds = kStageChunks * kChunkBytes + 16;  // 272 B: conflict-free 16 B reads
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ bf16x8_t as_bf(u32x4_t v) { return __builtin_bit_cast(bf16x8_t, v); }
+__device__ __forceinline__ u32x4_t neg_bf(u32x4_t v) {
+  return v ^ (u32x4_t){0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+}
+
+// Workgroup barrier for LDS hand-over: waits for this wave's LDS accesses
+// only (lgkmcnt), not for outstanding global loads, which __syncthreads()'s
+// fence would drain.  The memory clobber keeps the compiler from moving
+// memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Hardware workgroup id -> logical id such that consecutive logical ids land
+// on the same XCD (XCD = hw % 8).
+__device__ __forceinline__ int xcd_remap(int hw, int total) {
+  const int per = total / kXcds;
+  const int rem = total - per * kXcds;
+  if (hw >= per * kXcds) return hw;  // tail that does not fill all XCDs evenly
+  (void)rem;
+  return (hw % kXcds) * per + hw / kXcds;
+}
+
+template <int ROWS>
+struct Stage16 {
+  static constexpr int kPer = ROWS / 16;   // 16 threads per row (16 pieces)
+  uint4 v[kPer];
+
+  __device__ __forceinline__ void load(const uint8_t* __restrict__ base, long row_bytes,
+                                       int row0, int nrows, int kc0, int kchunks) {
+    const int t = threadIdx.x;
+    const int piece = t & 15;
+    const int chunk = kc0 + (piece >> 2);
+    const int plane = piece & 3;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      // Branch-free bounds handling: out-of-range lanes read row 0 / chunk 0 of
+      // the operand (always valid) and zero the value.
+      const int r = row0 + (t >> 4) + 16 * i;
+      const bool ok = r < nrows && chunk < kchunks;
+      const long off = ok ? (long)r * row_bytes + (long)chunk * kChunkBytes + plane * 16 : 0;
+      const uint4 x = *(const uint4*)(base + off);
+      v[i] = ok ? x : make_uint4(0u, 0u, 0u, 0u);
+    }
+  }
+
+  __device__ __forceinline__ void store(uint8_t* lds) const {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+      *(uint4*)(lds + ((t >> 4) + 16 * i) * kRowLds + (t & 15) * 16) = v[i];
+  }
+};
+
+// Epilogue shared by the GEMM variants.  32x32 MFMA layout: lane holds column
+// lane % 32, register r holds row 8*(r/4) + 4*(lane/32) + r%4.
+template <int TM, int TN, int EPI>
+__device__ __forceinline__ void epilogue_split(const GemmSplit& g, int b, int m0, int n0, int wm,
+                                               int wn, int lr, int lk,
+                                               f32x16_t (&accR)[TM][TN],
+                                               f32x16_t (&accI)[TM][TN]) {
+  const float alpha = g.alpha[b];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int n = n0 + wn + 32 * j + lr;
+      if (EPI == EPI_SPLIT_T) {
+        // rows n of a split4 matrix whose k is this GEMM's m: 4 consecutive m
+        // per register group -> half a chunk (8 bytes) per plane.
+        if (n >= g.N) continue;
+        const int m_pad = (g.M + 7) & ~7;   // the last chunk is zero-padded
+        uint8_t* crow = (uint8_t*)g.C + (long)b * g.c_batch + (long)n * g.ldc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int mq = m0 + wm + 32 * i + 8 * q + 4 * lk;   // first of 4 rows
+          if (mq >= m_pad) continue;
+          uint32_t rh[4], rl[4], ih[4], il[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const bool ok = mq + e < g.M;
+            split_bf16(ok ? alpha * accR[i][j][4 * q + e] : 0.f, rh[e], rl[e]);
+            split_bf16(ok ? alpha * accI[i][j][4 * q + e] : 0.f, ih[e], il[e]);
+          }
+          uint8_t* dst = crow + (long)(mq >> 3) * kChunkBytes + ((mq & 7) >> 2) * 8;
+          *(uint2*)(dst + 0) = make_uint2(rh[0] | (rh[1] << 16), rh[2] | (rh[3] << 16));
+          *(uint2*)(dst + 16) = make_uint2(rl[0] | (rl[1] << 16), rl[2] | (rl[3] << 16));
+          *(uint2*)(dst + 32) = make_uint2(ih[0] | (ih[1] << 16), ih[2] | (ih[3] << 16));
+          *(uint2*)(dst + 48) = make_uint2(il[0] | (il[1] << 16), il[2] | (il[3] << 16));
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int m = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * lk + (r & 3);
+          if (m < g.M && n < g.N) {
+            const float re = alpha * accR[i][j][r], im = alpha * accI[i][j][r];
+            if (EPI == EPI_STORE) {
+              ((float2*)g.C)[(long)b * g.c_batch + (long)m * g.ldc + n] = make_float2(re, im);
+            } else {
+              ((float*)g.C)[(long)b * g.c_batch + (long)m * g.ldc + n] = re * re + im * im;
+            }
+          }
+        }
+      }
+    }
+}
+
+template <int TM, int TN, int EPI>
+__global__ void __launch_bounds__(kThreads)
+cgemm_split4_kernel(GemmSplit g) {
+  constexpr int BM = 64 * TM, BN = 64 * TN;
+  constexpr int kBuf = (BM + BN) * kRowLds;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBuf];
+
+  const int tiles_n = (g.N + BN - 1) / BN, tiles_m = (g.M + BM - 1) / BM;
+  const int total = tiles_n * tiles_m * g.batch;
+  const int id = xcd_remap(blockIdx.x, total);
+  const int tn = id % tiles_n;
+  const int tm = (id / tiles_n) % tiles_m;
+  const int b = id / (tiles_n * tiles_m);
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  const uint8_t* __restrict__ A = g.A + (long)b * g.a_batch_bytes;
+  const uint8_t* __restrict__ B = g.B + (long)b * g.b_batch_bytes;
+  const int kchunks = (int)split4_chunks(g.K);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
+  const int lr = lane & 31, lk = lane >> 5;
+
+  f32x16_t accR[TM][TN], accI[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) { accR[i][j][r] = 0.f; accI[i][j][r] = 0.f; }
+
+  // Two register stages in flight: the loads of K-stage s + 2 are issued
+  // while stage s runs on the matrix cores, so a load has two stages of MFMA
+  // work to cover its latency before its data is written to LDS.
+  Stage16<BM> sa0, sa1;
+  Stage16<BN> sb0, sb1;
+  const int nst = (kchunks + kStageChunks - 1) / kStageChunks;
+  sa0.load(A, g.a_row_bytes, m0, g.M, 0, kchunks);
+  sb0.load(B, g.b_row_bytes, n0, g.N, 0, kchunks);
+  if (nst > 1) {
+    sa1.load(A, g.a_row_bytes, m0, g.M, kStageChunks, kchunks);
+    sb1.load(B, g.b_row_bytes, n0, g.N, kStageChunks, kchunks);
+  }
+  sa0.store(lds);
+  sb0.store(lds + BM * kRowLds);
+  lds_barrier();
+
+  auto mma_stage = [&](const uint8_t* la) {
+    const uint8_t* lb = la + BM * kRowLds;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int coff = (2 * ks + lk) * kChunkBytes;
+      u32x4_t fa[TM][4], fb[TN][4];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          fa[i][p] = *(const u32x4_t*)(la + (wm + 32 * i + lr) * kRowLds + coff + p * 16);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          fb[j][p] = *(const u32x4_t*)(lb + (wn + 32 * j + lr) * kRowLds + coff + p * 16);
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const bf16x8_t arh = as_bf(fa[i][0]), arl = as_bf(fa[i][1]);
+        const bf16x8_t aih = as_bf(fa[i][2]), ail = as_bf(fa[i][3]);
+        const bf16x8_t naih = as_bf(neg_bf(fa[i][2])), nail = as_bf(neg_bf(fa[i][3]));
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const bf16x8_t brh = as_bf(fb[j][0]), brl = as_bf(fb[j][1]);
+          const bf16x8_t bih = as_bf(fb[j][2]), bil = as_bf(fb[j][3]);
+          f32x16_t r = accR[i][j], im = accI[i][j];
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, brh, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, bih, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brl, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bil, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nail, bih, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ail, brh, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bil, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brl, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brh, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bih, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bih, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brh, im, 0, 0, 0);
+          accR[i][j] = r;
+          accI[i][j] = im;
+        }
+      }
+    }
+  };
+
+  // One K-stage: MFMAs on LDS buffer `cur`; reload `ld` with stage s + 2;
+  // write `st` (holding stage s + 1) into the other buffer.
+  auto step = [&](int s, Stage16<BM>& lda, Stage16<BN>& ldb, Stage16<BM>& sta,
+                  Stage16<BN>& stb) {
+    const uint8_t* cur = lds + (s & 1) * kBuf;
+    if (s + 2 < nst) {
+      lda.load(A, g.a_row_bytes, m0, g.M, (s + 2) * kStageChunks, kchunks);
+      ldb.load(B, g.b_row_bytes, n0, g.N, (s + 2) * kStageChunks, kchunks);
+    }
+    mma_stage(cur);
+    if (s + 1 < nst) {
+      uint8_t* nb = lds + ((s + 1) & 1) * kBuf;
+      sta.store(nb);
+      stb.store(nb + BM * kRowLds);
+    }
+    lds_barrier();
+  };
+
+  for (int s = 0; s < nst; s += 2) {
+    step(s, sa0, sb0, sa1, sb1);           // stage s+1 lives in set 1
+    if (s + 1 < nst) step(s + 1, sa1, sb1, sa0, sb0);   // stage s+2 lives in set 0
+  }
+
+  epilogue_split<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
+}
+
+// ---------------------------------------------------------------------------
+// Direct-to-LDS variant: global_load_lds_dwordx4 moves 16-byte pieces from HBM
+// straight into LDS (no VGPR staging, no ds_write).  One wave instruction fills
+// a contiguous 1 KiB = 4 tile rows x 16 slots; a row's 16 pieces are placed at
+// slot = piece ^ (row & 15), which keeps the 16-lane fragment reads of 16
+// consecutive rows on distinct banks without row padding.  Lanes outside the
+// operand read a zeroed line.
+// ---------------------------------------------------------------------------
+__device__ uint4 g_zero_line[4];
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef const __attribute__((address_space(1))) void gbl_void_t;
+
+constexpr int kRowDl = kStageChunks * kChunkBytes;   // 256 B, unpadded
+
+template <int ROWS>
+__device__ __forceinline__ void stage_dl(uint8_t* lds_tile, const uint8_t* __restrict__ base,
+                                         long row_bytes, int row0, int nrows, int kc0,
+                                         int kchunks) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < ROWS / 16; ++q) {
+    const int rb = 16 * q + 4 * wave;                 // first of this instruction's 4 rows
+    const int r = rb + (lane >> 4);
+    const int piece = (lane & 15) ^ (r & 15);
+    const int chunk = kc0 + (piece >> 2);
+    const int rg = row0 + r;
+    const uint8_t* src = (rg < nrows && chunk < kchunks)
+                             ? base + (long)rg * row_bytes + (long)chunk * kChunkBytes +
+                                   (piece & 3) * 16
+                             : (const uint8_t*)g_zero_line;
+    __builtin_amdgcn_global_load_lds((gbl_void_t*)src, (lds_void_t*)(lds_tile + rb * kRowDl),
+                                     16, 0, 0);
+  }
+}
+
+template <int TM, int TN, int EPI>
+__global__ void __launch_bounds__(kThreads)
+cgemm_split4_dl_kernel(GemmSplit g) {
+  constexpr int BM = 64 * TM, BN = 64 * TN;
+  constexpr int kBuf = (BM + BN) * kRowDl;
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kBuf];
+
+  const int tiles_n = (g.N + BN - 1) / BN, tiles_m = (g.M + BM - 1) / BM;
+  const int total = tiles_n * tiles_m * g.batch;
+  const int id = xcd_remap(blockIdx.x, total);
+  const int tn = id % tiles_n;
+  const int tm = (id / tiles_n) % tiles_m;
+  const int b = id / (tiles_n * tiles_m);
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  const uint8_t* __restrict__ A = g.A + (long)b * g.a_batch_bytes;
+  const uint8_t* __restrict__ B = g.B + (long)b * g.b_batch_bytes;
+  const int kchunks = (int)split4_chunks(g.K);
+  const int nst = (kchunks + kStageChunks - 1) / kStageChunks;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
+  const int lr = lane & 31, lk = lane >> 5;
+
+  f32x16_t accR[TM][TN], accI[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) { accR[i][j][r] = 0.f; accI[i][j][r] = 0.f; }
+
+  stage_dl<BM>(lds, A, g.a_row_bytes, m0, g.M, 0, kchunks);
+  stage_dl<BN>(lds + BM * kRowDl, B, g.b_row_bytes, n0, g.N, 0, kchunks);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+
+  for (int s = 0; s < nst; ++s) {
+    if (s + 1 < nst) {
+      uint8_t* nb = lds + ((s + 1) & 1) * kBuf;
+      stage_dl<BM>(nb, A, g.a_row_bytes, m0, g.M, (s + 1) * kStageChunks, kchunks);
+      stage_dl<BN>(nb + BM * kRowDl, B, g.b_row_bytes, n0, g.N, (s + 1) * kStageChunks,
+                   kchunks);
+    }
+    const uint8_t* la = lds + (s & 1) * kBuf;
+    const uint8_t* lb = la + BM * kRowDl;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      u32x4_t fa[TM][4], fb[TN][4];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int r = wm + 32 * i + lr;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          fa[i][p] = *(const u32x4_t*)(la + r * kRowDl + ((((2 * ks + lk) << 2) | p) ^ (r & 15)) * 16);
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int r = wn + 32 * j + lr;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          fb[j][p] = *(const u32x4_t*)(lb + r * kRowDl + ((((2 * ks + lk) << 2) | p) ^ (r & 15)) * 16);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const bf16x8_t arh = as_bf(fa[i][0]), arl = as_bf(fa[i][1]);
+        const bf16x8_t aih = as_bf(fa[i][2]), ail = as_bf(fa[i][3]);
+        const bf16x8_t naih = as_bf(neg_bf(fa[i][2])), nail = as_bf(neg_bf(fa[i][3]));
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const bf16x8_t brh = as_bf(fb[j][0]), brl = as_bf(fb[j][1]);
+          const bf16x8_t bih = as_bf(fb[j][2]), bil = as_bf(fb[j][3]);
+          f32x16_t r = accR[i][j], im = accI[i][j];
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, brh, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, bih, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brl, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bil, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nail, bih, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ail, brh, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bil, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brl, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brh, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bih, im, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bih, r, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brh, im, 0, 0, 0);
+          accR[i][j] = r;
+          accI[i][j] = im;
+        }
+      }
+    }
+    // this wave's loads of stage s+1 have landed; the barrier makes every
+    // wave's loads visible and retires every read of buffer s & 1
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+  epilogue_split<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
+}
+
+// ---------------------------------------------------------------------------
+// Occupancy-2 direct-to-LDS variant: 16-k stages (2 chunks, 128 B per tile
+// row) in two LDS buffers = 64 KiB per workgroup, so two workgroups share a
+// CU and each SIMD interleaves two waves: while one waits for its next stage
+// to land, the other issues MFMAs.  Row r stores piece p at slot
+// p ^ ((r >> 1) & 7): 16 consecutive rows reading the same piece cover all 16
+// bank groups.
+// ---------------------------------------------------------------------------
+constexpr int kDeepChunks = 2;
+constexpr int kRowDeep = kDeepChunks * kChunkBytes;   // 128 B
+
+template <int ROWS>
+__device__ __forceinline__ void stage_deep(uint8_t* lds_tile, const uint8_t* __restrict__ base,
+                                           long row_bytes, int row0, int nrows, int kc0,
+                                           int kchunks) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < ROWS / 32; ++q) {
+    const int rb = 32 * q + 8 * wave;                 // first of this instruction's 8 rows
+    const int r = rb + (lane >> 3);
+    const int piece = (lane & 7) ^ ((r >> 1) & 7);
+    const int chunk = kc0 + (piece >> 2);
+    const int rg = row0 + r;
+    const uint8_t* src = (rg < nrows && chunk < kchunks)
+                             ? base + (long)rg * row_bytes + (long)chunk * kChunkBytes +
+                                   (piece & 3) * 16
+                             : (const uint8_t*)g_zero_line;
+    __builtin_amdgcn_global_load_lds((gbl_void_t*)src, (lds_void_t*)(lds_tile + rb * kRowDeep),
+                                     16, 0, 0);
+  }
+}
+
+template <int TM, int TN, int EPI>
+__global__ void __launch_bounds__(kThreads, 2)
+cgemm_split4_deep_kernel(GemmSplit g) {
+  constexpr int BM = 64 * TM, BN = 64 * TN;
+  constexpr int kBuf = (BM + BN) * kRowDeep;
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kBuf];
+
+  const int tiles_n = (g.N + BN - 1) / BN, tiles_m = (g.M + BM - 1) / BM;
+  const int total = tiles_n * tiles_m * g.batch;
+  const int id = xcd_remap(blockIdx.x, total);
+  const int tn = id % tiles_n;
+  const int tm = (id / tiles_n) % tiles_m;
+  const int b = id / (tiles_n * tiles_m);
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  const uint8_t* __restrict__ A = g.A + (long)b * g.a_batch_bytes;
+  const uint8_t* __restrict__ B = g.B + (long)b * g.b_batch_bytes;
+  const int kchunks = (int)split4_chunks(g.K);
+  const int nst = (kchunks + kDeepChunks - 1) / kDeepChunks;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
+  const int lr = lane & 31, lk = lane >> 5;
+
+  f32x16_t accR[TM][TN], accI[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) { accR[i][j][r] = 0.f; accI[i][j][r] = 0.f; }
+
+  auto issue = [&](int st) {
+    uint8_t* buf = lds + (st & 1) * kBuf;
+    stage_deep<BM>(buf, A, g.a_row_bytes, m0, g.M, st * kDeepChunks, kchunks);
+    stage_deep<BN>(buf + BM * kRowDeep, B, g.b_row_bytes, n0, g.N, st * kDeepChunks, kchunks);
+  };
+  issue(0);
+
+  for (int s = 0; s < nst; ++s) {
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of stage s have landed
+    __syncthreads();                 // every wave's have; stage s-1 reads retired
+    if (s + 1 < nst) issue(s + 1);
+    const uint8_t* la = lds + (s & 1) * kBuf;
+    const uint8_t* lb = la + BM * kRowDeep;
+    u32x4_t fa[TM][4], fb[TN][4];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int r = wm + 32 * i + lr;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        fa[i][p] = *(const u32x4_t*)(la + r * kRowDeep + (((lk << 2) | p) ^ ((r >> 1) & 7)) * 16);
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int r = wn + 32 * j + lr;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        fb[j][p] = *(const u32x4_t*)(lb + r * kRowDeep + (((lk << 2) | p) ^ ((r >> 1) & 7)) * 16);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const bf16x8_t arh = as_bf(fa[i][0]), arl = as_bf(fa[i][1]);
+      const bf16x8_t aih = as_bf(fa[i][2]), ail = as_bf(fa[i][3]);
+      const bf16x8_t naih = as_bf(neg_bf(fa[i][2])), nail = as_bf(neg_bf(fa[i][3]));
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const bf16x8_t brh = as_bf(fb[j][0]), brl = as_bf(fb[j][1]);
+        const bf16x8_t bih = as_bf(fb[j][2]), bil = as_bf(fb[j][3]);
+        f32x16_t r = accR[i][j], im = accI[i][j];
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, brh, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, bih, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brl, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bil, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nail, bih, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ail, brh, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bil, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brl, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brh, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bih, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bih, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brh, im, 0, 0, 0);
+        accR[i][j] = r;
+        accI[i][j] = im;
+      }
+    }
+  }
+  epilogue_split<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
+}
+
+// Staging variants: 0 = deep direct-to-LDS (default), 1 = direct-to-LDS with
+// two 32-k buffers, 2 = register staging.  DLUX_AMD_GEMM_STAGING=deep|dl|regs
+// selects one for comparison runs.
+int staging_mode() {
+  static const int mode = [] {
+    const char* v = getenv("DLUX_AMD_GEMM_STAGING");
+    if (v && std::string(v) == "regs") return 2;
+    if (v && std::string(v) == "dl") return 1;
+    return 0;
+  }();
+  return mode;
+}
+
+template <int TM, int TN, int EPI>
+void launch_one(const GemmSplit& g, int tiles, int mode, hipStream_t s) {
+  if (mode == 0)
+    hipLaunchKernelGGL((cgemm_split4_deep_kernel<TM, TN, EPI>), dim3(tiles), dim3(kThreads), 0,
+                       s, g);
+  else if (mode == 1)
+    hipLaunchKernelGGL((cgemm_split4_dl_kernel<TM, TN, EPI>), dim3(tiles), dim3(kThreads), 0, s,
+                       g);
+  else
+    hipLaunchKernelGGL((cgemm_split4_kernel<TM, TN, EPI>), dim3(tiles), dim3(kThreads), 0, s, g);
+}
+
+template <int TM, int TN>
+hipError_t dispatch_split(const GemmSplit& g, hipStream_t s, int mode) {
+  const int tiles = ((g.N + 64 * TN - 1) / (64 * TN)) * ((g.M + 64 * TM - 1) / (64 * TM)) *
+                    g.batch;
+  switch (g.epilogue) {
+    case EPI_STORE: launch_one<TM, TN, EPI_STORE>(g, tiles, mode, s); break;
+    case EPI_SPLIT_T: launch_one<TM, TN, EPI_SPLIT_T>(g, tiles, mode, s); break;
+    default: launch_one<TM, TN, EPI_INTENSITY>(g, tiles, mode, s); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.batch <= 0) return hipSuccess;
+  if (g.batch > GemmSplit::kMaxBatch) return hipErrorInvalidValue;
+  // 128 x 128 tiles: the occupancy-2 kernel when they give two workgroups per
+  // CU (256 CUs), else the one-workgroup-per-CU direct-to-LDS kernel; 64 x 64
+  // tiles for small problems.
+  const long tiles128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch;
+  int mode = staging_mode();
+  if (mode == 0 && tiles128 < 512) mode = 1;
+  if (g.M <= 64 || g.N <= 64 || tiles128 < 128) return dispatch_split<1, 1>(g, s, mode);
+  return dispatch_split<2, 2>(g, s, mode);
+}
+
+}  // namespace dlx

@@ -88,19 +88,35 @@ enum Epilogue {
   EPI_STORE = 0,        // C = alpha * acc (complex64), ldc
   EPI_STORE_T = 1,      // C^T = alpha * acc: written at C[n*ldc + m]
   EPI_INTENSITY = 2,    // I = |alpha * acc|^2 (float32), ldc
+  EPI_SPLIT_T = 3,      // split4 matrix with rows n and k = m (ldc / c_batch in bytes)
 };
 
 struct GemmBatch {
+  static constexpr int kMaxBatch = 64;   // instances per launch
   const float2* A; long lda; long sA;
   const float2* B; long ldb; long sB;
   void* C; long ldc; long sC;
   int M, N, K, batch;
-  const float* alpha;   // device, per batch (NULL => 1)
   int epilogue;
   int precision;        // DLX_PREC_*
+  float alpha[kMaxBatch];   // per-instance scale of the result
 };
 
 hipError_t launch_cgemm_nt(const GemmBatch& g, hipStream_t s);
+
+// Split-BF16x3 complex NT GEMM on split4 operands (dlx_split.h).  Row strides
+// and batch strides of A and B are in bytes; C strides are in elements
+// (EPI_STORE / EPI_INTENSITY) or bytes (EPI_SPLIT_T).  alpha is per batch.
+struct GemmSplit {
+  static constexpr int kMaxBatch = 64;
+  const uint8_t* A; long a_row_bytes; long a_batch_bytes;
+  const uint8_t* B; long b_row_bytes; long b_batch_bytes;
+  void* C; long ldc; long c_batch;
+  int M, N, K, batch, epilogue;
+  float alpha[kMaxBatch];
+};
+
+hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s);
 
 // Per-launch instance parameters passed by value (kernel arguments), so no
 // device-side parameter buffer is needed.
@@ -119,6 +135,15 @@ hipError_t launch_pupil_chunk(const PupilChunk& c, int nb, int n, double dx, dou
                               double tx, double ty, const float* amp, const float* opd,
                               const float* phase, float scale, const float2* in,
                               float2* out, hipStream_t s);
+// split4 producers
+hipError_t launch_dft_split4(const DftChunk& c, int nb, int n_in, int n_out, int inverse,
+                             uint8_t* out, long batch_bytes, hipStream_t s);
+hipError_t launch_pupil_split4(const PupilChunk& c, int nb, int n, double dx, double x0,
+                               double tx, double ty, const float* amp, const float* opd,
+                               const float* phase, float scale, uint8_t* out, long batch_bytes,
+                               hipStream_t s);
+hipError_t launch_to_split4(const float2* in, int batch, long rows, int k, long in_batch,
+                            uint8_t* out, long out_batch_bytes, hipStream_t s);
 hipError_t launch_eval_basis(int nbasis, long npix, int nsets, const float* basis,
                              const float* coeffs, const float* base, float* out,
                              hipStream_t s);
@@ -127,5 +152,21 @@ hipError_t launch_intensity(long npix, int batch, const float2* f, int sum_batch
                             hipStream_t s);
 hipError_t launch_sum_planes(long npix, int nplanes, const float* planes, int accumulate,
                              float* psf, hipStream_t s);
+
+// Bytes of one batch instance of an R x K complex operand (complex64 or split4).
+size_t mat_bytes(long rows, long k, bool split);
+
+// Generate `nb` transfer matrices (optionally scaled by amp[b]) into `out`.
+int gen_dft(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
+            const double* pout, const double* shift, const double* amp, void* out, bool split,
+            hipStream_t s);
+
+// Device-resident transfer-matrix cache (dlx_cache.hip).  On success *ptr is
+// either the cached matrices (ready for work enqueued on s) or NULL when the
+// cache was bypassed; cache_release records the entry's last use on s.
+int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
+                  const double* pout, const double* shift, bool split, hipStream_t s,
+                  void** ptr, void** handle);
+int cache_release(void* handle, hipStream_t s);
 
 }  // namespace dlx

@@ -3,6 +3,7 @@
 // intensity reductions.  All are HBM-bound; they are written for coalesced
 // 8/16-byte accesses with one pixel (or four) per lane.
 #include "dlx_internal.h"
+#include "dlx_split.h"
 
 namespace dlx {
 
@@ -81,12 +82,136 @@ hipError_t launch_pupil_chunk(const PupilChunk& c, int nb, int n, double dx, dou
 }
 
 // ---------------------------------------------------------------------------
+// split4 producers: one lane per 8-element chunk, four 16-byte stores.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+dft_split4_kernel(DftChunk c, int n_in, int n_out, int inverse, uint8_t* __restrict__ out,
+                  long batch_bytes) {
+  const long kc = split4_chunks(n_in);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (t >= (long)n_out * kc) return;
+  const int l = (int)(t / kc), ch = (int)(t - (long)l * kc);
+  const double bb = (double)l * c.b_scale[b] + c.b_off[b];
+  const float amp = (float)c.amp[b];
+  float2 v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int k = ch * 8 + e;
+    if (k < n_in) {
+      const double a = (double)k * c.a_scale[b] + c.a_off[b];
+      double ph = a * bb;
+      ph -= rint(ph);
+      float s, co;
+      sincospif(2.0f * (float)ph, &s, &co);
+      v[e] = make_float2(amp * co, inverse ? amp * s : -amp * s);
+    } else {
+      v[e] = make_float2(0.f, 0.f);
+    }
+  }
+  store_chunk(out + (long)b * batch_bytes, l, split4_row_bytes(n_in), ch, v);
+}
+
+hipError_t launch_dft_split4(const DftChunk& c, int nb, int n_in, int n_out, int inverse,
+                             uint8_t* out, long batch_bytes, hipStream_t s) {
+  const long work = (long)n_out * split4_chunks(n_in);
+  hipLaunchKernelGGL(dft_split4_kernel, dim3((unsigned)((work + 255) / 256), nb), dim3(256), 0,
+                     s, c, n_in, n_out, inverse, out, batch_bytes);
+  return hipGetLastError();
+}
+
+// Pupil field in split4 (rows y, k = x).  Each lane loads its 8 pixels of
+// amplitude / OPD / phase once and emits them for up to 8 wavelengths.
+constexpr int kPupilWl = 8;
+
+__global__ void __launch_bounds__(256)
+pupil_split4_kernel(PupilChunk c, int nb, int n, double dx, double x0, double tx, double ty,
+                    const float* __restrict__ amp, const float* __restrict__ opd,
+                    const float* __restrict__ phase, float scale, uint8_t* __restrict__ out,
+                    long batch_bytes) {
+  const long kc = split4_chunks(n);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n * kc) return;
+  const int y = (int)(t / kc), ch = (int)(t - (long)y * kc);
+  const double Y = x0 + (double)y * dx;
+  double path[8], turns0[8];
+  float a[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int x = ch * 8 + e;
+    const bool in = x < n;
+    const long p = (long)y * n + x;
+    const double X = x0 + (double)x * dx;
+    path[e] = tx * X + ty * Y + ((in && opd) ? (double)opd[p] : 0.0);
+    turns0[e] = (in && phase) ? (double)phase[p] * 0.15915494309189535 : 0.0;
+    a[e] = in ? scale * (amp ? amp[p] : 1.0f) : 0.f;
+  }
+  const int b0 = blockIdx.y * kPupilWl;
+  const int b1 = b0 + kPupilWl < nb ? b0 + kPupilWl : nb;
+  const long rb = split4_row_bytes(n);
+  for (int b = b0; b < b1; ++b) {
+    float2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      double tt = path[e] * c.inv_wl[b] + turns0[e];
+      tt -= rint(tt);
+      float s, co;
+      sincospif(2.0f * (float)tt, &s, &co);
+      v[e] = make_float2(a[e] * co, a[e] * s);
+    }
+    store_chunk(out + (long)b * batch_bytes, y, rb, ch, v);
+  }
+}
+
+hipError_t launch_pupil_split4(const PupilChunk& c, int nb, int n, double dx, double x0,
+                               double tx, double ty, const float* amp, const float* opd,
+                               const float* phase, float scale, uint8_t* out, long batch_bytes,
+                               hipStream_t s) {
+  const long work = (long)n * split4_chunks(n);
+  dim3 grid((unsigned)((work + 255) / 256), (nb + kPupilWl - 1) / kPupilWl);
+  hipLaunchKernelGGL(pupil_split4_kernel, grid, dim3(256), 0, s, c, nb, n, dx, x0, tx, ty, amp,
+                     opd, phase, scale, out, batch_bytes);
+  return hipGetLastError();
+}
+
+// complex64 rows x k -> split4 (per batch instance).
+__global__ void __launch_bounds__(256)
+to_split4_kernel(const float2* __restrict__ in, long rows, int k, long in_batch,
+                 uint8_t* __restrict__ out, long out_batch_bytes) {
+  const long kc = split4_chunks(k);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (t >= rows * kc) return;
+  const long r = t / kc;
+  const int ch = (int)(t - r * kc);
+  const float2* src = in + (long)b * in_batch + r * k;
+  float2 v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int x = ch * 8 + e;
+    v[e] = x < k ? src[x] : make_float2(0.f, 0.f);
+  }
+  store_chunk(out + (long)b * out_batch_bytes, r, split4_row_bytes(k), ch, v);
+}
+
+hipError_t launch_to_split4(const float2* in, int batch, long rows, int k, long in_batch,
+                            uint8_t* out, long out_batch_bytes, hipStream_t s) {
+  const long work = rows * split4_chunks(k);
+  hipLaunchKernelGGL(to_split4_kernel, dim3((unsigned)((work + 255) / 256), batch), dim3(256),
+                     0, s, in, rows, k, in_batch, out, out_batch_bytes);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Basis evaluation: out[s][p] = sum_j coeffs[s][j] * basis[j][p] (+ base[p]).
 // One lane per 4 pixels (16-byte loads); the basis is streamed once for up to
 // kSets coefficient sets (utils/math.py:177-196 tensordot).
 // ---------------------------------------------------------------------------
 constexpr int kSets = 8;
 
+// NS coefficient sets per launch; the basis loop is unrolled so several
+// 16-byte loads per lane are in flight (the kernel streams the basis once).
+template <int NS>
 __global__ void __launch_bounds__(256)
 eval_basis_kernel(int nbasis, long npix, int nsets, const float* __restrict__ basis,
                   const float* __restrict__ coeffs, const float* __restrict__ base,
@@ -94,11 +219,12 @@ eval_basis_kernel(int nbasis, long npix, int nsets, const float* __restrict__ ba
   const long p4 = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (p4 >= npix) return;
   const bool full = vec_ok && (p4 + 4 <= npix);
-  float acc[kSets][4];
+  float acc[NS][4];
 #pragma unroll
-  for (int s = 0; s < kSets; ++s)
+  for (int s = 0; s < NS; ++s)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[s][q] = 0.f;
+#pragma unroll 6
   for (int j = 0; j < nbasis; ++j) {
     float v[4];
     const float* row = basis + (long)j * npix + p4;
@@ -110,8 +236,8 @@ eval_basis_kernel(int nbasis, long npix, int nsets, const float* __restrict__ ba
       for (int q = 0; q < 4; ++q) v[q] = (p4 + q < npix) ? row[q] : 0.f;
     }
 #pragma unroll
-    for (int s = 0; s < kSets; ++s) {
-      if (s < nsets) {
+    for (int s = 0; s < NS; ++s) {
+      if (NS == 1 || s < nsets) {
         const float cj = coeffs[(long)s * nbasis + j];
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[s][q] = fmaf(cj, v[q], acc[s][q]);
@@ -119,14 +245,23 @@ eval_basis_kernel(int nbasis, long npix, int nsets, const float* __restrict__ ba
     }
   }
 #pragma unroll
-  for (int s = 0; s < kSets; ++s) {
-    if (s >= nsets) break;
+  for (int s = 0; s < NS; ++s) {
+    if (NS > 1 && s >= nsets) break;
+    if (full) {
+      float4 r = make_float4(acc[s][0], acc[s][1], acc[s][2], acc[s][3]);
+      if (base) {
+        const float4 bb = *(const float4*)(base + p4);
+        r.x += bb.x; r.y += bb.y; r.z += bb.z; r.w += bb.w;
+      }
+      *(float4*)(out + (long)s * npix + p4) = r;
+    } else {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (p4 + q < npix) {
-        float r = acc[s][q];
-        if (base) r += base[p4 + q];
-        out[(long)s * npix + p4 + q] = r;
+      for (int q = 0; q < 4; ++q) {
+        if (p4 + q < npix) {
+          float r = acc[s][q];
+          if (base) r += base[p4 + q];
+          out[(long)s * npix + p4 + q] = r;
+        }
       }
     }
   }
@@ -135,14 +270,21 @@ eval_basis_kernel(int nbasis, long npix, int nsets, const float* __restrict__ ba
 hipError_t launch_eval_basis(int nbasis, long npix, int nsets, const float* basis,
                              const float* coeffs, const float* base, float* out,
                              hipStream_t s) {
-  // 16-byte loads need every basis row 16-byte aligned.
-  const int vec_ok = ((npix & 3) == 0) && ((reinterpret_cast<uintptr_t>(basis) & 15) == 0);
+  // 16-byte loads and stores need 16-byte aligned rows
+  const int vec_ok = ((npix & 3) == 0) && ((reinterpret_cast<uintptr_t>(basis) & 15) == 0) &&
+                     ((reinterpret_cast<uintptr_t>(out) & 15) == 0) &&
+                     (!base || (reinterpret_cast<uintptr_t>(base) & 15) == 0);
+  const long lanes = (npix + 3) / 4;
+  const dim3 grid((unsigned)((lanes + 255) / 256));
+  if (nsets == 1) {
+    hipLaunchKernelGGL(eval_basis_kernel<1>, grid, dim3(256), 0, s, nbasis, npix, 1, basis,
+                       coeffs, base, out, vec_ok);
+    return hipGetLastError();
+  }
   for (int s0 = 0; s0 < nsets; s0 += kSets) {
     const int ns = nsets - s0 < kSets ? nsets - s0 : kSets;
-    const long lanes = (npix + 3) / 4;
-    hipLaunchKernelGGL(eval_basis_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
-                       s, nbasis, npix, ns, basis, coeffs + (long)s0 * nbasis, base,
-                       out + (long)s0 * npix, vec_ok);
+    hipLaunchKernelGGL(eval_basis_kernel<kSets>, grid, dim3(256), 0, s, nbasis, npix, ns, basis,
+                       coeffs + (long)s0 * nbasis, base, out + (long)s0 * npix, vec_ok);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -208,21 +350,37 @@ hipError_t launch_intensity(long npix, int batch, const float2* f, int sum_batch
   return hipGetLastError();
 }
 
-// psf[p] (=|+=) sum_b planes[b][p], index order.  accumulate != 0 adds to psf.
+// psf[p] (=|+=) sum_b planes[b][p] in index order (deterministic); four
+// pixels per lane and an unrolled plane loop keep many loads in flight.
 __global__ void __launch_bounds__(256)
 sum_planes_kernel(long npix, int nplanes, const float* __restrict__ planes, int accumulate,
-                  float* __restrict__ psf) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npix) return;
-  float acc = accumulate ? psf[p] : 0.f;
-  for (int b = 0; b < nplanes; ++b) acc += planes[(long)b * npix + p];
-  psf[p] = acc;
+                  float* __restrict__ psf, int vec_ok) {
+  const long p4 = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (p4 >= npix) return;
+  if (vec_ok) {
+    float4 acc = accumulate ? *(const float4*)(psf + p4) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 8
+    for (int b = 0; b < nplanes; ++b) {
+      const float4 v = *(const float4*)(planes + (long)b * npix + p4);
+      acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+    }
+    *(float4*)(psf + p4) = acc;
+  } else {
+    for (long p = p4; p < p4 + 4 && p < npix; ++p) {
+      float acc = accumulate ? psf[p] : 0.f;
+      for (int b = 0; b < nplanes; ++b) acc += planes[(long)b * npix + p];
+      psf[p] = acc;
+    }
+  }
 }
 
 hipError_t launch_sum_planes(long npix, int nplanes, const float* planes, int accumulate,
                              float* psf, hipStream_t s) {
-  hipLaunchKernelGGL(sum_planes_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s,
-                     npix, nplanes, planes, accumulate, psf);
+  const long lanes = (npix + 3) / 4;
+  const int vec_ok = ((npix & 3) == 0) && ((reinterpret_cast<uintptr_t>(planes) & 15) == 0) &&
+                     ((reinterpret_cast<uintptr_t>(psf) & 15) == 0);
+  hipLaunchKernelGGL(sum_planes_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s,
+                     npix, nplanes, planes, accumulate, psf, vec_ok);
   return hipGetLastError();
 }
 

@@ -1,0 +1,65 @@
+// "split4" operand format of the split-BF16x3 contractions.
+//
+// A complex matrix R x K (row-major, k contiguous) is stored as rows of
+// Kc = ceil(K / 8) chunks; a chunk holds 8 consecutive k of four bf16 planes
+//   plane 0 = bf16(re), 1 = bf16(re - plane0), 2 = bf16(im), 3 = bf16(im - plane2)
+// as 4 x 16 bytes (64 B per chunk), zero beyond K.  Every producer of a
+// contraction operand (transfer-matrix generation, pupil synthesis, the
+// stage-1 epilogue, the complex64 converter) writes this format directly, so
+// the GEMM stages 16-byte pieces from HBM to LDS with no conversion work.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dlx {
+
+constexpr int kChunk = 8;          // k per chunk
+constexpr int kChunkBytes = 64;    // 4 planes x 8 bf16
+
+__host__ __device__ inline long split4_chunks(long k) { return (k + kChunk - 1) / kChunk; }
+__host__ __device__ inline long split4_row_bytes(long k) { return split4_chunks(k) * kChunkBytes; }
+
+// Round-to-nearest-even float -> bf16 bits (finite inputs).
+__device__ __forceinline__ uint32_t bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+// x = hi + lo + O(2^-17 |x|)
+__device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) {
+  hi = bf16_rne(x);
+  lo = bf16_rne(x - __uint_as_float(hi << 16));
+}
+
+// Pack 8 complex values (one chunk) into its four 16-byte planes.
+__device__ __forceinline__ void pack_chunk(const float2 (&v)[8], uint4 (&p)[4]) {
+  uint32_t rh[8], rl[8], ih[8], il[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    split_bf16(v[i].x, rh[i], rl[i]);
+    split_bf16(v[i].y, ih[i], il[i]);
+  }
+  p[0] = make_uint4(rh[0] | (rh[1] << 16), rh[2] | (rh[3] << 16), rh[4] | (rh[5] << 16),
+                    rh[6] | (rh[7] << 16));
+  p[1] = make_uint4(rl[0] | (rl[1] << 16), rl[2] | (rl[3] << 16), rl[4] | (rl[5] << 16),
+                    rl[6] | (rl[7] << 16));
+  p[2] = make_uint4(ih[0] | (ih[1] << 16), ih[2] | (ih[3] << 16), ih[4] | (ih[5] << 16),
+                    ih[6] | (ih[7] << 16));
+  p[3] = make_uint4(il[0] | (il[1] << 16), il[2] | (il[3] << 16), il[4] | (il[5] << 16),
+                    il[6] | (il[7] << 16));
+}
+
+// Store one chunk (8 complex values at k = 8*chunk .. 8*chunk+7 of `row`).
+__device__ __forceinline__ void store_chunk(uint8_t* base, long row, long row_bytes, long chunk,
+                                            const float2 (&v)[8]) {
+  uint4 p[4];
+  pack_chunk(v, p);
+  uint4* dst = (uint4*)(base + row * row_bytes + chunk * kChunkBytes);
+  dst[0] = p[0];
+  dst[1] = p[1];
+  dst[2] = p[2];
+  dst[3] = p[3];
+}
+
+}  // namespace dlx

@@ -337,3 +337,23 @@ def profile_read(stage: int) -> tuple[float, int]:
     _lib.check(_lib.load().dlx_profile_read(int(stage), ctypes.byref(t), ctypes.byref(c)),
                "dlx_profile_read")
     return float(t.value), int(c.value)
+
+
+def cache_clear() -> None:
+    """Free every cached transfer matrix (after their pending uses)."""
+    _lib.check(_lib.load().dlx_cache_clear(), "dlx_cache_clear")
+
+
+def cache_set_budget(nbytes: int) -> None:
+    """Device memory the transfer-matrix cache may hold (0 disables it)."""
+    _lib.check(_lib.load().dlx_cache_set_budget(int(nbytes)), "dlx_cache_set_budget")
+
+
+def cache_stats() -> dict:
+    """{'hits', 'misses', 'bytes', 'entries'} of the transfer-matrix cache."""
+    import ctypes
+    h, m = ctypes.c_uint64(), ctypes.c_uint64()
+    b, e = ctypes.c_size_t(), ctypes.c_int32()
+    _lib.check(_lib.load().dlx_cache_stats(ctypes.byref(h), ctypes.byref(m), ctypes.byref(b),
+                                           ctypes.byref(e)), "dlx_cache_stats")
+    return {"hits": h.value, "misses": m.value, "bytes": b.value, "entries": e.value}

@@ -221,6 +221,20 @@ int dlx_asm_prop(int32_t n, int32_t m, int32_t batch, const double* wavelength,
                  void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * Transfer-matrix cache (no counterpart in the reference: dLux rebuilds the
+ * matrices inside every traced call).  dlx_mft and dlx_psf_poly keep the
+ * matrices of each (geometry, wavelengths, pixel scales, shifts) batch
+ * resident on the device and reuse them; least-recently-used batches are
+ * evicted beyond the budget (default 4 GiB, environment DLUX_AMD_CACHE_MB).
+ * A budget of 0 disables caching.  The cache is bypassed for streams under
+ * graph capture.  dlx_cache_clear synchronises on outstanding uses and frees
+ * every entry.
+ */
+int dlx_cache_clear(void);
+int dlx_cache_set_budget(size_t bytes);
+int dlx_cache_stats(uint64_t* hits, uint64_t* misses, size_t* bytes, int32_t* entries);
+
+/*
  * Diagnostics (no counterpart in the reference): live timing of the MFT
  * contraction launches inside dlx_psf_poly.  While enabled, each stage-1
  * (pupil x transfer matrix) and stage-2 (transfer matrix x intermediate, with

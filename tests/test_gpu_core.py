@@ -150,3 +150,45 @@ def test_psf_poly_small(rt, prec):
     err = _peak_err(psf, ref)
     print(f"  peak-normalised error {err:.3e}")
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mft_batch_above_launch_chunk(rt, prec):
+    """More instances than one launch carries (64): per-instance parameters stay aligned."""
+    print(f"[mft-batch70] prec={prec}", flush=True)
+    rng = np.random.default_rng(70)
+    B, n_in, n_out = 70, 16, 12
+    E = rng.standard_normal((B, n_in, n_in)) + 1j * rng.standard_normal((B, n_in, n_in))
+    wls = np.linspace(0.5e-6, 1.5e-6, B)
+    out = rt.mft(E, wls, 1.0 / n_in, n_out, 3e-7, shift_x=0.25, shift_y=-0.5,
+                 precision=prec).cpu().numpy()
+    for b in (0, 33, 63, 64, 69):
+        ref = orc.mft(E[b], wls[b], 1.0 / n_in, n_out, 3e-7, shift=(0.25, -0.5))
+        assert _peak_err(out[b], ref) < FIELD_TOL[prec]
+
+
+def test_matrix_cache_hits_and_bypass(rt):
+    print("[cache]", flush=True)
+    n, n_out = 64, 32
+    mask = orc.circular_aperture(n, 1.0, oversample=1)
+    wls = np.linspace(0.9e-6, 1.1e-6, 5)
+    w = np.ones(5) / 5
+    rt.cache_clear()
+    s0 = rt.cache_stats()
+    a = rt.psf_poly(n, n_out, wls, w, 1.0 / n, 2.5e-7, amp=mask, scale=1.0 / n ** 2,
+                    precision="bf16x3").cpu().numpy()
+    b = rt.psf_poly(n, n_out, wls, w, 1.0 / n, 2.5e-7, amp=mask, scale=1.0 / n ** 2,
+                    precision="bf16x3").cpu().numpy()
+    s1 = rt.cache_stats()
+    assert s1["misses"] == s0["misses"] + 1 and s1["hits"] == s0["hits"] + 1
+    assert s1["entries"] == 1 and s1["bytes"] > 0
+    assert np.array_equal(a, b)
+    rt.cache_set_budget(0)
+    try:
+        c = rt.psf_poly(n, n_out, wls, w, 1.0 / n, 2.5e-7, amp=mask, scale=1.0 / n ** 2,
+                        precision="bf16x3").cpu().numpy()
+    finally:
+        rt.cache_set_budget(4 << 30)
+    assert np.array_equal(a, c)
+    rt.cache_clear()
+    assert rt.cache_stats()["entries"] == 0
