@@ -52,6 +52,13 @@ SIGNATURES = {
     "dlx_psf_poly": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, c_double, c_double,
                                c_double, c_double, c_double, _fp, _fp, _fp, c_double, _fp,
                                c_int32, c_void_p, c_size_t, c_void_p]),
+    "dlx_psf_poly_vjp_workspace_size": (c_size_t, [POINTER(MftGeom), c_int32, c_int32]),
+    "dlx_psf_poly_vjp": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, c_double, c_double,
+                                   c_double, c_double, c_double, _fp, _fp, _fp, c_double, _fp,
+                                   _fp, _fp, c_int32, c_void_p, c_size_t, c_void_p]),
+    "dlx_project_basis_workspace_size": (c_size_t, [c_int32, c_int64, c_int32]),
+    "dlx_project_basis": (c_int32, [c_int32, c_int64, c_int32, _fp, _fp, _fp, c_void_p,
+                                    c_size_t, c_void_p]),
     "dlx_noll_indices": (c_int32, [c_int32, POINTER(c_int32), POINTER(c_int32)]),
     "dlx_zernike_basis": (c_int32, [c_int32, POINTER(c_int32), c_int64, _fp, _fp, c_double,
                                     _fp, c_void_p]),

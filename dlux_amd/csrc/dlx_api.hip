@@ -83,11 +83,16 @@ size_t mat_bytes(long rows, long k, bool split) {
 
 // Generate `nb` transfer matrices with per-instance (lambda, dx, p_out, shift,
 // amplitude) into out[b] -- complex64 or split4 -- in launches of DftChunk::kMax.
+// The generator writes exp(sign 2 pi i u_r v_c) over rows r and columns c; the
+// adjoint W^H (rows k of the input plane, columns l of the output plane) is the
+// same phase with the two coordinate vectors swapped and the sign flipped.
 int gen_dft(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
             const double* pout, const double* shift, const double* amp, void* out, bool split,
-            hipStream_t s) {
-  const long stride = (long)g.n_out * g.n_in;
-  const long bbytes = (long)mat_bytes(g.n_out, g.n_in, true);
+            hipStream_t s, bool adjoint) {
+  const int rows = adjoint ? g.n_in : g.n_out, cols = adjoint ? g.n_out : g.n_in;
+  const int sign_inv = adjoint ? !g.inverse : g.inverse;
+  const long stride = (long)rows * cols;
+  const long bbytes = (long)mat_bytes(rows, cols, true);
   for (int b0 = 0; b0 < nb; b0 += DftChunk::kMax) {
     const int n = nb - b0 < DftChunk::kMax ? nb - b0 : DftChunk::kMax;
     DftChunk c;
@@ -97,17 +102,18 @@ int gen_dft(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
       DLX_CHECK_ARG(std::isfinite(dx[b]) && dx[b] != 0, "pixel_scale_in[%d] must be non-zero", b);
       DLX_CHECK_ARG(std::isfinite(pout[b]), "pixel_scale_out[%d] must be finite", b);
       const MftInst p = make_inst(g, wl[b], dx[b], pout[b], shift ? shift[b] : 0.0);
-      c.a_scale[i] = p.a_scale;
-      c.a_off[i] = p.a_off;
-      c.b_scale[i] = p.b_scale;
-      c.b_off[i] = p.b_off;
+      // columns carry the a (input-plane) vector, rows the b (output-plane) one
+      c.a_scale[i] = adjoint ? p.b_scale : p.a_scale;
+      c.a_off[i] = adjoint ? p.b_off : p.a_off;
+      c.b_scale[i] = adjoint ? p.a_scale : p.b_scale;
+      c.b_off[i] = adjoint ? p.a_off : p.b_off;
       c.amp[i] = amp ? amp[b] : 1.0;
     }
     if (split) {
-      DLX_HIP(launch_dft_split4(c, n, g.n_in, g.n_out, g.inverse,
+      DLX_HIP(launch_dft_split4(c, n, cols, rows, sign_inv,
                                 (uint8_t*)out + (long)b0 * bbytes, bbytes, s));
     } else {
-      DLX_HIP(launch_dft_chunk(c, n, g.n_in, g.n_out, g.inverse,
+      DLX_HIP(launch_dft_chunk(c, n, cols, rows, sign_inv,
                                (float2*)out + (long)b0 * stride, stride, s));
     }
   }
@@ -120,11 +126,12 @@ namespace {
 // into `fallback` when the cache is bypassed (graph capture, no memory).
 int get_matrices(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
                  const double* pout, const double* shift, bool split, void* fallback,
-                 hipStream_t s, void** out, void** handle) {
-  if (int st = cache_acquire(g, nb, wl, dx, pout, shift, split, s, out, handle)) return st;
+                 hipStream_t s, void** out, void** handle, bool adjoint = false) {
+  if (int st = cache_acquire(g, nb, wl, dx, pout, shift, split, s, out, handle, adjoint))
+    return st;
   if (*out) return DLX_OK;
   *out = fallback;
-  return gen_dft(g, nb, wl, dx, pout, shift, nullptr, fallback, split, s);
+  return gen_dft(g, nb, wl, dx, pout, shift, nullptr, fallback, split, s, adjoint);
 }
 
 // FP32 two-stage MFT of `batch` complex64 fields P (N_in x N_in) against
@@ -203,6 +210,25 @@ int mft_split4(const dlx_mft_geom& g, int batch, const uint8_t* P, const uint8_t
     if (pe) { if (int st = pe(slot, s)) return st; }
   }
   return DLX_OK;
+}
+
+// Workspace of the VJP for one chunk of c wavelengths: pupils P, forward and
+// adjoint matrices (used when the cache is bypassed), the stage-1 intermediate
+// (U^T, later V^T), Ebar, and the per-wavelength gradient planes.
+struct VjpLayout {
+  size_t P, Wf, Wa, T, Eb, H, total;
+};
+VjpLayout vjp_layout(const dlx_mft_geom& g, int c, bool split) {
+  VjpLayout L;
+  L.P = align_up((size_t)c * mat_bytes(g.n_in, g.n_in, split));
+  L.Wf = align_up((size_t)c * mat_bytes(g.n_out, g.n_in, split));
+  L.Wa = align_up((size_t)c * mat_bytes(g.n_in, g.n_out, split));
+  const size_t t1 = mat_bytes(g.n_out, g.n_in, split), t2 = mat_bytes(g.n_in, g.n_out, split);
+  L.T = align_up((size_t)c * (t1 > t2 ? t1 : t2));
+  L.Eb = align_up((size_t)c * mat_bytes(g.n_out, g.n_out, split));
+  L.H = align_up((size_t)c * g.n_in * g.n_in * sizeof(float));
+  L.total = L.P + L.Wf + L.Wa + L.T + L.Eb + L.H;
+  return L;
 }
 
 }  // namespace
@@ -444,6 +470,187 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
     if (int st = cache_release(h, s)) return st;
     DLX_HIP(launch_sum_planes(nout2, nb, I, l0 > 0, psf, s));
   }
+  return DLX_OK;
+}
+
+size_t dlx_psf_poly_vjp_workspace_size(const dlx_mft_geom* geom, int32_t nwl,
+                                       int32_t precision) {
+  if (!geom || nwl <= 0) return 0;
+  const int c = nwl < kWlChunk ? nwl : kWlChunk;
+  return vjp_layout(*geom, c, precision == DLX_PREC_BF16X3).total;
+}
+
+int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                     const double* weight, double pixel_scale_in, double pixel_scale_out,
+                     double center, double tilt_x, double tilt_y, const float* amp,
+                     const float* opd, const float* phase, double scale, const float* dpsf,
+                     float* grad_opd, float* grad_phase, int32_t precision, void* workspace,
+                     size_t workspace_bytes, void* stream) {
+  clear_error();
+  if (int st = check_geom(geom)) return st;
+  DLX_CHECK_ARG(nwl > 0, "nwl must be > 0, got %d", nwl);
+  DLX_CHECK_ARG(wavelength && weight && dpsf, "null argument to dlx_psf_poly_vjp");
+  DLX_CHECK_ARG(grad_opd || grad_phase, "dlx_psf_poly_vjp needs grad_opd or grad_phase");
+  DLX_CHECK_ARG(precision == DLX_PREC_FP32 || precision == DLX_PREC_BF16X3,
+                "unknown precision %d", precision);
+  DLX_CHECK_ARG(std::isfinite(pixel_scale_in) && pixel_scale_in != 0.0 &&
+                    std::isfinite(pixel_scale_out) && std::isfinite(center) &&
+                    std::isfinite(tilt_x) && std::isfinite(tilt_y) && std::isfinite(scale),
+                "non-finite scalar argument to dlx_psf_poly_vjp");
+  const size_t need = dlx_psf_poly_vjp_workspace_size(geom, nwl, precision);
+  if (!workspace || workspace_bytes < need) {
+    set_error("dlx_psf_poly_vjp workspace too small: need %zu bytes, got %zu", need,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  for (int l = 0; l < nwl; ++l) {
+    DLX_CHECK_ARG(std::isfinite(weight[l]) && weight[l] >= 0, "weight[%d] must be >= 0", l);
+    DLX_CHECK_ARG(std::isfinite(wavelength[l]) && wavelength[l] > 0,
+                  "wavelength[%d] must be > 0", l);
+  }
+  const dlx_mft_geom& g = *geom;
+  hipStream_t s = (hipStream_t)stream;
+  const bool split = precision == DLX_PREC_BF16X3;
+  const int c = nwl < kWlChunk ? nwl : kWlChunk;
+  const VjpLayout L = vjp_layout(g, c, split);
+  uint8_t* P = (uint8_t*)workspace;
+  uint8_t* Wf_ws = P + L.P;
+  uint8_t* Wa_ws = Wf_ws + L.Wf;
+  uint8_t* T = Wa_ws + L.Wa;
+  uint8_t* Eb = T + L.T;
+  float* H = (float*)(Eb + L.Eb);
+  const long nin = g.n_in, nout = g.n_out;
+  const double x0 = center - (g.n_in - 1) / 2.0 * pixel_scale_in;
+
+  std::vector<double> dx(c, pixel_scale_in), pout(c, pixel_scale_out);
+  for (int l0 = 0; l0 < nwl; l0 += c) {
+    const int nb = nwl - l0 < c ? nwl - l0 : c;
+    float alpha[kWlChunk];
+    GradChunk gc;
+    for (int i = 0; i < nb; ++i) {
+      alpha[i] = (float)(mft_norm(g, wavelength[l0 + i], pixel_scale_in, pixel_scale_out) *
+                         std::sqrt(weight[l0 + i]));
+      gc.kw[i] = (float)(2.0 * M_PI / wavelength[l0 + i]);
+    }
+    void *Wf = nullptr, *hf = nullptr, *Wa = nullptr, *ha = nullptr;
+    if (int st = get_matrices(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr, split,
+                              Wf_ws, s, &Wf, &hf))
+      return st;
+    if (int st = get_matrices(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr, split,
+                              Wa_ws, s, &Wa, &ha, true))
+      return st;
+    if (split) {
+      const long rb_in = split4_row_bytes(g.n_in), rb_out = split4_row_bytes(g.n_out);
+      const long pb = (long)mat_bytes(nin, nin, true), wfb = (long)mat_bytes(nout, nin, true);
+      const long wab = (long)mat_bytes(nin, nout, true), eb = (long)mat_bytes(nout, nout, true);
+      PupilChunk pc;
+      for (int i = 0; i < nb; ++i) pc.inv_wl[i] = 1.0 / wavelength[l0 + i];
+      DLX_HIP(launch_pupil_split4(pc, nb, g.n_in, pixel_scale_in, x0, tilt_x, tilt_y, amp, opd,
+                                  phase, (float)scale, P, pb, s));
+      // forward stage 1: U^T = (P W^T)^T, split4 rows n_out
+      GemmSplit g1{};
+      g1.A = P; g1.a_row_bytes = rb_in; g1.a_batch_bytes = pb;
+      g1.B = (const uint8_t*)Wf; g1.b_row_bytes = rb_in; g1.b_batch_bytes = wfb;
+      g1.C = T; g1.ldc = rb_in; g1.c_batch = wfb;
+      g1.M = g.n_in; g1.N = g.n_out; g1.K = g.n_in; g1.batch = nb; g1.epilogue = EPI_SPLIT_T;
+      for (int i = 0; i < nb; ++i) g1.alpha[i] = 1.0f;
+      DLX_HIP(launch_cgemm_split4(g1, s));
+      // forward stage 2 in transposed order, E^T = U^T W^T, so that the split4
+      // epilogue writes rows of Ebar = 2 G E (rows n_out, k = n_out)
+      GemmSplit g2{};
+      g2.A = T; g2.a_row_bytes = rb_in; g2.a_batch_bytes = wfb;
+      g2.B = (const uint8_t*)Wf; g2.b_row_bytes = rb_in; g2.b_batch_bytes = wfb;
+      g2.C = Eb; g2.ldc = rb_out; g2.c_batch = eb;
+      g2.M = g.n_out; g2.N = g.n_out; g2.K = g.n_in; g2.batch = nb;
+      g2.epilogue = EPI_VJP_SPLIT_T;
+      for (int i = 0; i < nb; ++i) g2.alpha[i] = alpha[i];
+      g2.G = dpsf; g2.ldg = g.n_out; g2.g_batch = 0;
+      DLX_HIP(launch_cgemm_split4(g2, s));
+      // adjoint stage 1: V = Ebar conj(W), stored V^T (rows n_in, k = n_out)
+      GemmSplit g3{};
+      g3.A = Eb; g3.a_row_bytes = rb_out; g3.a_batch_bytes = eb;
+      g3.B = (const uint8_t*)Wa; g3.b_row_bytes = rb_out; g3.b_batch_bytes = wab;
+      g3.C = T; g3.ldc = rb_out; g3.c_batch = wab;
+      g3.M = g.n_out; g3.N = g.n_in; g3.K = g.n_out; g3.batch = nb; g3.epilogue = EPI_SPLIT_T;
+      for (int i = 0; i < nb; ++i) g3.alpha[i] = 1.0f;
+      DLX_HIP(launch_cgemm_split4(g3, s));
+      // adjoint stage 2: Pbar = alpha W^H V, epilogue Im(conj(P) Pbar)
+      GemmSplit g4{};
+      g4.A = (const uint8_t*)Wa; g4.a_row_bytes = rb_out; g4.a_batch_bytes = wab;
+      g4.B = T; g4.b_row_bytes = rb_out; g4.b_batch_bytes = wab;
+      g4.C = H; g4.ldc = nin; g4.c_batch = nin * nin;
+      g4.M = g.n_in; g4.N = g.n_in; g4.K = g.n_out; g4.batch = nb; g4.epilogue = EPI_GRAD;
+      for (int i = 0; i < nb; ++i) g4.alpha[i] = alpha[i];
+      g4.P = P; g4.p_ld = rb_in; g4.p_batch = pb;
+      DLX_HIP(launch_cgemm_split4(g4, s));
+    } else {
+      const long pst = nin * nin, wst = nout * nin, est = nout * nout;
+      if (int st = dlx_pupil_field(g.n_in, nb, wavelength + l0, pixel_scale_in, center, tilt_x,
+                                   tilt_y, amp, opd, phase, scale, nullptr, P, stream))
+        return st;
+      GemmBatch g1{};
+      g1.A = (const float2*)P; g1.lda = nin; g1.sA = pst;
+      g1.B = (const float2*)Wf; g1.ldb = nin; g1.sB = wst;
+      g1.C = T; g1.ldc = nin; g1.sC = wst;
+      g1.M = g.n_in; g1.N = g.n_out; g1.K = g.n_in; g1.batch = nb;
+      g1.epilogue = EPI_STORE_T; g1.precision = DLX_PREC_FP32;
+      for (int i = 0; i < nb; ++i) g1.alpha[i] = 1.0f;
+      DLX_HIP(launch_cgemm_nt(g1, s));
+      GemmBatch g2{};
+      g2.A = (const float2*)Wf; g2.lda = nin; g2.sA = wst;
+      g2.B = (const float2*)T; g2.ldb = nin; g2.sB = wst;
+      g2.C = Eb; g2.ldc = nout; g2.sC = est;
+      g2.M = g.n_out; g2.N = g.n_out; g2.K = g.n_in; g2.batch = nb;
+      g2.epilogue = EPI_VJP; g2.precision = DLX_PREC_FP32;
+      for (int i = 0; i < nb; ++i) g2.alpha[i] = alpha[i];
+      g2.G = dpsf; g2.ldg = nout; g2.g_batch = 0;
+      DLX_HIP(launch_cgemm_nt(g2, s));
+      GemmBatch g3{};
+      g3.A = (const float2*)Eb; g3.lda = nout; g3.sA = est;
+      g3.B = (const float2*)Wa; g3.ldb = nout; g3.sB = wst;
+      g3.C = T; g3.ldc = nout; g3.sC = wst;
+      g3.M = g.n_out; g3.N = g.n_in; g3.K = g.n_out; g3.batch = nb;
+      g3.epilogue = EPI_STORE_T; g3.precision = DLX_PREC_FP32;
+      for (int i = 0; i < nb; ++i) g3.alpha[i] = 1.0f;
+      DLX_HIP(launch_cgemm_nt(g3, s));
+      GemmBatch g4{};
+      g4.A = (const float2*)Wa; g4.lda = nout; g4.sA = wst;
+      g4.B = (const float2*)T; g4.ldb = nout; g4.sB = wst;
+      g4.C = H; g4.ldc = nin; g4.sC = pst;
+      g4.M = g.n_in; g4.N = g.n_in; g4.K = g.n_out; g4.batch = nb;
+      g4.epilogue = EPI_GRAD; g4.precision = DLX_PREC_FP32;
+      for (int i = 0; i < nb; ++i) g4.alpha[i] = alpha[i];
+      g4.P = P; g4.p_ld = nin; g4.p_batch = pst;
+      DLX_HIP(launch_cgemm_nt(g4, s));
+    }
+    if (int st = cache_release(hf, s)) return st;
+    if (int st = cache_release(ha, s)) return st;
+    DLX_HIP(launch_grad_reduce(gc, nb, nin * nin, H, l0 > 0, grad_opd, grad_phase, s));
+  }
+  return DLX_OK;
+}
+
+size_t dlx_project_basis_workspace_size(int32_t nbasis, int64_t npix, int32_t nsets) {
+  if (nbasis <= 0 || npix <= 0 || nsets <= 0) return 0;
+  return align_up((size_t)nbasis * nsets * project_parts(npix) * sizeof(float));
+}
+
+int dlx_project_basis(int32_t nbasis, int64_t npix, int32_t nsets, const float* basis,
+                      const float* grad, float* out, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(nbasis >= 0 && npix >= 0 && nsets >= 0, "negative size to dlx_project_basis");
+  DLX_CHECK_ARG(nbasis <= 65535 && nsets <= 65535, "nbasis and nsets must be <= 65535");
+  if (nbasis == 0 || nsets == 0) return DLX_OK;
+  DLX_CHECK_ARG(basis && grad && out, "null argument to dlx_project_basis");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t need = dlx_project_basis_workspace_size(nbasis, npix, nsets);
+  if (need && (!workspace || workspace_bytes < need)) {
+    set_error("dlx_project_basis workspace too small: need %zu bytes, got %zu", need,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  DLX_HIP(launch_project_basis(nbasis, npix, nsets, basis, grad, (float*)workspace, out, s));
   return DLX_OK;
 }
 

@@ -77,7 +77,7 @@ void make_room(MatCache& c, size_t need) {
 
 int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
                   const double* pout, const double* shift, bool split, hipStream_t s,
-                  void** ptr, void** handle) {
+                  void** ptr, void** handle, bool adjoint) {
   *ptr = nullptr;
   *handle = nullptr;
   MatCache& c = cache();
@@ -94,7 +94,8 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
   key.reserve(9 + 4 * (size_t)nb);
   key.insert(key.end(), {(double)dev, (double)g.n_in, (double)g.n_out, (double)g.inverse,
                          (double)g.has_focal, g.has_focal ? g.focal_length : 0.0,
-                         g.has_focal ? g.focal_shift : 0.0, split ? 1.0 : 0.0, (double)nb});
+                         g.has_focal ? g.focal_shift : 0.0, (split ? 1.0 : 0.0) + (adjoint ? 2.0 : 0.0),
+                         (double)nb});
   key.insert(key.end(), wl, wl + nb);
   key.insert(key.end(), dx, dx + nb);
   key.insert(key.end(), pout, pout + nb);
@@ -113,8 +114,8 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
     }
   }
   ++c.misses;
-  const size_t per = split ? (size_t)g.n_out * split4_row_bytes(g.n_in)
-                           : (size_t)g.n_out * g.n_in * sizeof(float2);
+  const size_t per = adjoint ? mat_bytes(g.n_in, g.n_out, split)
+                             : mat_bytes(g.n_out, g.n_in, split);
   const size_t bytes = align_up(per * (size_t)nb);
   if (bytes > c.budget) return DLX_OK;  // larger than the whole cache: caller generates
   make_room(c, bytes);
@@ -135,7 +136,7 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
     set_error("hipEventCreate failed in the matrix cache");
     return DLX_ERR_HIP;
   }
-  if (int st = gen_dft(g, nb, wl, dx, pout, shift, nullptr, mem, split, s)) {
+  if (int st = gen_dft(g, nb, wl, dx, pout, shift, nullptr, mem, split, s, adjoint)) {
     free_entry(e);
     return st;
   }

@@ -134,6 +134,12 @@ cgemm_nt_fp32(GemmBatch g) {
             ((float2*)g.C)[(long)b * g.sC + (long)m * g.ldc + n] = make_float2(re, im);
           } else if (EPI == EPI_STORE_T) {
             ((float2*)g.C)[(long)b * g.sC + (long)n * g.ldc + m] = make_float2(re, im);
+          } else if (EPI == EPI_VJP) {
+            const float gm = 2.f * g.G[(long)b * g.g_batch + (long)m * g.ldg + n];
+            ((float2*)g.C)[(long)b * g.sC + (long)m * g.ldc + n] = make_float2(gm * re, gm * im);
+          } else if (EPI == EPI_GRAD) {
+            const float2 pv = ((const float2*)g.P)[(long)b * g.p_batch + (long)m * g.p_ld + n];
+            ((float*)g.C)[(long)b * g.sC + (long)m * g.ldc + n] = pv.x * im - pv.y * re;
           } else {
             ((float*)g.C)[(long)b * g.sC + (long)m * g.ldc + n] = re * re + im * im;
           }
@@ -148,6 +154,8 @@ hipError_t dispatch(const GemmBatch& g, hipStream_t s) {
   switch (g.epilogue) {
     case EPI_STORE: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_STORE>), grid, block, 0, s, g); break;
     case EPI_STORE_T: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_STORE_T>), grid, block, 0, s, g); break;
+    case EPI_VJP: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_VJP>), grid, block, 0, s, g); break;
+    case EPI_GRAD: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_GRAD>), grid, block, 0, s, g); break;
     default: hipLaunchKernelGGL((cgemm_nt_fp32<TM, TN, EPI_INTENSITY>), grid, block, 0, s, g); break;
   }
   return hipGetLastError();

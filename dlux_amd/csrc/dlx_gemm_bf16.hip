@@ -98,9 +98,10 @@ __device__ __forceinline__ void epilogue_split(const GemmSplit& g, int b, int m0
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       const int n = n0 + wn + 32 * j + lr;
-      if (EPI == EPI_SPLIT_T) {
+      if (EPI == EPI_SPLIT_T || EPI == EPI_VJP_SPLIT_T) {
         // rows n of a split4 matrix whose k is this GEMM's m: 4 consecutive m
-        // per register group -> half a chunk (8 bytes) per plane.
+        // per register group -> half a chunk (8 bytes) per plane.  The VJP form
+        // scales element (m, n) by 2 G(n, m).
         if (n >= g.N) continue;
         const int m_pad = (g.M + 7) & ~7;   // the last chunk is zero-padded
         uint8_t* crow = (uint8_t*)g.C + (long)b * g.c_batch + (long)n * g.ldc;
@@ -112,8 +113,11 @@ __device__ __forceinline__ void epilogue_split(const GemmSplit& g, int b, int m0
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const bool ok = mq + e < g.M;
-            split_bf16(ok ? alpha * accR[i][j][4 * q + e] : 0.f, rh[e], rl[e]);
-            split_bf16(ok ? alpha * accI[i][j][4 * q + e] : 0.f, ih[e], il[e]);
+            float f = alpha;
+            if (EPI == EPI_VJP_SPLIT_T && ok)
+              f *= 2.f * g.G[(long)b * g.g_batch + (long)n * g.ldg + mq + e];
+            split_bf16(ok ? f * accR[i][j][4 * q + e] : 0.f, rh[e], rl[e]);
+            split_bf16(ok ? f * accI[i][j][4 * q + e] : 0.f, ih[e], il[e]);
           }
           uint8_t* dst = crow + (long)(mq >> 3) * kChunkBytes + ((mq & 7) >> 2) * 8;
           *(uint2*)(dst + 0) = make_uint2(rh[0] | (rh[1] << 16), rh[2] | (rh[3] << 16));
@@ -129,6 +133,15 @@ __device__ __forceinline__ void epilogue_split(const GemmSplit& g, int b, int m0
             const float re = alpha * accR[i][j][r], im = alpha * accI[i][j][r];
             if (EPI == EPI_STORE) {
               ((float2*)g.C)[(long)b * g.c_batch + (long)m * g.ldc + n] = make_float2(re, im);
+            } else if (EPI == EPI_GRAD) {
+              // P(m, n) from its split4 row m: value = hi + lo per component
+              const uint8_t* pr = (const uint8_t*)g.P + (long)b * g.p_batch + (long)m * g.p_ld +
+                                  (long)(n >> 3) * kChunkBytes + (n & 7) * 2;
+              const float pre = __uint_as_float((uint32_t)*(const uint16_t*)(pr) << 16) +
+                                __uint_as_float((uint32_t)*(const uint16_t*)(pr + 16) << 16);
+              const float pim = __uint_as_float((uint32_t)*(const uint16_t*)(pr + 32) << 16) +
+                                __uint_as_float((uint32_t)*(const uint16_t*)(pr + 48) << 16);
+              ((float*)g.C)[(long)b * g.c_batch + (long)m * g.ldc + n] = pre * im - pim * re;
             } else {
               ((float*)g.C)[(long)b * g.c_batch + (long)m * g.ldc + n] = re * re + im * im;
             }
@@ -541,6 +554,8 @@ hipError_t dispatch_split(const GemmSplit& g, hipStream_t s, int mode) {
   switch (g.epilogue) {
     case EPI_STORE: launch_one<TM, TN, EPI_STORE>(g, tiles, mode, s); break;
     case EPI_SPLIT_T: launch_one<TM, TN, EPI_SPLIT_T>(g, tiles, mode, s); break;
+    case EPI_VJP_SPLIT_T: launch_one<TM, TN, EPI_VJP_SPLIT_T>(g, tiles, mode, s); break;
+    case EPI_GRAD: launch_one<TM, TN, EPI_GRAD>(g, tiles, mode, s); break;
     default: launch_one<TM, TN, EPI_INTENSITY>(g, tiles, mode, s); break;
   }
   return hipGetLastError();

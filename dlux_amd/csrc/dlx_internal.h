@@ -89,6 +89,11 @@ enum Epilogue {
   EPI_STORE_T = 1,      // C^T = alpha * acc: written at C[n*ldc + m]
   EPI_INTENSITY = 2,    // I = |alpha * acc|^2 (float32), ldc
   EPI_SPLIT_T = 3,      // split4 matrix with rows n and k = m (ldc / c_batch in bytes)
+  // gradient epilogues (G = dL/dPSF, float, ldg / g_batch in elements)
+  EPI_VJP = 4,          // C(m,n) = 2 G(m,n) alpha acc (complex64, ldc)
+  EPI_VJP_SPLIT_T = 5,  // as EPI_SPLIT_T with value 2 G(n,m) alpha acc (split4 rows n, k = m)
+  EPI_GRAD = 6,         // C(m,n) = Im(conj(P(m,n)) alpha acc) (float, ldc); P complex64
+                        // (fp32 kernels, p_ld elements) or split4 (p_ld bytes per row)
 };
 
 struct GemmBatch {
@@ -100,6 +105,8 @@ struct GemmBatch {
   int epilogue;
   int precision;        // DLX_PREC_*
   float alpha[kMaxBatch];   // per-instance scale of the result
+  const float* G; long ldg; long g_batch;          // EPI_VJP
+  const void* P; long p_ld; long p_batch;           // EPI_GRAD
 };
 
 hipError_t launch_cgemm_nt(const GemmBatch& g, hipStream_t s);
@@ -114,6 +121,8 @@ struct GemmSplit {
   void* C; long ldc; long c_batch;
   int M, N, K, batch, epilogue;
   float alpha[kMaxBatch];
+  const float* G; long ldg; long g_batch;          // EPI_VJP_SPLIT_T
+  const void* P; long p_ld; long p_batch;           // EPI_GRAD (split4, bytes)
 };
 
 hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s);
@@ -127,6 +136,11 @@ struct DftChunk {
 struct PupilChunk {
   static constexpr int kMax = 128;
   double inv_wl[kMax];
+};
+
+struct GradChunk {
+  static constexpr int kMax = 64;
+  float kw[kMax];   // 2 pi / lambda per plane
 };
 
 hipError_t launch_dft_chunk(const DftChunk& c, int nb, int n_in, int n_out, int inverse,
@@ -153,20 +167,29 @@ hipError_t launch_intensity(long npix, int batch, const float2* f, int sum_batch
 hipError_t launch_sum_planes(long npix, int nplanes, const float* planes, int accumulate,
                              float* psf, hipStream_t s);
 
+// Backward reductions (dlx_grad.hip).  partial holds nbasis * nsets * project_parts(npix)
+// floats.
+hipError_t launch_grad_reduce(const GradChunk& c, int nb, long npix, const float* h,
+                              int accumulate, float* g_opd, float* g_phase, hipStream_t s);
+int project_parts(long npix);
+hipError_t launch_project_basis(int nbasis, long npix, int nsets, const float* basis,
+                                const float* g, float* partial, float* out, hipStream_t s);
+
 // Bytes of one batch instance of an R x K complex operand (complex64 or split4).
 size_t mat_bytes(long rows, long k, bool split);
 
-// Generate `nb` transfer matrices (optionally scaled by amp[b]) into `out`.
+// Generate `nb` transfer matrices (optionally scaled by amp[b]) into `out`:
+// W (N_out x N_in) or, with `adjoint`, its conjugate transpose W^H (N_in x N_out).
 int gen_dft(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
             const double* pout, const double* shift, const double* amp, void* out, bool split,
-            hipStream_t s);
+            hipStream_t s, bool adjoint = false);
 
 // Device-resident transfer-matrix cache (dlx_cache.hip).  On success *ptr is
 // either the cached matrices (ready for work enqueued on s) or NULL when the
 // cache was bypassed; cache_release records the entry's last use on s.
 int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double* dx,
                   const double* pout, const double* shift, bool split, hipStream_t s,
-                  void** ptr, void** handle);
+                  void** ptr, void** handle, bool adjoint = false);
 int cache_release(void* handle, hipStream_t s);
 
 }  // namespace dlx

@@ -42,7 +42,8 @@ hipError_t launch_dft_chunk(const DftChunk& c, int nb, int n_in, int n_out, int 
 // Pupil field for a chunk of wavelengths:
 //   out[b](y,x) = s * amp * exp(2 pi i * ((opd + tx X + ty Y)/lambda_b + phase/(2 pi)))
 //                 [* in[b](y,x)]
-// The argument is accumulated in FP64 turns and reduced before the FP32 sincospi.
+// The argument is accumulated in FP64 turns and reduced to [-1/2, 1/2] before
+// the hardware sine / cosine (which take turns).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 pupil_field_kernel(PupilChunk c, int n, double dx, double x0, double tx, double ty,
@@ -59,8 +60,8 @@ pupil_field_kernel(PupilChunk c, int n, double dx, double x0, double tx, double 
   double t = path * c.inv_wl[b];
   if (phase) t += (double)phase[p] * 0.15915494309189535;  // 1 / (2 pi)
   t -= rint(t);
-  float s, co;
-  sincospif(2.0f * (float)t, &s, &co);
+  // hardware sin/cos take turns; max abs error 1.3e-7 on [-1/2, 1/2]
+  const float s = __builtin_amdgcn_sinf((float)t), co = __builtin_amdgcn_cosf((float)t);
   const float a = scale * (amp ? amp[p] : 1.0f);
   float2 v = make_float2(a * co, a * s);
   if (in) {
@@ -155,8 +156,7 @@ pupil_split4_kernel(PupilChunk c, int nb, int n, double dx, double x0, double tx
     for (int e = 0; e < 8; ++e) {
       double tt = path[e] * c.inv_wl[b] + turns0[e];
       tt -= rint(tt);
-      float s, co;
-      sincospif(2.0f * (float)tt, &s, &co);
+      const float s = __builtin_amdgcn_sinf((float)tt), co = __builtin_amdgcn_cosf((float)tt);
       v[e] = make_float2(a[e] * co, a[e] * s);
     }
     store_chunk(out + (long)b * batch_bytes, y, rb, ch, v);

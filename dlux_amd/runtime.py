@@ -184,8 +184,69 @@ def pupil_field(n: int, wavelengths, pixel_scale: float, center=0.0, tilt=(0.0, 
     return out
 
 
+def _eval_basis_impl(bt: torch.Tensor, ct: torch.Tensor, base_t, nb: int, npix: int,
+                     pix_shape) -> torch.Tensor:
+    out = torch.empty(pix_shape, dtype=torch.float32, device=bt.device)
+    st = _lib.load().dlx_eval_basis(nb, npix, 1, bt.data_ptr(), ct.reshape(-1).data_ptr(),
+                                    _ptr(base_t), out.data_ptr(), stream_handle())
+    _lib.check(st, "dlx_eval_basis")
+    return out
+
+
+def project_basis(basis, grad) -> torch.Tensor:
+    """Transpose of eval_basis: out[..., j] = sum_p basis[j, p] grad[..., p].
+
+    basis is (nbasis, *pix), grad is (*pix) or (nsets, *pix); the result is
+    (nbasis,) or (nsets, nbasis), float32, summed in a fixed order.
+    """
+    dev = device()
+    bt = as_float(basis, dev)
+    nbasis = int(bt.shape[0])
+    npix = int(np.prod(bt.shape[1:]))
+    gt = as_float(grad, dev)
+    if gt.numel() % max(npix, 1) != 0:
+        raise ValueError(f"grad of shape {tuple(gt.shape)} does not match basis pixels {npix}")
+    nsets = gt.numel() // npix if npix else 1
+    lib = _lib.load()
+    need = lib.dlx_project_basis_workspace_size(nbasis, npix, nsets)
+    ws = workspace(need, dev)
+    out = torch.empty((nsets, nbasis), dtype=torch.float32, device=dev)
+    st = lib.dlx_project_basis(nbasis, npix, nsets, bt.data_ptr(), gt.data_ptr(),
+                               out.data_ptr(), ws.data_ptr(), ws.numel(), stream_handle())
+    _lib.check(st, "dlx_project_basis")
+    return out[0] if gt.numel() == npix else out
+
+
+class _EvalBasisFn(torch.autograd.Function):
+    """eval_basis with the coefficient (and base) gradient on the device."""
+
+    @staticmethod
+    def forward(ctx, ct, bt, base_t, nb, npix, pix_shape):
+        ctx.save_for_backward(bt)
+        ctx.cshape = tuple(ct.shape)
+        ctx.nb, ctx.npix = nb, npix
+        return _eval_basis_impl(bt, ct, base_t, nb, npix, pix_shape)
+
+    @staticmethod
+    def backward(ctx, grad):
+        (bt,) = ctx.saved_tensors
+        if ctx.needs_input_grad[1]:
+            raise NotImplementedError("eval_basis: gradient with respect to the basis is not "
+                                      "implemented on the device path")
+        g_c = None
+        if ctx.needs_input_grad[0]:
+            g_c = project_basis(bt.reshape(ctx.nb, ctx.npix), grad.reshape(-1))
+            g_c = g_c.reshape(ctx.cshape)
+        g_base = grad if ctx.needs_input_grad[2] else None
+        return g_c, None, g_base, None, None, None
+
+
 def eval_basis(basis, coeffs, base=None) -> torch.Tensor:
-    """sum_j coeffs[..., j] basis[j] on the device; coeffs (nsets, nb) or (nb,)."""
+    """sum_j coeffs[..., j] basis[j] on the device; coeffs (nsets, nb) or (nb,).
+
+    Differentiable with respect to coeffs and base (torch autograd); the
+    backward is dlx_project_basis.
+    """
     dev = device()
     bt = as_float(basis, dev)
     ct = as_float(coeffs, dev)
@@ -196,15 +257,13 @@ def eval_basis(basis, coeffs, base=None) -> torch.Tensor:
             "The leading basis dimensions must match the coefficient shape, "
             f"received {tuple(bt.shape)} and {tuple(lead)}."
         )
-    pix_shape = bt.shape[ct.ndim:]
+    pix_shape = tuple(bt.shape[ct.ndim:])
     npix = int(np.prod(pix_shape))
-    lib = _lib.load()
-    out = torch.empty(pix_shape, dtype=torch.float32, device=dev)
     base_t = None if base is None else as_float(base, dev)
-    st = lib.dlx_eval_basis(nb, npix, 1, bt.data_ptr(), ct.reshape(-1).data_ptr(),
-                            _ptr(base_t), out.data_ptr(), stream_handle())
-    _lib.check(st, "dlx_eval_basis")
-    return out
+    if torch.is_grad_enabled() and any(
+            t is not None and t.requires_grad for t in (ct, bt, base_t)):
+        return _EvalBasisFn.apply(ct, bt, base_t, nb, npix, pix_shape)
+    return _eval_basis_impl(bt, ct, base_t, nb, npix, pix_shape)
 
 
 def power(fields: torch.Tensor) -> torch.Tensor:
@@ -238,37 +297,122 @@ def psf_poly_workspace(n_in, n_out, nwl, precision=None) -> int:
     return int(_lib.load().dlx_psf_poly_workspace_size(g, int(nwl), _prec_code(precision)))
 
 
+def _poly_args(n_in, n_out, wavelengths, weights, focal_length, inverse, precision):
+    wl = np.atleast_1d(np.asarray(wavelengths, np.float64))
+    w = np.atleast_1d(np.asarray(weights, np.float64))
+    if w.shape != wl.shape:
+        raise ValueError(f"Wavelength and weight shape mismatch: {wl.shape} vs {w.shape}")
+    return wl, w, geom(n_in, n_out, focal_length, inverse), _prec_code(precision)
+
+
+def _pupil_inputs(n_in, amp, opd, phase, dev):
+    ts = []
+    for name, x in (("amp", amp), ("opd", opd), ("phase", phase)):
+        t = None if x is None else as_float(x, dev)
+        if t is not None and tuple(t.shape) != (n_in, n_in):
+            raise ValueError(f"{name} must have shape {(n_in, n_in)}, got {tuple(t.shape)}")
+        ts.append(t)
+    return ts
+
+
+def _psf_poly_impl(p: dict, amp_t, opd_t, ph_t, out=None, ws=None) -> torch.Tensor:
+    lib = _lib.load()
+    dev = p["dev"]
+    need = lib.dlx_psf_poly_workspace_size(p["g"], p["wl"].size, p["prec"])
+    if ws is None or ws.numel() < need:
+        ws = workspace(need, dev)
+    if out is None:
+        out = torch.empty((p["n_out"], p["n_out"]), dtype=torch.float32, device=dev)
+    st = lib.dlx_psf_poly(p["g"], p["wl"].size, _lib.dptr(p["wl"]), _lib.dptr(p["w"]),
+                          p["dx"], p["pout"], p["center"], p["tilt"][0], p["tilt"][1],
+                          _ptr(amp_t), _ptr(opd_t), _ptr(ph_t), p["scale"], out.data_ptr(),
+                          p["prec"], ws.data_ptr(), ws.numel(), stream_handle())
+    _lib.check(st, "dlx_psf_poly")
+    return out
+
+
+def _psf_poly_vjp_impl(p: dict, amp_t, opd_t, ph_t, dpsf, need_opd=True, need_phase=False):
+    lib = _lib.load()
+    dev = p["dev"]
+    n = p["n_in"]
+    need = lib.dlx_psf_poly_vjp_workspace_size(p["g"], p["wl"].size, p["prec"])
+    ws = workspace(need, dev)
+    g_opd = torch.empty((n, n), dtype=torch.float32, device=dev) if need_opd else None
+    g_ph = torch.empty((n, n), dtype=torch.float32, device=dev) if need_phase else None
+    d = as_float(dpsf, dev)
+    if tuple(d.shape) != (p["n_out"], p["n_out"]):
+        raise ValueError(f"dpsf must have shape {(p['n_out'], p['n_out'])}, "
+                         f"got {tuple(d.shape)}")
+    st = lib.dlx_psf_poly_vjp(p["g"], p["wl"].size, _lib.dptr(p["wl"]), _lib.dptr(p["w"]),
+                              p["dx"], p["pout"], p["center"], p["tilt"][0], p["tilt"][1],
+                              _ptr(amp_t), _ptr(opd_t), _ptr(ph_t), p["scale"], d.data_ptr(),
+                              _ptr(g_opd), _ptr(g_ph), p["prec"], ws.data_ptr(), ws.numel(),
+                              stream_handle())
+    _lib.check(st, "dlx_psf_poly_vjp")
+    return g_opd, g_ph
+
+
+class _PsfPolyFn(torch.autograd.Function):
+    """Fused polychromatic PSF with the hand-written adjoint as its backward."""
+
+    @staticmethod
+    def forward(ctx, opd_t, ph_t, amp_t, p):
+        ctx.save_for_backward(opd_t, ph_t, amp_t)
+        ctx.p = p
+        return _psf_poly_impl(p, amp_t, opd_t, ph_t)
+
+    @staticmethod
+    def backward(ctx, dpsf):
+        opd_t, ph_t, amp_t = ctx.saved_tensors
+        g_opd, g_ph = _psf_poly_vjp_impl(ctx.p, amp_t, opd_t, ph_t, dpsf.contiguous(),
+                                         need_opd=ctx.needs_input_grad[0],
+                                         need_phase=ctx.needs_input_grad[1])
+        return g_opd, g_ph, None, None
+
+
 def psf_poly(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: float,
              pixel_scale_out: float, amp=None, opd=None, phase=None, scale=1.0,
              tilt=(0.0, 0.0), center=0.0, focal_length=None, inverse=False,
              precision: str | None = None, out: torch.Tensor | None = None,
              ws: torch.Tensor | None = None) -> torch.Tensor:
-    """Fused polychromatic PSF sum_l w_l |MFT_l(P_l)|^2 -> (n_out, n_out) float32."""
-    wl = np.atleast_1d(np.asarray(wavelengths, np.float64))
-    w = np.atleast_1d(np.asarray(weights, np.float64))
-    if w.shape != wl.shape:
-        raise ValueError(f"Wavelength and weight shape mismatch: {wl.shape} vs {w.shape}")
+    """Fused polychromatic PSF sum_l w_l |MFT_l(P_l)|^2 -> (n_out, n_out) float32.
+
+    Differentiable (torch autograd) with respect to opd and phase: the
+    backward is dlx_psf_poly_vjp.  Gradients with respect to amp are not
+    implemented and raise.
+    """
+    wl, w, g, prec = _poly_args(n_in, n_out, wavelengths, weights, focal_length, inverse,
+                                precision)
     dev = device()
-    lib = _lib.load()
-    g = geom(n_in, n_out, focal_length, inverse)
-    prec = _prec_code(precision)
-    amp_t = None if amp is None else as_float(amp, dev)
-    opd_t = None if opd is None else as_float(opd, dev)
-    ph_t = None if phase is None else as_float(phase, dev)
-    for name, t in (("amp", amp_t), ("opd", opd_t), ("phase", ph_t)):
-        if t is not None and tuple(t.shape) != (n_in, n_in):
-            raise ValueError(f"{name} must have shape {(n_in, n_in)}, got {tuple(t.shape)}")
-    need = lib.dlx_psf_poly_workspace_size(g, wl.size, prec)
-    if ws is None or ws.numel() < need:
-        ws = workspace(need, dev)
-    if out is None:
-        out = torch.empty((int(n_out), int(n_out)), dtype=torch.float32, device=dev)
-    st = lib.dlx_psf_poly(g, wl.size, _lib.dptr(wl), _lib.dptr(w), float(pixel_scale_in),
-                          float(pixel_scale_out), float(center), float(tilt[0]),
-                          float(tilt[1]), _ptr(amp_t), _ptr(opd_t), _ptr(ph_t), float(scale),
-                          out.data_ptr(), prec, ws.data_ptr(), ws.numel(), stream_handle())
-    _lib.check(st, "dlx_psf_poly")
-    return out
+    amp_t, opd_t, ph_t = _pupil_inputs(n_in, amp, opd, phase, dev)
+    p = dict(dev=dev, g=g, wl=wl, w=w, prec=prec, n_in=int(n_in), n_out=int(n_out),
+             dx=float(pixel_scale_in), pout=float(pixel_scale_out), center=float(center),
+             tilt=(float(tilt[0]), float(tilt[1])), scale=float(scale))
+    if torch.is_grad_enabled() and any(t is not None and t.requires_grad
+                                       for t in (amp_t, opd_t, ph_t)):
+        if amp_t is not None and amp_t.requires_grad:
+            raise NotImplementedError("psf_poly: gradient with respect to the amplitude is not "
+                                      "implemented on the fused path")
+        if out is not None:
+            raise ValueError("psf_poly: out= cannot be used when gradients are recorded")
+        return _PsfPolyFn.apply(opd_t, ph_t, amp_t, p)
+    return _psf_poly_impl(p, amp_t, opd_t, ph_t, out=out, ws=ws)
+
+
+def psf_poly_vjp(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: float,
+                 pixel_scale_out: float, dpsf, amp=None, opd=None, phase=None, scale=1.0,
+                 tilt=(0.0, 0.0), center=0.0, focal_length=None, inverse=False,
+                 precision: str | None = None, need_opd: bool = True,
+                 need_phase: bool = False):
+    """(dL/dOPD, dL/dphase) of the fused PSF for dL/dPSF = dpsf (either may be None)."""
+    wl, w, g, prec = _poly_args(n_in, n_out, wavelengths, weights, focal_length, inverse,
+                                precision)
+    dev = device()
+    amp_t, opd_t, ph_t = _pupil_inputs(n_in, amp, opd, phase, dev)
+    p = dict(dev=dev, g=g, wl=wl, w=w, prec=prec, n_in=int(n_in), n_out=int(n_out),
+             dx=float(pixel_scale_in), pout=float(pixel_scale_out), center=float(center),
+             tilt=(float(tilt[0]), float(tilt[1])), scale=float(scale))
+    return _psf_poly_vjp_impl(p, amp_t, opd_t, ph_t, dpsf, need_opd, need_phase)
 
 
 def _pow2(m: int) -> bool:

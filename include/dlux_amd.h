@@ -163,6 +163,41 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
                  void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * Vector-Jacobian product of dlx_psf_poly: given dpsf = dL/dpsf (float32
+ * N_out x N_out), writes
+ *   grad_opd[p]   = sum_l (2 pi/lambda_l) Im(conj(P_l[p]) Pbar_l[p])
+ *   grad_phase[p] = sum_l Im(conj(P_l[p]) Pbar_l[p])
+ * with Pbar_l = sqrt(w_l) c_l conj(Y_l) (2 dpsf * E_l) X_l^H, the conjugate-
+ * transpose contractions of the forward MFT (either output may be NULL, not
+ * both).  The per-wavelength fields are recomputed, not stored: the work is
+ * four contractions per wavelength.  Arguments as dlx_psf_poly.
+ * Replaces the reverse-mode derivative JAX builds of the same chain
+ * (jax.grad through OpticalSystem.propagate; MFT at utils/propagation.py:178-256,
+ * phase/OPD application at wavefronts.py:332-367, |E|^2 at wavefronts.py:268-283).
+ */
+size_t dlx_psf_poly_vjp_workspace_size(const dlx_mft_geom* geom, int32_t nwl,
+                                       int32_t precision);
+int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                     const double* weight, double pixel_scale_in,
+                     double pixel_scale_out, double center, double tilt_x,
+                     double tilt_y, const float* amp, const float* opd,
+                     const float* phase, double scale, const float* dpsf,
+                     float* grad_opd, float* grad_phase, int32_t precision,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Transpose of dlx_eval_basis: out[s][j] = sum_p basis[j][p] * grad[s][p]
+ * (nsets x nbasis float32), summed in a fixed order (deterministic).
+ * Replaces the reverse-mode derivative of eval_basis (utils/math.py:177-196,
+ * jnp.tensordot) that jax.grad builds for BasisLayer coefficients
+ * (layers/optical_layers.py:304-327).
+ */
+size_t dlx_project_basis_workspace_size(int32_t nbasis, int64_t npix, int32_t nsets);
+int dlx_project_basis(int32_t nbasis, int64_t npix, int32_t nsets, const float* basis,
+                      const float* grad, float* out, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/*
  * Noll index j -> radial order n and signed azimuthal order m.
  * Replaces: noll_indices, utils/zernikes.py:99-119.
  */

@@ -438,35 +438,53 @@ def gaussian_loss(psf, data, flux):
     return 0.5 * np.sum((flux * psf - data) ** 2 / var)
 
 
-def poly_psf_grad(mask, basis, coeffs, wavelengths, weights, diameter, npixels_out,
-                  pixel_scale_out, data, flux):
-    """(loss, dloss/dcoeffs) by the adjoint of the MFT chain, in FP64.
+def poly_psf_vjp(mask, basis, coeffs, wavelengths, weights, diameter, npixels_out,
+                 pixel_scale_out, G, offset=(0.0, 0.0), normalise_power=True):
+    """(dL/dOPD, dL/dphase) of poly_psf for the cotangent G = dL/dPSF, in FP64.
 
     With P_l = A exp(i k_l OPD) and E_l = sqrt(w_l) s Y^T P_l X:
-      Ebar_l = 2 G E_l,  G = dL/dPSF
+      Ebar_l = 2 G E_l
       Pbar_l = sqrt(w_l) s conj(Y) Ebar_l X^H
       dL/dOPD = sum_l k_l Im(conj(P_l) Pbar_l)   (real-cotangent convention)
-      dL/dc_j = sum_px Z_j dL/dOPD
+      dL/dphase = sum_l Im(conj(P_l) Pbar_l)
+    The reference differentiates the same chain by JAX reverse mode
+    (optical_systems.py:147-223, utils/propagation.py:178-256,
+    wavefronts.py:332-367); this is its closed form.
     """
     wavelengths = np.atleast_1d(np.asarray(wavelengths, np.float64))
     weights = np.atleast_1d(np.asarray(weights, np.float64))
     n = mask.shape[-1]
     d = diameter / n
-    psf, fields = poly_psf(mask, basis, coeffs, wavelengths, weights, diameter,
-                           npixels_out, pixel_scale_out, return_fields=True)
-    var = np.maximum(data, 1.0)
-    loss = 0.5 * np.sum((flux * psf - data) ** 2 / var)
-    G = flux * (flux * psf - data) / var
     g_opd = np.zeros((n, n))
-    for wl, w, E in zip(wavelengths, weights, fields):
-        P = pupil_field(wl, mask, basis, coeffs, diameter)
+    g_phase = np.zeros((n, n))
+    for wl, w in zip(wavelengths, weights):
+        P = pupil_field(wl, mask, basis, coeffs, diameter, offset, normalise_power)
         Y = transfer_matrix(wl, n, d, npixels_out, pixel_scale_out)
         s = np.exp(np.log(calc_nfringes(wl, n, d, npixels_out, pixel_scale_out))
                    - (np.log(n) + np.log(npixels_out)))
+        E = (w ** 0.5) * s * (Y.T @ P @ Y)
         Ebar = 2.0 * G * E
         Pbar = (w ** 0.5) * s * (np.conj(Y) @ Ebar @ np.conj(Y).T)
-        k = 2 * np.pi / wl
-        g_opd += k * np.imag(np.conj(P) * Pbar)
+        h = np.imag(np.conj(P) * Pbar)
+        g_opd += (2 * np.pi / wl) * h
+        g_phase += h
+    return g_opd, g_phase
+
+
+def poly_psf_grad(mask, basis, coeffs, wavelengths, weights, diameter, npixels_out,
+                  pixel_scale_out, data, flux):
+    """(loss, dloss/dcoeffs) of the Gaussian loss by the adjoint chain, in FP64.
+
+    G = dL/dPSF = flux (flux PSF - data) / max(data, 1); then poly_psf_vjp and
+    dL/dc_j = sum_px Z_j dL/dOPD.
+    """
+    psf = poly_psf(mask, basis, coeffs, wavelengths, weights, diameter, npixels_out,
+                   pixel_scale_out)
+    var = np.maximum(data, 1.0)
+    loss = 0.5 * np.sum((flux * psf - data) ** 2 / var)
+    G = flux * (flux * psf - data) / var
+    g_opd, _ = poly_psf_vjp(mask, basis, coeffs, wavelengths, weights, diameter, npixels_out,
+                            pixel_scale_out, G)
     grad = np.tensordot(basis, g_opd, axes=([1, 2], [0, 1]))
     return loss, grad
 
