@@ -40,6 +40,7 @@ def _parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     return ap.parse_args()
 
 
@@ -92,6 +93,88 @@ def _cpu_baseline(cfg, coeffs0, budget_s):
     return ref, len(times) / sum(times), threads, len(times)
 
 
+def _grad_leg(torch, runtime, workloads, prec, steps, warmup, with_oracle):
+    """Phase-retrieval gradient step (the "grad" configuration): forward PSF,
+    Gaussian loss against a Poisson target, backward to the 36 coefficients."""
+    cfg = workloads.GRAD
+    dev = runtime.device()
+    mask, basis = workloads.build_inputs(cfg)
+    truth = workloads.build_system(cfg, cfg.coefficients(seed=cfg.extra["target_seed"]),
+                                   mask, basis)
+    flux = cfg.extra["photons"]
+    with runtime.precision("fp32"), torch.no_grad():
+        model = truth.propagate(cfg.wavelengths, weights=cfg.weights).double().cpu().numpy()
+    rng = np.random.default_rng(cfg.extra["noise_seed"])
+    data_np = rng.poisson(flux * np.clip(model, 0, None)).astype(np.float64)
+    data = torch.as_tensor(data_np, dtype=torch.float32, device=dev)
+    var = torch.clamp(data, min=1.0)
+    c0 = cfg.coefficients()
+    coeffs = torch.tensor(c0, dtype=torch.float32, device=dev, requires_grad=True)
+    system = workloads.build_system(cfg, coeffs, mask, basis)
+
+    def step():
+        psf = system.propagate(cfg.wavelengths, weights=cfg.weights)
+        loss = 0.5 * torch.sum((flux * psf - data) ** 2 / var)
+        coeffs.grad = None
+        loss.backward()
+        return loss
+
+    with runtime.precision(prec):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        loss = float(step())
+    g = coeffs.grad.double().cpu().numpy()
+    err = None
+    if with_oracle:
+        from oracle import dlux_oracle as orc
+        mask_o = orc.circular_aperture(cfg.n_in, 1.0, oversample=1)
+        basis_o = orc.zernike_basis(cfg.js, orc.pixel_coords(cfg.n_in, 1.0), 1.0)
+        _, g_ref = orc.poly_psf_grad(mask_o, basis_o, c0, cfg.wavelengths, cfg.weights, 1.0,
+                                     cfg.n_out, cfg.p_out, data_np, flux)
+        err = float(np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref)))
+    fwd = cfg.mft_flops
+    return {"workload": "grad: 512^2 pupil, 36 Zernikes, 16 lambda 0.9-1.1 um, 128^2 output, "
+                        "Gaussian loss vs Poisson(1e6 photons) target",
+            "precision": prec, "steps_per_s": 1e3 / ms, "ms_per_step": ms, "loss": loss,
+            "grad_max_rel_err_vs_fp64": err,
+            "mft_tflops_per_step": 3 * fwd / 1e12,
+            "note": "forward + VJP (field recomputed: 3 MFTs' worth of contractions per step)"}
+
+
+def _batch_leg(torch, runtime, workloads, steps, warmup):
+    """Uneven batched MFT (the "uneven" configuration): 4096 instances per call."""
+    cfg = workloads.BATCH
+    instances, sources = workloads.build_batch(cfg)
+    batch = runtime.MftBatch(instances, n_outputs=len(sources))
+    for _ in range(warmup):
+        batch()
+    torch.cuda.synchronize()
+    beg, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    beg.record()
+    for _ in range(steps):
+        batch()
+    end.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    dev_ms = beg.elapsed_time(end) / steps
+    flops = cfg.flops(sources)
+    return {"workload": "uneven: 256 sources x 16 lambda = 4096 instances, N_in in "
+                        "{256,512,1024}, N_out in {64,128,256}, tilts +-2 arcsec, "
+                        "log-uniform fluxes, one call (grouped persistent launches)",
+            "precision": "bf16x3", "instances": len(instances), "ms_per_batch": wall,
+            "device_ms_per_batch": dev_ms, "psf_per_s": len(sources) * 1e3 / wall,
+            "instances_per_s": len(instances) * 1e3 / wall,
+            "algorithmic_tflops": flops / (wall * 1e-3) / 1e12,
+            "workspace_gib": batch.ws_bytes / 2 ** 30}
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -138,6 +221,13 @@ def main():
                "sample": f"{nsamp} full PSFs of the same workload, FP64 NumPy oracle "
                          f"(numpy BLAS, {threads} threads)"}
 
+    print("[bench] gradient leg", file=sys.stderr, flush=True)
+    grad = _grad_leg(torch, runtime, workloads, "bf16x3", max(args.steps // 5, 5), 3,
+                     not args.no_cpu)
+    print("[bench] uneven batch leg", file=sys.stderr, flush=True)
+    batched = None if args.no_batch else _batch_leg(torch, runtime, workloads,
+                                                    max(args.steps // 20, 3), 2)
+
     ok = {p: (errs[p] is not None and errs[p] <= ERR_BOUND) for p in psfs}
     head = "bf16x3" if ok["bf16x3"] or errs["bf16x3"] is None else "fp32"
     ms = res[head]["ms"]
@@ -169,8 +259,10 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[head],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head],
                      "traffic": None,
-                     "kernel": f"cgemm_nt_{head} stage 1 (pupil x transfer matrix), "
-                               f"{cfg.n_wl} wavelengths per launch",
+                     "kernel": ("cgemm_split4_deep_kernel" if head == "bf16x3"
+                                else "cgemm_nt_fp32") +
+                               f" stage 1 (pupil x transfer matrix), {cfg.n_wl} "
+                               "wavelengths per launch",
                      "avg_launch_ms": avg1, "launches": c1,
                      "mfma_util": achieved * terms / PEAK_TFLOPS[head],
                      "mft_pair_tflops": pair},
@@ -181,6 +273,8 @@ def main():
                       "stage2_avg_ms": res[p]["s2"][0] / max(res[p]["s2"][1], 1)}
                   for p in res},
         "speedup_bf16x3_over_fp32": res["fp32"]["ms"] / res["bf16x3"]["ms"],
+        "grad": grad,
+        "batched": batched,
     }
     print(json.dumps(line))
 

@@ -34,6 +34,28 @@ class MftGeom(ctypes.Structure):
 
 
 # name -> (restype, argtypes); must match include/dlux_amd.h
+class MftInstance(ctypes.Structure):
+    """``dlx_mft_instance``."""
+
+    _fields_ = [
+        ("field", c_void_p),
+        ("n_in", c_int32),
+        ("n_out", c_int32),
+        ("out_index", c_int32),
+        ("inverse", c_int32),
+        ("has_focal", c_int32),
+        ("reserved_", c_int32),
+        ("focal_length", c_double),
+        ("focal_shift", c_double),
+        ("wavelength", c_double),
+        ("pixel_scale_in", c_double),
+        ("pixel_scale_out", c_double),
+        ("shift_x", c_double),
+        ("shift_y", c_double),
+        ("weight", c_double),
+    ]
+
+
 SIGNATURES = {
     "dlx_version": (c_char_p, []),
     "dlx_last_error": (c_char_p, []),
@@ -59,6 +81,9 @@ SIGNATURES = {
     "dlx_project_basis_workspace_size": (c_size_t, [c_int32, c_int64, c_int32]),
     "dlx_project_basis": (c_int32, [c_int32, c_int64, c_int32, _fp, _fp, _fp, c_void_p,
                                     c_size_t, c_void_p]),
+    "dlx_mft_batched_workspace_size": (c_size_t, [c_int32, POINTER(MftInstance), c_int32]),
+    "dlx_mft_batched_psf": (c_int32, [c_int32, POINTER(MftInstance), c_int32,
+                                      POINTER(c_void_p), c_void_p, c_size_t, c_void_p]),
     "dlx_noll_indices": (c_int32, [c_int32, POINTER(c_int32), POINTER(c_int32)]),
     "dlx_zernike_basis": (c_int32, [c_int32, POINTER(c_int32), c_int64, _fp, _fp, c_double,
                                     _fp, c_void_p]),

@@ -433,30 +433,24 @@ __device__ __forceinline__ void stage_deep(uint8_t* lds_tile, const uint8_t* __r
   }
 }
 
-template <int TM, int TN, int EPI>
-__global__ void __launch_bounds__(kThreads, 2)
-cgemm_split4_deep_kernel(GemmSplit g) {
+// Main loop of the occupancy-2 kernel for one BM x BN output tile at (m0, n0):
+// accumulates sum_k A(m, k) B(n, k) into accR / accI.  `lds` is the
+// workgroup's 2 x (BM + BN) x 128 B staging area.
+template <int TM, int TN>
+__device__ __forceinline__ void deep_mainloop(uint8_t* lds, const uint8_t* __restrict__ A,
+                                              long a_row_bytes, int M,
+                                              const uint8_t* __restrict__ B, long b_row_bytes,
+                                              int N, int K, int m0, int n0,
+                                              f32x16_t (&accR)[TM][TN],
+                                              f32x16_t (&accI)[TM][TN]) {
   constexpr int BM = 64 * TM, BN = 64 * TN;
   constexpr int kBuf = (BM + BN) * kRowDeep;
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kBuf];
-
-  const int tiles_n = (g.N + BN - 1) / BN, tiles_m = (g.M + BM - 1) / BM;
-  const int total = tiles_n * tiles_m * g.batch;
-  const int id = xcd_remap(blockIdx.x, total);
-  const int tn = id % tiles_n;
-  const int tm = (id / tiles_n) % tiles_m;
-  const int b = id / (tiles_n * tiles_m);
-  const int m0 = tm * BM, n0 = tn * BN;
-
-  const uint8_t* __restrict__ A = g.A + (long)b * g.a_batch_bytes;
-  const uint8_t* __restrict__ B = g.B + (long)b * g.b_batch_bytes;
-  const int kchunks = (int)split4_chunks(g.K);
+  const int kchunks = (int)split4_chunks(K);
   const int nst = (kchunks + kDeepChunks - 1) / kDeepChunks;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
   const int lr = lane & 31, lk = lane >> 5;
 
-  f32x16_t accR[TM][TN], accI[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -466,8 +460,8 @@ cgemm_split4_deep_kernel(GemmSplit g) {
 
   auto issue = [&](int st) {
     uint8_t* buf = lds + (st & 1) * kBuf;
-    stage_deep<BM>(buf, A, g.a_row_bytes, m0, g.M, st * kDeepChunks, kchunks);
-    stage_deep<BN>(buf + BM * kRowDeep, B, g.b_row_bytes, n0, g.N, st * kDeepChunks, kchunks);
+    stage_deep<BM>(buf, A, a_row_bytes, m0, M, st * kDeepChunks, kchunks);
+    stage_deep<BN>(buf + BM * kRowDeep, B, b_row_bytes, n0, N, st * kDeepChunks, kchunks);
   };
   issue(0);
 
@@ -519,7 +513,74 @@ cgemm_split4_deep_kernel(GemmSplit g) {
       }
     }
   }
+}
+
+template <int TM, int TN, int EPI>
+__global__ void __launch_bounds__(kThreads, 2)
+cgemm_split4_deep_kernel(GemmSplit g) {
+  constexpr int BM = 64 * TM, BN = 64 * TN;
+  constexpr int kBuf = (BM + BN) * kRowDeep;
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kBuf];
+
+  const int tiles_n = (g.N + BN - 1) / BN, tiles_m = (g.M + BM - 1) / BM;
+  const int total = tiles_n * tiles_m * g.batch;
+  const int id = xcd_remap(blockIdx.x, total);
+  const int tn = id % tiles_n;
+  const int tm = (id / tiles_n) % tiles_m;
+  const int b = id / (tiles_n * tiles_m);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
+  const int lr = lane & 31, lk = lane >> 5;
+
+  f32x16_t accR[TM][TN], accI[TM][TN];
+  deep_mainloop<TM, TN>(lds, g.A + (long)b * g.a_batch_bytes, g.a_row_bytes, g.M,
+                        g.B + (long)b * g.b_batch_bytes, g.b_row_bytes, g.N, g.K, m0, n0,
+                        accR, accI);
   epilogue_split<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped persistent variant for uneven batches: one launch covers the
+// 128 x 128 tiles of many problems of different shapes.  The tile list is cut
+// into 8 contiguous ranges, one per XCD (hardware workgroup w runs on XCD
+// w % 8), so all tiles of a problem share one L2; the workgroups of an XCD
+// stride over its range.  Problems are ordered by the host (largest first
+// within each range), and every workgroup leaves its loop after its range.
+// ---------------------------------------------------------------------------
+template <int EPI>
+__global__ void __launch_bounds__(kThreads, 2)
+cgemm_split4_group_kernel(GroupLaunch gl) {
+  constexpr int TM = 2, TN = 2, BM = 128, BN = 128;
+  constexpr int kBuf = (BM + BN) * kRowDeep;
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kBuf];
+  const int x = blockIdx.x % kXcds;
+  const int per = gridDim.x / kXcds;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
+  const int lr = lane & 31, lk = lane >> 5;
+  const int t_end = gl.xcd_start[x + 1];
+  for (int t = gl.xcd_start[x] + (int)blockIdx.x / kXcds; t < t_end; t += per) {
+    // problem owning tile t: the last one whose tile_start <= t
+    int lo = 0, hi = gl.nprobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (gl.probs[mid].tile_start <= t) lo = mid; else hi = mid - 1;
+    }
+    const GroupProblem& pr = gl.probs[lo];
+    const int local = t - pr.tile_start;
+    const int tn = local % pr.tiles_n, tm = local / pr.tiles_n;
+    const int m0 = tm * BM, n0 = tn * BN;
+    __syncthreads();   // the previous tile's LDS reads have retired
+    f32x16_t accR[TM][TN], accI[TM][TN];
+    deep_mainloop<TM, TN>(lds, pr.A, pr.a_row_bytes, pr.M, pr.B, pr.b_row_bytes, pr.N, pr.K,
+                          m0, n0, accR, accI);
+    GemmSplit v{};
+    v.C = pr.C; v.ldc = pr.ldc; v.c_batch = 0;
+    v.M = pr.M; v.N = pr.N; v.K = pr.K; v.batch = 1; v.epilogue = EPI;
+    v.alpha[0] = pr.alpha;
+    epilogue_split<TM, TN, EPI>(v, 0, m0, n0, wm, wn, lr, lk, accR, accI);
+  }
 }
 
 // Staging variants: 0 = deep direct-to-LDS (default), 1 = direct-to-LDS with
@@ -562,6 +623,26 @@ hipError_t dispatch_split(const GemmSplit& g, hipStream_t s, int mode) {
 }
 
 }  // namespace
+
+hipError_t launch_cgemm_split4_group(const GroupLaunch& gl, int epilogue, hipStream_t s) {
+  const int total = gl.xcd_start[8];
+  if (total <= 0 || gl.nprobs <= 0) return hipSuccess;
+  // two workgroups per CU, 32 CUs per XCD; no more per XCD than its largest range
+  int most = 0;
+  for (int x = 0; x < 8; ++x) {
+    const int n = gl.xcd_start[x + 1] - gl.xcd_start[x];
+    most = n > most ? n : most;
+  }
+  const int per = most < 64 ? most : 64;
+  const dim3 grid(8 * per), block(kThreads);
+  switch (epilogue) {
+    case EPI_SPLIT_T: hipLaunchKernelGGL(cgemm_split4_group_kernel<EPI_SPLIT_T>, grid, block, 0, s, gl); break;
+    case EPI_INTENSITY: hipLaunchKernelGGL(cgemm_split4_group_kernel<EPI_INTENSITY>, grid, block, 0, s, gl); break;
+    case EPI_STORE: hipLaunchKernelGGL(cgemm_split4_group_kernel<EPI_STORE>, grid, block, 0, s, gl); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.batch <= 0) return hipSuccess;

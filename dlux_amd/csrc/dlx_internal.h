@@ -127,6 +127,23 @@ struct GemmSplit {
 
 hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s);
 
+// One problem of a grouped (uneven) split4 GEMM, in device memory.  Its
+// 128 x 128 tiles are [tile_start, tile_start + tiles_m * tiles_n) of the
+// launch's tile list, row-major over (tm, tn).
+struct GroupProblem {
+  const uint8_t* A; const uint8_t* B; void* C;
+  long a_row_bytes, b_row_bytes, ldc;
+  int M, N, K, tiles_n, tiles_m, tile_start;
+  float alpha; int pad_;
+};
+struct GroupLaunch {
+  const GroupProblem* probs;   // device array, sorted by tile_start
+  int nprobs;
+  int xcd_start[9];            // tile range of XCD x: [xcd_start[x], xcd_start[x+1])
+};
+// Epilogues supported: EPI_STORE, EPI_SPLIT_T, EPI_INTENSITY.
+hipError_t launch_cgemm_split4_group(const GroupLaunch& gl, int epilogue, hipStream_t s);
+
 // Per-launch instance parameters passed by value (kernel arguments), so no
 // device-side parameter buffer is needed.
 struct DftChunk {

@@ -8,6 +8,7 @@ converted to contiguous device tensors (complex64 / float32) before the call.
 from __future__ import annotations
 
 import contextlib
+import ctypes
 import os
 
 import numpy as np
@@ -413,6 +414,79 @@ def psf_poly_vjp(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: fl
              dx=float(pixel_scale_in), pout=float(pixel_scale_out), center=float(center),
              tilt=(float(tilt[0]), float(tilt[1])), scale=float(scale))
     return _psf_poly_vjp_impl(p, amp_t, opd_t, ph_t, dpsf, need_opd, need_phase)
+
+
+class MftBatch:
+    """A prepared uneven batch: the descriptor array, outputs and workspace are
+    built once; each call runs dlx_mft_batched_psf on the current stream.
+
+    ``instances`` as for mft_batched_psf.
+    """
+
+    def __init__(self, instances, n_outputs: int | None = None, outputs=None):
+        self.dev = device()
+        self.lib = _lib.load()
+        n = len(instances)
+        self.n = n
+        self.arr = (_lib.MftInstance * max(n, 1))()
+        self._keep = []
+        nout_of = {}
+        for i, d in enumerate(instances):
+            f = as_complex(d["field"], self.dev)
+            if f.ndim != 2 or f.shape[0] != f.shape[1]:
+                raise ValueError(f"instance {i}: field must be square, got {tuple(f.shape)}")
+            self._keep.append(f)
+            e = self.arr[i]
+            e.field = f.data_ptr()
+            e.n_in = int(f.shape[0])
+            e.n_out = int(d["n_out"])
+            e.out_index = int(d.get("out_index", 0))
+            fl = d.get("focal_length")
+            e.inverse = int(bool(d.get("inverse", False)))
+            e.has_focal = 0 if fl is None else 1
+            e.focal_length = 0.0 if fl is None else float(fl)
+            e.focal_shift = float(d.get("focal_shift", 0.0))
+            e.wavelength = float(d["wavelength"])
+            e.pixel_scale_in = float(d["pixel_scale_in"])
+            e.pixel_scale_out = float(d["pixel_scale_out"])
+            sh = d.get("shift", (0.0, 0.0))
+            e.shift_x, e.shift_y = float(sh[0]), float(sh[1])
+            e.weight = float(d.get("weight", 1.0))
+            nout_of.setdefault(e.out_index, e.n_out)
+        if n_outputs is None:
+            n_outputs = (max(nout_of) + 1) if nout_of else 0
+        self.n_outputs = int(n_outputs)
+        if outputs is None:
+            outputs = [torch.empty((nout_of[o], nout_of[o]), dtype=torch.float32,
+                                   device=self.dev) if o in nout_of else None
+                       for o in range(self.n_outputs)]
+        self.outputs = outputs
+        self.ptrs = (ctypes.c_void_p * max(self.n_outputs, 1))(
+            *[(0 if t is None else t.data_ptr()) for t in outputs])
+        self.ws_bytes = int(self.lib.dlx_mft_batched_workspace_size(n, self.arr,
+                                                                    self.n_outputs))
+        self.ws = workspace(self.ws_bytes, self.dev)
+
+    def __call__(self) -> list:
+        st = self.lib.dlx_mft_batched_psf(self.n, self.arr, self.n_outputs, self.ptrs,
+                                          self.ws.data_ptr(), self.ws.numel(), stream_handle())
+        _lib.check(st, "dlx_mft_batched_psf")
+        return self.outputs
+
+
+def mft_batched_psf(instances, n_outputs: int | None = None, outputs=None,
+                    ws: torch.Tensor | None = None) -> list:
+    """Uneven batched PSFs (dlx_mft_batched_psf).
+
+    ``instances`` is a sequence of dicts with keys ``field`` (complex64
+    n_in x n_in device tensor; instances may share one), ``wavelength``,
+    ``pixel_scale_in``, ``n_out``, ``pixel_scale_out`` and optionally
+    ``shift`` ((sx, sy) output pixels), ``weight`` (1), ``out_index`` (0),
+    ``focal_length`` (None), ``inverse`` (False).  Returns the list of output
+    images: output o is the sum over its instances of weight * |MFT(field)|^2
+    (None for an output no instance maps to, unless ``outputs`` is given).
+    """
+    return MftBatch(instances, n_outputs, outputs)()
 
 
 def _pow2(m: int) -> bool:

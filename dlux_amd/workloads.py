@@ -12,6 +12,12 @@ coefficients in metres, flat weights, wavelengths np.linspace(lo, hi, n), and
   grad    512^2 pupil, Noll 2..37 ~ N(0, 50 nm), seed 2, 16 wavelengths on
           0.9-1.1 um, 128^2 output; data = Poisson(1e6-photon PSF of seed-3
           coefficients, seed 4)
+  uneven  256 point sources (seed 5), each with its own complex64 pupil field
+          (N_in drawn from {256, 512, 1024}; circular aperture, Noll 2..11
+          ~ N(0, 30 nm) phase at 1 um), output size from {64, 128, 256}, tilt
+          U(+-2 arcsec) per axis (an MFT shift of tilt / p_out pixels), flux
+          log-uniform on 1e3-1e6, 16 wavelengths on 0.8-1.2 um with flat
+          weights: 4096 (source x wavelength) instances, split-BF16x3
 """
 
 from __future__ import annotations
@@ -93,3 +99,78 @@ def build_system(cfg: PolyConfig, coeffs=None, mask=None, basis=None):
                 ("aberrations", layers.BasisLayer(basis, coeffs))],
         psf_npixels=cfg.n_out, psf_pixel_scale=float(units.rad2arcsec(cfg.p_out)),
         oversample=1)
+
+
+@dataclass(frozen=True)
+class BatchConfig:
+    name: str = "uneven"
+    n_sources: int = 256
+    n_wl: int = 16
+    n_in_choices: tuple = (256, 512, 1024)
+    n_out_choices: tuple = (64, 128, 256)
+    tilt_arcsec: float = 2.0
+    flux_lo: float = 1e3
+    flux_hi: float = 1e6
+    wl_lo: float = 0.8e-6
+    wl_hi: float = 1.2e-6
+    js: tuple = tuple(range(2, 12))
+    coeff_std: float = 30e-9
+    seed: int = 5
+    p_out: float = P_OUT
+
+    @property
+    def wavelengths(self) -> np.ndarray:
+        return np.linspace(self.wl_lo, self.wl_hi, self.n_wl)
+
+    def sources(self) -> list:
+        """Per-source draws: n_in, n_out, tilt (rad, x/y), flux, phase coefficients."""
+        rng = np.random.default_rng(self.seed)
+        out = []
+        for _ in range(self.n_sources):
+            n_in = int(rng.choice(self.n_in_choices))
+            n_out = int(rng.choice(self.n_out_choices))
+            tilt = rng.uniform(-self.tilt_arcsec, self.tilt_arcsec, 2) * np.pi / (180 * 3600)
+            flux = float(np.exp(rng.uniform(np.log(self.flux_lo), np.log(self.flux_hi))))
+            coeffs = rng.normal(0.0, self.coeff_std, len(self.js))
+            out.append(dict(n_in=n_in, n_out=n_out, tilt=tilt, flux=flux, coeffs=coeffs))
+        return out
+
+    def flops(self, sources) -> float:
+        """Algorithmic MFT FLOPs of one batch: sum over instances of F(N_in, N_out)."""
+        return float(sum(8.0 * (s["n_out"] * s["n_in"] ** 2 + s["n_out"] ** 2 * s["n_in"])
+                         for s in sources) * self.n_wl)
+
+
+BATCH = BatchConfig()
+
+
+def build_batch(cfg: BatchConfig, sources=None):
+    """(instances, sources): device pupil fields and per-instance descriptors for
+    runtime.mft_batched_psf (output index = source index)."""
+    import torch
+    from . import runtime
+    from .utils import apertures, coordinates, zernikes
+    sources = cfg.sources() if sources is None else sources
+    dev = runtime.device()
+    geo = {}
+    instances = []
+    for si, src in enumerate(sources):
+        n = src["n_in"]
+        if n not in geo:
+            mask = apertures.circular_aperture(n, D, oversample=1)
+            basis = zernikes.zernike_basis(cfg.js, coordinates.pixel_coords(n, diameter=D), D)
+            geo[n] = (torch.as_tensor(mask, device=dev).float(),
+                      torch.as_tensor(basis, device=dev).float())
+        mask, basis = geo[n]
+        c = torch.as_tensor(src["coeffs"], dtype=torch.float32, device=dev)
+        opd = runtime.eval_basis(basis, c)
+        phase = (2 * np.pi / LAMBDA_REF) * opd
+        field = mask * torch.exp(1j * phase.to(torch.float32))
+        field = (field / torch.sqrt(torch.sum(mask ** 2))).to(torch.complex64).contiguous()
+        for wl in cfg.wavelengths:
+            instances.append(dict(field=field, wavelength=float(wl), pixel_scale_in=D / n,
+                                  n_out=src["n_out"], pixel_scale_out=cfg.p_out,
+                                  shift=(float(src["tilt"][0] / cfg.p_out),
+                                         float(src["tilt"][1] / cfg.p_out)),
+                                  weight=src["flux"] / cfg.n_wl, out_index=si))
+    return instances, sources

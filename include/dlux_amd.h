@@ -198,6 +198,45 @@ int dlx_project_basis(int32_t nbasis, int64_t npix, int32_t nsets, const float* 
                       size_t workspace_bytes, void* stream);
 
 /*
+ * One instance of an uneven batched MFT: a complex64 n_in x n_in field
+ * (device) propagated at one wavelength with its own pixel scales and output
+ * shift (output pixels, per axis), whose intensity times `weight` is added
+ * into output image `out_index`.  Geometry flags as in dlx_mft_geom.
+ */
+typedef struct dlx_mft_instance {
+  const void* field;
+  int32_t n_in, n_out;
+  int32_t out_index;
+  int32_t inverse;
+  int32_t has_focal;
+  int32_t reserved_;
+  double focal_length, focal_shift;
+  double wavelength, pixel_scale_in, pixel_scale_out;
+  double shift_x, shift_y;
+  double weight;
+} dlx_mft_instance;
+
+/*
+ * Uneven batched polychromatic PSFs: for every instance i,
+ *   I_i = weight_i * |c_i Y_i^T field_i X_i|^2   (X_i = T(shift_x), Y_i = T(shift_y))
+ * and outputs[o] = sum of I_i over the instances with out_index == o
+ * (float32 n_out x n_out, all its instances share n_out; an output with no
+ * instance is not written).  Shapes may differ between instances; all
+ * instances run in grouped persistent launches (split-BF16x3 contractions).
+ * Instances that share a field pointer share its conversion.  `instances`
+ * and `outputs` are HOST arrays (outputs holds device pointers).  Cannot be
+ * captured into a graph (it uploads per-call descriptors).
+ * Replaces the vmapped source x wavelength propagation of
+ * PointSources.model (sources.py:392-411) and OpticalSystem.propagate
+ * (optical_systems.py:211-223) with MFT shifts (utils/propagation.py:178-256).
+ */
+size_t dlx_mft_batched_workspace_size(int32_t n, const dlx_mft_instance* instances,
+                                      int32_t n_outputs);
+int dlx_mft_batched_psf(int32_t n, const dlx_mft_instance* instances, int32_t n_outputs,
+                        float* const* outputs, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
+/*
  * Noll index j -> radial order n and signed azimuthal order m.
  * Replaces: noll_indices, utils/zernikes.py:99-119.
  */

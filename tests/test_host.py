@@ -86,6 +86,46 @@ def test_workspace_queries(lib):
     assert lib.dlx_fft2_workspace_size(1024, 2048, 1) >= 2048 * 2048 * 8
 
 
+def test_batched_and_vjp_abi_validation_without_gpu(lib):
+    from dlux_amd import _lib
+    arr = (_lib.MftInstance * 2)()
+    for e in arr:
+        e.field, e.n_in, e.n_out, e.out_index = 4096, 64, 32, 0
+        e.wavelength, e.pixel_scale_in, e.pixel_scale_out, e.weight = 1e-6, 1 / 64, 2.5e-7, 1.0
+    outs = (ctypes.c_void_p * 1)(8192)
+    arr[1].out_index = 1
+    st = lib.dlx_mft_batched_psf(2, arr, 1, outs, None, 0, None)
+    assert st == 1 and b"out_index" in lib.dlx_last_error()
+    arr[1].out_index, arr[1].n_out = 0, 16
+    st = lib.dlx_mft_batched_psf(2, arr, 1, outs, None, 0, None)
+    assert st == 1 and b"different n_out" in lib.dlx_last_error()
+    arr[1].n_out, arr[1].wavelength = 32, -1.0
+    st = lib.dlx_mft_batched_psf(2, arr, 1, outs, None, 0, None)
+    assert st == 1 and b"wavelength" in lib.dlx_last_error()
+    arr[1].wavelength = 1e-6
+    # two instances sharing one 64 x 64 field: one conversion, two matrices each
+    need = lib.dlx_mft_batched_workspace_size(2, arr, 1)
+    assert need >= 64 * 64 * 8 + 2 * 32 * 64 * 8 + 2 * 32 * 64 * 8 + 2 * 32 * 32 * 4
+    st = lib.dlx_mft_batched_psf(2, arr, 1, outs, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+    assert lib.dlx_mft_batched_psf(0, None, 0, None, None, 0, None) == 0
+
+    g = _lib.MftGeom(n_in=64, n_out=32, inverse=0, has_focal=0, focal_length=0.0,
+                     focal_shift=0.0)
+    one = _lib.dptr([1e-6])
+    w = _lib.dptr([1.0])
+    st = lib.dlx_psf_poly_vjp(g, 1, one, w, 1 / 64, 2.5e-7, 0.0, 0.0, 0.0, None, None, None,
+                              1.0, 4096, None, None, 0, None, 0, None)
+    assert st == 1 and b"grad_opd or grad_phase" in lib.dlx_last_error()
+    st = lib.dlx_psf_poly_vjp(g, 1, one, w, 1 / 64, 2.5e-7, 0.0, 0.0, 0.0, None, None, None,
+                              1.0, 4096, 8192, None, 0, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+    assert lib.dlx_psf_poly_vjp_workspace_size(g, 1, 1) > lib.dlx_psf_poly_workspace_size(g, 1, 1)
+    assert lib.dlx_project_basis_workspace_size(36, 512 * 512, 1) >= 36 * 4
+    st = lib.dlx_project_basis(3, 100, 1, 4096, 8192, 12288, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+
+
 def test_wavefront_constructor_errors():
     import dlux_amd as dl
     with pytest.raises(ValueError, match="Provide one"):
