@@ -541,6 +541,131 @@ cgemm_split4_deep_kernel(GemmSplit g) {
 }
 
 // ---------------------------------------------------------------------------
+// Wide variant: one 8-wave workgroup per CU on a 256 x 128 tile (waves 4 x 2,
+// each 64 x 64), three 48 KiB LDS buffers (144 KiB), so the loads of stage
+// s + 2 are in flight while stage s is computed.  The per-stage wait keeps the
+// six newest loads of this wave (stage s + 1) outstanding.
+// ---------------------------------------------------------------------------
+constexpr int kWideThreads = 512;
+constexpr int kWideBufs = 3;
+
+template <int ROWS>
+__device__ __forceinline__ void stage_wide(uint8_t* lds_tile, const uint8_t* __restrict__ base,
+                                           long row_bytes, int row0, int nrows, int kc0,
+                                           int kchunks) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < ROWS / 64; ++q) {
+    const int rb = 64 * q + 8 * wave;                 // first of this instruction's 8 rows
+    const int r = rb + (lane >> 3);
+    const int piece = (lane & 7) ^ ((r >> 1) & 7);
+    const int chunk = kc0 + (piece >> 2);
+    const int rg = row0 + r;
+    const uint8_t* src = (rg < nrows && chunk < kchunks)
+                             ? base + (long)rg * row_bytes + (long)chunk * kChunkBytes +
+                                   (piece & 3) * 16
+                             : (const uint8_t*)g_zero_line;
+    __builtin_amdgcn_global_load_lds((gbl_void_t*)src, (lds_void_t*)(lds_tile + rb * kRowDeep),
+                                     16, 0, 0);
+  }
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(kWideThreads, 1)
+cgemm_split4_wide_kernel(GemmSplit g) {
+  constexpr int TM = 2, TN = 2, BM = 256, BN = 128;
+  constexpr int kBuf = (BM + BN) * kRowDeep;
+  constexpr int kLoads = BM / 64 + BN / 64;           // global_load_lds per wave per stage
+  static_assert(kLoads == 6, "the stage wait below encodes six loads");
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kWideBufs * kBuf];
+
+  const int tiles_n = (g.N + BN - 1) / BN, tiles_m = (g.M + BM - 1) / BM;
+  const int total = tiles_n * tiles_m * g.batch;
+  const int id = xcd_remap(blockIdx.x, total);
+  const int tn = id % tiles_n;
+  const int tm = (id / tiles_n) % tiles_m;
+  const int b = id / (tiles_n * tiles_m);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const uint8_t* __restrict__ A = g.A + (long)b * g.a_batch_bytes;
+  const uint8_t* __restrict__ B = g.B + (long)b * g.b_batch_bytes;
+  const int kchunks = (int)split4_chunks(g.K);
+  const int nst = (kchunks + kDeepChunks - 1) / kDeepChunks;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
+  const int lr = lane & 31, lk = lane >> 5;
+
+  f32x16_t accR[TM][TN], accI[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) { accR[i][j][r] = 0.f; accI[i][j][r] = 0.f; }
+
+  auto issue = [&](int st) {
+    uint8_t* buf = lds + (st % kWideBufs) * kBuf;
+    stage_wide<BM>(buf, A, g.a_row_bytes, m0, g.M, st * kDeepChunks, kchunks);
+    stage_wide<BN>(buf + BM * kRowDeep, B, g.b_row_bytes, n0, g.N, st * kDeepChunks, kchunks);
+  };
+  issue(0);
+  if (nst > 1) issue(1);
+
+  for (int s = 0; s < nst; ++s) {
+    // stage s has landed for this wave (stage s + 1 may stay in flight); the
+    // barrier makes every wave's loads visible and retires stage s - 1 reads
+    if (s + 1 < nst)
+      asm volatile("s_waitcnt vmcnt(6)\n\ts_barrier" ::: "memory");
+    else
+      asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+    if (s + 2 < nst) issue(s + 2);
+    const uint8_t* la = lds + (s % kWideBufs) * kBuf;
+    const uint8_t* lb = la + BM * kRowDeep;
+    u32x4_t fa[TM][4], fb[TN][4];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int r = wm + 32 * i + lr;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        fa[i][p] = *(const u32x4_t*)(la + r * kRowDeep + (((lk << 2) | p) ^ ((r >> 1) & 7)) * 16);
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int r = wn + 32 * j + lr;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        fb[j][p] = *(const u32x4_t*)(lb + r * kRowDeep + (((lk << 2) | p) ^ ((r >> 1) & 7)) * 16);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const bf16x8_t arh = as_bf(fa[i][0]), arl = as_bf(fa[i][1]);
+      const bf16x8_t aih = as_bf(fa[i][2]), ail = as_bf(fa[i][3]);
+      const bf16x8_t naih = as_bf(neg_bf(fa[i][2])), nail = as_bf(neg_bf(fa[i][3]));
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const bf16x8_t brh = as_bf(fb[j][0]), brl = as_bf(fb[j][1]);
+        const bf16x8_t bih = as_bf(fb[j][2]), bil = as_bf(fb[j][3]);
+        f32x16_t r = accR[i][j], im = accI[i][j];
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, brh, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, bih, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brl, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bil, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nail, bih, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ail, brh, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bil, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brl, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brh, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bih, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bih, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brh, im, 0, 0, 0);
+        accR[i][j] = r;
+        accI[i][j] = im;
+      }
+    }
+  }
+  epilogue_split<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
+}
+
+// ---------------------------------------------------------------------------
 // Grouped persistent variant for uneven batches: one launch covers the
 // 128 x 128 tiles of many problems of different shapes.  The tile list is cut
 // into 8 contiguous ranges, one per XCD (hardware workgroup w runs on XCD
@@ -589,6 +714,7 @@ cgemm_split4_group_kernel(GroupLaunch gl) {
 int staging_mode() {
   static const int mode = [] {
     const char* v = getenv("DLUX_AMD_GEMM_STAGING");
+    if (v && std::string(v) == "wide") return 3;
     if (v && std::string(v) == "regs") return 2;
     if (v && std::string(v) == "dl") return 1;
     return 0;
@@ -644,14 +770,33 @@ hipError_t launch_cgemm_split4_group(const GroupLaunch& gl, int epilogue, hipStr
   return hipGetLastError();
 }
 
+template <int EPI>
+void launch_wide(const GemmSplit& g, hipStream_t s) {
+  const int tiles = ((g.N + 127) / 128) * ((g.M + 255) / 256) * g.batch;
+  hipLaunchKernelGGL(cgemm_split4_wide_kernel<EPI>, dim3(tiles), dim3(kWideThreads), 0, s, g);
+}
+
 hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.batch <= 0) return hipSuccess;
   if (g.batch > GemmSplit::kMaxBatch) return hipErrorInvalidValue;
+  // one 256 x 128 tile per CU: only when there are enough tiles for every CU
+  const long tiles_wide = (long)((g.N + 127) / 128) * ((g.M + 255) / 256) * g.batch;
+  if (staging_mode() == 3 && g.M > 128 && g.N > 64 && tiles_wide >= 256) {
+    switch (g.epilogue) {
+      case EPI_STORE: launch_wide<EPI_STORE>(g, s); break;
+      case EPI_SPLIT_T: launch_wide<EPI_SPLIT_T>(g, s); break;
+      case EPI_VJP_SPLIT_T: launch_wide<EPI_VJP_SPLIT_T>(g, s); break;
+      case EPI_GRAD: launch_wide<EPI_GRAD>(g, s); break;
+      default: launch_wide<EPI_INTENSITY>(g, s); break;
+    }
+    return hipGetLastError();
+  }
   // 128 x 128 tiles: the occupancy-2 kernel when they give two workgroups per
   // CU (256 CUs), else the one-workgroup-per-CU direct-to-LDS kernel; 64 x 64
   // tiles for small problems.
   const long tiles128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch;
   int mode = staging_mode();
+  if (mode == 3) mode = 0;
   if (mode == 0 && tiles128 < 512) mode = 1;
   if (g.M <= 64 || g.N <= 64 || tiles128 < 128) return dispatch_split<1, 1>(g, s, mode);
   return dispatch_split<2, 2>(g, s, mode);

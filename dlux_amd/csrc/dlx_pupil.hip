@@ -121,8 +121,9 @@ hipError_t launch_dft_split4(const DftChunk& c, int nb, int n_in, int n_out, int
   return hipGetLastError();
 }
 
-// Pupil field in split4 (rows y, k = x).  Each lane loads its 8 pixels of
-// amplitude / OPD / phase once and emits them for up to 8 wavelengths.
+// Pupil field in split4 (rows y, k = x).  A quad of lanes produces one chunk
+// (8 pixels): lane q loads pixels 2q, 2q+1 of amplitude / OPD / phase once and
+// emits them for up to 8 wavelengths with quad-cooperative 16-byte stores.
 constexpr int kPupilWl = 8;
 
 __global__ void __launch_bounds__(256)
@@ -132,14 +133,17 @@ pupil_split4_kernel(PupilChunk c, int nb, int n, double dx, double x0, double tx
                     long batch_bytes) {
   const long kc = split4_chunks(n);
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)n * kc) return;
-  const int y = (int)(t / kc), ch = (int)(t - (long)y * kc);
+  // the work size is a multiple of 4 and of the quad alignment: whole quads exit together
+  if (t >= (long)n * kc * 4) return;
+  const int q = (int)(t & 3);
+  const long cidx = t >> 2;
+  const int y = (int)(cidx / kc), ch = (int)(cidx - (long)y * kc);
   const double Y = x0 + (double)y * dx;
-  double path[8], turns0[8];
-  float a[8];
+  double path[2], turns0[2];
+  float a[2];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const int x = ch * 8 + e;
+  for (int e = 0; e < 2; ++e) {
+    const int x = ch * 8 + 2 * q + e;
     const bool in = x < n;
     const long p = (long)y * n + x;
     const double X = x0 + (double)x * dx;
@@ -151,15 +155,16 @@ pupil_split4_kernel(PupilChunk c, int nb, int n, double dx, double x0, double tx
   const int b1 = b0 + kPupilWl < nb ? b0 + kPupilWl : nb;
   const long rb = split4_row_bytes(n);
   for (int b = b0; b < b1; ++b) {
-    float2 v[8];
+    float2 v[2];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < 2; ++e) {
       double tt = path[e] * c.inv_wl[b] + turns0[e];
       tt -= rint(tt);
+      // hardware sin/cos take turns; max abs error 1.3e-7 on [-1/2, 1/2]
       const float s = __builtin_amdgcn_sinf((float)tt), co = __builtin_amdgcn_cosf((float)tt);
       v[e] = make_float2(a[e] * co, a[e] * s);
     }
-    store_chunk(out + (long)b * batch_bytes, y, rb, ch, v);
+    store_chunk_quad(out + (long)b * batch_bytes, y, rb, ch, q, v[0], v[1]);
   }
 }
 
@@ -167,7 +172,7 @@ hipError_t launch_pupil_split4(const PupilChunk& c, int nb, int n, double dx, do
                                double tx, double ty, const float* amp, const float* opd,
                                const float* phase, float scale, uint8_t* out, long batch_bytes,
                                hipStream_t s) {
-  const long work = (long)n * split4_chunks(n);
+  const long work = (long)n * split4_chunks(n) * 4;
   dim3 grid((unsigned)((work + 255) / 256), (nb + kPupilWl - 1) / kPupilWl);
   hipLaunchKernelGGL(pupil_split4_kernel, grid, dim3(256), 0, s, c, nb, n, dx, x0, tx, ty, amp,
                      opd, phase, scale, out, batch_bytes);

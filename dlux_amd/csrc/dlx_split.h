@@ -62,4 +62,42 @@ __device__ __forceinline__ void store_chunk(uint8_t* base, long row, long row_by
   dst[3] = p[3];
 }
 
+// Quad-cooperative chunk store: the four lanes q = 0..3 of a quad hold
+// elements 2q and 2q+1 of one chunk (v0, v1).  Lane q receives plane q of the
+// chunk by a 4 x 4 exchange inside the quad (DPP quad permutations: round m
+// reads lane q ^ m) and stores it as one 16-byte write, so the lanes of a
+// wave write consecutive 16-byte pieces.  All four lanes must be active.
+template <int M>
+__device__ __forceinline__ uint32_t quad_xor(uint32_t v) {
+  // quad_perm [q ^ M for q in 0..3]
+  constexpr int ctrl = M == 1 ? 0xB1 : (M == 2 ? 0x4E : 0x1B);
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t sel4(const uint32_t (&a)[4], int i) {
+  return (i & 2) ? ((i & 1) ? a[3] : a[2]) : ((i & 1) ? a[1] : a[0]);
+}
+
+__device__ __forceinline__ void store_chunk_quad(uint8_t* base, long row, long row_bytes,
+                                                 long chunk, int q, float2 v0, float2 v1) {
+  uint32_t h0, l0, h1, l1, w[4];
+  split_bf16(v0.x, h0, l0);
+  split_bf16(v1.x, h1, l1);
+  w[0] = h0 | (h1 << 16);
+  w[1] = l0 | (l1 << 16);
+  split_bf16(v0.y, h0, l0);
+  split_bf16(v1.y, h1, l1);
+  w[2] = h0 | (h1 << 16);
+  w[3] = l0 | (l1 << 16);
+  // g[m] = the word lane q ^ m holds for plane q
+  uint32_t g[4];
+  g[0] = sel4(w, q);
+  g[1] = quad_xor<1>(sel4(w, q ^ 1));
+  g[2] = quad_xor<2>(sel4(w, q ^ 2));
+  g[3] = quad_xor<3>(sel4(w, q ^ 3));
+  // word j of plane q comes from lane j = q ^ m
+  const uint4 o = make_uint4(sel4(g, q), sel4(g, q ^ 1), sel4(g, q ^ 2), sel4(g, q ^ 3));
+  *(uint4*)(base + row * row_bytes + chunk * kChunkBytes + q * 16) = o;
+}
+
 }  // namespace dlx

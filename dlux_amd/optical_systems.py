@@ -361,6 +361,40 @@ class ParametricLayeredOpticalSystem(ParametricOpticalSystem, LayeredOpticalSyst
         return psf, p_out
 
 
+    def _sources_psf(self, wavelengths, positions, weights):
+        """Sum over point sources of sum_l w[s, l] |MFT_l(P_l)|^2, each source's
+        position (radians) entering as an MFT shift of position / p_out output
+        pixels (equal in intensity to the pupil tilt), as ONE uneven batched
+        call over every (source, wavelength) instance.  None when the layers do
+        not lower, gradients are being recorded, or the FP32 form is selected
+        (the batched launches run the split-BF16x3 contractions)."""
+        if runtime.get_precision() != "bf16x3":
+            return None
+        lowered = self._lower(list(self.layers.values()), np.zeros(2))
+        if lowered is None:
+            return None
+        plan, tilt = lowered
+        if torch.is_grad_enabled() and any(
+                t is not None and isinstance(t, torch.Tensor) and t.requires_grad
+                for t in (plan.amp, plan.opd, plan.phase)):
+            return None
+        n_out, p_out, f = self._focus_params()
+        if f is not None:
+            return None
+        wavelengths = np.atleast_1d(np.asarray(wavelengths, float))
+        fields = runtime.pupil_field(plan.n, wavelengths, plan.dx, tilt=tilt, amp=plan.amp,
+                                     opd=plan.opd, phase=plan.phase, scale=plan.scale)
+        instances = []
+        for pos, w in zip(np.asarray(positions, float), np.asarray(weights, float)):
+            for l, wl in enumerate(wavelengths):
+                instances.append(dict(field=fields[l], wavelength=float(wl),
+                                      pixel_scale_in=plan.dx, n_out=int(n_out),
+                                      pixel_scale_out=float(p_out),
+                                      shift=(float(pos[0] / p_out), float(pos[1] / p_out)),
+                                      weight=float(w[l]), out_index=0))
+        return runtime.mft_batched_psf(instances, n_outputs=1)[0], p_out
+
+
 class AngularOpticalSystem(ParametricLayeredOpticalSystem):
     """Focal plane sampled in arcseconds (optical_systems.py:597-680)."""
 

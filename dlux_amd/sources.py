@@ -85,6 +85,12 @@ class PointSources(Source):
         if return_wf:
             return [optics.propagate(src.wavelengths, pos, w, return_wf=True)
                     for pos, w in zip(src.position, weights)]
+        # every (source, wavelength) pair in one batched call where the system allows it
+        batched = getattr(optics, "_sources_psf", None)
+        out = batched(src.wavelengths, src.position, weights) if batched else None
+        if out is not None:
+            psf, p_out = out
+            return PSF(psf, p_out) if return_psf else psf
         psf = None
         for pos, w in zip(src.position, weights):
             p = optics.propagate(src.wavelengths, pos, w)

@@ -184,3 +184,25 @@ def test_point_sources_sum(dl):
     ref = orc.point_sources_psf(mask_r, basis_r, cfg.coefficients(), [0.9e-6, 1.0e-6],
                                 [1, 1], pos, flux, 1.0, cfg.n_out, cfg.p_out)
     assert _err(psf, ref) < 1e-5
+
+
+def test_point_sources_batched_bf16x3(dl):
+    """PointSources.model under the split-BF16x3 form: all (source, wavelength)
+    instances in one batched call, positions as MFT shifts."""
+    print("[point-sources-batched]", flush=True)
+    from dlux_amd import runtime, workloads
+    cfg = workloads.MONO
+    optics = workloads.build_system(cfg)
+    rng = np.random.default_rng(21)
+    pos = rng.uniform(-4e-6, 4e-6, (7, 2))
+    flux = np.exp(rng.uniform(0, 4, 7))
+    wls = [0.9e-6, 1.0e-6, 1.1e-6]
+    src = dl.PointSources(wavelengths=wls, position=pos, flux=flux)
+    with runtime.precision("bf16x3"):
+        assert optics._sources_psf(wls, pos, np.ones((7, 3))) is not None
+        psf = src.model(optics)
+    mask_r = orc.circular_aperture(cfg.n_in, 1.0, oversample=1)
+    basis_r = orc.zernike_basis(cfg.js, orc.pixel_coords(cfg.n_in, 1.0), 1.0)
+    ref = orc.point_sources_psf(mask_r, basis_r, cfg.coefficients(), wls, [1, 1, 1], pos,
+                                flux, 1.0, cfg.n_out, cfg.p_out)
+    assert _err(psf, ref) < 1e-5

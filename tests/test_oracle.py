@@ -301,3 +301,40 @@ def test_asm_invariants():
     assert np.isclose(np.sum(np.abs(a) ** 2), np.sum(np.abs(u) ** 2))
     b = orc.asm_prop(orc.asm_prop(u, dx, wl, 3e-5, 32), dx, wl, 7e-5, 32)
     assert np.allclose(a, b, atol=1e-12)
+
+
+# ---------------------------------------------------------------- golden fixtures
+GOLDEN = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+def _g(name):
+    return np.load(GOLDEN / name)
+
+
+def test_oracle_reproduces_golden_fixtures():
+    """The committed FP64 fixtures (tests/golden/make_golden.py) still come out
+    of the oracle: a change to the oracle that alters them fails here."""
+    d = _g("mft_32to16.npz")
+    for i in range(len(d["out"])):
+        f = None if np.isnan(d["focal"][i]) else float(d["focal"][i])
+        out = orc.mft(d["field"], 1.0, 0.1, 16, 0.05, focal_length=f,
+                      shift=tuple(d["shift"][i]), inverse=bool(d["inverse"][i]))
+        assert np.allclose(out, d["out"][i], rtol=0, atol=1e-13)
+    d = _g("psf_mono_64to32.npz")
+    psf = orc.poly_psf(d["mask"], d["basis"], d["coeffs"], [1e-6], [1.0], 1.0, 32, 2.5e-7)
+    assert np.allclose(psf, d["psf"], rtol=1e-12, atol=0)
+    d = _g("psf_poly3_48to24.npz")
+    psf = orc.poly_psf(d["mask"], d["basis"], d["coeffs"], d["wavelengths"], d["weights"], 1.0,
+                       24, 2.5e-7)
+    assert np.allclose(psf, d["psf"], rtol=1e-12, atol=0)
+    d = _g("zernike_16.npz")
+    zb = orc.zernike_basis(d["js"], orc.pixel_coords(16, 1.0), 1.0)
+    assert np.allclose(zb, d["basis"], rtol=1e-12, atol=1e-14, equal_nan=True)
+    d = _g("fft_16to32.npz")
+    assert np.allclose(orc.fft(d["field"], 1.0, 1.0, pad=2)[0], d["forward"], atol=1e-14)
+    d = _g("grad_32to16.npz")
+    loss, grad = orc.poly_psf_grad(d["mask"], d["basis"], d["coeffs"], d["wavelengths"],
+                                   d["weights"], 1.0, 16, float(d["p_out"]), d["data"],
+                                   float(d["flux"]))
+    assert np.isclose(loss, d["loss"], rtol=1e-12)
+    assert np.allclose(grad, d["grad"], rtol=1e-10)
