@@ -30,6 +30,11 @@ import numpy as np
 METRIC = "MFT PSFs/s at <=1e-5 rel err vs FP64; serving p50/p99 latency"
 PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0}   # MI355X dense MFMA (fp32, bf16)
 ERR_BOUND = 1e-5
+# HBM bytes per launch of the stage-1 kernel at this workload, from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE runs scaled by the same counters on a streaming kernel of
+# known size (profiles/mft_poly_512to256_64wl_hbm_traffic.md).  The counters cannot
+# be read from inside this process.
+TRAFFIC_BYTES = {"bf16x3": 269.0e6}
 
 
 def _parse():
@@ -42,6 +47,10 @@ def _parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-grad", action="store_true", help="skip the gradient leg")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a streaming kernel of known size (intensity of 8 Mi "
+                         "complex64: 64 MiB read, 32 MiB written) to scale PMC byte counters")
     ap.add_argument("--serving-rates", type=float, nargs="+", default=[100.0, 300.0, 1000.0])
     ap.add_argument("--serving-seconds", type=float, default=2.0)
     return ap.parse_args()
@@ -212,6 +221,13 @@ def main():
     import torch
     from dlux_amd import runtime, workloads
 
+    if args.calibrate:
+        f = torch.ones((1, 8 * 1024 * 1024), dtype=torch.complex64, device=runtime.device())
+        for _ in range(3):
+            runtime.intensity(f)
+        torch.cuda.synchronize()
+        del f
+
     cfg = workloads.POLY
     print(f"[bench] building {cfg.name}: {cfg.n_in}^2 -> {cfg.n_out}^2 x {cfg.n_wl} lambda",
           file=sys.stderr, flush=True)
@@ -249,8 +265,8 @@ def main():
                          f"(numpy BLAS, {threads} threads)"}
 
     print("[bench] gradient leg", file=sys.stderr, flush=True)
-    grad = _grad_leg(torch, runtime, workloads, "bf16x3", max(args.steps // 5, 5), 3,
-                     not args.no_cpu)
+    grad = None if args.no_grad else _grad_leg(torch, runtime, workloads, "bf16x3",
+                                                max(args.steps // 5, 5), 3, not args.no_cpu)
     print("[bench] uneven batch leg", file=sys.stderr, flush=True)
     batched = None if args.no_batch else _batch_leg(torch, runtime, workloads,
                                                     max(args.steps // 20, 3), 2)
@@ -289,7 +305,10 @@ def main():
                    "precision": head},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[head],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head],
-                     "traffic": None,
+                     "traffic": TRAFFIC_BYTES.get(head),
+                     "traffic_unit": "bytes per launch (rocprofv3 PMC, calibrated)",
+                     "algorithmic_bytes": float(cfg.n_wl * 8 * (cfg.n_in * cfg.n_in +
+                                                                2 * cfg.n_out * cfg.n_in)),
                      "kernel": ("cgemm_split4_deep_kernel" if head == "bf16x3"
                                 else "cgemm_nt_fp32") +
                                f" stage 1 (pupil x transfer matrix), {cfg.n_wl} "

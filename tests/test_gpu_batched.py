@@ -34,7 +34,8 @@ def _pupil(n, seed):
     """Unit-power circular pupil with a random low-order aberration (complex128)."""
     mask, basis = orc.circular_aperture(n, 1.0, oversample=1, zernike_nolls=range(2, 9))
     c = np.random.default_rng(seed).normal(0, 40e-9, basis.shape[0])
-    opd = np.tensordot(basis, c, axes=(0, 0))
+    # odd grids put a pixel at rho = 0, where some reference Zernike terms are NaN
+    opd = np.tensordot(np.nan_to_num(basis), c, axes=(0, 0))
     f = mask * np.exp(2j * np.pi * opd / 1e-6)
     return f / np.sqrt(np.sum(np.abs(f) ** 2))
 

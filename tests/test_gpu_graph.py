@@ -1,0 +1,74 @@
+"""HIP graph capture of the fused PSF and of its VJP: a captured call replays
+with new inputs copied into its static buffers, with eager work between the
+replays, and every replay equals the eager result (same kernels, same inputs;
+tolerance 1e-6 peak-normalised)."""
+
+import numpy as np
+import pytest
+
+from oracle import dlux_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def rt():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dlux_amd import runtime
+    return runtime
+
+
+def _peak_err(a, b):
+    a = a.double().cpu().numpy()
+    b = b.double().cpu().numpy()
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_psf_poly_and_vjp_graph_replays(rt, prec):
+    print(f"[graph] prec={prec}", flush=True)
+    n, n_out, nwl = 96, 48, 5
+    mask, basis = orc.circular_aperture(n, 1.0, oversample=1, zernike_nolls=range(2, 9))
+    wls = np.linspace(0.9e-6, 1.1e-6, nwl)
+    w = np.ones(nwl) / nwl
+    scale = 1.0 / np.sqrt(np.sum(mask ** 2))
+    amp = torch.as_tensor(mask, dtype=torch.float32, device="cuda")
+    rng = np.random.default_rng(5)
+    opds = [torch.as_tensor(np.tensordot(basis, rng.normal(0, 40e-9, basis.shape[0]),
+                                         axes=(0, 0)), dtype=torch.float32, device="cuda")
+            for _ in range(4)]
+    G = torch.as_tensor(rng.standard_normal((n_out, n_out)), dtype=torch.float32,
+                        device="cuda")
+    kw = dict(amp=amp, scale=scale, precision=prec)
+
+    def eager(o):
+        with torch.no_grad():
+            psf = rt.psf_poly(n, n_out, wls, w, 1.0 / n, 2.5e-7, opd=o, **kw)
+            g_opd, _ = rt.psf_poly_vjp(n, n_out, wls, w, 1.0 / n, 2.5e-7, G, opd=o, **kw)
+        return psf.clone(), g_opd.clone()
+
+    refs = [eager(o) for o in opds]
+    static_opd = opds[0].clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        eager(static_opd)                       # warm-up outside the capture
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.no_grad(), torch.cuda.graph(graph):
+        psf_s = rt.psf_poly(n, n_out, wls, w, 1.0 / n, 2.5e-7, opd=static_opd, **kw)
+        g_s, _ = rt.psf_poly_vjp(n, n_out, wls, w, 1.0 / n, 2.5e-7, G, opd=static_opd, **kw)
+    for rep in range(2):
+        for i, o in enumerate(opds):
+            static_opd.copy_(o)
+            graph.replay()
+            torch.cuda.synchronize()
+            e_psf = _peak_err(psf_s, refs[i][0])
+            e_g = _peak_err(g_s, refs[i][1])
+            assert e_psf <= 1e-6 and e_g <= 1e-6, (rep, i, e_psf, e_g)
+            eager(opds[(i + 1) % len(opds)])    # eager work between replays
+            torch.cuda.synchronize()

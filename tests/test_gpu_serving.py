@@ -66,8 +66,9 @@ def test_alone_equals_in_mix_and_matches_oracle(sv):
     for r in reqs:
         if r.kind != "psf":
             continue
-        mask, basis = orc.circular_aperture(r.n_in, 1.0, oversample=1,
-                                            zernike_nolls=cfg.js)
+        # the serving loop's geometry: hard circle, Zernikes on pixel_coords(n, D) with D
+        mask = orc.circular_aperture(r.n_in, 1.0, oversample=1)
+        basis = orc.zernike_basis(cfg.js, orc.pixel_coords(r.n_in, 1.0), 1.0)
         w = np.ones(len(r.wavelengths)) / len(r.wavelengths)
         ref = orc.poly_psf(mask, basis, r.coeffs, r.wavelengths, w, 1.0, r.n_out, 2.5e-7)
         err = _peak_err(r.result.cpu().numpy(), ref)
