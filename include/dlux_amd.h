@@ -296,10 +296,11 @@ int dlx_asm_prop(int32_t n, int32_t m, int32_t batch, const double* wavelength,
 
 /*
  * Transfer-matrix cache (no counterpart in the reference: dLux rebuilds the
- * matrices inside every traced call).  dlx_mft and dlx_psf_poly keep the
- * matrices of each (geometry, wavelengths, pixel scales, shifts) batch
- * resident on the device and reuse them; least-recently-used batches are
- * evicted beyond the budget (default 4 GiB, environment DLUX_AMD_CACHE_MB).
+ * matrices inside every traced call).  dlx_mft, dlx_psf_poly,
+ * dlx_psf_poly_vjp and dlx_mft_batched_psf keep the matrices of each
+ * (geometry, wavelengths, pixel scales, shifts) batch resident on the device
+ * and reuse them; least-recently-used batches are
+ * evicted beyond the budget (default 16 GiB, environment DLUX_AMD_CACHE_MB).
  * A budget of 0 disables caching.  The cache is bypassed for streams under
  * graph capture.  dlx_cache_clear synchronises on outstanding uses and frees
  * every entry.

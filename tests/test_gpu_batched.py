@@ -141,3 +141,25 @@ def test_batched_leaves_empty_outputs_and_rejects_bad_input(rt):
     bad = dict(inst[0], n_out=12)
     with pytest.raises(RuntimeError, match="different n_out"):
         rt.mft_batched_psf([inst[0], bad])
+
+
+def test_batched_matrices_stay_resident(rt):
+    """A repeated batch finds its transfer matrices in the device cache and gives
+    the same images; with the cache disabled the matrices are generated in the
+    workspace and the images are unchanged."""
+    print("[batched] matrix cache", flush=True)
+    rng = np.random.default_rng(13)
+    inst = _make(rng, [32, 48], 10, 3)
+    rt.cache_clear()
+    a = [o.clone() for o in rt.mft_batched_psf(inst, n_outputs=3)]
+    s0 = rt.cache_stats()
+    b = rt.mft_batched_psf(inst, n_outputs=3)
+    s1 = rt.cache_stats()
+    assert s1["hits"] == s0["hits"] + 1 and s1["misses"] == s0["misses"]
+    rt.cache_set_budget(0)
+    try:
+        c = rt.mft_batched_psf(inst, n_outputs=3)
+    finally:
+        rt.cache_set_budget(16 << 30)
+    for x, y, z in zip(a, b, c):
+        assert torch.equal(x, y) and torch.equal(x, z)
