@@ -37,6 +37,9 @@ struct MatEntry {
 struct MatCache {
   std::mutex mu;
   std::vector<MatEntry*> entries;
+  // raw keys seen once (hash -> tick): a raw block is cached only on its second
+  // sighting, so one-off batches (a serving loop's) do not allocate or evict
+  std::vector<std::pair<uint64_t, uint64_t>> seen;
   size_t budget = (size_t)16 << 30;  // 16 GiB by default (DLUX_AMD_CACHE_MB)
   size_t used = 0;
   uint64_t tick = 0, hits = 0, misses = 0;
@@ -149,11 +152,21 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
   return DLX_OK;
 }
 
+uint64_t hash_key(const std::vector<double>& key) {
+  uint64_t h = 1469598103934665603ull;   // FNV-1a over the key bytes
+  const uint8_t* p = (const uint8_t*)key.data();
+  for (size_t i = 0; i < key.size() * sizeof(double); ++i) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
 int cache_acquire_raw(const std::vector<double>& key_in, size_t bytes, hipStream_t s,
                       void** ptr, void** handle, bool* fresh) {
   *ptr = nullptr;
   *handle = nullptr;
-  *fresh = false;
+  *fresh = true;   // bypass and miss: the caller generates the contents
   MatCache& c = cache();
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
@@ -176,6 +189,7 @@ int cache_acquire_raw(const std::vector<double>& key_in, size_t bytes, hipStream
       if (e->pending) return DLX_OK;   // being filled by another caller: bypass
       e->tick = ++c.tick;
       ++c.hits;
+      *fresh = false;
       DLX_HIP(hipStreamWaitEvent(s, e->ready, 0));
       *ptr = e->ptr;
       *handle = e;
@@ -185,6 +199,20 @@ int cache_acquire_raw(const std::vector<double>& key_in, size_t bytes, hipStream
   ++c.misses;
   const size_t need = align_up(bytes);
   if (need > c.budget) return DLX_OK;
+  const uint64_t h = hash_key(key);
+  bool again = false;
+  for (auto& kv : c.seen)
+    if (kv.first == h) { again = true; kv.second = ++c.tick; break; }
+  if (!again) {
+    if (c.seen.size() >= 256) {
+      size_t old = 0;
+      for (size_t i = 1; i < c.seen.size(); ++i)
+        if (c.seen[i].second < c.seen[old].second) old = i;
+      c.seen.erase(c.seen.begin() + old);
+    }
+    c.seen.push_back({h, ++c.tick});
+    return DLX_OK;   // first sighting: the caller generates into its workspace
+  }
   make_room(c, need);
   void* mem = nullptr;
   if (hipMalloc(&mem, need) != hipSuccess) {

@@ -436,13 +436,40 @@ __device__ __forceinline__ void stage_deep(uint8_t* lds_tile, const uint8_t* __r
 // Main loop of the occupancy-2 kernel for one BM x BN output tile at (m0, n0):
 // accumulates sum_k A(m, k) B(n, k) into accR / accI.  `lds` is the
 // workgroup's 2 x (BM + BN) x 128 B staging area.
-template <int TM, int TN>
-__device__ __forceinline__ void deep_mainloop(uint8_t* lds, const uint8_t* __restrict__ A,
-                                              long a_row_bytes, int M,
-                                              const uint8_t* __restrict__ B, long b_row_bytes,
-                                              int N, int K, int m0, int n0,
-                                              f32x16_t (&accR)[TM][TN],
-                                              f32x16_t (&accI)[TM][TN]) {
+// Bounds-free loader for tiles entirely inside both operands with an even
+// number of k-chunks: each thread's source addresses are formed once per tile
+// and advanced by one k-slice (128 B) per stage.
+template <int ROWS>
+struct FastRows {
+  const uint8_t* src[ROWS / 32];
+  __device__ __forceinline__ void init(const uint8_t* __restrict__ base, long row_bytes,
+                                       int row0) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < ROWS / 32; ++q) {
+      const int r = 32 * q + 8 * wave + (lane >> 3);
+      const int piece = (lane & 7) ^ ((r >> 1) & 7);
+      src[q] = base + (long)(row0 + r) * row_bytes + (piece >> 2) * kChunkBytes +
+               (piece & 3) * 16;
+    }
+  }
+  __device__ __forceinline__ void issue(uint8_t* lds_tile, int kc0) const {
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < ROWS / 32; ++q)
+      __builtin_amdgcn_global_load_lds((gbl_void_t*)(src[q] + (long)kc0 * kChunkBytes),
+                                       (lds_void_t*)(lds_tile + (32 * q + 8 * wave) * kRowDeep),
+                                       16, 0, 0);
+  }
+};
+
+template <int TM, int TN, bool FAST, bool PRIO = true>
+__device__ __forceinline__ void deep_mainloop_impl(uint8_t* lds, const uint8_t* __restrict__ A,
+                                                   long a_row_bytes, int M,
+                                                   const uint8_t* __restrict__ B,
+                                                   long b_row_bytes, int N, int K, int m0,
+                                                   int n0, f32x16_t (&accR)[TM][TN],
+                                                   f32x16_t (&accI)[TM][TN]) {
   constexpr int BM = 64 * TM, BN = 64 * TN;
   constexpr int kBuf = (BM + BN) * kRowDeep;
   const int kchunks = (int)split4_chunks(K);
@@ -458,10 +485,21 @@ __device__ __forceinline__ void deep_mainloop(uint8_t* lds, const uint8_t* __res
 #pragma unroll
       for (int r = 0; r < 16; ++r) { accR[i][j][r] = 0.f; accI[i][j][r] = 0.f; }
 
+  FastRows<BM> fa_rows;
+  FastRows<BN> fb_rows;
+  if (FAST) {
+    fa_rows.init(A, a_row_bytes, m0);
+    fb_rows.init(B, b_row_bytes, n0);
+  }
   auto issue = [&](int st) {
     uint8_t* buf = lds + (st & 1) * kBuf;
-    stage_deep<BM>(buf, A, a_row_bytes, m0, M, st * kDeepChunks, kchunks);
-    stage_deep<BN>(buf + BM * kRowDeep, B, b_row_bytes, n0, N, st * kDeepChunks, kchunks);
+    if (FAST) {
+      fa_rows.issue(buf, st * kDeepChunks);
+      fb_rows.issue(buf + BM * kRowDeep, st * kDeepChunks);
+    } else {
+      stage_deep<BM>(buf, A, a_row_bytes, m0, M, st * kDeepChunks, kchunks);
+      stage_deep<BN>(buf + BM * kRowDeep, B, b_row_bytes, n0, N, st * kDeepChunks, kchunks);
+    }
   };
   issue(0);
 
@@ -486,6 +524,9 @@ __device__ __forceinline__ void deep_mainloop(uint8_t* lds, const uint8_t* __res
       for (int p = 0; p < 4; ++p)
         fb[j][p] = *(const u32x4_t*)(lb + r * kRowDeep + (((lk << 2) | p) ^ ((r >> 1) & 7)) * 16);
     }
+    // raise this wave's issue priority while it feeds the matrix cores, so the
+    // SIMD's other wave does its loads and LDS reads in the gaps
+    if (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
       const bf16x8_t arh = as_bf(fa[i][0]), arl = as_bf(fa[i][1]);
@@ -512,7 +553,25 @@ __device__ __forceinline__ void deep_mainloop(uint8_t* lds, const uint8_t* __res
         accI[i][j] = im;
       }
     }
+    if (PRIO) __builtin_amdgcn_s_setprio(0);
   }
+}
+
+template <int TM, int TN>
+__device__ __forceinline__ void deep_mainloop(uint8_t* lds, const uint8_t* __restrict__ A,
+                                              long a_row_bytes, int M,
+                                              const uint8_t* __restrict__ B, long b_row_bytes,
+                                              int N, int K, int m0, int n0,
+                                              f32x16_t (&accR)[TM][TN],
+                                              f32x16_t (&accI)[TM][TN]) {
+  constexpr int BM = 64 * TM, BN = 64 * TN;
+  const bool interior = m0 + BM <= M && n0 + BN <= N && (K % (8 * kDeepChunks)) == 0;
+  if (interior)
+    deep_mainloop_impl<TM, TN, true>(lds, A, a_row_bytes, M, B, b_row_bytes, N, K, m0, n0,
+                                     accR, accI);
+  else
+    deep_mainloop_impl<TM, TN, false>(lds, A, a_row_bytes, M, B, b_row_bytes, N, K, m0, n0,
+                                      accR, accI);
 }
 
 template <int TM, int TN, int EPI>
@@ -698,8 +757,10 @@ cgemm_split4_group_kernel(GroupLaunch gl) {
     const int m0 = tm * BM, n0 = tn * BN;
     __syncthreads();   // the previous tile's LDS reads have retired
     f32x16_t accR[TM][TN], accI[TM][TN];
-    deep_mainloop<TM, TN>(lds, pr.A, pr.a_row_bytes, pr.M, pr.B, pr.b_row_bytes, pr.N, pr.K,
-                          m0, n0, accR, accI);
+    // the generic loader: the persistent loop's own state leaves no registers
+    // for the bounds-free one
+    deep_mainloop_impl<TM, TN, false, false>(lds, pr.A, pr.a_row_bytes, pr.M, pr.B,
+                                             pr.b_row_bytes, pr.N, pr.K, m0, n0, accR, accI);
     GemmSplit v{};
     v.C = pr.C; v.ldc = pr.ldc; v.c_batch = 0;
     v.M = pr.M; v.N = pr.N; v.K = pr.K; v.batch = 1; v.epilogue = EPI;

@@ -211,7 +211,8 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
 int cache_release(void* handle, hipStream_t s);
 // Raw entries keyed by the caller: on a hit (*fresh false) the block is ready
 // for work on s; on a miss (*fresh true) the caller fills it on s and then
-// calls cache_ready.  *ptr == NULL when bypassed (capture, budget, memory).
+// calls cache_ready.  *ptr == NULL (and *fresh true) when bypassed (capture,
+// budget, memory): the caller generates into its own buffer.
 int cache_acquire_raw(const std::vector<double>& key, size_t bytes, hipStream_t s,
                       void** ptr, void** handle, bool* fresh);
 int cache_ready(void* handle, hipStream_t s);
