@@ -140,7 +140,7 @@ def _grad_leg(torch, runtime, workloads, prec, steps, warmup, with_oracle):
             step()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
-        loss = float(step())
+        loss = float(step().detach())
     g = coeffs.grad.double().cpu().numpy()
     err = None
     if with_oracle:

@@ -10,6 +10,8 @@ import ctypes
 from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_size_t, c_void_p
 from pathlib import Path
 
+import numpy as np
+
 LIB_PATH = Path(__file__).resolve().parent / "_native" / "libdlux_amd.so"
 
 DLX_OK = 0
@@ -54,6 +56,16 @@ class MftInstance(ctypes.Structure):
         ("shift_y", c_double),
         ("weight", c_double),
     ]
+
+
+# numpy view of dlx_mft_instance with the same field offsets (vectorised filling)
+INSTANCE_DTYPE = np.dtype({
+    "names": [f[0] for f in MftInstance._fields_],
+    "formats": [np.uint64 if f[1] is c_void_p else (np.int32 if f[1] is c_int32 else np.float64)
+                for f in MftInstance._fields_],
+    "offsets": [getattr(MftInstance, f[0]).offset for f in MftInstance._fields_],
+    "itemsize": ctypes.sizeof(MftInstance),
+})
 
 
 SIGNATURES = {

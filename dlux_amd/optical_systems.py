@@ -384,15 +384,15 @@ class ParametricLayeredOpticalSystem(ParametricOpticalSystem, LayeredOpticalSyst
         wavelengths = np.atleast_1d(np.asarray(wavelengths, float))
         fields = runtime.pupil_field(plan.n, wavelengths, plan.dx, tilt=tilt, amp=plan.amp,
                                      opd=plan.opd, phase=plan.phase, scale=plan.scale)
-        instances = []
-        for pos, w in zip(np.asarray(positions, float), np.asarray(weights, float)):
-            for l, wl in enumerate(wavelengths):
-                instances.append(dict(field=fields[l], wavelength=float(wl),
-                                      pixel_scale_in=plan.dx, n_out=int(n_out),
-                                      pixel_scale_out=float(p_out),
-                                      shift=(float(pos[0] / p_out), float(pos[1] / p_out)),
-                                      weight=float(w[l]), out_index=0))
-        return runtime.mft_batched_psf(instances, n_outputs=1)[0], p_out
+        pos = np.asarray(positions, float).reshape(-1, 2)
+        w = np.asarray(weights, float).reshape(len(pos), len(wavelengths))
+        ns, nl = w.shape
+        batch = runtime.MftBatch.from_arrays(
+            [fields], np.zeros(ns * nl, np.int64), np.tile(wavelengths, ns), plan.dx,
+            int(n_out), float(p_out), w.reshape(-1), 0,
+            shift_x=np.repeat(pos[:, 0] / p_out, nl), shift_y=np.repeat(pos[:, 1] / p_out, nl),
+            slice_index=np.tile(np.arange(nl), ns), n_outputs=1)
+        return batch()[0], p_out
 
 
 class AngularOpticalSystem(ParametricLayeredOpticalSystem):

@@ -467,6 +467,58 @@ class MftBatch:
                                                                     self.n_outputs))
         self.ws = workspace(self.ws_bytes, self.dev)
 
+    @classmethod
+    def from_arrays(cls, fields, field_index, wavelength, pixel_scale_in, n_out,
+                    pixel_scale_out, weight, out_index, shift_x=0.0, shift_y=0.0,
+                    slice_index=0, n_outputs: int | None = None):
+        """Vectorised construction: `fields` is a list of complex64 device
+        tensors, each (N, N) or a stack (B, N, N); instance i uses slice
+        slice_index[i] of fields[field_index[i]].  Every other argument is a
+        scalar or one value per instance.  The descriptor array is filled from a
+        numpy structured array with the C layout, without per-instance Python."""
+        self = cls.__new__(cls)
+        self.dev = device()
+        self.lib = _lib.load()
+        self._keep = [as_complex(f, self.dev) for f in fields]
+        for f in self._keep:
+            if f.ndim not in (2, 3) or f.shape[-1] != f.shape[-2]:
+                raise ValueError(f"fields must be (N, N) or (B, N, N), got {tuple(f.shape)}")
+        fi = np.asarray(field_index, np.int64).reshape(-1)
+        n = fi.size
+        self.n = n
+        si = np.broadcast_to(np.asarray(slice_index, np.int64), (n,))
+        depth = np.array([f.shape[0] if f.ndim == 3 else 1 for f in self._keep], np.int64)
+        if n and (np.any(fi < 0) or np.any(fi >= len(self._keep)) or np.any(si < 0) or
+                  np.any(si >= depth[fi])):
+            raise ValueError("field_index / slice_index outside the given fields")
+        rec = np.zeros(max(n, 1), dtype=_lib.INSTANCE_DTYPE)[:n]
+        ptr = np.array([f.data_ptr() for f in self._keep], np.uint64)
+        side = np.array([f.shape[-1] for f in self._keep], np.int64)
+        rec["field"] = ptr[fi] + (si * side[fi] * side[fi] * 8).astype(np.uint64)
+        rec["n_in"] = side[fi].astype(np.int32)
+        rec["n_out"] = np.broadcast_to(np.asarray(n_out, np.int32), (n,))
+        rec["out_index"] = np.broadcast_to(np.asarray(out_index, np.int32), (n,))
+        for name, v in (("wavelength", wavelength), ("pixel_scale_in", pixel_scale_in),
+                        ("pixel_scale_out", pixel_scale_out), ("shift_x", shift_x),
+                        ("shift_y", shift_y), ("weight", weight)):
+            rec[name] = np.broadcast_to(np.asarray(v, np.float64), (n,))
+        self._rec = np.ascontiguousarray(rec)
+        self.arr = self._rec.ctypes.data_as(ctypes.POINTER(_lib.MftInstance))
+        if n_outputs is None:
+            n_outputs = int(rec["out_index"].max()) + 1 if n else 0
+        self.n_outputs = int(n_outputs)
+        sizes = {}
+        for o, k in zip(rec["out_index"].tolist(), rec["n_out"].tolist()):
+            sizes.setdefault(o, k)
+        self.outputs = [torch.empty((sizes[o], sizes[o]), dtype=torch.float32, device=self.dev)
+                        if o in sizes else None for o in range(self.n_outputs)]
+        self.ptrs = (ctypes.c_void_p * max(self.n_outputs, 1))(
+            *[(0 if t is None else t.data_ptr()) for t in self.outputs])
+        self.ws_bytes = int(self.lib.dlx_mft_batched_workspace_size(n, self.arr,
+                                                                    self.n_outputs))
+        self.ws = workspace(self.ws_bytes, self.dev)
+        return self
+
     def __call__(self) -> list:
         st = self.lib.dlx_mft_batched_psf(self.n, self.arr, self.n_outputs, self.ptrs,
                                           self.ws.data_ptr(), self.ws.numel(), stream_handle())

@@ -184,20 +184,28 @@ class Server:
         return s
 
     def _launch_psf(self, reqs):
-        keep, instances = [], []
+        keep, fields = [], []
+        cols = {k: [] for k in ("fi", "si", "wl", "dx", "nout", "w", "oi")}
         for j, r in enumerate(reqs):
             mask, basis, scale = self.geometry(r.n_in)
             c = torch.as_tensor(r.coeffs, dtype=torch.float32, device=self.dev)
             opd = runtime.eval_basis(basis, c)
-            fields = runtime.pupil_field(r.n_in, r.wavelengths, D / r.n_in, amp=mask, opd=opd,
-                                         scale=scale)
-            keep += [c, opd, fields]
-            w = 1.0 / len(r.wavelengths)
-            for l, wl in enumerate(r.wavelengths):
-                instances.append(dict(field=fields[l], wavelength=float(wl),
-                                      pixel_scale_in=D / r.n_in, n_out=r.n_out,
-                                      pixel_scale_out=P_OUT, weight=w, out_index=j))
-        batch = runtime.MftBatch(instances, n_outputs=len(reqs))
+            f = runtime.pupil_field(r.n_in, r.wavelengths, D / r.n_in, amp=mask, opd=opd,
+                                    scale=scale)
+            keep += [c, opd]
+            nwl = len(r.wavelengths)
+            cols["fi"].append(np.full(nwl, len(fields)))
+            cols["si"].append(np.arange(nwl))
+            cols["wl"].append(r.wavelengths)
+            cols["dx"].append(np.full(nwl, D / r.n_in))
+            cols["nout"].append(np.full(nwl, r.n_out))
+            cols["w"].append(np.full(nwl, 1.0 / nwl))
+            cols["oi"].append(np.full(nwl, j))
+            fields.append(f)
+        cat = {k: np.concatenate(v) for k, v in cols.items()}
+        batch = runtime.MftBatch.from_arrays(
+            fields, cat["fi"], cat["wl"], cat["dx"], cat["nout"], P_OUT, cat["w"], cat["oi"],
+            slice_index=cat["si"], n_outputs=len(reqs))
         outs = batch()
         keep.append(batch)
 

@@ -144,22 +144,50 @@ def test_batched_leaves_empty_outputs_and_rejects_bad_input(rt):
 
 
 def test_batched_matrices_stay_resident(rt):
-    """A repeated batch finds its transfer matrices in the device cache and gives
-    the same images; with the cache disabled the matrices are generated in the
-    workspace and the images are unchanged."""
+    """A batch seen twice keeps its transfer matrices in the device cache: the
+    third call hits.  Every call (first sighting generating into the
+    workspace, second filling the cache, third reading it, and with the cache
+    disabled) gives the same images."""
     print("[batched] matrix cache", flush=True)
     rng = np.random.default_rng(13)
     inst = _make(rng, [32, 48], 10, 3)
     rt.cache_clear()
     a = [o.clone() for o in rt.mft_batched_psf(inst, n_outputs=3)]
+    b = [o.clone() for o in rt.mft_batched_psf(inst, n_outputs=3)]
     s0 = rt.cache_stats()
-    b = rt.mft_batched_psf(inst, n_outputs=3)
+    c = [o.clone() for o in rt.mft_batched_psf(inst, n_outputs=3)]
     s1 = rt.cache_stats()
     assert s1["hits"] == s0["hits"] + 1 and s1["misses"] == s0["misses"]
     rt.cache_set_budget(0)
     try:
-        c = rt.mft_batched_psf(inst, n_outputs=3)
+        d = rt.mft_batched_psf(inst, n_outputs=3)
     finally:
         rt.cache_set_budget(16 << 30)
-    for x, y, z in zip(a, b, c):
-        assert torch.equal(x, y) and torch.equal(x, z)
+    for x, y, z, u in zip(a, b, c, d):
+        assert torch.equal(x, y) and torch.equal(x, z) and torch.equal(x, u)
+
+
+def test_batched_from_arrays_equals_dict_path(rt):
+    """The vectorised descriptor construction gives the same images as the
+    per-instance dictionaries (stacked fields addressed by slice)."""
+    print("[batched] from_arrays", flush=True)
+    rng = np.random.default_rng(14)
+    n, nl = 40, 5
+    stack = torch.stack([torch.as_tensor(_pupil(n, 200 + l).astype(np.complex64))
+                         for l in range(nl)]).cuda()
+    wls = np.linspace(0.9e-6, 1.1e-6, nl)
+    sx, sy = rng.uniform(-2, 2, 3), rng.uniform(-2, 2, 3)
+    w = rng.uniform(0.1, 1.0, (3, nl))
+    inst = [dict(field=stack[l], wavelength=wls[l], pixel_scale_in=1.0 / n, n_out=24,
+                 pixel_scale_out=2.5e-7, shift=(sx[s], sy[s]), weight=w[s, l], out_index=s)
+            for s in range(3) for l in range(nl)]
+    a = rt.mft_batched_psf(inst, n_outputs=3)
+    b = rt.MftBatch.from_arrays([stack], np.zeros(3 * nl, np.int64), np.tile(wls, 3), 1.0 / n,
+                                24, 2.5e-7, w.reshape(-1), np.repeat(np.arange(3), nl),
+                                shift_x=np.repeat(sx, nl), shift_y=np.repeat(sy, nl),
+                                slice_index=np.tile(np.arange(nl), 3), n_outputs=3)()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+    with pytest.raises(ValueError, match="outside"):
+        rt.MftBatch.from_arrays([stack], [0], 1e-6, 1.0 / n, 24, 2.5e-7, 1.0, 0,
+                                slice_index=nl)
