@@ -51,7 +51,7 @@ def _parse():
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a streaming kernel of known size (intensity of 8 Mi "
                          "complex64: 64 MiB read, 32 MiB written) to scale PMC byte counters")
-    ap.add_argument("--serving-rates", type=float, nargs="+", default=[100.0, 300.0, 1000.0])
+    ap.add_argument("--serving-rates", type=float, nargs="+", default=[100.0, 300.0, 500.0, 700.0, 1000.0])
     ap.add_argument("--serving-seconds", type=float, default=2.0)
     return ap.parse_args()
 
@@ -187,6 +187,34 @@ def _batch_leg(torch, runtime, workloads, steps, warmup):
             "workspace_gib": batch.ws_bytes / 2 ** 30}
 
 
+def _asm_leg(torch, runtime, steps, warmup=3):
+    """Angular-spectrum propagation of a 1024^2 complex64 field padded to 2048^2
+    (the serving mix's ASM request): forward centred FFT with the transfer
+    function fused into its second pass, then the inverse FFT."""
+    n, m = 1024, 2048
+    g = torch.Generator(device="cuda").manual_seed(7)
+    f = torch.polar(torch.ones((n, n), device="cuda"),
+                    torch.rand((n, n), generator=g, device="cuda")).contiguous()
+    for _ in range(warmup):
+        runtime.asm_prop(f, 1e-6, 2e-6, 50e-3, m)
+    torch.cuda.synchronize()
+    beg, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    beg.record()
+    for _ in range(steps):
+        runtime.asm_prop(f, 1e-6, 2e-6, 50e-3, m)
+    end.record()
+    torch.cuda.synchronize()
+    ms = beg.elapsed_time(end) / steps
+    # two 2-D transforms, each two passes that read and write m^2 complex64 values
+    nbytes = 2 * 2 * 2 * 8 * m * m
+    flops = 2 * 5 * m * m * np.log2(m * m)
+    return {"workload": "asm: 1024^2 complex64 padded to 2048^2, z = 50 mm, lambda = 1 um",
+            "ms_per_call": ms, "algorithmic_bytes": nbytes,
+            "achieved_GBps": nbytes / (ms * 1e-3) / 1e9,
+            "frac_of_hbm_peak": nbytes / (ms * 1e-3) / 8e12,
+            "gflops": flops / (ms * 1e-3) / 1e9}
+
+
 def _serving_leg(torch, rates, duration):
     """Poisson request stream (75% PSF, 20% gradient, 5% ASM) at each rate."""
     from dlux_amd import serving
@@ -271,6 +299,8 @@ def main():
     batched = None if args.no_batch else _batch_leg(torch, runtime, workloads,
                                                     max(args.steps // 20, 3), 2)
 
+    print("[bench] angular-spectrum leg", file=sys.stderr, flush=True)
+    asm = _asm_leg(torch, runtime, max(args.steps // 5, 10))
     print("[bench] serving leg", file=sys.stderr, flush=True)
     serve = None if args.no_serving else _serving_leg(torch, args.serving_rates,
                                                       args.serving_seconds)
@@ -325,6 +355,7 @@ def main():
         "speedup_bf16x3_over_fp32": res["fp32"]["ms"] / res["bf16x3"]["ms"],
         "grad": grad,
         "batched": batched,
+        "asm": asm,
         "serving": serve,
     }
     print(json.dumps(line))

@@ -49,13 +49,19 @@ struct SumJob {              // out[p] = sum_{q < count} planes[q * npix + p]
   int count, pad_;
 };
 
+// Job of work item w: a binary search for the wave's first item (wave-uniform,
+// so it runs on scalar loads), then at most a few steps per lane -- a wave's 64
+// consecutive items span one job or the start of the next.
 template <typename Job>
 __device__ __forceinline__ int find_job(const Job* jobs, int njobs, long w) {
+  const long w0 = __builtin_amdgcn_readfirstlane((int)(w >> 32)) * 4294967296L +
+                  (long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(w & 0xffffffff));
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].work_start <= w) lo = mid; else hi = mid - 1;
+    if (jobs[mid].work_start <= w0) lo = mid; else hi = mid - 1;
   }
+  while (lo + 1 < njobs && jobs[lo + 1].work_start <= w) ++lo;
   return lo;
 }
 
