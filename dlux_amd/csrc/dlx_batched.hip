@@ -30,13 +30,13 @@ namespace {
 struct CvtJob {              // complex64 rows x k -> split4
   const float2* in;
   uint8_t* out;
-  long work_start;           // first work item (row, chunk) of this job
+  long row_start;            // first row of this job in the launch's row list
   int rows, k;
 };
 
 struct MatJob {              // split4 rows x cols of exp(sign 2 pi i u_r v_c)
   uint8_t* out;
-  long work_start;
+  long row_start;
   int rows, cols, sign_inv, pad_;
   double c_scale, c_off;     // column coordinate v_c = c * c_scale + c_off
   double r_scale, r_off;     // row coordinate    u_r = r * r_scale + r_off
@@ -49,71 +49,63 @@ struct SumJob {              // out[p] = sum_{q < count} planes[q * npix + p]
   int count, pad_;
 };
 
-// Job of work item w: a binary search for the wave's first item (wave-uniform,
-// so it runs on scalar loads), then at most a few steps per lane -- a wave's 64
-// consecutive items span one job or the start of the next.
+// Workgroup-per-row kernels: a workgroup takes whole rows (grid-stride over the
+// launch's row list), finds the row's job once with a workgroup-uniform binary
+// search on scalar loads, and its lanes cover the row's chunks, a quad of
+// lanes per chunk with quad-cooperative 16-byte stores.  Chunk counts times 4
+// and the workgroup size are multiples of 4, so whole quads take each chunk.
 template <typename Job>
-__device__ __forceinline__ int find_job(const Job* jobs, int njobs, long w) {
-  const long w0 = __builtin_amdgcn_readfirstlane((int)(w >> 32)) * 4294967296L +
-                  (long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(w & 0xffffffff));
+__device__ __forceinline__ int find_row_job(const Job* jobs, int njobs, long row) {
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].work_start <= w0) lo = mid; else hi = mid - 1;
+    if (jobs[mid].row_start <= row) lo = mid; else hi = mid - 1;
   }
-  while (lo + 1 < njobs && jobs[lo + 1].work_start <= w) ++lo;
   return lo;
 }
 
 __global__ void __launch_bounds__(256)
-group_to_split4_kernel(const CvtJob* __restrict__ jobs, int njobs, long total) {
-  // a quad of lanes per chunk (see group_dft_split4_kernel)
-  for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < total;
-       w += (long)gridDim.x * blockDim.x) {
-    const CvtJob& j = jobs[find_job(jobs, njobs, w)];
-    const long local = w - j.work_start;
-    const int q = (int)(local & 3);
-    const long cidx = local >> 2;
-    const long kc = split4_chunks(j.k);
-    const long r = cidx / kc;
-    const int ch = (int)(cidx - r * kc);
+group_to_split4_kernel(const CvtJob* __restrict__ jobs, int njobs, long total_rows) {
+  for (long row = blockIdx.x; row < total_rows; row += gridDim.x) {
+    const CvtJob& j = jobs[find_row_job(jobs, njobs, row)];
+    const long r = row - j.row_start;
+    const int kc = (int)split4_chunks(j.k);
     const float2* src = j.in + r * j.k;
-    const int x = ch * 8 + 2 * q;
-    const float2 v0 = x < j.k ? src[x] : make_float2(0.f, 0.f);
-    const float2 v1 = x + 1 < j.k ? src[x + 1] : make_float2(0.f, 0.f);
-    store_chunk_quad(j.out, r, split4_row_bytes(j.k), ch, q, v0, v1);
+    for (int item = threadIdx.x; item < 4 * kc; item += blockDim.x) {
+      const int q = item & 3, ch = item >> 2;
+      const int x = ch * 8 + 2 * q;
+      const float2 v0 = x < j.k ? src[x] : make_float2(0.f, 0.f);
+      const float2 v1 = x + 1 < j.k ? src[x + 1] : make_float2(0.f, 0.f);
+      store_chunk_quad(j.out, r, split4_row_bytes(j.k), ch, q, v0, v1);
+    }
   }
 }
 
-// Work items are quads of lanes (4 per chunk, quad-cooperative stores); the
-// total and the grid stride are multiples of 4, so whole quads run each item.
 __global__ void __launch_bounds__(256)
-group_dft_split4_kernel(const MatJob* __restrict__ jobs, int njobs, long total) {
-  for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < total;
-       w += (long)gridDim.x * blockDim.x) {
-    const MatJob& j = jobs[find_job(jobs, njobs, w)];
-    const long local = w - j.work_start;
-    const int q = (int)(local & 3);
-    const long cidx = local >> 2;
-    const long kc = split4_chunks(j.cols);
-    const long r = cidx / kc;
-    const int ch = (int)(cidx - r * kc);
+group_dft_split4_kernel(const MatJob* __restrict__ jobs, int njobs, long total_rows) {
+  for (long row = blockIdx.x; row < total_rows; row += gridDim.x) {
+    const MatJob& j = jobs[find_row_job(jobs, njobs, row)];
+    const long r = row - j.row_start;
+    const int kc = (int)split4_chunks(j.cols);
     const double u = (double)r * j.r_scale + j.r_off;
-    float2 v[2];
+    for (int item = threadIdx.x; item < 4 * kc; item += blockDim.x) {
+      const int q = item & 3, ch = item >> 2;
+      float2 v[2];
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int c = ch * 8 + 2 * q + e;
-      if (c < j.cols) {
-        double t = u * ((double)c * j.c_scale + j.c_off);
-        t -= rint(t);
-        // hardware sin/cos take turns; max abs error 1.3e-7 on [-1/2, 1/2]
-        const float sn = __builtin_amdgcn_sinf((float)t), cs = __builtin_amdgcn_cosf((float)t);
-        v[e] = make_float2(cs, j.sign_inv ? sn : -sn);
-      } else {
-        v[e] = make_float2(0.f, 0.f);
+      for (int e = 0; e < 2; ++e) {
+        const int c = ch * 8 + 2 * q + e;
+        if (c < j.cols) {
+          double t = u * ((double)c * j.c_scale + j.c_off);
+          t -= rint(t);
+          // hardware sin/cos take turns; max abs error 1.3e-7 on [-1/2, 1/2]
+          const float sn = __builtin_amdgcn_sinf((float)t), cs = __builtin_amdgcn_cosf((float)t);
+          v[e] = make_float2(cs, j.sign_inv ? sn : -sn);
+        } else {
+          v[e] = make_float2(0.f, 0.f);
+        }
       }
+      store_chunk_quad(j.out, r, split4_row_bytes(j.cols), ch, q, v[0], v[1]);
     }
-    store_chunk_quad(j.out, r, split4_row_bytes(j.cols), ch, q, v[0], v[1]);
   }
 }
 
@@ -187,7 +179,7 @@ struct Plan {
   std::vector<GroupProblem> g1, g2;
   std::vector<SumJob> sum;
   int g1_xcd[9], g2_xcd[9];
-  long cvt_work = 0, mat_work = 0;
+  long cvt_work = 0, mat_work = 0;   // rows of the conversion / matrix launches
   std::vector<int> field_of;       // instance -> distinct-field index
   std::vector<const void*> fields;
   std::vector<int> field_n;
@@ -355,8 +347,8 @@ int plan_batched(int n, const dlx_mft_instance* inst, int n_outputs, float* cons
     j.out = ws + P.P_off[f];
     j.rows = P.field_n[f];
     j.k = P.field_n[f];
-    j.work_start = w;
-    w += (long)j.rows * split4_chunks(j.k) * 4;
+    j.row_start = w;
+    w += j.rows;
     P.cvt.push_back(j);
   }
   P.cvt_work = w;
@@ -376,8 +368,8 @@ int plan_batched(int n, const dlx_mft_instance* inst, int n_outputs, float* cons
       j.sign_inv = inst[i].inverse ? 1 : 0;
       j.c_scale = p.a_scale; j.c_off = p.a_off;
       j.r_scale = p.b_scale; j.r_off = p.b_off;
-      j.work_start = w;
-      w += (long)j.rows * split4_chunks(j.cols) * 4;   // a quad of lanes per chunk
+      j.row_start = w;
+      w += j.rows;
       P.mat.push_back(j);
     }
   }
@@ -491,10 +483,7 @@ int dlx_mft_batched_psf(int32_t n, const dlx_mft_instance* instances, int32_t n_
   put(host, Q.off_sum, Q.sum);
   if (int st = upload(host.data(), host.size(), ws, s)) return st;
 
-  auto blocks = [](long work) {
-    const long b = (work + 255) / 256;
-    return (unsigned)(b < 8192 ? b : 8192);
-  };
+  auto blocks = [](long rows) { return (unsigned)(rows < 16384 ? rows : 16384); };
   hipLaunchKernelGGL(group_to_split4_kernel, dim3(blocks(Q.cvt_work)), dim3(256), 0, s,
                      (const CvtJob*)(ws + Q.off_cvt), (int)Q.cvt.size(), Q.cvt_work);
   if (int st = launch_status("group_to_split4_kernel")) return st;

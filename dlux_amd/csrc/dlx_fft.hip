@@ -19,11 +19,13 @@
 namespace dlx {
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kLdsElems = 8192;   // R * M complex64 per workgroup (64 KiB)
+// R * M complex64 per workgroup (128 KiB + twiddles): R = 8 rows of 2048, so the
+// transposed store writes 64-byte runs; 8 waves per workgroup.
+constexpr int kThreads = 512;
+constexpr int kLdsElems = 16384;
 
 struct FftParams {
-  int M, log2M, R, inverse;
+  int M, log2M, R, log2R, inverse;
   int in_mode;      // 0: centred + zero-padded from an N x N source; 1: plain M x M rows
   int N, p;         // source side and pad offset (in_mode 0)
   long in_bstride, out_bstride;
@@ -48,10 +50,11 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
     sincospif(2.0f * (float)q / (float)M, &s, &c);
     tw[q] = make_float2(c, sgn * s);
   }
-  // load with bit-reversed placement
+  // load with bit-reversed placement (M and R are powers of two: shifts and masks)
   const int total = R * M;
+  const int log2M = P.log2M, log2H = P.log2M - 1;
   for (int t = threadIdx.x; t < total; t += kThreads) {
-    const int rr = t / M, nc = t - rr * M;
+    const int rr = t >> log2M, nc = t & (M - 1);
     const int row = r0 + rr;
     float2 v = make_float2(0.f, 0.f);
     if (P.in_mode == 0) {
@@ -68,13 +71,13 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
 
   // in-place radix-2 decimation-in-time stages
   const int nbfly = R * half_m;
-  for (int len = 2, tws = half_m; len <= M; len <<= 1, tws >>= 1) {
-    const int h = len >> 1;
+  for (int lh = 0, ltw = log2H; lh < log2M; ++lh, --ltw) {
+    const int h = 1 << lh;   // half the butterfly span at this stage
     for (int t = threadIdx.x; t < nbfly; t += kThreads) {
-      const int rr = t / half_m, j = t - rr * half_m;
-      const int grp = j / h, pos = j - grp * h;
-      const int i0 = rr * M + grp * len + pos, i1 = i0 + h;
-      const float2 w = tw[pos * tws];
+      const int rr = t >> log2H, j = t & (half_m - 1);
+      const int grp = j >> lh, pos = j & (h - 1);
+      const int i0 = (rr << log2M) + (grp << (lh + 1)) + pos, i1 = i0 + h;
+      const float2 w = tw[pos << ltw];
       const float2 a = lds[i0], x = lds[i1];
       const float2 bw = make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
       lds[i0] = make_float2(a.x + bw.x, a.y + bw.y);
@@ -86,7 +89,7 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
   // transposed store: out[k][row] (consecutive lanes -> consecutive rows)
   float2* dst = out + (long)b * P.out_bstride;
   for (int t = threadIdx.x; t < total; t += kThreads) {
-    const int k = t / R, rr = t - k * R;
+    const int k = t >> P.log2R, rr = t & (R - 1);
     const int row = r0 + rr;
     float2 v = lds[rr * M + k];
     if (P.out_mode == 0) {
@@ -124,7 +127,9 @@ hipError_t launch_rows(const float2* in, float2* out, int batch, FftParams P, hi
   P.R = kLdsElems / P.M;
   if (P.R < 1) P.R = 1;
   if (P.R > P.M) P.R = P.M;
+  P.log2R = ilog2(P.R);
   const size_t lds = ((size_t)P.R * P.M + P.M / 2) * sizeof(float2);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
   if (lds > 64 * 1024) {
     // Opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
     (void)hipFuncSetAttribute((const void*)fft_rows_kernel,
