@@ -6,8 +6,10 @@
 //
 // A 2-D transform is two launches of one kernel.  Each workgroup loads R
 // rows of length M (power of two) into LDS -- applying the zero pad and
-// ifftshift as an index map on the fly -- runs an in-place radix-2 DIT FFT on
-// every row, and writes the rows out TRANSPOSED.  The second launch therefore
+// ifftshift as an index map on the fly -- runs a Stockham auto-sort FFT on
+// every row (radix-8 stages, then one radix-4 or radix-2 stage; each stage
+// reads its butterfly inputs into registers, synchronises, and writes the
+// outputs in place), and writes the rows out TRANSPOSED.  The second launch therefore
 // reads the first's columns as rows and its transposed write restores the
 // orientation; that write also applies fftshift, the 1/M scale and, for the
 // angular-spectrum method, the transfer function H(fx, fy).  HBM traffic is
@@ -19,10 +21,105 @@
 namespace dlx {
 namespace {
 
-// R * M complex64 per workgroup (128 KiB + twiddles): R = 8 rows of 2048, so the
-// transposed store writes 64-byte runs; 8 waves per workgroup.
+// At most kLdsElems complex64 of rows per workgroup plus an M-entry twiddle
+// table, within the 160 KiB of LDS; 8 waves per workgroup.
 constexpr int kThreads = 512;
 constexpr int kLdsElems = 16384;
+constexpr int kLdsBudget = 20480;   // complex64 values in 160 KiB
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// (a) * (sgn i)
+__device__ __forceinline__ float2 mul_si(float2 a, float sgn) {
+  return make_float2(-sgn * a.y, sgn * a.x);
+}
+
+// In-register DFT of size RAD: y_q = sum_r x_r exp(sgn 2 pi i r q / RAD).
+template <int RAD>
+__device__ __forceinline__ void dft_reg(float2 (&x)[RAD], float sgn);
+
+template <>
+__device__ __forceinline__ void dft_reg<2>(float2 (&x)[2], float) {
+  const float2 a = x[0], b = x[1];
+  x[0] = make_float2(a.x + b.x, a.y + b.y);
+  x[1] = make_float2(a.x - b.x, a.y - b.y);
+}
+
+template <>
+__device__ __forceinline__ void dft_reg<4>(float2 (&x)[4], float sgn) {
+  const float2 a0 = make_float2(x[0].x + x[2].x, x[0].y + x[2].y);
+  const float2 a1 = make_float2(x[0].x - x[2].x, x[0].y - x[2].y);
+  const float2 a2 = make_float2(x[1].x + x[3].x, x[1].y + x[3].y);
+  const float2 a3 = mul_si(make_float2(x[1].x - x[3].x, x[1].y - x[3].y), sgn);
+  x[0] = make_float2(a0.x + a2.x, a0.y + a2.y);
+  x[2] = make_float2(a0.x - a2.x, a0.y - a2.y);
+  x[1] = make_float2(a1.x + a3.x, a1.y + a3.y);
+  x[3] = make_float2(a1.x - a3.x, a1.y - a3.y);
+}
+
+template <>
+__device__ __forceinline__ void dft_reg<8>(float2 (&x)[8], float sgn) {
+  float2 e[4] = {x[0], x[2], x[4], x[6]};
+  float2 o[4] = {x[1], x[3], x[5], x[7]};
+  dft_reg<4>(e, sgn);
+  dft_reg<4>(o, sgn);
+  const float h = 0.70710678118654752f;
+  // o_q * exp(sgn 2 pi i q / 8)
+  const float2 o1 = make_float2(h * (o[1].x - sgn * o[1].y), h * (o[1].y + sgn * o[1].x));
+  const float2 o2 = mul_si(o[2], sgn);
+  const float2 o3 = make_float2(h * (-o[3].x - sgn * o[3].y), h * (-o[3].y + sgn * o[3].x));
+  const float2 ot[4] = {o[0], o1, o2, o3};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    x[q] = make_float2(e[q].x + ot[q].x, e[q].y + ot[q].y);
+    x[q + 4] = make_float2(e[q].x - ot[q].x, e[q].y - ot[q].y);
+  }
+}
+
+// One Stockham stage of radix RAD on `rows` rows of length M held in LDS:
+// item j (of M/RAD per row) reads x_r = row[j + r M/RAD], twiddles them by
+// exp(sgn 2 pi i r k / (p RAD)) with k = j mod p, transforms, and writes
+// y_q to row[(j - k) RAD + k + q p].  p is the product of earlier radices.
+template <int RAD, int LOG2RAD>
+__device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, int total,
+                                               int log2M, int log2p, float sgn) {
+  constexpr int kItems = kLdsElems / RAD / kThreads;
+  const int log2_per_row = log2M - LOG2RAD;
+  const int per_row = 1 << log2_per_row;
+  const int p = 1 << log2p;
+  float2 v[kItems][RAD];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int t = threadIdx.x + it * kThreads;
+    if (t < total) {
+      const int rr = t >> log2_per_row, j = t & (per_row - 1);
+      const float2* row = buf + (rr << log2M);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) v[it][r] = row[j + r * per_row];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int t = threadIdx.x + it * kThreads;
+    if (t < total) {
+      const int rr = t >> log2_per_row, j = t & (per_row - 1);
+      const int k = j & (p - 1);
+      if (p > 1) {
+        const int sh = log2M - log2p - LOG2RAD;   // twiddle index r k M / (p RAD)
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) v[it][r] = cmul(v[it][r], tw[(r * k) << sh]);
+      }
+      dft_reg<RAD>(v[it], sgn);
+      float2* row = buf + (rr << log2M);
+      const int i0 = ((j - k) << LOG2RAD) + k;
+#pragma unroll
+      for (int q = 0; q < RAD; ++q) row[i0 + (q << log2p)] = v[it][q];
+    }
+  }
+  __syncthreads();
+}
 
 struct FftParams {
   int M, log2M, R, log2R, inverse;
@@ -39,20 +136,20 @@ __global__ void __launch_bounds__(kThreads)
 fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftParams P) {
   extern __shared__ float2 lds[];
   const int M = P.M, R = P.R, half_m = M >> 1;
-  float2* tw = lds + (long)R * M;  // M/2 twiddles exp(sign 2 pi i q / M)
+  float2* tw = lds + (long)R * M;  // M twiddles exp(sign 2 pi i q / M)
   const int b = blockIdx.y;
   const int r0 = blockIdx.x * R;
   const float sgn = P.inverse ? 1.f : -1.f;
   const float2* src = in + (long)b * P.in_bstride;
 
-  for (int q = threadIdx.x; q < half_m; q += kThreads) {
+  for (int q = threadIdx.x; q < M; q += kThreads) {
     float s, c;
     sincospif(2.0f * (float)q / (float)M, &s, &c);
     tw[q] = make_float2(c, sgn * s);
   }
-  // load with bit-reversed placement (M and R are powers of two: shifts and masks)
+  // load in natural order (M and R are powers of two: shifts and masks)
   const int total = R * M;
-  const int log2M = P.log2M, log2H = P.log2M - 1;
+  const int log2M = P.log2M;
   for (int t = threadIdx.x; t < total; t += kThreads) {
     const int rr = t >> log2M, nc = t & (M - 1);
     const int row = r0 + rr;
@@ -64,27 +161,18 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
     } else {
       v = src[(long)row * M + nc];
     }
-    const int rev = (int)(__brev((unsigned)nc) >> (32 - P.log2M));
-    lds[rr * M + rev] = v;
+    lds[rr * M + nc] = v;
   }
   __syncthreads();
 
-  // in-place radix-2 decimation-in-time stages
-  const int nbfly = R * half_m;
-  for (int lh = 0, ltw = log2H; lh < log2M; ++lh, --ltw) {
-    const int h = 1 << lh;   // half the butterfly span at this stage
-    for (int t = threadIdx.x; t < nbfly; t += kThreads) {
-      const int rr = t >> log2H, j = t & (half_m - 1);
-      const int grp = j >> lh, pos = j & (h - 1);
-      const int i0 = (rr << log2M) + (grp << (lh + 1)) + pos, i1 = i0 + h;
-      const float2 w = tw[pos << ltw];
-      const float2 a = lds[i0], x = lds[i1];
-      const float2 bw = make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
-      lds[i0] = make_float2(a.x + bw.x, a.y + bw.y);
-      lds[i1] = make_float2(a.x - bw.x, a.y - bw.y);
-    }
-    __syncthreads();
+  // Stockham stages: radix 8 while it divides, then one radix-4 or radix-2 stage
+  int log2p = 0;
+  while (log2M - log2p >= 3) {
+    stockham_stage<8, 3>(lds, tw, total >> 3, log2M, log2p, sgn);
+    log2p += 3;
   }
+  if (log2M - log2p == 2) stockham_stage<4, 2>(lds, tw, total >> 2, log2M, log2p, sgn);
+  else if (log2M - log2p == 1) stockham_stage<2, 1>(lds, tw, total >> 1, log2M, log2p, sgn);
 
   // transposed store: out[k][row] (consecutive lanes -> consecutive rows)
   float2* dst = out + (long)b * P.out_bstride;
@@ -124,11 +212,22 @@ int ilog2(int m) {
 
 hipError_t launch_rows(const float2* in, float2* out, int batch, FftParams P, hipStream_t s) {
   P.log2M = ilog2(P.M);
-  P.R = kLdsElems / P.M;
-  if (P.R < 1) P.R = 1;
+  // rows per workgroup: about two workgroups per CU (512) when the rows fit,
+  // within kLdsElems and the LDS left beside the M-entry twiddle table
+  int cap = (kLdsBudget - P.M) / P.M;
+  if (cap > kLdsElems / P.M) cap = kLdsElems / P.M;
+  if (cap < 1) cap = 1;
+  // (at least 4 rows -- 32-byte runs in the transposed store -- while that
+  // still leaves one workgroup per CU)
+  int r = P.M / 512;
+  const int r_min = P.M / 256 < 4 ? P.M / 256 : 4;
+  if (r < r_min) r = r_min;
+  if (r < 1) r = 1;
+  if (r > cap) r = cap;
+  P.R = 1 << (ilog2(r + 1) - 1);   // largest power of two <= r
   if (P.R > P.M) P.R = P.M;
   P.log2R = ilog2(P.R);
-  const size_t lds = ((size_t)P.R * P.M + P.M / 2) * sizeof(float2);
+  const size_t lds = ((size_t)P.R * P.M + P.M) * sizeof(float2);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   if (lds > 64 * 1024) {
     // Opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
