@@ -176,9 +176,9 @@ struct Plan {
   size_t off_cvt, off_mat, off_g1, off_g2, off_sum, off_P, off_W, off_U, off_I, total;
   std::vector<CvtJob> cvt;
   std::vector<MatJob> mat;
-  std::vector<GroupProblem> g1, g2;
+  std::vector<GroupProblem> g1, g2, g1n, g2n;   // n: problems with N <= 64 (128 x 64 tiles)
   std::vector<SumJob> sum;
-  int g1_xcd[9], g2_xcd[9];
+  int g1_xcd[9], g2_xcd[9], g1n_xcd[9], g2n_xcd[9];
   long cvt_work = 0, mat_work = 0;   // rows of the conversion / matrix launches
   std::vector<int> field_of;       // instance -> distinct-field index
   std::vector<const void*> fields;
@@ -273,10 +273,18 @@ GroupProblem make_problem(const uint8_t* A, long arb, const uint8_t* B, long brb
   p.B = B; p.b_row_bytes = brb;
   p.C = C; p.ldc = ldc;
   p.M = M; p.N = N; p.K = K;
+  const int tile_n = N <= 64 ? 64 : 128;
   p.tiles_m = (M + 127) / 128;
-  p.tiles_n = (N + 127) / 128;
+  p.tiles_n = (N + tile_n - 1) / tile_n;
   p.alpha = alpha;
   return p;
+}
+
+// Problems with N <= 64 go to a launch with 128 x 64 tiles (no half-empty tiles).
+void split_narrow(std::vector<GroupProblem>& all, std::vector<GroupProblem>& narrow) {
+  std::vector<GroupProblem> wide;
+  for (const GroupProblem& p : all) (p.N <= 64 ? narrow : wide).push_back(p);
+  all.swap(wide);
 }
 
 // Sizes and offsets; with ws != nullptr also the device pointers in the jobs.
@@ -309,6 +317,7 @@ int plan_batched(int n, const dlx_mft_instance* inst, int n_outputs, float* cons
   const size_t nf = P.fields.size();
   P.off_cvt = take(nf * sizeof(CvtJob));
   P.off_mat = take(2 * (size_t)n * sizeof(MatJob));
+  // each stage's problems: wide list then narrow list (n in total)
   P.off_g1 = take((size_t)n * sizeof(GroupProblem));
   P.off_g2 = take((size_t)n * sizeof(GroupProblem));
   P.off_sum = take((size_t)n_outputs * sizeof(SumJob));
@@ -386,8 +395,12 @@ int plan_batched(int n, const dlx_mft_instance* inst, int n_outputs, float* cons
                                 ws + P.I_off[i], no, no, no, ni,
                                 (float)(c * std::sqrt(inst[i].weight))));
   }
+  split_narrow(P.g1, P.g1n);
+  split_narrow(P.g2, P.g2n);
   order_problems(P.g1, P.g1_xcd);
   order_problems(P.g2, P.g2_xcd);
+  order_problems(P.g1n, P.g1n_xcd);
+  order_problems(P.g2n, P.g2n_xcd);
   // sums
   for (int o = 0; o < n_outputs; ++o) {
     if (members[o].empty()) continue;
@@ -473,13 +486,17 @@ int dlx_mft_batched_psf(int32_t n, const dlx_mft_instance* instances, int32_t n_
     for (MatJob& j : Q.mat) j.out = rebase(j.out);
     for (GroupProblem& g : Q.g1) g.B = rebase(g.B);
     for (GroupProblem& g : Q.g2) g.A = rebase(g.A);
+    for (GroupProblem& g : Q.g1n) g.B = rebase(g.B);
+    for (GroupProblem& g : Q.g2n) g.A = rebase(g.A);
   }
   // all descriptors sit at the front of the workspace: one upload
   std::vector<uint8_t> host(Q.off_P, 0);
   put(host, Q.off_cvt, Q.cvt);
   put(host, Q.off_mat, Q.mat);
   put(host, Q.off_g1, Q.g1);
+  put(host, Q.off_g1 + Q.g1.size() * sizeof(GroupProblem), Q.g1n);
   put(host, Q.off_g2, Q.g2);
+  put(host, Q.off_g2 + Q.g2.size() * sizeof(GroupProblem), Q.g2n);
   put(host, Q.off_sum, Q.sum);
   if (int st = upload(host.data(), host.size(), ws, s)) return st;
 
@@ -494,16 +511,22 @@ int dlx_mft_batched_psf(int32_t n, const dlx_mft_instance* instances, int32_t n_
     if (int st = cache_ready(whandle, s)) return st;
   }
 
-  GroupLaunch l1{};
-  l1.probs = (const GroupProblem*)(ws + Q.off_g1);
-  l1.nprobs = (int)Q.g1.size();
-  std::memcpy(l1.xcd_start, Q.g1_xcd, sizeof(l1.xcd_start));
-  DLX_HIP(launch_cgemm_split4_group(l1, EPI_SPLIT_T, s));
-  GroupLaunch l2{};
-  l2.probs = (const GroupProblem*)(ws + Q.off_g2);
-  l2.nprobs = (int)Q.g2.size();
-  std::memcpy(l2.xcd_start, Q.g2_xcd, sizeof(l2.xcd_start));
-  DLX_HIP(launch_cgemm_split4_group(l2, EPI_INTENSITY, s));
+  auto grouped = [&](size_t off, const std::vector<GroupProblem>& probs, const int (&xcd)[9],
+                     int tile_n, int epi) -> int {
+    if (probs.empty()) return DLX_OK;
+    GroupLaunch l{};
+    l.probs = (const GroupProblem*)(ws + off);
+    l.nprobs = (int)probs.size();
+    l.tile_n = tile_n;
+    std::memcpy(l.xcd_start, xcd, sizeof(l.xcd_start));
+    DLX_HIP(launch_cgemm_split4_group(l, epi, s));
+    return DLX_OK;
+  };
+  const size_t n1 = Q.g1.size() * sizeof(GroupProblem), n2 = Q.g2.size() * sizeof(GroupProblem);
+  if (int st = grouped(Q.off_g1, Q.g1, Q.g1_xcd, 128, EPI_SPLIT_T)) return st;
+  if (int st = grouped(Q.off_g1 + n1, Q.g1n, Q.g1n_xcd, 64, EPI_SPLIT_T)) return st;
+  if (int st = grouped(Q.off_g2, Q.g2, Q.g2_xcd, 128, EPI_INTENSITY)) return st;
+  if (int st = grouped(Q.off_g2 + n2, Q.g2n, Q.g2n_xcd, 64, EPI_INTENSITY)) return st;
 
   long most = 0;
   for (const SumJob& j : Q.sum) most = j.npix > most ? j.npix : most;

@@ -291,6 +291,7 @@ extern "C" int dlx_fft2(int32_t n, int32_t m, int32_t batch, int32_t inverse, co
                         void* out, void* workspace, size_t workspace_bytes, void* stream) {
   clear_error();
   DLX_CHECK_ARG(n > 0 && m >= n && batch >= 0, "need 0 < n <= m and batch >= 0");
+  DLX_CHECK_ARG(batch <= 65535, "batch must be <= 65535, got %d", batch);
   DLX_CHECK_ARG(((m - n) % 2) == 0, "m - n must be even (centre-preserving pad)");
   if (batch == 0) return DLX_OK;
   DLX_CHECK_ARG(in && out, "null argument to dlx_fft2");
@@ -319,6 +320,7 @@ extern "C" int dlx_asm_prop(int32_t n, int32_t m, int32_t batch, const double* w
                             void* workspace, size_t workspace_bytes, void* stream) {
   clear_error();
   DLX_CHECK_ARG(n > 0 && m >= n && batch >= 0, "need 0 < n <= m and batch >= 0");
+  DLX_CHECK_ARG(batch <= 65535, "batch must be <= 65535, got %d", batch);
   DLX_CHECK_ARG(((m - n) % 2) == 0, "m - n must be even (centre-preserving pad)");
   if (batch == 0) return DLX_OK;
   DLX_CHECK_ARG(in && out && wavelength, "null argument to dlx_asm_prop");

@@ -732,10 +732,10 @@ cgemm_split4_wide_kernel(GemmSplit g) {
 // stride over its range.  Problems are ordered by the host (largest first
 // within each range), and every workgroup leaves its loop after its range.
 // ---------------------------------------------------------------------------
-template <int EPI>
+template <int EPI, int TN>
 __global__ void __launch_bounds__(kThreads, 2)
 cgemm_split4_group_kernel(GroupLaunch gl) {
-  constexpr int TM = 2, TN = 2, BM = 128, BN = 128;
+  constexpr int TM = 2, BM = 128, BN = 64 * TN;
   constexpr int kBuf = (BM + BN) * kRowDeep;
   __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kBuf];
   const int x = blockIdx.x % kXcds;
@@ -811,6 +811,18 @@ hipError_t dispatch_split(const GemmSplit& g, hipStream_t s, int mode) {
 
 }  // namespace
 
+template <int TN>
+hipError_t launch_group_tn(const GroupLaunch& gl, int epilogue, const dim3& grid, hipStream_t s) {
+  const dim3 block(kThreads);
+  switch (epilogue) {
+    case EPI_SPLIT_T: hipLaunchKernelGGL((cgemm_split4_group_kernel<EPI_SPLIT_T, TN>), grid, block, 0, s, gl); break;
+    case EPI_INTENSITY: hipLaunchKernelGGL((cgemm_split4_group_kernel<EPI_INTENSITY, TN>), grid, block, 0, s, gl); break;
+    case EPI_STORE: hipLaunchKernelGGL((cgemm_split4_group_kernel<EPI_STORE, TN>), grid, block, 0, s, gl); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_cgemm_split4_group(const GroupLaunch& gl, int epilogue, hipStream_t s) {
   const int total = gl.xcd_start[8];
   if (total <= 0 || gl.nprobs <= 0) return hipSuccess;
@@ -821,14 +833,9 @@ hipError_t launch_cgemm_split4_group(const GroupLaunch& gl, int epilogue, hipStr
     most = n > most ? n : most;
   }
   const int per = most < 64 ? most : 64;
-  const dim3 grid(8 * per), block(kThreads);
-  switch (epilogue) {
-    case EPI_SPLIT_T: hipLaunchKernelGGL(cgemm_split4_group_kernel<EPI_SPLIT_T>, grid, block, 0, s, gl); break;
-    case EPI_INTENSITY: hipLaunchKernelGGL(cgemm_split4_group_kernel<EPI_INTENSITY>, grid, block, 0, s, gl); break;
-    case EPI_STORE: hipLaunchKernelGGL(cgemm_split4_group_kernel<EPI_STORE>, grid, block, 0, s, gl); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  const dim3 grid(8 * per);
+  return gl.tile_n == 64 ? launch_group_tn<1>(gl, epilogue, grid, s)
+                         : launch_group_tn<2>(gl, epilogue, grid, s);
 }
 
 template <int EPI>

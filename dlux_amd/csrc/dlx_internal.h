@@ -129,7 +129,7 @@ struct GemmSplit {
 hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s);
 
 // One problem of a grouped (uneven) split4 GEMM, in device memory.  Its
-// 128 x 128 tiles are [tile_start, tile_start + tiles_m * tiles_n) of the
+// 128 x tile_n tiles are [tile_start, tile_start + tiles_m * tiles_n) of the
 // launch's tile list, row-major over (tm, tn).
 struct GroupProblem {
   const uint8_t* A; const uint8_t* B; void* C;
@@ -140,6 +140,7 @@ struct GroupProblem {
 struct GroupLaunch {
   const GroupProblem* probs;   // device array, sorted by tile_start
   int nprobs;
+  int tile_n;                  // 128, or 64 for launches of problems with N <= 64
   int xcd_start[9];            // tile range of XCD x: [xcd_start[x], xcd_start[x+1])
 };
 // Epilogues supported: EPI_STORE, EPI_SPLIT_T, EPI_INTENSITY.
