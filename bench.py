@@ -220,7 +220,7 @@ def _serving_leg(torch, rates, duration):
     from dlux_amd import serving
     server = serving.Server(hbm_budget=64 << 30, max_inflight=256, n_streams=4)
     # warm the resident geometry and the kernels once
-    server.run(serving.make_requests(rate=0, duration=0, seed=99, n_max=12), realtime=False)
+    server.run(serving.make_requests(rate=0, duration=0, seed=99, n_max=64), realtime=False)
     torch.cuda.synchronize()
     sweep = []
     for rate in rates:

@@ -41,7 +41,11 @@ struct MatCache {
   // sighting, so one-off batches (a serving loop's) do not allocate or evict
   std::vector<std::pair<uint64_t, uint64_t>> seen;
   size_t budget = (size_t)16 << 30;  // 16 GiB by default (DLUX_AMD_CACHE_MB)
-  size_t used = 0;
+  size_t used = 0;                    // bytes of live entries
+  // blocks of evicted entries, kept for reuse: hipFree synchronises the whole
+  // device, which would stall every stream of a serving loop on each eviction
+  std::vector<std::pair<void*, size_t>> spare;
+  size_t spare_bytes = 0;
   uint64_t tick = 0, hits = 0, misses = 0;
   bool init = false;
 };
@@ -55,16 +59,30 @@ MatCache& cache() {
   return c;
 }
 
-void free_entry(MatEntry* e) {
+// Retire an entry: wait for its last use (only that entry's work, not the
+// device), then keep its block as a spare (or free it when `release`).
+void free_entry(MatCache& c, MatEntry* e, bool release) {
   if (e->used) (void)hipEventSynchronize(e->last_use);
   (void)hipEventSynchronize(e->ready);
-  (void)hipFree(e->ptr);
+  if (release) {
+    (void)hipFree(e->ptr);
+  } else {
+    c.spare.push_back({e->ptr, e->bytes});
+    c.spare_bytes += e->bytes;
+  }
   (void)hipEventDestroy(e->ready);
   (void)hipEventDestroy(e->last_use);
   delete e;
 }
 
-// Evict least-recently-used entries until `need` more bytes fit the budget.
+void release_spares(MatCache& c) {
+  for (auto& b : c.spare) (void)hipFree(b.first);
+  c.spare.clear();
+  c.spare_bytes = 0;
+}
+
+// Evict least-recently-used entries until `need` more bytes fit the budget
+// (live entries plus spare blocks).
 void make_room(MatCache& c, size_t need) {
   while (!c.entries.empty() && c.used + need > c.budget) {
     size_t lru = 0;
@@ -73,8 +91,38 @@ void make_room(MatCache& c, size_t need) {
     MatEntry* e = c.entries[lru];
     c.used -= e->bytes;
     c.entries.erase(c.entries.begin() + lru);
-    free_entry(e);
+    free_entry(c, e, false);
   }
+}
+
+// A block of at least `need` bytes: the smallest spare that is no more than
+// twice as large, else a new allocation (spares are released first when the
+// budget would be exceeded).  Returns its size in *bytes; NULL on failure.
+void* take_block(MatCache& c, size_t need, size_t* bytes) {
+  size_t best = c.spare.size();
+  for (size_t i = 0; i < c.spare.size(); ++i)
+    if (c.spare[i].second >= need && c.spare[i].second <= 2 * need &&
+        (best == c.spare.size() || c.spare[i].second < c.spare[best].second))
+      best = i;
+  if (best < c.spare.size()) {
+    void* p = c.spare[best].first;
+    *bytes = c.spare[best].second;
+    c.spare_bytes -= *bytes;
+    c.spare.erase(c.spare.begin() + best);
+    return p;
+  }
+  if (c.used + c.spare_bytes + need > c.budget) release_spares(c);
+  void* mem = nullptr;
+  if (hipMalloc(&mem, need) != hipSuccess) {
+    (void)hipGetLastError();
+    release_spares(c);
+    if (hipMalloc(&mem, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  *bytes = need;
+  return mem;
 }
 
 }  // namespace
@@ -120,14 +168,12 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
   ++c.misses;
   const size_t per = adjoint ? mat_bytes(g.n_in, g.n_out, split)
                              : mat_bytes(g.n_out, g.n_in, split);
-  const size_t bytes = align_up(per * (size_t)nb);
-  if (bytes > c.budget) return DLX_OK;  // larger than the whole cache: caller generates
-  make_room(c, bytes);
-  void* mem = nullptr;
-  if (hipMalloc(&mem, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    return DLX_OK;  // no memory for the cache: caller generates into its workspace
-  }
+  const size_t need = align_up(per * (size_t)nb);
+  if (need > c.budget) return DLX_OK;  // larger than the whole cache: caller generates
+  make_room(c, need);
+  size_t bytes = 0;
+  void* mem = take_block(c, need, &bytes);
+  if (!mem) return DLX_OK;  // no memory for the cache: caller generates into its workspace
   MatEntry* e = new MatEntry();
   e->key = std::move(key);
   e->ptr = mem;
@@ -141,7 +187,7 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
     return DLX_ERR_HIP;
   }
   if (int st = gen_dft(g, nb, wl, dx, pout, shift, nullptr, mem, split, s, adjoint)) {
-    free_entry(e);
+    free_entry(c, e, true);
     return st;
   }
   DLX_HIP(hipEventRecord(e->ready, s));
@@ -214,15 +260,13 @@ int cache_acquire_raw(const std::vector<double>& key_in, size_t bytes, hipStream
     return DLX_OK;   // first sighting: the caller generates into its workspace
   }
   make_room(c, need);
-  void* mem = nullptr;
-  if (hipMalloc(&mem, need) != hipSuccess) {
-    (void)hipGetLastError();
-    return DLX_OK;
-  }
+  size_t got = 0;
+  void* mem = take_block(c, need, &got);
+  if (!mem) return DLX_OK;
   MatEntry* e = new MatEntry();
   e->key = std::move(key);
   e->ptr = mem;
-  e->bytes = need;
+  e->bytes = got;
   e->tick = ++c.tick;
   e->pending = true;
   if (hipEventCreateWithFlags(&e->ready, hipEventDisableTiming) != hipSuccess ||
@@ -233,7 +277,7 @@ int cache_acquire_raw(const std::vector<double>& key_in, size_t bytes, hipStream
     return DLX_ERR_HIP;
   }
   c.entries.push_back(e);
-  c.used += need;
+  c.used += got;
   *ptr = mem;
   *handle = e;
   *fresh = true;
@@ -268,9 +312,10 @@ extern "C" int dlx_cache_clear(void) {
   clear_error();
   MatCache& c = cache();
   std::lock_guard<std::mutex> lock(c.mu);
-  for (MatEntry* e : c.entries) free_entry(e);
+  for (MatEntry* e : c.entries) free_entry(c, e, true);
   c.entries.clear();
   c.used = 0;
+  release_spares(c);
   return DLX_OK;
 }
 
@@ -280,6 +325,7 @@ extern "C" int dlx_cache_set_budget(size_t bytes) {
   std::lock_guard<std::mutex> lock(c.mu);
   c.budget = bytes;
   make_room(c, 0);
+  if (c.used + c.spare_bytes > c.budget) release_spares(c);
   return DLX_OK;
 }
 

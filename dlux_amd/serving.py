@@ -178,6 +178,14 @@ class Server:
             12 * r.n_out ** 2
 
     # ---- launches ---------------------------------------------------------
+    def _upload(self, x, keep):
+        """Asynchronous host-to-device copy on the current stream through pinned
+        memory (a pageable copy would make the host wait for the stream's
+        earlier work); the pinned staging tensor is kept until the group ends."""
+        h = torch.from_numpy(np.ascontiguousarray(x, np.float32)).pin_memory()
+        keep.append(h)
+        return h.to(self.dev, non_blocking=True)
+
     def _stream(self):
         s = self.streams[self._next_stream]
         self._next_stream = (self._next_stream + 1) % len(self.streams)
@@ -188,7 +196,7 @@ class Server:
         cols = {k: [] for k in ("fi", "si", "wl", "dx", "nout", "w", "oi")}
         for j, r in enumerate(reqs):
             mask, basis, scale = self.geometry(r.n_in)
-            c = torch.as_tensor(r.coeffs, dtype=torch.float32, device=self.dev)
+            c = self._upload(r.coeffs, keep)
             opd = runtime.eval_basis(basis, c)
             f = runtime.pupil_field(r.n_in, r.wavelengths, D / r.n_in, amp=mask, opd=opd,
                                     scale=scale)
@@ -216,12 +224,13 @@ class Server:
 
     def _launch_grad(self, r: Request):
         mask, basis, scale = self.geometry(r.n_in)
-        c = torch.as_tensor(r.coeffs, dtype=torch.float32, device=self.dev)
+        keep = []
+        c = self._upload(r.coeffs, keep)
         opd = runtime.eval_basis(basis, c)
         w = np.full(len(r.wavelengths), 1.0 / len(r.wavelengths))
         kw = dict(amp=mask, opd=opd, scale=scale, precision=self.precision)
         psf = runtime.psf_poly(r.n_in, r.n_out, r.wavelengths, w, D / r.n_in, P_OUT, **kw)
-        d = torch.as_tensor(r.data, dtype=torch.float32).to(self.dev, non_blocking=False)
+        d = self._upload(r.data, keep)
         var = torch.clamp(d, min=1.0)
         resid = r.flux * psf - d
         loss = 0.5 * torch.sum(resid.double() ** 2 / var)
@@ -229,7 +238,7 @@ class Server:
         g_opd, _ = runtime.psf_poly_vjp(r.n_in, r.n_out, r.wavelengths, w, D / r.n_in, P_OUT,
                                         G, **kw)
         g_c = runtime.project_basis(basis, g_opd)
-        keep = [c, opd, psf, d, var, resid, G, g_opd]
+        keep += [c, opd, psf, d, var, resid, G, g_opd]
 
         def finish():
             r.result = (loss, g_c)
