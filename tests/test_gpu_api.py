@@ -37,6 +37,26 @@ def test_fft_vs_oracle(dl, n, pad, inverse):
     assert _err(out, ref) < 3e-6
 
 
+def test_fft_large_sides(dl):
+    """The largest row blocks: M = 4096 (four rows, full 160 KiB of LDS) against the
+    FP64 oracle, and M = 8192 (one row per workgroup) by the forward-inverse round trip
+    and Parseval (sum |X|^2 = sum |x|^2 / 1 for the unitary centred transform)."""
+    print("[fft-large]", flush=True)
+    rng = np.random.default_rng(41)
+    E = rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))
+    out, _ = dl.utils.FFT(E, 1e-6, 1e-3, pad=128)
+    ref, _ = orc.fft(E, 1e-6, 1e-3, pad=128)
+    assert _err(out, ref) < 3e-6
+    x = torch.as_tensor((rng.standard_normal((8192, 8192)) +
+                         1j * rng.standard_normal((8192, 8192))).astype(np.complex64)).cuda()
+    fwd = dl.runtime.fft2(x, 8192)
+    back = dl.runtime.fft2(fwd, 8192, inverse=True)
+    assert float(torch.max(torch.abs(back - x))) < 2e-4 * float(torch.max(torch.abs(x)))
+    e_in = float(torch.sum(torch.abs(x.double()) ** 2))
+    e_out = float(torch.sum(torch.abs(fwd.double()) ** 2))
+    assert abs(e_out - e_in) < 1e-4 * e_in
+
+
 def test_fft_round_trip_reference_fixture(dl):
     """tests/utils/test_propagation.py:74-92: forward then inverse (pad=1) is identity."""
     print("[fft-roundtrip]", flush=True)

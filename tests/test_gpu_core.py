@@ -192,3 +192,31 @@ def test_matrix_cache_hits_and_bypass(rt):
     assert np.array_equal(a, c)
     rt.cache_clear()
     assert rt.cache_stats()["entries"] == 0
+
+
+def test_matrix_cache_eviction_reuses_blocks(rt):
+    """A budget that holds about two entries: later wavelength grids evict earlier
+    ones, their blocks are reused, and every result equals the uncached one."""
+    print("[cache-evict]", flush=True)
+    n, n_out = 64, 32
+    mask = orc.circular_aperture(n, 1.0, oversample=1)
+    grids = [np.linspace(0.8e-6 + 0.01e-6 * k, 1.2e-6, 6) for k in range(5)]
+    w = np.ones(6) / 6
+    kw = dict(amp=mask, scale=1.0 / n ** 2, precision="bf16x3")
+    rt.cache_set_budget(0)
+    try:
+        refs = [rt.psf_poly(n, n_out, g, w, 1.0 / n, 2.5e-7, **kw).cpu().numpy() for g in grids]
+    finally:
+        rt.cache_set_budget(16 << 30)
+    rt.cache_clear()
+    per_entry = 6 * n_out * n * 8
+    rt.cache_set_budget(int(2.5 * per_entry) + 4096)
+    try:
+        for rep in range(2):
+            for g, ref in zip(grids, refs):
+                out = rt.psf_poly(n, n_out, g, w, 1.0 / n, 2.5e-7, **kw).cpu().numpy()
+                assert np.array_equal(out, ref)
+        assert rt.cache_stats()["entries"] <= 2
+    finally:
+        rt.cache_set_budget(16 << 30)
+        rt.cache_clear()
