@@ -21,6 +21,8 @@ from .optical_systems import (BaseOpticalSystem, OpticalSystem, ParametricOptica
                               LayeredOpticalSystem, AngularOpticalSystem,
                               CartesianOpticalSystem)
 from .spectra import Spectrum
-from .sources import PointSource, PointSources
+from .sources import BaseSource, Source, PointSource, PointSources, BinarySource, Scene
+from .detectors import BaseDetector, LayeredDetector
+from .instruments import Instrument, Telescope, Dither
 
 __version__ = "0.1.0"

@@ -68,6 +68,24 @@ INSTANCE_DTYPE = np.dtype({
 })
 
 
+DLX_DET_PIXEL_RESPONSE = 1
+DLX_DET_CONVOLVE = 2
+DLX_DET_DOWNSAMPLE = 3
+DLX_DET_SATURATE = 4
+DLX_DET_ADD_CONSTANT = 5
+
+
+class DetectorOp(ctypes.Structure):
+    """``dlx_detector_op``."""
+
+    _fields_ = [
+        ("kind", c_int32),
+        ("size", c_int32),
+        ("value", c_double),
+        ("data", c_void_p),
+    ]
+
+
 SIGNATURES = {
     "dlx_version": (c_char_p, []),
     "dlx_last_error": (c_char_p, []),
@@ -93,6 +111,14 @@ SIGNATURES = {
     "dlx_project_basis_workspace_size": (c_size_t, [c_int32, c_int64, c_int32]),
     "dlx_project_basis": (c_int32, [c_int32, c_int64, c_int32, _fp, _fp, _fp, c_void_p,
                                     c_size_t, c_void_p]),
+    "dlx_detector_out_side": (c_int32, [c_int32, POINTER(DetectorOp), c_int32,
+                                        POINTER(c_int32)]),
+    "dlx_detector_workspace_size": (c_size_t, [c_int32, c_int32, POINTER(DetectorOp), c_int32,
+                                               c_int32]),
+    "dlx_detector_apply": (c_int32, [c_int32, c_int32, _fp, POINTER(DetectorOp), c_int32, _fp,
+                                     c_void_p, c_size_t, c_void_p]),
+    "dlx_detector_vjp": (c_int32, [c_int32, c_int32, _fp, POINTER(DetectorOp), c_int32, _fp,
+                                   _fp, c_void_p, c_size_t, c_void_p]),
     "dlx_mft_batched_workspace_size": (c_size_t, [c_int32, POINTER(MftInstance), c_int32]),
     "dlx_mft_batched_psf": (c_int32, [c_int32, POINTER(MftInstance), c_int32,
                                       POINTER(c_void_p), c_void_p, c_size_t, c_void_p]),

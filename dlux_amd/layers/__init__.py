@@ -1,6 +1,8 @@
-"""Optical layers (dLux.layers subset on the hot path)."""
+"""Optical and detector layers (dLux.layers subset on the hot path)."""
 
 from .optical_layers import (OpticalLayer, TransmissiveLayer, AberratedLayer, BasisLayer,
                              Tilt, Normalise)
 from .optics import Optic, BasisOptic
 from .propagators import Propagator, FFT, MFT, ASMPropagator
+from .detector_layers import (DetectorLayer, ApplyPixelResponse, ApplyJitter, ApplySaturation,
+                              AddConstant, Downsample)

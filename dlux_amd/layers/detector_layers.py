@@ -1,0 +1,135 @@
+"""Detector layers (layers/detector_layers.py:23-293): PSF -> PSF transforms.
+
+Each layer is one op of the HIP detector chain (dlx_detector_apply): applied
+alone it is one launch; a ``LayeredDetector`` hands its whole layer list to
+the chain, which fuses the elementwise layers into the convolution and
+downsampling passes.  Gradients flow to the PSF through dlx_detector_vjp.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import runtime
+from ..base import Base
+from ..psfs import PSF
+from ..utils.math import gaussian
+
+__all__ = ["DetectorLayer", "ApplyPixelResponse", "ApplyJitter", "ApplySaturation",
+           "AddConstant", "Downsample"]
+
+
+class DetectorLayer(Base):
+    """Base detector layer (detector_layers.py:23-65)."""
+
+    def __call__(self, psf: PSF) -> PSF:  # pragma: no cover - abstract
+        raise NotImplementedError
+
+    def apply(self, psf: PSF) -> PSF:
+        """Backwards-compatible alias of __call__ (detector_layers.py:51-65)."""
+        return self(psf)
+
+    # (op name, argument) of the HIP chain, or None if the layer has no chain form
+    def _det_op(self):
+        return None
+
+    # factor applied to the PSF pixel scale
+    def _scale_factor(self) -> int:
+        return 1
+
+    def _run(self, psf: PSF) -> PSF:
+        data = runtime.detector(psf.data, [self._det_op()])
+        return PSF(data, psf.pixel_scale * self._scale_factor())
+
+
+class ApplyPixelResponse(DetectorLayer):
+    """psf * pixel_response (detector_layers.py:100-131)."""
+
+    def __init__(self, pixel_response):
+        if isinstance(pixel_response, torch.Tensor):
+            self.pixel_response = pixel_response.to(torch.float32)
+        else:
+            self.pixel_response = np.asarray(pixel_response, dtype=np.float32)
+        if self.pixel_response.ndim != 2:
+            raise ValueError("pixel_response must be a 2d array.")
+
+    def _det_op(self):
+        return ("pixel_response", self.pixel_response)
+
+    def __call__(self, psf: PSF) -> PSF:
+        return self._run(psf)
+
+
+class ApplyJitter(DetectorLayer):
+    """Convolution with a normalised Gaussian of standard deviation `sigma`
+    pixels: gaussian on kernel_size * oversample points, sum-downsampled by
+    oversample (detector_layers.py:134-199)."""
+
+    def __init__(self, sigma, kernel_size: int = 9, oversample: int = 3):
+        self.kernel_size = int(kernel_size)
+        self.sigma = np.asarray(sigma, float)
+        self.oversample = int(oversample)
+        if self.kernel_size <= 0:
+            raise ValueError("kernel_size must be greater than 0.")
+
+    @property
+    def kernel(self) -> np.ndarray:
+        """detector_layers.py:180-196 (a float32 host array of side kernel_size)."""
+        s = float(self.sigma)
+        k = gaussian(mean=np.zeros(2), std=np.array([s, s]),
+                     npixels=self.kernel_size * self.oversample).astype(np.float64)
+        n, f = self.kernel_size, self.oversample
+        return k.reshape(n, f, n, f).sum((1, 3)).astype(np.float32)
+
+    def _det_op(self):
+        return ("convolve", self.kernel)
+
+    def __call__(self, psf: PSF) -> PSF:
+        return self._run(psf)
+
+
+class ApplySaturation(DetectorLayer):
+    """minimum(psf, threshold) (detector_layers.py:202-229)."""
+
+    def __init__(self, threshold):
+        self.threshold = np.asarray(threshold, float)
+
+    def _det_op(self):
+        return ("saturate", float(self.threshold))
+
+    def __call__(self, psf: PSF) -> PSF:
+        return self._run(psf)
+
+
+class AddConstant(DetectorLayer):
+    """psf + value (detector_layers.py:232-259)."""
+
+    def __init__(self, value):
+        self.value = np.asarray(value, float)
+
+    def _det_op(self):
+        return ("add", float(self.value))
+
+    def __call__(self, psf: PSF) -> PSF:
+        return self._run(psf)
+
+
+class Downsample(DetectorLayer):
+    """Sum over kernel_size x kernel_size blocks; the pixel scale grows by
+    kernel_size (detector_layers.py:262-293, psfs.py:74-91)."""
+
+    def __init__(self, kernel_size: int):
+        self.kernel_size = int(kernel_size)
+        if self.kernel_size <= 0:
+            raise ValueError("kernel_size must be greater than 0.")
+
+    def _det_op(self):
+        return ("downsample", self.kernel_size)
+
+    def _scale_factor(self) -> int:
+        return self.kernel_size
+
+    def __call__(self, psf: PSF) -> PSF:
+        return self._run(psf)
+

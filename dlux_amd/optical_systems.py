@@ -361,11 +361,15 @@ class ParametricLayeredOpticalSystem(ParametricOpticalSystem, LayeredOpticalSyst
         return psf, p_out
 
 
-    def _sources_psf(self, wavelengths, positions, weights):
-        """Sum over point sources of sum_l w[s, l] |MFT_l(P_l)|^2, each source's
-        position (radians) entering as an MFT shift of position / p_out output
-        pixels (equal in intensity to the pupil tilt), as ONE uneven batched
-        call over every (source, wavelength) instance.  None when the layers do
+    def _sources_psf(self, terms, n_outputs: int = 1):
+        """Point-source images as ONE uneven batched MFT call over every (star,
+        wavelength) instance.  `terms` lists (wavelengths (L,), positions (S, 2)
+        in radians, weights (S, L), output index); image o is
+        sum over the terms sent to o of sum_l w[s, l] |MFT_l(P_l)|^2, each
+        star's position entering as an MFT shift of position / p_out output
+        pixels (equal in intensity to the pupil tilt of sources.py:316-327).
+        The pupil fields are synthesised once per distinct wavelength.
+        Returns (list of n_outputs images, p_out), or None when the layers do
         not lower, gradients are being recorded, or the FP32 form is selected
         (the batched launches run the split-BF16x3 contractions)."""
         if runtime.get_precision() != "bf16x3":
@@ -381,18 +385,35 @@ class ParametricLayeredOpticalSystem(ParametricOpticalSystem, LayeredOpticalSyst
         n_out, p_out, f = self._focus_params()
         if f is not None:
             return None
-        wavelengths = np.atleast_1d(np.asarray(wavelengths, float))
-        fields = runtime.pupil_field(plan.n, wavelengths, plan.dx, tilt=tilt, amp=plan.amp,
+        wls, pos, wts, outs = [], [], [], []
+        for wavelengths, positions, weights, out_index in terms:
+            wl = np.atleast_1d(np.asarray(wavelengths, float))
+            p = np.asarray(positions, float).reshape(-1, 2)
+            w = np.asarray(weights, float).reshape(len(p), len(wl))
+            out_index = int(out_index)
+            if not 0 <= out_index < n_outputs:
+                raise ValueError(f"output index {out_index} outside [0, {n_outputs})")
+            wls.append(np.tile(wl, len(p)))
+            pos.append(np.repeat(p, len(wl), axis=0))
+            wts.append(w.reshape(-1))
+            outs.append(np.full(w.size, out_index, np.int64))
+        if not wls:
+            return None
+        wl_all = np.concatenate(wls)
+        unique_wl, slice_index = np.unique(wl_all, return_inverse=True)
+        fields = runtime.pupil_field(plan.n, unique_wl, plan.dx, tilt=tilt, amp=plan.amp,
                                      opd=plan.opd, phase=plan.phase, scale=plan.scale)
-        pos = np.asarray(positions, float).reshape(-1, 2)
-        w = np.asarray(weights, float).reshape(len(pos), len(wavelengths))
-        ns, nl = w.shape
+        pos_all = np.concatenate(pos)
         batch = runtime.MftBatch.from_arrays(
-            [fields], np.zeros(ns * nl, np.int64), np.tile(wavelengths, ns), plan.dx,
-            int(n_out), float(p_out), w.reshape(-1), 0,
-            shift_x=np.repeat(pos[:, 0] / p_out, nl), shift_y=np.repeat(pos[:, 1] / p_out, nl),
-            slice_index=np.tile(np.arange(nl), ns), n_outputs=1)
-        return batch()[0], p_out
+            [fields], np.zeros(len(wl_all), np.int64), wl_all, plan.dx, int(n_out),
+            float(p_out), np.concatenate(wts), np.concatenate(outs),
+            shift_x=pos_all[:, 0] / p_out, shift_y=pos_all[:, 1] / p_out,
+            slice_index=slice_index, n_outputs=n_outputs)
+        images = batch()
+        dev = fields.device
+        images = [torch.zeros((int(n_out), int(n_out)), device=dev) if im is None else im
+                  for im in images]
+        return images, p_out
 
 
 class AngularOpticalSystem(ParametricLayeredOpticalSystem):

@@ -541,6 +541,130 @@ def mft_batched_psf(instances, n_outputs: int | None = None, outputs=None,
     return MftBatch(instances, n_outputs, outputs)()
 
 
+# ---------------------------------------------------------------------------
+# Detector chain (dlx_detector_apply / dlx_detector_vjp)
+# ---------------------------------------------------------------------------
+
+DETECTOR_KINDS = {
+    "pixel_response": _lib.DLX_DET_PIXEL_RESPONSE,
+    "convolve": _lib.DLX_DET_CONVOLVE,
+    "downsample": _lib.DLX_DET_DOWNSAMPLE,
+    "saturate": _lib.DLX_DET_SATURATE,
+    "add": _lib.DLX_DET_ADD_CONSTANT,
+}
+
+
+def _detector_ops(ops, dev):
+    """[(kind name, array or scalar)] -> (dlx_detector_op array, tensors kept alive).
+    pixel_response / convolve take a 2-D array, downsample an int factor,
+    saturate / add a scalar."""
+    arr = (_lib.DetectorOp * max(len(ops), 1))()
+    keep = []
+    for i, (kind, arg) in enumerate(ops):
+        if kind not in DETECTOR_KINDS:
+            raise ValueError(f"unknown detector op {kind!r}")
+        op = arr[i]
+        op.kind = DETECTOR_KINDS[kind]
+        if kind in ("pixel_response", "convolve"):
+            t = as_float(arg, dev).contiguous()
+            if t.ndim != 2 or t.shape[0] != t.shape[1]:
+                raise ValueError(f"{kind} needs a square 2d array, got {tuple(t.shape)}")
+            keep.append(t)
+            op.size = int(t.shape[0])
+            op.data = t.data_ptr()
+        elif kind == "downsample":
+            op.size = int(arg)
+        else:
+            op.value = float(np.asarray(arg, np.float32))
+    return arr, keep
+
+
+def detector_out_side(n: int, ops) -> int:
+    """Side of the image the chain produces from an n x n input (validates it)."""
+    arr, _keep = _detector_ops(ops, device())
+    side = ctypes.c_int32(0)
+    _lib.check(_lib.load().dlx_detector_out_side(int(n), arr, len(ops), ctypes.byref(side)),
+               "dlx_detector_out_side")
+    return int(side.value)
+
+
+def _detector_impl(images, ops) -> torch.Tensor:
+    dev = device()
+    x = as_float(images, dev).contiguous()
+    if x.ndim < 2 or x.shape[-1] != x.shape[-2]:
+        raise ValueError(f"detector input must be (..., n, n), got {tuple(x.shape)}")
+    n = int(x.shape[-1])
+    lead = tuple(x.shape[:-2])
+    batch = int(np.prod(lead)) if lead else 1
+    lib = _lib.load()
+    arr, keep = _detector_ops(ops, dev)
+    side = ctypes.c_int32(0)
+    _lib.check(lib.dlx_detector_out_side(n, arr, len(ops), ctypes.byref(side)),
+               "dlx_detector_out_side")
+    out = torch.empty(lead + (side.value, side.value), dtype=torch.float32, device=dev)
+    ws = workspace(lib.dlx_detector_workspace_size(n, batch, arr, len(ops), 0), dev)
+    st = lib.dlx_detector_apply(n, batch, x.data_ptr(), arr, len(ops), out.data_ptr(),
+                                ws.data_ptr(), ws.numel(), stream_handle())
+    _lib.check(st, "dlx_detector_apply")
+    del keep
+    return out
+
+
+def _detector_vjp_impl(images, ops, dout) -> torch.Tensor:
+    dev = device()
+    x = as_float(images, dev).contiguous()
+    g = as_float(dout, dev).contiguous()
+    n = int(x.shape[-1])
+    lead = tuple(x.shape[:-2])
+    batch = int(np.prod(lead)) if lead else 1
+    lib = _lib.load()
+    arr, keep = _detector_ops(ops, dev)
+    side = ctypes.c_int32(0)
+    _lib.check(lib.dlx_detector_out_side(n, arr, len(ops), ctypes.byref(side)),
+               "dlx_detector_out_side")
+    if tuple(g.shape) != lead + (side.value, side.value):
+        raise ValueError(f"cotangent shape {tuple(g.shape)} != output shape "
+                         f"{lead + (side.value, side.value)}")
+    din = torch.empty_like(x)
+    ws = workspace(lib.dlx_detector_workspace_size(n, batch, arr, len(ops), 1), dev)
+    st = lib.dlx_detector_vjp(n, batch, x.data_ptr(), arr, len(ops), g.data_ptr(),
+                              din.data_ptr(), ws.data_ptr(), ws.numel(), stream_handle())
+    _lib.check(st, "dlx_detector_vjp")
+    del keep
+    return din
+
+
+class _DetectorFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, images, ops):
+        ctx.ops = ops
+        ctx.save_for_backward(images)
+        return _detector_impl(images, ops)
+
+    @staticmethod
+    def backward(ctx, g):
+        (images,) = ctx.saved_tensors
+        return _detector_vjp_impl(images, ctx.ops, g), None
+
+
+def detector(images, ops) -> torch.Tensor:
+    """The detector chain `ops` applied to float32 images (..., n, n), as one
+    dlx_detector_apply call; differentiable in `images` (dlx_detector_vjp)."""
+    for kind, arg in ops:
+        if isinstance(arg, torch.Tensor) and arg.requires_grad and torch.is_grad_enabled():
+            raise NotImplementedError(
+                f"gradients with respect to the {kind} array are not implemented")
+    if (torch.is_grad_enabled() and isinstance(images, torch.Tensor)
+            and images.requires_grad):
+        return _DetectorFn.apply(images, list(ops))
+    return _detector_impl(images, ops)
+
+
+def detector_vjp(images, ops, dout) -> torch.Tensor:
+    """(d detector(images) / d images)^T dout (dlx_detector_vjp)."""
+    return _detector_vjp_impl(images, ops, dout)
+
+
 def _pow2(m: int) -> bool:
     return m >= 2 and (m & (m - 1)) == 0 and m <= 8192
 

@@ -4,11 +4,12 @@ from __future__ import annotations
 
 from math import lgamma, exp
 
+import numpy as np
 import torch
 
 from .. import runtime
 
-__all__ = ["eval_basis", "factorial", "nandiv", "triangular_number"]
+__all__ = ["eval_basis", "factorial", "nandiv", "triangular_number", "gaussian"]
 
 
 def eval_basis(basis, coefficients) -> torch.Tensor:
@@ -39,3 +40,20 @@ def nandiv(a, b, fill=0.0):
     b = torch.as_tensor(b, device=a.device)
     safe = torch.where(b == 0, torch.ones_like(b), b)
     return torch.where(b == 0, torch.full_like(a / safe, fill), a / safe)
+
+
+def gaussian(mean=0.0, std=1.0, npixels=64, extent: float = 5.0) -> np.ndarray:
+    """Normalised separable Gaussian (utils/math.py:20-67): per axis the normal
+    pdf (loc=mean, scale=std) sampled on linspace(-extent, extent, n), the
+    outer product in 'xy' order, divided by its sum.  A small host array (a
+    detector kernel), returned as float32."""
+    npix = (int(npixels),) if np.ndim(npixels) == 0 else tuple(int(n) for n in npixels)
+    ndim = max(len(npix), np.size(mean), np.size(std))
+    mean = np.broadcast_to(np.asarray(mean, np.float64), (ndim,))
+    std = np.broadcast_to(np.asarray(std, np.float64), (ndim,))
+    if len(npix) != ndim:
+        npix = npix * ndim
+    one_d = [np.exp(-0.5 * ((np.linspace(-extent, extent, n) - m) / s) ** 2)
+             / (s * np.sqrt(2.0 * np.pi)) for n, m, s in zip(npix, mean, std)]
+    kernel = np.array(np.meshgrid(*one_d, indexing="xy")).prod(0)
+    return (kernel / np.sum(kernel)).astype(np.float32)

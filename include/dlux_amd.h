@@ -295,6 +295,67 @@ int dlx_asm_prop(int32_t n, int32_t m, int32_t batch, const double* wavelength,
                  void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * Detector chain: the reference's detector layers applied in order to
+ * `batch` float32 images (n x n each, stride n*n):
+ *   DLX_DET_PIXEL_RESPONSE  img *= map     (map: device f32, size x size, size =
+ *                                           the image side at that point)
+ *                           ApplyPixelResponse, layers/detector_layers.py:100-131
+ *   DLX_DET_CONVOLVE        img = convolve(img, kernel, mode="same"), zero padded,
+ *                           kernel device f32 size x size (1 <= size <= 64);
+ *                           output index i reads input i + (size-1)/2 - k
+ *                           ApplyJitter / PSF.convolve, layers/detector_layers.py:
+ *                           134-199, psfs.py:93-110
+ *   DLX_DET_DOWNSAMPLE      size x size block sums (side must divide by size)
+ *                           Downsample / PSF.downsample,
+ *                           layers/detector_layers.py:262-293, psfs.py:74-91
+ *   DLX_DET_SATURATE        img = min(img, value)
+ *                           ApplySaturation, layers/detector_layers.py:202-229
+ *   DLX_DET_ADD_CONSTANT    img += value
+ *                           AddConstant, layers/detector_layers.py:232-259
+ * (LayeredDetector.__call__, detectors.py:103-128.)  Elementwise ops fuse into
+ * the load or the store of the nearest convolve / downsample pass, so the
+ * chain costs one pass over the image per convolve or downsample.  `value`
+ * is used as float32.  dlx_detector_out_side validates the chain and gives
+ * the output side; `out` holds batch x out_side^2 floats.
+ */
+enum dlx_detector_kind {
+  DLX_DET_PIXEL_RESPONSE = 1,
+  DLX_DET_CONVOLVE = 2,
+  DLX_DET_DOWNSAMPLE = 3,
+  DLX_DET_SATURATE = 4,
+  DLX_DET_ADD_CONSTANT = 5
+};
+
+typedef struct dlx_detector_op {
+  int32_t kind;        /* dlx_detector_kind                                   */
+  int32_t size;        /* map side, kernel side or downsample factor          */
+  double value;        /* threshold or constant                               */
+  const float* data;   /* device map or kernel; NULL for the other kinds      */
+} dlx_detector_op;
+
+int dlx_detector_out_side(int32_t n, const dlx_detector_op* ops, int32_t nops,
+                          int32_t* out_side);
+size_t dlx_detector_workspace_size(int32_t n, int32_t batch, const dlx_detector_op* ops,
+                                   int32_t nops, int32_t for_vjp);
+int dlx_detector_apply(int32_t n, int32_t batch, const float* in, const dlx_detector_op* ops,
+                       int32_t nops, float* out, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
+/*
+ * Vector-Jacobian product of dlx_detector_apply with respect to its input
+ * images: din = (d out / d in)^T dout.  The chain is re-run forward to keep
+ * each pass's input and pre-store values (workspace sized with for_vjp = 1).
+ * Adjoints: convolve -> correlation with the same kernel, downsample ->
+ * replication, pixel response -> the same map, saturation -> 1 below the
+ * threshold, 1/2 at it, 0 above (jnp.minimum's derivative), constant -> 1.
+ * Replaces the reverse-mode derivative JAX builds of the detector layers
+ * (layers/detector_layers.py:100-293).
+ */
+int dlx_detector_vjp(int32_t n, int32_t batch, const float* in, const dlx_detector_op* ops,
+                     int32_t nops, const float* dout, float* din, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/*
  * Transfer-matrix cache (no counterpart in the reference: dLux rebuilds the
  * matrices inside every traced call).  dlx_mft, dlx_psf_poly,
  * dlx_psf_poly_vjp and dlx_mft_batched_psf keep the matrices of each

@@ -21,6 +21,8 @@ Parity status per function (all pinned unless marked):
   * circle / spider / combine / circular_aperture                pinned
   * eval_basis, wavefront chain, poly_psf, point_sources_psf     pinned
   * poly_psf_grad (adjoint gradient)                             pinned (FD)
+  * gaussian / jitter_kernel / convolve_same / detector           pinned
+  * detector_vjp                                                 pinned (FD, adjoint identity)
   * asm_prop (angular spectrum)                                  parity unpinned
       -- its conventions live in abcdLux (absent); only textbook invariants
          (z=0 identity, unitarity, z1+z2 composition) are checked.
@@ -531,3 +533,116 @@ def propagate_fft(phasor, wavelength, pixel_scale, center=0.0, pad=2, focal_leng
     out, ps = fft(np.asarray(phasor) * in_ramp, wavelength, pixel_scale, focal_length, pad,
                   inverse)
     return out * out_ramp, ps, c_out
+
+
+# ---------------------------------------------------------------------------
+# Detector layers  (src/dLux/layers/detector_layers.py, src/dLux/psfs.py,
+# src/dLux/utils/math.py, src/dLux/utils/array_ops.py)
+# ---------------------------------------------------------------------------
+
+
+def gaussian(mean=0.0, std=1.0, npixels=64, extent=5.0):
+    """Separable normalised Gaussian on linspace(-extent, extent, n) per axis,
+    pdf with loc=mean, scale=std, outer product in 'xy' order, divided by its
+    sum.  utils/math.py:20-67 (the grid spans +-extent in the pdf's own units)."""
+    mean = np.broadcast_to(np.asarray(mean, np.float64), (2,))
+    std = np.broadcast_to(np.asarray(std, np.float64), (2,))
+    npix = (npixels, npixels) if np.ndim(npixels) == 0 else tuple(npixels)
+    axes = [np.linspace(-extent, extent, n) for n in npix]
+    g1 = [np.exp(-0.5 * ((a - m) / s) ** 2) / (s * np.sqrt(2 * np.pi))
+          for a, m, s in zip(axes, mean, std)]
+    kernel = np.array(np.meshgrid(*g1, indexing="xy")).prod(0)
+    return kernel / np.sum(kernel)
+
+
+def jitter_kernel(sigma, kernel_size=9, oversample=3):
+    """ApplyJitter.kernel: gaussian on kernel_size*oversample pixels, then a
+    sum-downsample by oversample.  layers/detector_layers.py:180-196."""
+    k = gaussian(0.0, [sigma, sigma], kernel_size * oversample)
+    return downsample(k, oversample, mean=False)
+
+
+def convolve_same(data, kernel):
+    """Direct 2-D convolution, output the size of `data`, zero padded:
+    out[i, j] = sum_{a,b} data[i + (K0-1)//2 - a, j + (K1-1)//2 - b] kernel[a, b]
+    (scipy / jax.scipy.signal.convolve mode="same", method="direct", the call
+    of PSF.convolve, psfs.py:93-110)."""
+    data = np.asarray(data, np.float64)
+    kernel = np.asarray(kernel, np.float64)
+    n0, n1 = data.shape
+    k0, k1 = kernel.shape
+    o0, o1 = (k0 - 1) // 2, (k1 - 1) // 2
+    padded = np.zeros((n0 + k0 - 1, n1 + k1 - 1))
+    # padded[u, v] = data[u - (k0 - 1 - o0), v - (k1 - 1 - o1)]
+    padded[k0 - 1 - o0:k0 - 1 - o0 + n0, k1 - 1 - o1:k1 - 1 - o1 + n1] = data
+    out = np.zeros((n0, n1))
+    for a in range(k0):
+        for b in range(k1):
+            # data index i + o0 - a  ->  padded index i + (k0 - 1) - a
+            out += kernel[a, b] * padded[k0 - 1 - a:k0 - 1 - a + n0, k1 - 1 - b:k1 - 1 - b + n1]
+    return out
+
+
+def correlate_same_adjoint(g, kernel):
+    """Adjoint of convolve_same in its first argument:
+    din[q] = sum_{a,b} g[q0 - (K0-1)//2 + a, q1 - (K1-1)//2 + b] kernel[a, b]."""
+    g = np.asarray(g, np.float64)
+    kernel = np.asarray(kernel, np.float64)
+    n0, n1 = g.shape
+    k0, k1 = kernel.shape
+    o0, o1 = (k0 - 1) // 2, (k1 - 1) // 2
+    padded = np.zeros((n0 + k0 - 1, n1 + k1 - 1))
+    padded[o0:o0 + n0, o1:o1 + n1] = g
+    out = np.zeros((n0, n1))
+    for a in range(k0):
+        for b in range(k1):
+            out += kernel[a, b] * padded[a:a + n0, b:b + n1]
+    return out
+
+
+def detector(psf, layers):
+    """LayeredDetector.__call__ (detectors.py:103-128) over a list of layers:
+    ("pixel_response", map)  psf * map                  detector_layers.py:130-131
+    ("convolve", kernel)     convolve_same(psf, kernel) detector_layers.py:198-199
+    ("downsample", k)        block sums                 detector_layers.py:292-293
+    ("saturate", t)          minimum(psf, t)            detector_layers.py:228-229
+    ("add", c)               psf + c                    detector_layers.py:258-259"""
+    x = np.asarray(psf, np.float64)
+    for layer in layers:
+        kind = layer[0]
+        if kind == "pixel_response":
+            x = x * np.asarray(layer[1], np.float64)
+        elif kind == "convolve":
+            x = convolve_same(x, layer[1])
+        elif kind == "downsample":
+            x = downsample(x, int(layer[1]), mean=False)
+        elif kind == "saturate":
+            x = np.minimum(x, float(layer[1]))
+        elif kind == "add":
+            x = x + float(layer[1])
+        else:
+            raise ValueError(f"unknown detector layer {kind!r}")
+    return x
+
+
+def detector_vjp(psf, layers, g):
+    """(d detector / d psf)^T g, the reverse pass JAX builds: adjoint convolution
+    (correlation), block replication, map multiply, minimum's derivative
+    (1 below, 1/2 at, 0 above the threshold), identity for the constant."""
+    xs = [np.asarray(psf, np.float64)]
+    for layer in layers:
+        xs.append(detector(xs[-1], [layer]))
+    g = np.asarray(g, np.float64)
+    for layer, x in zip(reversed(layers), reversed(xs[:-1])):
+        kind = layer[0]
+        if kind == "pixel_response":
+            g = g * np.asarray(layer[1], np.float64)
+        elif kind == "convolve":
+            g = correlate_same_adjoint(g, layer[1])
+        elif kind == "downsample":
+            k = int(layer[1])
+            g = np.repeat(np.repeat(g, k, axis=0), k, axis=1)
+        elif kind == "saturate":
+            t = float(layer[1])
+            g = g * np.where(x < t, 1.0, np.where(x == t, 0.5, 0.0))
+    return g

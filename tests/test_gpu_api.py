@@ -220,7 +220,7 @@ def test_point_sources_batched_bf16x3(dl):
     wls = [0.9e-6, 1.0e-6, 1.1e-6]
     src = dl.PointSources(wavelengths=wls, position=pos, flux=flux)
     with runtime.precision("bf16x3"):
-        assert optics._sources_psf(wls, pos, np.ones((7, 3))) is not None
+        assert optics._sources_psf([(wls, pos, np.ones((7, 3)), 0)]) is not None
         psf = src.model(optics)
     mask_r = orc.circular_aperture(cfg.n_in, 1.0, oversample=1)
     basis_r = orc.zernike_basis(cfg.js, orc.pixel_coords(cfg.n_in, 1.0), 1.0)

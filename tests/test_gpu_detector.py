@@ -1,0 +1,222 @@
+"""GPU parity of the detector chain (dlx_detector_apply / dlx_detector_vjp),
+the detector layers, LayeredDetector, and the source-to-detector models
+(BinarySource, Scene, Telescope, Dither) against the FP64 oracle.
+
+Error metric: peak-normalised max |x - ref| / max |ref|.  Bounds: 2e-6 for the
+detector chain and its VJP (float32 sums of at most a few hundred terms),
+1e-5 for anything that contains a PSF (the PSF bound of the hot path).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import dlux_oracle as orc
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+CHAIN_TOL = 2e-6
+PSF_TOL = 1e-5
+
+
+def _err(x, ref):
+    x = x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
+    return float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
+
+
+@pytest.fixture(scope="module")
+def dl():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import dlux_amd
+    return dlux_amd
+
+
+def _threshold_in_gap(values, q):
+    """A threshold near quantile q at least 1e-4 (relative) from every value,
+    so float32 and float64 put each pixel on the same side of it."""
+    v = np.sort(values.ravel())
+    scale = np.max(np.abs(v))
+    i = int(q * (len(v) - 1))
+    for j in range(i, len(v) - 1):
+        if v[j + 1] - v[j] > 2e-4 * scale:
+            return 0.5 * (v[j] + v[j + 1])
+    raise AssertionError("no gap found")
+
+
+def _chains(rng):
+    """(name, n, batch, oracle chain) cases; thresholds put in gaps of the data."""
+    cases = []
+    cases.append(("jitter", 64, 3, [("convolve", orc.jitter_kernel(0.8, 9, 3))]))
+    r64, r16 = rng.uniform(0.5, 1.5, (64, 64)), rng.uniform(0.5, 1.5, (16, 16))
+    cases.append(("full", 64, 2, [("add", 0.5), ("pixel_response", r64),
+                                  ("convolve", rng.normal(size=(4, 4))), ("saturate", None),
+                                  ("downsample", 4), ("pixel_response", r16), ("add", -0.1)]))
+    cases.append(("odd", 45, 1, [("convolve", rng.uniform(0, 1, (7, 7))), ("downsample", 3),
+                                 ("saturate", None)]))
+    cases.append(("wide-kernel", 20, 2, [("convolve", rng.normal(size=(33, 33)))]))
+    cases.append(("max-kernel", 70, 1, [("convolve", rng.normal(size=(64, 64)))]))
+    many = []
+    for k in range(11):
+        many.append(("add", 0.01 * (k + 1)))
+        many.append(("pixel_response", rng.uniform(0.9, 1.1, (32, 32))))
+    cases.append(("ew-overflow", 32, 2, many))
+    cases.append(("down-down", 96, 2, [("downsample", 2), ("add", 1.0), ("downsample", 3),
+                                       ("pixel_response", rng.uniform(0.5, 1.5, (16, 16)))]))
+    cases.append(("identity", 17, 2, []))
+    cases.append(("large", 1024, 2, [("convolve", orc.jitter_kernel(0.6, 9, 3)),
+                                     ("downsample", 4), ("saturate", None)]))
+    return cases
+
+
+def _resolve(chain, x):
+    """Fill each saturation threshold with a value in a gap of the data
+    reaching it (per image batch, taken over the whole batch)."""
+    out = []
+    for kind, arg in chain:
+        if kind == "saturate" and arg is None:
+            pre = np.stack([orc.detector(xi, out) for xi in x])
+            arg = float(np.float32(_threshold_in_gap(pre, 0.8)))
+        out.append((kind, arg))
+    return out
+
+
+def _oracle(x, chain):
+    return np.stack([orc.detector(xi, chain) for xi in x])
+
+
+@pytest.mark.parametrize("case", range(9))
+def test_detector_chain_and_vjp_vs_oracle(dl, case):
+    rng = np.random.default_rng(40 + case)
+    name, n, batch, chain = _chains(rng)[case]
+    print(f"[detector] {name} n={n} batch={batch} ops={len(chain)}", flush=True)
+    x = rng.uniform(0.0, 1.0, (batch, n, n))
+    chain = _resolve(chain, x)
+    ref = _oracle(x, chain)
+    got = dl.runtime.detector(torch.as_tensor(x, dtype=torch.float32).cuda(), chain)
+    assert tuple(got.shape) == ref.shape
+    err = _err(got, ref)
+    print(f"  forward err {err:.2e}")
+    assert err < CHAIN_TOL
+    g = rng.normal(size=ref.shape)
+    vjp = dl.runtime.detector_vjp(torch.as_tensor(x, dtype=torch.float32).cuda(), chain,
+                                  torch.as_tensor(g, dtype=torch.float32).cuda())
+    vref = np.stack([orc.detector_vjp(xi, chain, gi) for xi, gi in zip(x, g)])
+    err = _err(vjp, vref)
+    print(f"  vjp err {err:.2e}")
+    assert err < CHAIN_TOL
+
+
+def test_detector_autograd_and_fusion_invariance(dl):
+    """torch autograd through LayeredDetector uses dlx_detector_vjp; the fused
+    chain equals the layers applied one at a time."""
+    print("[detector] autograd / fusion", flush=True)
+    from dlux_amd import layers
+    rng = np.random.default_rng(3)
+    n = 96
+    x = torch.as_tensor(rng.uniform(0, 1, (n, n)), dtype=torch.float32).cuda()
+    resp = rng.uniform(0.8, 1.2, (n, n)).astype(np.float32)
+    det = dl.LayeredDetector([layers.ApplyPixelResponse(resp), layers.ApplyJitter(0.7),
+                              layers.Downsample(3), layers.AddConstant(2.0)])
+    psf = dl.PSF(x, 0.1)
+    fused = det.model(psf, return_psf=True)
+    step = psf
+    for layer in det.layers.values():
+        step = layer(step)
+    assert torch.allclose(fused.data, step.data, rtol=1e-6, atol=1e-6)
+    assert np.isclose(float(fused.pixel_scale), 0.3) and np.isclose(float(step.pixel_scale), 0.3)
+    xg = x.clone().requires_grad_(True)
+    w = torch.as_tensor(rng.normal(size=(32, 32)), dtype=torch.float32).cuda()
+    (grad,) = torch.autograd.grad((det(dl.PSF(xg, 0.1)) * w).sum(), xg)
+    chain = [layer._det_op() for layer in det.layers.values()]
+    ref = orc.detector_vjp(x.double().cpu().numpy(), chain, w.double().cpu().numpy())
+    assert _err(grad, ref) < CHAIN_TOL
+
+
+def test_detector_rejects_mismatched_response(dl):
+    print("[detector] errors", flush=True)
+    from dlux_amd import layers
+    psf = dl.PSF(torch.zeros((64, 64), device="cuda"), 1.0)
+    det = dl.LayeredDetector([layers.Downsample(2), layers.ApplyPixelResponse(np.ones((64, 64)))])
+    with pytest.raises(RuntimeError, match="64x64 but the image is 32x32"):
+        det(psf)
+    with pytest.raises(RuntimeError, match="not divisible by 5"):
+        layers.Downsample(5)(psf)
+
+
+def _optics_and_oracle(dl):
+    from dlux_amd import workloads
+    cfg = workloads.MONO
+    optics = workloads.build_system(cfg)
+    mask = orc.circular_aperture(cfg.n_in, 1.0, oversample=1)
+    basis = orc.zernike_basis(cfg.js, orc.pixel_coords(cfg.n_in, 1.0), 1.0)
+
+    def star(wls, weights, pos):
+        return orc.poly_psf(mask, basis, cfg.coefficients(), wls, weights, 1.0, cfg.n_out,
+                            cfg.p_out, offset=pos)
+    return cfg, optics, star
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_telescope_binary_scene_with_detector_vs_oracle(dl, precision):
+    """Scene(BinarySource, PointSources on another wavelength grid) through the
+    optics and a jitter + pixel-response + downsample detector; under bf16x3 the
+    whole scene is one batched launch."""
+    print(f"[telescope] scene + detector, {precision}", flush=True)
+    from dlux_amd import layers, runtime
+    cfg, optics, star = _optics_and_oracle(dl)
+    p = cfg.p_out
+    wl_a = np.array([0.95e-6, 1.05e-6])
+    wl_b = np.array([0.9e-6, 1.0e-6, 1.1e-6])
+    binary = dl.BinarySource(wl_a, position=[1.5 * p, -0.5 * p], mean_flux=2.0,
+                             separation=6.3 * p, position_angle=0.7, contrast=3.0,
+                             weights=np.array([[1.0, 2.0], [2.0, 1.0]]))
+    pts = dl.PointSources(wl_b, position=np.array([[-4.2 * p, 3.1 * p], [2.2 * p, 2.9 * p]]),
+                          flux=[0.5, 1.5])
+    resp = np.random.default_rng(8).uniform(0.9, 1.1, (cfg.n_out, cfg.n_out))
+    det = dl.LayeredDetector([layers.ApplyJitter(0.9, 5, 3), layers.ApplyPixelResponse(resp),
+                              layers.Downsample(2)])
+    tel = dl.Telescope(optics, [("binary", binary), ("pts", pts)], det)
+    with runtime.precision(precision):
+        out = tel.model(return_psf=True)
+    # oracle: every star, then the detector chain
+    f = 2.0 * 2 * np.array([3.0, 1.0]) / 4.0
+    sep = np.array([np.sin(0.7), np.cos(0.7)]) * 6.3 * p / 2
+    centre = np.array([1.5 * p, -0.5 * p])
+    wa = np.array([[1.0, 2.0], [2.0, 1.0]]) / 3.0
+    img = star(wl_a, wa[0] * f[0], centre + sep) + star(wl_a, wa[1] * f[1], centre - sep)
+    for pos, flux in zip(pts.position, [0.5, 1.5]):
+        img = img + star(wl_b, np.ones(3) / 3 * flux, pos)
+    ref = orc.detector(img, [("convolve", orc.jitter_kernel(0.9, 5, 3)),
+                             ("pixel_response", resp), ("downsample", 2)])
+    err = _err(out.data, ref)
+    print(f"  err {err:.2e}")
+    assert err < PSF_TOL
+    assert np.isclose(float(out.pixel_scale), 2 * cfg.p_out)
+
+
+def test_dither_single_launch_matches_per_dither_models(dl):
+    """Dither: every dither in one batched launch equals the Telescope modelled
+    with the source moved by that dither (instruments.py:231-256)."""
+    print("[dither]", flush=True)
+    from dlux_amd import layers, runtime
+    cfg, optics, star = _optics_and_oracle(dl)
+    p = cfg.p_out
+    dithers = np.array([[0.0, 0.0], [2.5 * p, -1.0 * p], [-3.0 * p, 4.0 * p]])
+    src = dl.BinarySource([1e-6, 1.1e-6], separation=5 * p, contrast=2.0)
+    det = dl.LayeredDetector([layers.Downsample(4), layers.AddConstant(0.25)])
+    dith = dl.Dither(dithers, optics, src, det)
+    with runtime.precision("bf16x3"):
+        got = dith.model(return_psf=True)
+        assert tuple(got.data.shape) == (3, cfg.n_out // 4, cfg.n_out // 4)
+        for k, d in enumerate(dithers):
+            one = dl.Telescope(optics, src.add("position", d), det).model()
+            assert _err(got.data[k], one.cpu().numpy()) < 1e-6
+    # dither 1 against the oracle
+    f = 2 * np.array([2.0, 1.0]) / 3.0
+    half = np.array([np.sin(np.pi / 2), np.cos(np.pi / 2)]) * 5 * p / 2
+    c = dithers[1]
+    img = star([1e-6, 1.1e-6], np.full(2, 0.5) * f[0], c + half) + \
+        star([1e-6, 1.1e-6], np.full(2, 0.5) * f[1], c - half)
+    ref = orc.detector(img, [("downsample", 4), ("add", 0.25)])
+    assert _err(got.data[1], ref) < PSF_TOL

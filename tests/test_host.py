@@ -206,3 +206,152 @@ def test_workload_definitions():
     assert len(w.POLY.js) == 36 and w.POLY.n_wl == 64
     c1, c2 = w.POLY.coefficients(), w.POLY.coefficients()
     assert np.array_equal(c1, c2)
+
+
+def test_detector_abi_plans_without_gpu(lib):
+    """dlx_detector_out_side / workspace_size plan the chain on the host; the
+    entry points reject what the reference's layers reject."""
+    from dlux_amd import _lib
+
+    def ops(*specs):
+        arr = (_lib.DetectorOp * max(len(specs), 1))()
+        for e, (kind, size, value, data) in zip(arr, specs):
+            e.kind, e.size, e.value, e.data = kind, size, value, data
+        return arr, len(specs)
+
+    side = ctypes.c_int32()
+    a, n = ops((_lib.DLX_DET_CONVOLVE, 9, 0.0, 4096), (_lib.DLX_DET_PIXEL_RESPONSE, 64, 0.0, 8192),
+               (_lib.DLX_DET_DOWNSAMPLE, 4, 0.0, None), (_lib.DLX_DET_SATURATE, 0, 1e4, None),
+               (_lib.DLX_DET_ADD_CONSTANT, 0, 3.0, None), (_lib.DLX_DET_DOWNSAMPLE, 2, 0.0, None))
+    assert lib.dlx_detector_out_side(64, a, n, ctypes.byref(side)) == 0 and side.value == 8
+    # two structural passes: one intermediate image pair of 64x64 floats per batch
+    assert lib.dlx_detector_workspace_size(64, 3, a, n, 0) >= 2 * 3 * 64 * 64 * 4
+    assert (lib.dlx_detector_workspace_size(64, 3, a, n, 1)
+            > lib.dlx_detector_workspace_size(64, 3, a, n, 0))
+    a, n = ops((_lib.DLX_DET_DOWNSAMPLE, 5, 0.0, None),)
+    assert lib.dlx_detector_out_side(64, a, n, ctypes.byref(side)) == 1
+    assert b"not divisible by 5" in lib.dlx_last_error()
+    a, n = ops((_lib.DLX_DET_DOWNSAMPLE, 2, 0.0, None), (_lib.DLX_DET_PIXEL_RESPONSE, 64, 0.0, 8))
+    assert lib.dlx_detector_out_side(64, a, n, ctypes.byref(side)) == 1
+    assert b"32x32" in lib.dlx_last_error()
+    a, n = ops((_lib.DLX_DET_CONVOLVE, 65, 0.0, 8),)
+    assert lib.dlx_detector_out_side(64, a, n, ctypes.byref(side)) == 1
+    a, n = ops((9, 1, 0.0, None),)
+    assert lib.dlx_detector_out_side(64, a, n, ctypes.byref(side)) == 1
+    assert b"unknown" in lib.dlx_last_error()
+    # an empty chain is the identity; empty batches are no-ops
+    assert lib.dlx_detector_out_side(7, None, 0, ctypes.byref(side)) == 0 and side.value == 7
+    assert lib.dlx_detector_apply(7, 0, None, None, 0, None, None, 0, None) == 0
+    a, n = ops((_lib.DLX_DET_CONVOLVE, 3, 0.0, 8), (_lib.DLX_DET_DOWNSAMPLE, 2, 0.0, None))
+    st = lib.dlx_detector_apply(16, 1, 4096, a, n, 8192, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+    st = lib.dlx_detector_vjp(16, 1, 4096, a, n, 8192, 12288, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+
+
+def test_detector_layer_construction():
+    """tests/layers/test_detector_layers.py:21-51 and tests/test_detectors.py:10-31."""
+    import dlux_amd as dl
+    from dlux_amd import layers
+    with pytest.raises(ValueError):
+        layers.ApplyPixelResponse(np.ones(16))
+    jit = layers.ApplyJitter(0.1, 5)
+    assert jit.sigma.shape == () and jit.kernel.shape == (5, 5)
+    with pytest.raises(ValueError):
+        layers.ApplyJitter(0.1, 0)
+    assert layers.ApplySaturation(1e4).threshold.shape == ()
+    assert layers.AddConstant(1).value.shape == ()
+    with pytest.raises(ValueError):
+        layers.Downsample(0)
+    det = dl.LayeredDetector([layers.Downsample(4)])
+    assert isinstance(det.Downsample, layers.Downsample) and det.kernel_size == 4
+    with pytest.raises(AttributeError):
+        det.not_an_attr
+    inserted = det.insert_layer(layers.Downsample(4), 1)
+    assert list(inserted.layers.keys()) == ["Downsample_0", "Downsample_1"]
+    removed = inserted.remove_layer("Downsample_0")
+    assert list(removed.layers.keys()) == ["Downsample_1"]
+    assert list(det.layers.keys()) == ["Downsample"]
+
+
+def test_jitter_kernel_matches_oracle():
+    from dlux_amd import layers
+    from dlux_amd.utils import gaussian
+    from oracle import dlux_oracle as orc
+    assert np.allclose(gaussian(0.0, [0.7, 1.3], 11), orc.gaussian(0.0, [0.7, 1.3], 11),
+                       rtol=1e-6)
+    for sigma, k, os_ in [(0.5, 9, 3), (1.7, 5, 1), (0.1, 7, 4)]:
+        ref = orc.jitter_kernel(sigma, k, os_)
+        got = layers.ApplyJitter(sigma, k, os_).kernel
+        assert got.dtype == np.float32 and np.allclose(got, ref, rtol=1e-6, atol=1e-9)
+
+
+def test_binary_source_and_scene_host_logic():
+    """tests/test_sources.py:76-129 (construction, weights, keys) and
+    utils/source.py:10-52."""
+    import dlux_amd as dl
+    from dlux_amd import sources as S
+    from dlux_amd.utils import fluxes_from_contrast, positions_from_sep
+    wl = np.ones(2)
+    src = dl.BinarySource(wl, separation=1.0)
+    assert src.mean_flux.shape == () and src.separation.shape == ()
+    assert src.position_angle.shape == () and src.contrast.shape == ()
+    with pytest.raises(ValueError):
+        dl.BinarySource(wl, separation=1.0, position=np.ones(1))
+    src = dl.BinarySource(wl, weights=np.ones((2, 2)))
+    assert src.weights.shape == (2, 2) and np.allclose(src.weights.sum(-1), 1)
+    # fluxes average to the mean and keep the contrast; positions straddle the centre
+    f = fluxes_from_contrast(10.0, 4.0)
+    assert np.allclose(f, [16.0, 4.0]) and np.isclose(f.mean(), 10.0)
+    p = positions_from_sep(np.array([1.0, 2.0]), 2.0, np.pi / 2)
+    assert np.allclose(p, [[2.0, 2.0], [0.0, 2.0]])
+    p = positions_from_sep(np.zeros(2), 2.0, 0.0)
+    assert np.allclose(p, [[0.0, 1.0], [0.0, -1.0]])
+    (wls, pos, w), = dl.BinarySource([1e-6, 2e-6], mean_flux=3.0, contrast=2.0,
+                                     separation=1e-6)._point_terms()
+    assert pos.shape == (2, 2) and np.allclose(w.sum(-1), [4.0, 2.0])
+
+    keyed = dl.Scene([("source", dl.PointSource([1e-6]))])
+    assert isinstance(keyed.source, dl.PointSource)
+    assert isinstance(dl.Scene(dl.PointSource([1e-6])).PointSource, dl.PointSource)
+    tup = dl.Scene((dl.PointSource([1e-6]), dl.PointSource([1e-6])))
+    assert isinstance(tup.PointSource_0, dl.PointSource)
+    with pytest.raises(AttributeError):
+        tup.not_an_attr
+    assert len(tup._point_terms()) == 2
+
+    assert S._as_wavelengths_1d(None) is None
+    with pytest.raises(ValueError):
+        S._as_wavelengths_1d(np.ones((2, 2)))
+    assert S._infer_n_wavelengths(np.ones(3), None) == 3
+    assert S._infer_n_wavelengths(None, dl.Spectrum([1e-6])) == 1
+    with pytest.raises(ValueError):
+        S._infer_n_wavelengths(None, None)
+    with pytest.raises(TypeError):
+        S._infer_n_wavelengths(None, 1)
+    with pytest.raises(TypeError):
+        dl.PointSource([1e-6], spectrum=1)
+    with pytest.raises(ValueError, match="Cannot return both"):
+        dl.PointSources([1e-6]).model(None, return_wf=True, return_psf=True)
+
+
+def test_instrument_construction():
+    """instruments.py:65-114 and :211-229 validation."""
+    import dlux_amd as dl
+    from dlux_amd import layers
+    optics = dl.LayeredOpticalSystem(16, 1.0, [dl.Optic()])
+    with pytest.raises(TypeError, match="OpticalSystem"):
+        dl.Telescope(1, dl.PointSource([1e-6]))
+    with pytest.raises(TypeError, match="source tuple"):
+        dl.Telescope(optics, ("a", 1))
+    with pytest.raises(TypeError, match="Detector"):
+        dl.Telescope(optics, dl.PointSource([1e-6]), detector=1)
+    tel = dl.Telescope(optics, [dl.PointSource([1e-6]), ("b", dl.PointSource([1e-6]))],
+                       dl.LayeredDetector([layers.Downsample(2)]))
+    assert isinstance(tel.source, dl.Scene) and list(tel.source.sources) == ["PointSource", "b"]
+    assert tel.wf_npixels == 16 and tel.kernel_size == 2
+    with pytest.raises(ValueError, match="ndithers"):
+        dl.Dither(np.zeros(2), optics, dl.PointSource([1e-6]))
+    d = dl.Dither(np.zeros((3, 2)), optics, dl.PointSource([1e-6]))
+    moved = d.add("source.position", np.ones(2))
+    assert np.allclose(moved.source.position, 1.0) and np.allclose(d.source.position, 0.0)

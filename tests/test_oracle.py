@@ -338,3 +338,86 @@ def test_oracle_reproduces_golden_fixtures():
                                    float(d["flux"]))
     assert np.isclose(loss, d["loss"], rtol=1e-12)
     assert np.allclose(grad, d["grad"], rtol=1e-10)
+
+
+# ---------------------------------------------------------------- detector
+@pytest.mark.parametrize("n,k", [(16, 5), (17, 4), (8, 9), (12, 1), (10, 6), (33, 9)])
+def test_convolve_same_is_scipy_direct_convolution(n, k):
+    """PSF.convolve calls jax.scipy.signal.convolve(mode="same"), whose output
+    alignment is scipy's: both odd and even kernels, and kernels wider than the
+    image."""
+    import scipy.signal as ss
+    rng = np.random.default_rng(n * 100 + k)
+    d, w = rng.normal(size=(n, n)), rng.normal(size=(k, k))
+    ref = ss.convolve(d, w, mode="same", method="direct")
+    assert np.max(np.abs(orc.convolve_same(d, w) - ref)) < 1e-12
+
+
+def test_convolve_same_delta_and_shift():
+    """A centred delta kernel is the identity; a delta one pixel right of the
+    kernel centre moves the image one pixel right (true convolution, not
+    correlation)."""
+    rng = np.random.default_rng(1)
+    d = rng.normal(size=(9, 9))
+    delta = np.zeros((5, 5))
+    delta[2, 2] = 1.0
+    assert np.allclose(orc.convolve_same(d, delta), d)
+    right = np.zeros((5, 5))
+    right[2, 3] = 1.0
+    out = orc.convolve_same(d, right)
+    assert np.allclose(out[:, 1:], d[:, :-1]) and np.allclose(out[:, 0], 0)
+
+
+def test_gaussian_and_jitter_kernel():
+    """utils/math.py:20-67 with scipy's normal pdf; ApplyJitter's kernel is
+    normalised, symmetric and narrows with sigma."""
+    from scipy.stats import norm
+    g = orc.gaussian(0.0, [0.7, 1.3], 11)
+    ax = np.linspace(-5, 5, 11)
+    ref = np.outer(norm.pdf(ax, scale=1.3), norm.pdf(ax, scale=0.7))   # rows = y
+    assert np.allclose(g, ref / ref.sum(), rtol=1e-12)
+    k = orc.jitter_kernel(0.8, 9, 3)
+    assert k.shape == (9, 9) and abs(k.sum() - 1) < 1e-12
+    assert np.allclose(k, k.T) and np.allclose(k, k[::-1, ::-1])
+    assert orc.jitter_kernel(0.3)[4, 4] > orc.jitter_kernel(1.5)[4, 4]
+
+
+def test_detector_chain_known_answers():
+    """The reference tests' detector inputs (tests/test_detectors.py:7,
+    tests/layers/test_detector_layers.py:16): a 16x16 image of ones."""
+    ones = np.ones((16, 16))
+    assert np.array_equal(orc.detector(ones, [("downsample", 4)]), np.full((4, 4), 16.0))
+    out = orc.detector(ones, [("add", 1.0), ("saturate", 1.5), ("pixel_response", 2 * ones)])
+    assert np.array_equal(out, np.full((16, 16), 3.0))
+    # a normalised kernel keeps the interior of a flat image
+    out = orc.detector(ones, [("convolve", orc.jitter_kernel(0.5, 5, 3))])
+    assert np.allclose(out[2:-2, 2:-2], 1.0)
+    assert out[0, 0] < 1.0
+
+
+def test_detector_vjp_adjoint_identity_and_differences():
+    """<D x, y> = <x, D^T y> for a linear chain; central differences through a
+    chain with a saturation (no input within h of the threshold)."""
+    rng = np.random.default_rng(5)
+    n = 24
+    resp = rng.uniform(0.5, 1.5, (n, n))
+    lin = [("pixel_response", resp), ("convolve", rng.normal(size=(4, 4))), ("add", 0.3),
+           ("downsample", 3), ("pixel_response", rng.uniform(0.5, 1.5, (8, 8)))]
+    x = rng.normal(size=(n, n))
+    y = rng.normal(size=(8, 8))
+    lhs = np.sum((orc.detector(x, lin) - orc.detector(np.zeros_like(x), lin)) * y)
+    rhs = np.sum(x * orc.detector_vjp(x, lin, y))
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+    chain = [("convolve", orc.jitter_kernel(0.9, 5, 3)), ("saturate", 0.6), ("downsample", 2)]
+    x = rng.uniform(0, 1, (12, 12))
+    g = rng.normal(size=(6, 6))
+    vjp = orc.detector_vjp(x, chain, g)
+    h = 1e-6
+    conv = orc.detector(x, chain[:1])
+    assert np.min(np.abs(conv - 0.6)) > 1e-3
+    for idx in [(0, 0), (5, 7), (11, 3), (6, 6)]:
+        e = np.zeros_like(x)
+        e[idx] = h
+        fd = np.sum((orc.detector(x + e, chain) - orc.detector(x - e, chain)) * g) / (2 * h)
+        assert abs(fd - vjp[idx]) < 1e-7
