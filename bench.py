@@ -46,6 +46,7 @@ def _parse():
                     help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--no-detector", action="store_true", help="skip the detector-chain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient leg")
     ap.add_argument("--calibrate", action="store_true",
@@ -215,6 +216,62 @@ def _asm_leg(torch, runtime, steps, warmup=3):
             "gflops": flops / (ms * 1e-3) / 1e9}
 
 
+def _detector_leg(torch, steps, check, warmup=3):
+    """Detector chain over a stack of oversampled PSFs: 64 frames of 1024^2
+    (4x-oversampled 256^2 detector images) through ApplyJitter(sigma 0.5 px,
+    9 x 9) -> ApplyPixelResponse -> Downsample(4) -> ApplySaturation ->
+    AddConstant.  The chain runs as one fused launch (separable convolution,
+    pixel response, block sums, saturation and constant in one pass)."""
+    import dlux_amd as dl
+    from dlux_amd import layers
+    frames, n, k = 64, 1024, 4
+    g = torch.Generator(device="cuda").manual_seed(11)
+    data = torch.rand((frames, n, n), generator=g, device="cuda")
+    resp = 0.9 + 0.2 * torch.rand((n, n), generator=g, device="cuda")
+    det = dl.LayeredDetector([layers.ApplyJitter(0.5, 9, 3), layers.ApplyPixelResponse(resp),
+                              layers.Downsample(k), layers.ApplySaturation(14.0),
+                              layers.AddConstant(0.01)])
+    psf = dl.PSF(data, 1.0)
+    for _ in range(warmup):
+        det(psf)
+    torch.cuda.synchronize()
+    beg, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    beg.record()
+    for _ in range(steps):
+        out = det(psf)
+    end.record()
+    torch.cuda.synchronize()
+    ms = beg.elapsed_time(end) / steps
+    ops = [layer._det_op() for layer in det.layers.values()]
+    dout = torch.ones_like(out)
+    from dlux_amd import runtime
+    for _ in range(warmup):
+        runtime.detector_vjp(data, ops, dout)
+    torch.cuda.synchronize()
+    beg.record()
+    for _ in range(steps):
+        runtime.detector_vjp(data, ops, dout)
+    end.record()
+    torch.cuda.synchronize()
+    ms_vjp = beg.elapsed_time(end) / steps
+    # read every frame and the response map once, write the detector images
+    nbytes = frames * n * n * 4 + n * n * 4 + frames * (n // k) ** 2 * 4
+    res = {"workload": "detector: 64 frames of 1024^2 float32 -> jitter (9x9, sigma 0.5 px) "
+                       "-> pixel response -> downsample 4 -> saturation -> constant",
+           "ms_per_call": ms, "frames_per_s": frames / (ms * 1e-3),
+           "algorithmic_bytes": nbytes, "achieved_GBps": nbytes / (ms * 1e-3) / 1e9,
+           "frac_of_hbm_peak": nbytes / (ms * 1e-3) / 8e12, "vjp_ms_per_call": ms_vjp,
+           "max_rel_err_vs_fp64": None}
+    if check:
+        from oracle import dlux_oracle as orc
+        chain = [(kind, np.asarray(arg.cpu() if hasattr(arg, "cpu") else arg)
+                  if kind == "pixel_response" else arg) for kind, arg in ops]
+        ref = orc.detector(data[0].double().cpu().numpy(), chain)
+        res["max_rel_err_vs_fp64"] = float(np.max(np.abs(out[0].double().cpu().numpy() - ref))
+                                           / np.max(np.abs(ref)))
+    return res
+
+
 def _serving_leg(torch, rates, duration):
     """Poisson request stream (75% PSF, 20% gradient, 5% ASM) at each rate."""
     from dlux_amd import serving
@@ -301,6 +358,9 @@ def main():
 
     print("[bench] angular-spectrum leg", file=sys.stderr, flush=True)
     asm = _asm_leg(torch, runtime, max(args.steps // 5, 10))
+    print("[bench] detector leg", file=sys.stderr, flush=True)
+    detector = None if args.no_detector else _detector_leg(torch, max(args.steps // 5, 10),
+                                                           not args.no_cpu)
     print("[bench] serving leg", file=sys.stderr, flush=True)
     serve = None if args.no_serving else _serving_leg(torch, args.serving_rates,
                                                       args.serving_seconds)
@@ -356,6 +416,7 @@ def main():
         "grad": grad,
         "batched": batched,
         "asm": asm,
+        "detector": detector,
         "serving": serve,
     }
     print(json.dumps(line))

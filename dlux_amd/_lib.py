@@ -73,6 +73,7 @@ DLX_DET_CONVOLVE = 2
 DLX_DET_DOWNSAMPLE = 3
 DLX_DET_SATURATE = 4
 DLX_DET_ADD_CONSTANT = 5
+DLX_DET_CONVOLVE_SEPARABLE = 6
 
 
 class DetectorOp(ctypes.Structure):

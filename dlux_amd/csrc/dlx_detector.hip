@@ -14,6 +14,10 @@
 //             "same", as PSF.convolve calls it, psfs.py:93-110); run as a
 //             correlation with the flipped kernel starting at row y + o-(K-1).
 //   downsample out[Y][X] = sum of the k x k block (utils/array_ops.py:124-162).
+//   A separable kernel w[ky][kx] = wy[ky] wx[kx] (ApplyJitter's Gaussian) runs
+//   as a horizontal then a vertical pass through LDS: 2K instead of K^2
+//   multiply-adds per pixel, which keeps the pass HBM-bound.
+//   A convolve followed by a downsample runs as one pass (det_conv_down_kernel).
 //
 // The vector-Jacobian product re-runs the chain keeping each pass's input and
 // its pre-store values, then walks the passes backwards: the store-side ops'
@@ -53,6 +57,52 @@ __device__ __forceinline__ float ew_apply(const EwList& l, float v, long p) {
   return v;
 }
 
+// Index of the first pixel-response op of l (-1 if none).
+__device__ __forceinline__ int first_map(const EwList& l) {
+  for (int i = 0; i < l.n; ++i)
+    if (l.kind[i] == DLX_DET_PIXEL_RESPONSE) return i;
+  return -1;
+}
+
+// ew_apply with the map value of op `pi` already loaded (mv).
+__device__ __forceinline__ float ew_apply_pf(const EwList& l, float v, long p, int pi,
+                                             float mv) {
+  for (int i = 0; i < l.n; ++i) {
+    const int k = l.kind[i];
+    if (k == DLX_DET_PIXEL_RESPONSE) {
+      v *= i == pi ? mv : l.map[i][p];
+    } else if (k == DLX_DET_SATURATE) {
+      v = v > l.value[i] ? l.value[i] : v;
+    } else {
+      v += l.value[i];
+    }
+  }
+  return v;
+}
+
+// The four map values of op `pi` at (y, x0..x0+3), clamped into the n x n map;
+// one 16-byte load when the four are in the image and aligned.
+__device__ __forceinline__ void load_map4(const EwList& l, int pi, int n, int y, int x0,
+                                          float (&mv)[4]) {
+  if (pi < 0) {
+    mv[0] = mv[1] = mv[2] = mv[3] = 1.f;
+    return;
+  }
+  const float* m = l.map[pi];
+  y = min(y, n - 1);
+  const long row = (long)y * n;
+  if (n % 4 == 0 && x0 + 3 < n && ((uintptr_t)m & 15) == 0) {
+    const float4 v = *(const float4*)(m + row + x0);
+    mv[0] = v.x;
+    mv[1] = v.y;
+    mv[2] = v.z;
+    mv[3] = v.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) mv[j] = m[row + min(x0 + j, n - 1)];
+  }
+}
+
 // d(ops of l)(v)/dv.  jnp.minimum splits its derivative 1/2, 1/2 at a tie.
 __device__ __forceinline__ float ew_deriv(const EwList& l, float v, long p) {
   float d = 1.f;
@@ -73,123 +123,430 @@ __device__ __forceinline__ float ew_deriv(const EwList& l, float v, long p) {
   return d;
 }
 
-// out[y][x] = post( sum_{ky,kx} pre(src[y + lo + ky][x + lo + kx]) * w'[ky][kx] )
-//             [* dpre'(aux[y][x])]
-// with w' = w flipped in both axes when flip != 0, and zero outside the image.
-// One workgroup per 64 x 16 tile and image (blockIdx.y = image).
+// Tile kernels: 256 threads, 64 output columns (4 consecutive per thread)
+// and 16 * RPT output rows (RPT rows per thread, 16 apart).
+// LDS floats: weights, the (16 RPT + K - 1) x (64 + K - 1) input tile and, for
+// separable weights, the (16 RPT + K - 1) x 64 horizontal-pass rows, which
+// start at a 16-byte boundary (float4 accesses).
+// Tile row stride: at least 64 + K - 1 columns, a multiple of 4, and wide
+// enough for every thread's float4 window of K + 3 columns.
+__host__ __device__ inline int conv_tile_w(int K) { return kTileX - 4 + 4 * ((K + 6) / 4); }
+
+// Offset of the input tile: after the weights, at a 16-byte boundary.
+__host__ __device__ inline int conv_tile_offset(int K, int sep) {
+  return ((sep ? 2 * K : K * K) + 3) / 4 * 4;
+}
+
+__host__ __device__ inline int conv_hrow_offset(int K, int rpt) {
+  return conv_tile_offset(K, 1) + (kTileY * rpt + K - 1) * conv_tile_w(K);
+}
+
+__host__ __device__ inline int conv_lds_floats(int K, int sep, int rpt) {
+  const int TH = kTileY * rpt + K - 1;
+  return sep ? conv_hrow_offset(K, rpt) + TH * kTileX
+             : conv_tile_offset(K, 0) + TH * conv_tile_w(K);
+}
+
+// Rows per thread of the tile kernels: 2 (32-row tiles, less halo per output)
+// when the tile fits the 64 KiB of LDS a workgroup may allocate, else 1.
+inline int conv_rpt(int K, int sep) {
+  return conv_lds_floats(K, sep, 2) * sizeof(float) <= 65536 ? 2 : 1;
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (hardware id % 8).  Remap
+// the (tile, image) pairs so each XCD takes a contiguous run of them: the halo
+// rows that vertically adjacent tiles share then stay in one L2.
+__device__ __forceinline__ void xcd_tile(int& tile, int& image) {
+  const int nx = gridDim.x, total = nx * gridDim.y;
+  int id = blockIdx.y * nx + blockIdx.x;
+  if (total % 8 == 0) id = (id % 8) * (total / 8) + id / 8;
+  tile = id % nx;
+  image = id / nx;
+}
+
+// Kernels of up to kWinK taps read each thread's K + 3 tile values as float4s
+// into registers (one 16-byte LDS read per four values) and run the taps
+// fully unrolled.
+constexpr int kWinK = 13;
+constexpr int kWin = kWinK + 3;
+
+__device__ __forceinline__ void load_window(const float* row, int K, float (&a)[kWin]) {
+  const float4* r4 = (const float4*)row;   // 16-byte aligned: c0 and the row stride are x4
+#pragma unroll
+  for (int q = 0; q < kWin / 4; ++q) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (4 * q < K + 3) v = r4[q];          // stays inside the row stride
+    a[4 * q] = v.x;
+    a[4 * q + 1] = v.y;
+    a[4 * q + 2] = v.z;
+    a[4 * q + 3] = v.w;
+  }
+}
+
+// The 4 x RPT correlation outputs of this thread for the tile at (ty0, tx0):
+//   acc[u][j] = sum_{ky,kx} pre(src[ty0 + row + 16u + lo + ky][tx0 + c0 + j + lo + kx])
+//               * w'[ky][kx]
+// (zero outside the image; w' = w flipped in both axes when flip != 0).  With
+// sep != 0, w holds K column weights then K row weights (w[ky][kx] = wy[ky] wx[kx]):
+// a horizontal pass over every tile row into LDS, then a vertical pass.
+// Ends with the input tile no longer read by any thread of the workgroup.
+template <int RPT>
+__device__ __forceinline__ void conv_tile(const float* __restrict__ s, int n, int ty0, int tx0,
+                                          int K, int lo, const float* __restrict__ w, int flip,
+                                          int sep, const EwList& pre, float* lds, int row,
+                                          int c0, float (&acc)[RPT][4]) {
+  const int TW = conv_tile_w(K), TH = kTileY * RPT + K - 1;
+  float* wl = lds;
+  float* tl = lds + conv_tile_offset(K, sep);
+  // Thread t loads tile columns t % 64 and t % 64 + 64 (when < TW) of rows
+  // t / 64, t / 64 + 4, ...: twelve rows of loads in flight (every row of a
+  // tile up to 48 rows high), addresses clamped into the image (every load
+  // valid) and out-of-image elements zeroed after.  The first batch is issued
+  // before the weights are staged, so the two latencies overlap.
+  constexpr int kRowStep = kThreads / kTileX;   // 4
+  constexpr int kRows = 12;
+  const int cx = threadIdx.x % kTileX;
+  const int r0 = threadIdx.x / kTileX;
+  const int gx0 = tx0 + lo + cx, gx1 = gx0 + kTileX;
+  const bool has1 = cx + kTileX < TW;
+  const bool okx0 = gx0 >= 0 && gx0 < n;
+  const bool okx1 = has1 && gx1 >= 0 && gx1 < n;
+  const int cx0 = min(max(gx0, 0), n - 1), cx1 = min(max(gx1, 0), n - 1);
+  for (int rb = r0; rb < TH; rb += kRows * kRowStep) {
+    float v0[kRows], v1[kRows];
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) {
+      const int gy = min(max(ty0 + lo + rb + u * kRowStep, 0), n - 1);
+      v0[u] = s[(long)gy * n + cx0];
+      v1[u] = s[(long)gy * n + cx1];
+    }
+    if (rb == r0) {
+      if (sep) {
+        for (int i = threadIdx.x; i < 2 * K; i += kThreads) {
+          const int seg = i / K, j = i - seg * K;
+          wl[i] = w[seg * K + (flip ? K - 1 - j : j)];
+        }
+      } else {
+        for (int i = threadIdx.x; i < K * K; i += kThreads) {
+          const int ky = i / K, kx = i - ky * K;
+          wl[i] = flip ? w[(K - 1 - ky) * K + (K - 1 - kx)] : w[i];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) {
+      const int r = rb + u * kRowStep;
+      if (r >= TH) break;
+      const int gy = ty0 + lo + r;
+      const bool oky = gy >= 0 && gy < n;
+      const long p = (long)gy * n;
+      tl[r * TW + cx] = oky && okx0 ? ew_apply(pre, v0[u], p + gx0) : 0.f;
+      if (has1) tl[r * TW + cx + kTileX] = oky && okx1 ? ew_apply(pre, v1[u], p + gx1) : 0.f;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) acc[u][0] = acc[u][1] = acc[u][2] = acc[u][3] = 0.f;
+  if (sep) {
+    const float* wy = wl;
+    const float* wx = wl + K;
+    float* hl = lds + conv_hrow_offset(K, RPT);   // TH x 64
+    if (K <= kWinK) {
+      for (int r = row; r < TH; r += kTileY) {
+        float a[kWin];
+        load_window(tl + r * TW + c0, K, a);
+        float h[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kx = 0; kx < kWinK; ++kx) {
+          if (kx < K) {
+            const float wv = wx[kx];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[j] += a[kx + j] * wv;
+          }
+        }
+        *(float4*)(hl + r * kTileX + c0) = make_float4(h[0], h[1], h[2], h[3]);
+      }
+    } else {
+      for (int r = row; r < TH; r += kTileY) {
+        const float* trow = tl + r * TW + c0;   // columns c0 .. c0+K+2 < TW
+        float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f;
+        float a0 = trow[0], a1 = trow[1], a2 = trow[2];
+#pragma unroll 4
+        for (int kx = 0; kx < K; ++kx) {
+          const float a3 = trow[kx + 3];
+          const float wv = wx[kx];
+          h0 += a0 * wv;
+          h1 += a1 * wv;
+          h2 += a2 * wv;
+          h3 += a3 * wv;
+          a0 = a1;
+          a1 = a2;
+          a2 = a3;
+        }
+        *(float4*)(hl + r * kTileX + c0) = make_float4(h0, h1, h2, h3);
+      }
+    }
+    __syncthreads();
+    for (int ky = 0; ky < K; ++ky) {
+      const float wv = wy[ky];
+#pragma unroll
+      for (int u = 0; u < RPT; ++u) {
+        const float4 h = *(const float4*)(hl + (row + kTileY * u + ky) * kTileX + c0);
+        acc[u][0] += h.x * wv;
+        acc[u][1] += h.y * wv;
+        acc[u][2] += h.z * wv;
+        acc[u][3] += h.w * wv;
+      }
+    }
+  } else if (K <= kWinK) {
+    for (int ky = 0; ky < K; ++ky) {
+      const float* wrow = wl + ky * K;
+#pragma unroll
+      for (int u = 0; u < RPT; ++u) {
+        float a[kWin];
+        load_window(tl + (row + kTileY * u + ky) * TW + c0, K, a);
+#pragma unroll
+        for (int kx = 0; kx < kWinK; ++kx) {
+          if (kx < K) {
+            const float wv = wrow[kx];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[u][j] += a[kx + j] * wv;
+          }
+        }
+      }
+    }
+  } else {
+    for (int ky = 0; ky < K; ++ky) {
+      const float* wrow = wl + ky * K;
+#pragma unroll
+      for (int u = 0; u < RPT; ++u) {
+        const float* trow = tl + (row + kTileY * u + ky) * TW + c0;   // cols < c0+K+3 <= TW
+        float a0 = trow[0], a1 = trow[1], a2 = trow[2];
+#pragma unroll 4
+        for (int kx = 0; kx < K; ++kx) {
+          const float a3 = trow[kx + 3];
+          const float wv = wrow[kx];
+          acc[u][0] += a0 * wv;
+          acc[u][1] += a1 * wv;
+          acc[u][2] += a2 * wv;
+          acc[u][3] += a3 * wv;
+          a0 = a1;
+          a1 = a2;
+          a2 = a3;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// out[y][x] = post( correlation of pre(src) with w' at (y, x) )  [* dpre'(aux[y][x])]
+// One workgroup per 64 x 16 RPT tile and image (blockIdx.y = image).
+template <int RPT>
 __global__ void __launch_bounds__(kThreads)
 det_corr_kernel(const float* __restrict__ src, int n, int tiles_x, int K, int lo,
-                const float* __restrict__ w, int flip, EwList pre, EwList post,
+                const float* __restrict__ w, int flip, int sep, EwList pre, EwList post,
                 const float* __restrict__ aux, EwList dpre, float* __restrict__ r_save,
                 float* __restrict__ out) {
   extern __shared__ float lds[];
-  const int TW = kTileX + K - 1, TH = kTileY + K - 1;
-  float* wl = lds;           // K*K weights
-  float* tl = lds + K * K;   // TH x TW input tile
-  const long img = (long)n * n;
-  const long base = (long)blockIdx.y * img;
-  const int ty0 = (blockIdx.x / tiles_x) * kTileY;
-  const int tx0 = (blockIdx.x % tiles_x) * kTileX;
-  const float* s = src + base;
-  for (int i = threadIdx.x; i < K * K; i += kThreads) {
-    const int ky = i / K, kx = i - ky * K;
-    wl[i] = flip ? w[(K - 1 - ky) * K + (K - 1 - kx)] : w[i];
-  }
-  for (int i = threadIdx.x; i < TH * TW; i += kThreads) {
-    const int yy = i / TW, xx = i - yy * TW;
-    const int gy = ty0 + lo + yy, gx = tx0 + lo + xx;
-    float v = 0.f;
-    if (gy >= 0 && gy < n && gx >= 0 && gx < n) {
-      const long p = (long)gy * n + gx;
-      v = ew_apply(pre, s[p], p);
-    }
-    tl[i] = v;
-  }
-  __syncthreads();
+  int tile, image;
+  xcd_tile(tile, image);
+  const long base = (long)image * n * n;
+  const int ty0 = (tile / tiles_x) * (kTileY * RPT);
+  const int tx0 = (tile % tiles_x) * kTileX;
   const int row = threadIdx.x / (kTileX / 4);
   const int c0 = (threadIdx.x % (kTileX / 4)) * 4;
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int ky = 0; ky < K; ++ky) {
-    const float* trow = tl + (row + ky) * TW + c0;   // columns c0 .. c0+K+2 < TW
-    const float* wrow = wl + ky * K;
-    float a0 = trow[0], a1 = trow[1], a2 = trow[2];
-    for (int kx = 0; kx < K; ++kx) {
-      const float a3 = trow[kx + 3];
-      const float wv = wrow[kx];
-      acc[0] += a0 * wv;
-      acc[1] += a1 * wv;
-      acc[2] += a2 * wv;
-      acc[3] += a3 * wv;
-      a0 = a1;
-      a1 = a2;
-      a2 = a3;
+  // the store-side response map is loaded before the tile, so its latency
+  // overlaps the tile loads
+  const int pi = first_map(post);
+  float mv[RPT][4];
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) load_map4(post, pi, n, ty0 + row + kTileY * u, tx0 + c0, mv[u]);
+  float acc[RPT][4];
+  conv_tile<RPT>(src + base, n, ty0, tx0, K, lo, w, flip, sep, pre, lds, row, c0, acc);
+  // four consecutive outputs go out as one 16-byte store where they can
+  const bool vec = n % 4 == 0 && (((uintptr_t)out | (uintptr_t)r_save) & 15) == 0;
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    const int y = ty0 + row + kTileY * u;
+    if (y >= n) break;
+    if (vec && tx0 + c0 + 3 < n) {
+      const long p = (long)y * n + tx0 + c0;
+      float4 v;
+      float* vv = &v.x;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        vv[j] = ew_apply_pf(post, acc[u][j], p + j, pi, mv[u][j]);
+        if (aux) vv[j] *= ew_deriv(dpre, aux[base + p + j], p + j);
+      }
+      if (r_save)
+        *(float4*)(r_save + base + p) = make_float4(acc[u][0], acc[u][1], acc[u][2], acc[u][3]);
+      *(float4*)(out + base + p) = v;
+      continue;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int x = tx0 + c0 + j;
+      if (x >= n) break;
+      const long p = (long)y * n + x;
+      const float r = acc[u][j];
+      if (r_save) r_save[base + p] = r;
+      float v = ew_apply_pf(post, r, p, pi, mv[u][j]);
+      if (aux) v *= ew_deriv(dpre, aux[base + p], p);
+      out[base + p] = v;
     }
   }
-  const int y = ty0 + row;
-  if (y >= n) return;
+}
+
+// Convolve followed by a block sum, in one pass: the convolution over a
+// th x tw tile (multiples of the block factor k, th <= 16 RPT, tw <= 64) with
+// the convolve pass's store-side ops, then the k x k block sums of that tile
+// out of LDS with the block-sum pass's store-side ops.  The intermediate image
+// is never written.  Blocks never straddle tiles (k divides th, tw and n).
+template <int RPT>
+__global__ void __launch_bounds__(kThreads)
+det_conv_down_kernel(const float* __restrict__ src, int n, int th, int tw, int tiles_x, int K,
+                     int lo, const float* __restrict__ w, int sep, EwList pre, EwList post,
+                     int k, int m, EwList post2, float* __restrict__ out) {
+  extern __shared__ float lds[];
+  int tile, image;
+  xcd_tile(tile, image);
+  const long base = (long)image * n * n;
+  const int ty0 = (tile / tiles_x) * th;
+  const int tx0 = (tile % tiles_x) * tw;
+  const int row = threadIdx.x / (kTileX / 4);
+  const int c0 = (threadIdx.x % (kTileX / 4)) * 4;
+  const int pi = first_map(post);   // prefetched as in det_corr_kernel
+  float mv[RPT][4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int x = tx0 + c0 + j;
-    if (x >= n) break;
-    const long p = (long)y * n + x;
-    const float r = acc[j];
-    if (r_save) r_save[base + p] = r;
-    float v = ew_apply(post, r, p);
-    if (aux) v *= ew_deriv(dpre, aux[base + p], p);
-    out[base + p] = v;
+  for (int u = 0; u < RPT; ++u) load_map4(post, pi, n, ty0 + row + kTileY * u, tx0 + c0, mv[u]);
+  float acc[RPT][4];
+  conv_tile<RPT>(src + base, n, ty0, tx0, K, lo, w, 1, sep, pre, lds, row, c0, acc);
+  float* ot = lds;   // (16 RPT) x 64 convolve outputs (the tile is no longer read)
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    const int rr = row + kTileY * u;
+    const int y = ty0 + rr;
+    float4 v;
+    float* vv = &v.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int x = tx0 + c0 + j;
+      vv[j] = 0.f;
+      if (rr < th && c0 + j < tw && y < n && x < n)
+        vv[j] = ew_apply_pf(post, acc[u][j], (long)y * n + x, pi, mv[u][j]);
+    }
+    *(float4*)(ot + rr * kTileX + c0) = v;
+  }
+  __syncthreads();
+  const int bw = tw / k, bh = th / k;
+  const long obase = (long)image * m * m;
+  for (int o = threadIdx.x; o < bw * bh; o += kThreads) {
+    const int oy = o / bw, ox = o - oy * bw;
+    const int gy = ty0 / k + oy, gx = tx0 / k + ox;
+    if (gy >= m || gx >= m) continue;
+    float a = 0.f;
+    for (int dy = 0; dy < k; ++dy) {
+      const float* orow = ot + (oy * k + dy) * kTileX + ox * k;
+      float rs = 0.f;
+      if (k % 4 == 0) {   // 16-byte aligned groups of four
+        for (int dx = 0; dx < k; dx += 4) {
+          const float4 q = *(const float4*)(orow + dx);
+          rs += ((q.x + q.y) + q.z) + q.w;
+        }
+      } else {
+        for (int dx = 0; dx < k; ++dx) rs += orow[dx];
+      }
+      a += rs;
+    }
+    const long q = (long)gy * m + gx;
+    out[obase + q] = ew_apply(post2, a, q);
   }
 }
 
 // out[b][Y][X] = post( sum_{dy,dx < k} pre(src[b][Y k + dy][X k + dx]) ), m = n / k.
 // Row sums first, then the sum of the rows (the order of utils/array_ops.py:151-161).
+// Grid: (pixel blocks of the output image, image); in-image indices are 32-bit.
 __global__ void __launch_bounds__(256)
-det_down_kernel(const float* __restrict__ src, int n, int k, int m, long total, EwList pre,
-                EwList post, float* __restrict__ r_save, float* __restrict__ out) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= total) return;
-  const long mm = (long)m * m;
-  const long b = i / mm;
-  const long q = i - b * mm;
-  const int oy = (int)(q / m), ox = (int)(q - (long)oy * m);
-  const float* s = src + b * (long)n * n;
-  float acc = 0.f;
-  for (int dy = 0; dy < k; ++dy) {
-    const long rp = (long)(oy * k + dy) * n + (long)ox * k;
-    float rs = 0.f;
-    for (int dx = 0; dx < k; ++dx) rs += ew_apply(pre, s[rp + dx], rp + dx);
-    acc += rs;
+det_down_kernel(const float* __restrict__ src, int n, int k, int m, EwList pre, EwList post,
+                float* __restrict__ r_save, float* __restrict__ out) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= m * m) return;
+  const float* s = src + (long)blockIdx.y * n * n;
+  const long o = (long)blockIdx.y * m * m + q;
+  float acc;
+  if (k == 1) {
+    acc = ew_apply(pre, s[q], q);
+  } else {
+    const int oy = q / m, ox = q - oy * m;
+    acc = 0.f;
+    for (int dy = 0; dy < k; ++dy) {
+      const int rp = (oy * k + dy) * n + ox * k;
+      float rs = 0.f;
+      for (int dx = 0; dx < k; ++dx) rs += ew_apply(pre, s[rp + dx], rp + dx);
+      acc += rs;
+    }
   }
-  if (r_save) r_save[i] = acc;
-  out[i] = ew_apply(post, acc, q);
+  if (r_save) r_save[o] = acc;
+  out[o] = ew_apply(post, acc, q);
+}
+
+// Elementwise pass, four pixels per thread (npix % 4 == 0: float4 aligned):
+// out = post(pre(src)), r_save = pre(src).
+__global__ void __launch_bounds__(256)
+det_ew_kernel(const float* __restrict__ src, int npix, EwList pre, EwList post,
+              float* __restrict__ r_save, float* __restrict__ out) {
+  const int q = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (q >= npix) return;
+  const long o = (long)blockIdx.y * npix + q;
+  const float4 v = *(const float4*)(src + o);
+  float4 r;
+  r.x = ew_apply(pre, v.x, q);
+  r.y = ew_apply(pre, v.y, q + 1);
+  r.z = ew_apply(pre, v.z, q + 2);
+  r.w = ew_apply(pre, v.w, q + 3);
+  if (r_save) *(float4*)(r_save + o) = r;
+  float4 w;
+  w.x = ew_apply(post, r.x, q);
+  w.y = ew_apply(post, r.y, q + 1);
+  w.z = ew_apply(post, r.z, q + 2);
+  w.w = ew_apply(post, r.w, q + 3);
+  *(float4*)(out + o) = w;
 }
 
 // Adjoint of the block sum: out[b][y][x] = g[b][y/k][x/k] * dpre'(aux[b][y][x]).
 __global__ void __launch_bounds__(256)
-det_up_kernel(const float* __restrict__ g, int n, int k, int m, long total,
-              const float* __restrict__ aux, EwList dpre, float* __restrict__ out) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= total) return;
-  const long nn = (long)n * n;
-  const long b = i / nn;
-  const long p = i - b * nn;
-  const int y = (int)(p / n), x = (int)(p - (long)y * n);
-  float v = g[b * (long)m * m + (long)(y / k) * m + x / k];
+det_up_kernel(const float* __restrict__ g, int n, int k, int m, const float* __restrict__ aux,
+              EwList dpre, float* __restrict__ out) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n * n) return;
+  const long i = (long)blockIdx.y * n * n + p;
+  const long gb = (long)blockIdx.y * m * m;
+  float v;
+  if (k == 1) {
+    v = g[gb + p];
+  } else {
+    const int y = p / n, x = p - y * n;
+    v = g[gb + (y / k) * m + x / k];
+  }
   out[i] = v * ew_deriv(dpre, aux[i], p);
 }
 
-// out[i] = g[i] * post'(r[i]), elementwise (in place allowed).
+// out = g * post'(r), elementwise over (pixel block, image) (in place allowed).
 __global__ void __launch_bounds__(256)
-det_ew_grad_kernel(const float* g, const float* __restrict__ r, long npix, long total,
-                   EwList post, float* out) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= total) return;
-  out[i] = g[i] * ew_deriv(post, r[i], i % npix);
+det_ew_grad_kernel(const float* g, const float* __restrict__ r, int npix, EwList post,
+                   float* out) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  const long i = (long)blockIdx.y * npix + p;
+  out[i] = g[i] * ew_deriv(post, r[i], p);
 }
 
 struct Pass {
   int conv;        // 1: convolve, 0: block sum (k = 1: elementwise only)
   int n_in, n_out;
   int K;           // convolve kernel side
+  int sep;         // 1: w holds K column then K row weights
   const float* w;  // convolve kernel (device)
   int k;           // block-sum factor
   EwList pre, post;
@@ -241,12 +598,14 @@ int plan(int n, const dlx_detector_op* ops, int nops, std::vector<Pass>& passes)
         }
         break;
       }
-      case DLX_DET_CONVOLVE: {
+      case DLX_DET_CONVOLVE:
+      case DLX_DET_CONVOLVE_SEPARABLE: {
         DLX_CHECK_ARG(op.data, "convolution %d has no kernel", i);
         DLX_CHECK_ARG(op.size >= 1 && op.size <= kMaxKernel,
                       "convolution kernel side must be in [1, %d], got %d", kMaxKernel, op.size);
         Pass p = make_pass(1, cur);
         p.K = op.size;
+        p.sep = op.kind == DLX_DET_CONVOLVE_SEPARABLE;
         p.w = op.data;
         if (passes.empty()) {
           p.pre = pending;
@@ -326,10 +685,9 @@ Layout layout(const std::vector<Pass>& passes, int batch, bool for_vjp) {
 
 int check_grid(const std::vector<Pass>& passes, int batch) {
   DLX_CHECK_ARG(batch <= 65535, "detector batch must be <= 65535, got %d", batch);
-  for (const Pass& p : passes) {
-    const long total = (long)std::max(p.n_in, p.n_out) * std::max(p.n_in, p.n_out) * batch;
-    DLX_CHECK_ARG(total / 256 < (1L << 31) - 1, "detector images too large for one launch");
-  }
+  // in-image pixel indices are 32-bit
+  DLX_CHECK_ARG(passes.front().n_in <= 46340, "detector image side must be <= 46340, got %d",
+                passes.front().n_in);
   return DLX_OK;
 }
 
@@ -337,20 +695,59 @@ int check_grid(const std::vector<Pass>& passes, int batch) {
 int run_pass(const Pass& p, int batch, const float* src, float* r_save, float* dst,
              hipStream_t st) {
   if (p.conv) {
+    const int rpt = conv_rpt(p.K, p.sep);
     const int tiles_x = (p.n_in + kTileX - 1) / kTileX;
-    const int tiles_y = (p.n_in + kTileY - 1) / kTileY;
+    const int tiles_y = (p.n_in + kTileY * rpt - 1) / (kTileY * rpt);
     const int lo = (p.K - 1) / 2 - (p.K - 1);
-    const size_t lds = sizeof(float) * ((size_t)p.K * p.K +
-                                        (size_t)(kTileY + p.K - 1) * (kTileX + p.K - 1));
-    hipLaunchKernelGGL(det_corr_kernel, dim3(tiles_x * tiles_y, batch), dim3(kThreads), lds,
-                       st, src, p.n_in, tiles_x, p.K, lo, p.w, 1, p.pre, p.post,
-                       (const float*)nullptr, EwList{}, r_save, dst);
+    const size_t lds = sizeof(float) * (size_t)conv_lds_floats(p.K, p.sep, rpt);
+    const dim3 grid(tiles_x * tiles_y, batch);
+    if (rpt == 2)
+      hipLaunchKernelGGL(det_corr_kernel<2>, grid, dim3(kThreads), lds, st, src, p.n_in, tiles_x,
+                         p.K, lo, p.w, 1, p.sep, p.pre, p.post, (const float*)nullptr, EwList{},
+                         r_save, dst);
+    else
+      hipLaunchKernelGGL(det_corr_kernel<1>, grid, dim3(kThreads), lds, st, src, p.n_in, tiles_x,
+                         p.K, lo, p.w, 1, p.sep, p.pre, p.post, (const float*)nullptr, EwList{},
+                         r_save, dst);
     return launch_status("det_corr_kernel");
   }
-  const long total = (long)p.n_out * p.n_out * batch;
-  hipLaunchKernelGGL(det_down_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                     src, p.n_in, p.k, p.n_out, total, p.pre, p.post, r_save, dst);
+  const int npix = p.n_out * p.n_out;
+  const bool aligned = ((uintptr_t)src | (uintptr_t)dst | (uintptr_t)r_save) % 16 == 0;
+  if (p.k == 1 && npix % 4 == 0 && aligned) {
+    hipLaunchKernelGGL(det_ew_kernel, dim3((npix / 4 + 255) / 256, batch), dim3(256), 0, st, src,
+                       npix, p.pre, p.post, r_save, dst);
+    return launch_status("det_ew_kernel");
+  }
+  hipLaunchKernelGGL(det_down_kernel, dim3((npix + 255) / 256, batch), dim3(256), 0, st, src,
+                     p.n_in, p.k, p.n_out, p.pre, p.post, r_save, dst);
   return launch_status("det_down_kernel");
+}
+
+// A convolve pass followed by a block-sum pass whose factor fits the tile
+// height runs as one det_conv_down_kernel launch.
+bool can_fuse(const std::vector<Pass>& passes, size_t i) {
+  return passes[i].conv && i + 1 < passes.size() && !passes[i + 1].conv &&
+         passes[i + 1].k <= kTileY * conv_rpt(passes[i].K, passes[i].sep) &&
+         passes[i + 1].pre.n == 0;
+}
+
+int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, float* dst,
+                  hipStream_t st) {
+  const int rpt = conv_rpt(c.K, c.sep);
+  const int k = d.k;
+  const int th = (kTileY * rpt / k) * k, tw = (kTileX / k) * k;
+  const int tiles_x = (c.n_in + tw - 1) / tw;
+  const int tiles_y = (c.n_in + th - 1) / th;
+  const int lo = (c.K - 1) / 2 - (c.K - 1);
+  const size_t lds = sizeof(float) * (size_t)conv_lds_floats(c.K, c.sep, rpt);
+  const dim3 grid(tiles_x * tiles_y, batch);
+  if (rpt == 2)
+    hipLaunchKernelGGL(det_conv_down_kernel<2>, grid, dim3(kThreads), lds, st, src, c.n_in, th,
+                       tw, tiles_x, c.K, lo, c.w, c.sep, c.pre, c.post, k, d.n_out, d.post, dst);
+  else
+    hipLaunchKernelGGL(det_conv_down_kernel<1>, grid, dim3(kThreads), lds, st, src, c.n_in, th,
+                       tw, tiles_x, c.K, lo, c.w, c.sep, c.pre, c.post, k, d.n_out, d.post, dst);
+  return launch_status("det_conv_down_kernel");
 }
 
 }  // namespace
@@ -397,10 +794,18 @@ int dlx_detector_apply(int32_t n, int32_t batch, const float* in, const dlx_dete
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
   const float* src = in;
-  for (size_t i = 0; i < passes.size(); ++i) {
-    float* dst = i + 1 == passes.size() ? out : (float*)(ws + L.g_off[i & 1]);
-    if ((st = run_pass(passes[i], batch, src, nullptr, dst, s))) return st;
+  const size_t np = passes.size();
+  int buf = 0;
+  for (size_t i = 0; i < np;) {
+    const bool fuse = can_fuse(passes, i);
+    const size_t next = i + (fuse ? 2 : 1);
+    float* dst = next == np ? out : (float*)(ws + L.g_off[buf]);
+    buf ^= 1;
+    st = fuse ? run_conv_down(passes[i], passes[i + 1], batch, src, dst, s)
+              : run_pass(passes[i], batch, src, nullptr, dst, s);
+    if (st) return st;
     src = dst;
+    i = next;
   }
   return DLX_OK;
 }
@@ -440,33 +845,38 @@ int dlx_detector_vjp(int32_t n, int32_t batch, const float* in, const dlx_detect
   int cur_idx = -1;   // -1: cur is the caller's dout
   for (size_t ii = np; ii-- > 0;) {
     const Pass& p = passes[ii];
-    const long out_total = (long)p.n_out * p.n_out * batch;
     const float* gr = cur;
     int gr_idx = cur_idx;
     if (p.post.n > 0) {
       gr_idx = cur_idx < 0 ? 0 : cur_idx;
-      hipLaunchKernelGGL(det_ew_grad_kernel, dim3((unsigned)((out_total + 255) / 256)),
-                         dim3(256), 0, s, cur, (const float*)(ws + L.r_off[ii]),
-                         (long)p.n_out * p.n_out, out_total, p.post, g[gr_idx]);
+      const int npix = p.n_out * p.n_out;
+      hipLaunchKernelGGL(det_ew_grad_kernel, dim3((npix + 255) / 256, batch), dim3(256), 0, s,
+                         cur, (const float*)(ws + L.r_off[ii]), npix, p.post, g[gr_idx]);
       if ((st = launch_status("det_ew_grad_kernel"))) return st;
       gr = g[gr_idx];
     }
     const int dst_idx = gr_idx == 0 ? 1 : 0;
     float* dst = ii == 0 ? din : g[dst_idx];
     if (p.conv) {
+      const int rpt = conv_rpt(p.K, p.sep);
       const int tiles_x = (p.n_in + kTileX - 1) / kTileX;
-      const int tiles_y = (p.n_in + kTileY - 1) / kTileY;
+      const int tiles_y = (p.n_in + kTileY * rpt - 1) / (kTileY * rpt);
       const int lo = -((p.K - 1) / 2);
-      const size_t lds = sizeof(float) * ((size_t)p.K * p.K +
-                                          (size_t)(kTileY + p.K - 1) * (kTileX + p.K - 1));
-      hipLaunchKernelGGL(det_corr_kernel, dim3(tiles_x * tiles_y, batch), dim3(kThreads), lds,
-                         s, gr, p.n_in, tiles_x, p.K, lo, p.w, 0, EwList{}, EwList{}, x_in(ii),
-                         p.pre, (float*)nullptr, dst);
+      const size_t lds = sizeof(float) * (size_t)conv_lds_floats(p.K, p.sep, rpt);
+      const dim3 grid(tiles_x * tiles_y, batch);
+      if (rpt == 2)
+        hipLaunchKernelGGL(det_corr_kernel<2>, grid, dim3(kThreads), lds, s, gr, p.n_in, tiles_x,
+                           p.K, lo, p.w, 0, p.sep, EwList{}, EwList{}, x_in(ii), p.pre,
+                           (float*)nullptr, dst);
+      else
+        hipLaunchKernelGGL(det_corr_kernel<1>, grid, dim3(kThreads), lds, s, gr, p.n_in, tiles_x,
+                           p.K, lo, p.w, 0, p.sep, EwList{}, EwList{}, x_in(ii), p.pre,
+                           (float*)nullptr, dst);
       if ((st = launch_status("det_corr_kernel"))) return st;
     } else {
-      const long in_total = (long)p.n_in * p.n_in * batch;
-      hipLaunchKernelGGL(det_up_kernel, dim3((unsigned)((in_total + 255) / 256)), dim3(256), 0,
-                         s, gr, p.n_in, p.k, p.n_out, in_total, x_in(ii), p.pre, dst);
+      const int npix = p.n_in * p.n_in;
+      hipLaunchKernelGGL(det_up_kernel, dim3((npix + 255) / 256, batch), dim3(256), 0, s, gr,
+                         p.n_in, p.k, p.n_out, x_in(ii), p.pre, dst);
       if ((st = launch_status("det_up_kernel"))) return st;
     }
     cur = dst;

@@ -551,13 +551,15 @@ DETECTOR_KINDS = {
     "downsample": _lib.DLX_DET_DOWNSAMPLE,
     "saturate": _lib.DLX_DET_SATURATE,
     "add": _lib.DLX_DET_ADD_CONSTANT,
+    "convolve_separable": _lib.DLX_DET_CONVOLVE_SEPARABLE,
 }
 
 
 def _detector_ops(ops, dev):
     """[(kind name, array or scalar)] -> (dlx_detector_op array, tensors kept alive).
-    pixel_response / convolve take a 2-D array, downsample an int factor,
-    saturate / add a scalar."""
+    pixel_response / convolve take a 2-D array, convolve_separable a pair
+    (column factor, row factor) of 1-D arrays of one length, downsample an int
+    factor, saturate / add a scalar."""
     arr = (_lib.DetectorOp * max(len(ops), 1))()
     keep = []
     for i, (kind, arg) in enumerate(ops):
@@ -571,6 +573,14 @@ def _detector_ops(ops, dev):
                 raise ValueError(f"{kind} needs a square 2d array, got {tuple(t.shape)}")
             keep.append(t)
             op.size = int(t.shape[0])
+            op.data = t.data_ptr()
+        elif kind == "convolve_separable":
+            wy, wx = (as_float(a, dev).reshape(-1) for a in arg)
+            if wy.numel() != wx.numel():
+                raise ValueError("convolve_separable factors must have the same length")
+            t = torch.cat([wy, wx]).contiguous()
+            keep.append(t)
+            op.size = int(wy.numel())
             op.data = t.data_ptr()
         elif kind == "downsample":
             op.size = int(arg)
@@ -651,7 +661,9 @@ def detector(images, ops) -> torch.Tensor:
     """The detector chain `ops` applied to float32 images (..., n, n), as one
     dlx_detector_apply call; differentiable in `images` (dlx_detector_vjp)."""
     for kind, arg in ops:
-        if isinstance(arg, torch.Tensor) and arg.requires_grad and torch.is_grad_enabled():
+        args = arg if isinstance(arg, (tuple, list)) else (arg,)
+        if torch.is_grad_enabled() and any(isinstance(a, torch.Tensor) and a.requires_grad
+                                           for a in args):
             raise NotImplementedError(
                 f"gradients with respect to the {kind} array are not implemented")
     if (torch.is_grad_enabled() and isinstance(images, torch.Tensor)

@@ -305,6 +305,10 @@ int dlx_asm_prop(int32_t n, int32_t m, int32_t batch, const double* wavelength,
  *                           output index i reads input i + (size-1)/2 - k
  *                           ApplyJitter / PSF.convolve, layers/detector_layers.py:
  *                           134-199, psfs.py:93-110
+ *   DLX_DET_CONVOLVE_SEPARABLE  as DLX_DET_CONVOLVE with kernel[ky][kx] =
+ *                           data[ky] * data[size + kx] (2*size floats: the column
+ *                           then the row factor), computed in two 1-D passes
+ *                           (ApplyJitter's Gaussian kernel is such an outer product)
  *   DLX_DET_DOWNSAMPLE      size x size block sums (side must divide by size)
  *                           Downsample / PSF.downsample,
  *                           layers/detector_layers.py:262-293, psfs.py:74-91
@@ -323,7 +327,8 @@ enum dlx_detector_kind {
   DLX_DET_CONVOLVE = 2,
   DLX_DET_DOWNSAMPLE = 3,
   DLX_DET_SATURATE = 4,
-  DLX_DET_ADD_CONSTANT = 5
+  DLX_DET_ADD_CONSTANT = 5,
+  DLX_DET_CONVOLVE_SEPARABLE = 6
 };
 
 typedef struct dlx_detector_op {

@@ -604,6 +604,7 @@ def detector(psf, layers):
     """LayeredDetector.__call__ (detectors.py:103-128) over a list of layers:
     ("pixel_response", map)  psf * map                  detector_layers.py:130-131
     ("convolve", kernel)     convolve_same(psf, kernel) detector_layers.py:198-199
+    ("convolve_separable", (ky, kx))  the same with kernel = outer(ky, kx)
     ("downsample", k)        block sums                 detector_layers.py:292-293
     ("saturate", t)          minimum(psf, t)            detector_layers.py:228-229
     ("add", c)               psf + c                    detector_layers.py:258-259"""
@@ -614,6 +615,8 @@ def detector(psf, layers):
             x = x * np.asarray(layer[1], np.float64)
         elif kind == "convolve":
             x = convolve_same(x, layer[1])
+        elif kind == "convolve_separable":
+            x = convolve_same(x, np.outer(layer[1][0], layer[1][1]))
         elif kind == "downsample":
             x = downsample(x, int(layer[1]), mean=False)
         elif kind == "saturate":
@@ -639,6 +642,8 @@ def detector_vjp(psf, layers, g):
             g = g * np.asarray(layer[1], np.float64)
         elif kind == "convolve":
             g = correlate_same_adjoint(g, layer[1])
+        elif kind == "convolve_separable":
+            g = correlate_same_adjoint(g, np.outer(layer[1][0], layer[1][1]))
         elif kind == "downsample":
             k = int(layer[1])
             g = np.repeat(np.repeat(g, k, axis=0), k, axis=1)

@@ -66,6 +66,28 @@ def _chains(rng):
     cases.append(("identity", 17, 2, []))
     cases.append(("large", 1024, 2, [("convolve", orc.jitter_kernel(0.6, 9, 3)),
                                      ("downsample", 4), ("saturate", None)]))
+    # convolve + block sum fused with 15 x 60 tiles; then a factor too big to fuse
+    cases.append(("conv-down5", 80, 2, [("convolve", rng.normal(size=(6, 6))), ("add", 0.2),
+                                        ("downsample", 5), ("pixel_response",
+                                                            rng.uniform(0.5, 1.5, (16, 16)))]))
+    cases.append(("conv-down32", 96, 1, [("convolve", rng.normal(size=(3, 3))),
+                                         ("downsample", 32)]))
+    cases.append(("conv-down16-odd-tiles", 144, 1, [("convolve", rng.uniform(0, 1, (9, 9))),
+                                                    ("downsample", 16), ("saturate", None)]))
+    # separable kernels: unequal factors, alone, fused with a block sum, at the size limit
+    cases.append(("sep", 70, 2, [("pixel_response", rng.uniform(0.5, 1.5, (70, 70))),
+                                 ("convolve_separable", (rng.normal(size=7), rng.normal(size=7))),
+                                 ("add", 0.3)]))
+    cases.append(("sep-down3", 99, 2, [("convolve_separable", (rng.uniform(0, 1, 10),
+                                                               rng.uniform(0, 1, 10))),
+                                       ("saturate", None), ("downsample", 3)]))
+    cases.append(("sep-max", 40, 1, [("convolve_separable", (rng.normal(size=64),
+                                                             rng.normal(size=64)))]))
+    # the register-window limit (13 taps) and one past it, back to back
+    cases.append(("k13-k14", 50, 2, [("convolve_separable", (rng.normal(size=13),
+                                                             rng.normal(size=13))),
+                                     ("convolve", rng.normal(size=(14, 14))),
+                                     ("convolve", rng.normal(size=(13, 13)))]))
     return cases
 
 
@@ -85,7 +107,7 @@ def _oracle(x, chain):
     return np.stack([orc.detector(xi, chain) for xi in x])
 
 
-@pytest.mark.parametrize("case", range(9))
+@pytest.mark.parametrize("case", range(16))
 def test_detector_chain_and_vjp_vs_oracle(dl, case):
     rng = np.random.default_rng(40 + case)
     name, n, batch, chain = _chains(rng)[case]

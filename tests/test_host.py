@@ -282,8 +282,13 @@ def test_jitter_kernel_matches_oracle():
                        rtol=1e-6)
     for sigma, k, os_ in [(0.5, 9, 3), (1.7, 5, 1), (0.1, 7, 4)]:
         ref = orc.jitter_kernel(sigma, k, os_)
-        got = layers.ApplyJitter(sigma, k, os_).kernel
+        layer = layers.ApplyJitter(sigma, k, os_)
+        got = layer.kernel
         assert got.dtype == np.float32 and np.allclose(got, ref, rtol=1e-6, atol=1e-9)
+        # the separable form the HIP chain runs is the same kernel
+        f = layer._factor()
+        assert np.max(np.abs(np.outer(f, f) - ref)) < 2e-7 * ref.max()
+        assert layer._det_op()[0] == "convolve_separable"
 
 
 def test_binary_source_and_scene_host_logic():
