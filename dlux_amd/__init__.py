@@ -21,7 +21,8 @@ from .optical_systems import (BaseOpticalSystem, OpticalSystem, ParametricOptica
                               LayeredOpticalSystem, AngularOpticalSystem,
                               CartesianOpticalSystem)
 from .spectra import Spectrum
-from .sources import BaseSource, Source, PointSource, PointSources, BinarySource, Scene
+from .sources import (BaseSource, Source, PointSource, PointSources, BinarySource, ResolvedSource,
+                      PointResolvedSource, Scene)
 from .detectors import BaseDetector, LayeredDetector
 from .instruments import Instrument, Telescope, Dither
 

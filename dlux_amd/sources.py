@@ -1,6 +1,7 @@
-"""Sources (sources.py:27-850): point sources, binaries and scenes.
+"""Sources (sources.py:27-850): point sources, binaries, resolved sources and scenes.
 
-Every supported source reduces to point terms: for each star, its on-sky
+Resolved sources are a fused polychromatic PSF followed by the convolve pass
+of the HIP detector kernels.  Every other source reduces to point terms: for each star, its on-sky
 position and its spectral weights already multiplied by its flux.  A
 ``Scene`` concatenates the terms of its members, so a whole scene -- every
 (star, wavelength) instance, whatever wavelength grid each member uses --
@@ -19,7 +20,8 @@ from .psfs import PSF
 from .spectra import Spectrum
 from .utils.source import fluxes_from_contrast, positions_from_sep
 
-__all__ = ["BaseSource", "Source", "PointSource", "PointSources", "BinarySource", "Scene"]
+__all__ = ["BaseSource", "Source", "PointSource", "PointSources", "BinarySource",
+           "ResolvedSource", "PointResolvedSource", "Scene"]
 
 
 def _validate_return_mode(return_wf, return_psf):
@@ -224,6 +226,87 @@ class BinarySource(Source):
     def model(self, optics, return_wf: bool = False, return_psf: bool = False):
         _validate_return_mode(return_wf, return_psf)
         return _model_terms(optics, self._point_terms(), return_wf, return_psf)
+
+
+_NO_WF_MESSAGE = (
+    "Wavefront information cannot be preserved through convolution. "
+    "Convolution can only operate on PSFs (incoherent light). "
+    "Please use return_wf=False to get the PSF array or return_psf=True "
+    "to get a PSF object."
+)
+
+
+def _convolve_distribution(psf, distribution):
+    """convolve(psf, distribution, mode="same") on the device (the convolve
+    pass of the detector kernels; square distributions up to 64 px)."""
+    from . import runtime
+    d = np.asarray(distribution, np.float32)
+    if d.ndim != 2 or d.shape[0] != d.shape[1]:
+        raise NotImplementedError(
+            f"only square distributions are supported, got shape {d.shape}")
+    return runtime.detector(psf, [("convolve", d)])
+
+
+class ResolvedSource(PointSource):
+    """A point source's PSF convolved with a normalised intensity distribution
+    (sources.py:414-521)."""
+
+    def __init__(self, wavelengths=None, position=np.zeros(2), flux=1.0,
+                 distribution=np.ones((3, 3)), weights=None, spectrum=None):
+        distribution = np.asarray(distribution, dtype=float)
+        self.distribution = distribution / distribution.sum()
+        if self.distribution.ndim != 2:
+            raise ValueError("distribution must be a 2d array.")
+        super().__init__(position=position, flux=flux, spectrum=spectrum,
+                         wavelengths=wavelengths, weights=weights)
+
+    def normalise(self):
+        """Spectrum normalised; distribution floored at 0 and normalised (sources.py:475-488)."""
+        floor = np.maximum(self.distribution, 0.0)
+        return self.set(["spectrum", "distribution"],
+                        [self.spectrum.normalise(), floor / floor.sum()])
+
+    def _point_terms(self):
+        raise NotImplementedError("a resolved source has no point-source form")
+
+    def model(self, optics, return_wf: bool = False, return_psf: bool = False):
+        _validate_return_mode(return_wf, return_psf)
+        if return_wf:
+            raise NotImplementedError(_NO_WF_MESSAGE)
+        src = self.normalise()
+        psf = optics.propagate(src.wavelengths, src.position, src.weights * src.flux,
+                               return_psf=True)
+        conv = _convolve_distribution(psf.data, src.distribution)
+        return PSF(conv, float(np.mean(psf.pixel_scale))) if return_psf else conv
+
+
+class PointResolvedSource(ResolvedSource):
+    """A point source plus a resolved component about it sharing its spectrum:
+    flux split by contrast, each with its own spectral weights (sources.py:638-748)."""
+
+    def __init__(self, wavelengths=None, position=np.zeros(2), flux=1.0,
+                 distribution=np.ones((3, 3)), contrast=1.0, weights=None, spectrum=None):
+        wavelengths = _as_wavelengths_1d(wavelengths)
+        if weights is None:
+            weights = np.ones((2, _infer_n_wavelengths(wavelengths, spectrum)))
+        self.contrast = np.asarray(contrast, float)
+        super().__init__(wavelengths=wavelengths, position=position, flux=flux,
+                         distribution=distribution, spectrum=spectrum, weights=weights)
+
+    def model(self, optics, return_wf: bool = False, return_psf: bool = False):
+        _validate_return_mode(return_wf, return_psf)
+        if return_wf:
+            raise NotImplementedError(_NO_WF_MESSAGE)
+        src = self.normalise()
+        flux = fluxes_from_contrast(src.flux, src.contrast)
+        weights = np.broadcast_to(src.weights, (2, len(src.wavelengths))) * flux[:, None]
+        # the reference propagates with the default weights 1/L (sources.py:728) and then
+        # weights each wavelength's PSF, so every term carries that extra 1/L
+        weights = weights / len(src.wavelengths)
+        point = optics.propagate(src.wavelengths, src.position, weights[0], return_psf=True)
+        resolved = optics.propagate(src.wavelengths, src.position, weights[1])
+        psf = point.data + _convolve_distribution(resolved, src.distribution)
+        return PSF(psf, float(np.mean(point.pixel_scale))) if return_psf else psf
 
 
 class Scene(BaseSource):

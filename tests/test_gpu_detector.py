@@ -242,3 +242,33 @@ def test_dither_single_launch_matches_per_dither_models(dl):
         star([1e-6, 1.1e-6], np.full(2, 0.5) * f[1], c - half)
     ref = orc.detector(img, [("downsample", 4), ("add", 0.25)])
     assert _err(got.data[1], ref) < PSF_TOL
+
+
+def test_resolved_sources_vs_oracle(dl):
+    """ResolvedSource = convolve(PSF, distribution, "same") (sources.py:490-521);
+    PointResolvedSource = point PSF + convolved resolved PSF with the flux split
+    by contrast (sources.py:711-748); in a Scene beside a point source."""
+    print("[resolved sources]", flush=True)
+    cfg, optics, star = _optics_and_oracle(dl)
+    p = cfg.p_out
+    rng = np.random.default_rng(17)
+    dist = rng.uniform(0, 1, (9, 9))
+    wls = np.array([0.95e-6, 1.05e-6])
+    pos = np.array([1.2 * p, -0.7 * p])
+    res = dl.ResolvedSource(wls, position=pos, flux=2.0, distribution=dist)
+    got = res.model(optics, return_psf=True)
+    ref = orc.convolve_same(star(wls, np.full(2, 0.5) * 2.0, pos), dist / dist.sum())
+    assert _err(got.data, ref) < PSF_TOL
+    assert np.isclose(float(got.pixel_scale), cfg.p_out)
+
+    prs = dl.PointResolvedSource(wls, position=pos, flux=3.0, distribution=dist, contrast=4.0,
+                                 weights=np.array([[1.0, 3.0], [2.0, 2.0]]))
+    # the reference's wavefront carries the default 1/L weights too (sources.py:728-741)
+    f = 2 * 3.0 * np.array([4.0, 1.0]) / 5.0 / len(wls)
+    ref2 = (star(wls, np.array([0.25, 0.75]) * f[0], pos)
+            + orc.convolve_same(star(wls, np.array([0.5, 0.5]) * f[1], pos), dist / dist.sum()))
+    assert _err(prs.model(optics), ref2) < PSF_TOL
+
+    scene = dl.Scene([("disk", prs), ("star", dl.PointSource(wls, position=-pos, flux=1.5))])
+    ref3 = ref2 + star(wls, np.full(2, 0.5) * 1.5, -pos)
+    assert _err(scene.model(optics), ref3) < PSF_TOL

@@ -360,3 +360,25 @@ def test_instrument_construction():
     d = dl.Dither(np.zeros((3, 2)), optics, dl.PointSource([1e-6]))
     moved = d.add("source.position", np.ones(2))
     assert np.allclose(moved.source.position, 1.0) and np.allclose(d.source.position, 0.0)
+
+
+def test_resolved_source_construction():
+    """tests/test_sources.py:69-101."""
+    import dlux_amd as dl
+    wl = np.ones(2)
+    src = dl.ResolvedSource(wl)
+    assert np.isclose(src.distribution.sum(), 1.0)
+    with pytest.raises(ValueError):
+        dl.ResolvedSource(wl, distribution=np.ones(1))
+    # normalise floors negative intensities before renormalising (sources.py:475-488)
+    d = np.array([[1.0, -1.0], [2.0, 1.0]])
+    n = dl.ResolvedSource(wl, distribution=d).normalise()
+    assert np.allclose(n.distribution, np.array([[1.0, 0.0], [2.0, 1.0]]) / 4.0)
+    prs = dl.PointResolvedSource(wl)
+    assert prs.contrast.shape == ()
+    prs = dl.PointResolvedSource(wl, weights=np.ones((2, 2)))
+    assert prs.weights.shape == (2, 2) and np.allclose(prs.weights.sum(-1), 1)
+    with pytest.raises(NotImplementedError, match="Wavefront information"):
+        prs.model(None, return_wf=True)
+    with pytest.raises(NotImplementedError):
+        src._point_terms()
