@@ -264,8 +264,10 @@ def _detector_leg(torch, steps, check, warmup=3):
            "max_rel_err_vs_fp64": None}
     if check:
         from oracle import dlux_oracle as orc
-        chain = [(kind, np.asarray(arg.cpu() if hasattr(arg, "cpu") else arg)
-                  if kind == "pixel_response" else arg) for kind, arg in ops]
+        def host(a):
+            return a.double().cpu().numpy() if hasattr(a, "cpu") else a
+        chain = [(kind, tuple(host(a) for a in arg) if isinstance(arg, tuple) else host(arg))
+                 for kind, arg in ops]
         ref = orc.detector(data[0].double().cpu().numpy(), chain)
         res["max_rel_err_vs_fp64"] = float(np.max(np.abs(out[0].double().cpu().numpy() - ref))
                                            / np.max(np.abs(ref)))

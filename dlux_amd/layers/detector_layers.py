@@ -42,6 +42,19 @@ class DetectorLayer(Base):
         data = runtime.detector(psf.data, [self._det_op()])
         return PSF(data, psf.pixel_scale * self._scale_factor())
 
+    def _device(self, key, make):
+        """A device tensor built by make(), kept while `key` stays the same
+        (host arrays are uploaded once, not on every call).  `key` holds the
+        objects or values the tensor was made from; a layer updated through
+        set() gets new ones and so a new tensor."""
+        cached = self.__dict__.get("_dev")
+        if cached is not None and len(cached[0]) == len(key) and all(
+                a is b or (np.isscalar(a) and a == b) for a, b in zip(cached[0], key)):
+            return cached[1]
+        t = make()
+        self.__dict__["_dev"] = (key, t)
+        return t
+
 
 class ApplyPixelResponse(DetectorLayer):
     """psf * pixel_response (detector_layers.py:100-131)."""
@@ -55,7 +68,10 @@ class ApplyPixelResponse(DetectorLayer):
             raise ValueError("pixel_response must be a 2d array.")
 
     def _det_op(self):
-        return ("pixel_response", self.pixel_response)
+        resp = self.pixel_response
+        if isinstance(resp, torch.Tensor) and resp.is_cuda:
+            return ("pixel_response", resp)
+        return ("pixel_response", self._device((resp,), lambda: runtime.as_float(resp)))
 
     def __call__(self, psf: PSF) -> PSF:
         return self._run(psf)
@@ -94,7 +110,8 @@ class ApplyJitter(DetectorLayer):
 
     def _det_op(self):
         # two 1-D passes (2K multiply-adds per pixel) instead of K^2
-        f = self._factor()
+        key = (float(self.sigma), self.kernel_size, self.oversample)
+        f = self._device(key, lambda: runtime.as_float(self._factor()))
         return ("convolve_separable", (f, f))
 
     def __call__(self, psf: PSF) -> PSF:

@@ -103,6 +103,15 @@ def _resolve(chain, x):
     return out
 
 
+def _host_op(op):
+    """A chain op with its device arrays copied to the host (for the oracle)."""
+    kind, arg = op
+
+    def host(a):
+        return a.double().cpu().numpy() if isinstance(a, torch.Tensor) else a
+    return (kind, tuple(host(a) for a in arg) if isinstance(arg, tuple) else host(arg))
+
+
 def _oracle(x, chain):
     return np.stack([orc.detector(xi, chain) for xi in x])
 
@@ -150,7 +159,7 @@ def test_detector_autograd_and_fusion_invariance(dl):
     xg = x.clone().requires_grad_(True)
     w = torch.as_tensor(rng.normal(size=(32, 32)), dtype=torch.float32).cuda()
     (grad,) = torch.autograd.grad((det(dl.PSF(xg, 0.1)) * w).sum(), xg)
-    chain = [layer._det_op() for layer in det.layers.values()]
+    chain = [_host_op(layer._det_op()) for layer in det.layers.values()]
     ref = orc.detector_vjp(x.double().cpu().numpy(), chain, w.double().cpu().numpy())
     assert _err(grad, ref) < CHAIN_TOL
 
