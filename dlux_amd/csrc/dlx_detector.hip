@@ -57,6 +57,13 @@ __device__ __forceinline__ float ew_apply(const EwList& l, float v, long p) {
   return v;
 }
 
+// Whether l multiplies by a pixel-response map.
+__device__ __forceinline__ bool has_map(const EwList& l) {
+  for (int i = 0; i < l.n; ++i)
+    if (l.kind[i] == DLX_DET_PIXEL_RESPONSE) return true;
+  return false;
+}
+
 // Index of the first pixel-response op of l (-1 if none).
 __device__ __forceinline__ int first_map(const EwList& l) {
   for (int i = 0; i < l.n; ++i)
@@ -154,14 +161,23 @@ inline int conv_rpt(int K, int sep) {
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin (hardware id % 8).  Remap
-// the (tile, image) pairs so each XCD takes a contiguous run of them: the halo
-// rows that vertically adjacent tiles share then stay in one L2.
-__device__ __forceinline__ void xcd_tile(int& tile, int& image) {
-  const int nx = gridDim.x, total = nx * gridDim.y;
+// the (tile, image) pairs so each XCD takes a contiguous run of them.  With
+// the tile index fastest (map_major == 0), the halo rows that vertically
+// adjacent tiles of an image share stay in one L2.  When the pass reads a
+// pixel-response map (map_major != 0) the image index goes fastest instead:
+// one tile position is done for every image in a row, so its slice of the map
+// is fetched from HBM once per launch rather than once per image.
+__device__ __forceinline__ void xcd_tile(int map_major, int& tile, int& image) {
+  const int nx = gridDim.x, nb = gridDim.y, total = nx * nb;
   int id = blockIdx.y * nx + blockIdx.x;
   if (total % 8 == 0) id = (id % 8) * (total / 8) + id / 8;
-  tile = id % nx;
-  image = id / nx;
+  if (map_major) {
+    image = id % nb;
+    tile = id / nb;
+  } else {
+    tile = id % nx;
+    image = id / nx;
+  }
 }
 
 // Kernels of up to kWinK taps read each thread's K + 3 tile values as float4s
@@ -350,7 +366,7 @@ det_corr_kernel(const float* __restrict__ src, int n, int tiles_x, int K, int lo
                 float* __restrict__ out) {
   extern __shared__ float lds[];
   int tile, image;
-  xcd_tile(tile, image);
+  xcd_tile(has_map(pre) || has_map(post) || has_map(dpre), tile, image);
   const long base = (long)image * n * n;
   const int ty0 = (tile / tiles_x) * (kTileY * RPT);
   const int tx0 = (tile % tiles_x) * kTileX;
@@ -410,7 +426,7 @@ det_conv_down_kernel(const float* __restrict__ src, int n, int th, int tw, int t
                      int k, int m, EwList post2, float* __restrict__ out) {
   extern __shared__ float lds[];
   int tile, image;
-  xcd_tile(tile, image);
+  xcd_tile(has_map(pre) || has_map(post) || has_map(post2), tile, image);
   const long base = (long)image * n * n;
   const int ty0 = (tile / tiles_x) * th;
   const int tx0 = (tile % tiles_x) * tw;
