@@ -11,7 +11,7 @@ optical-system / layer / propagator API:
 from . import runtime, utils
 from .runtime import set_precision, get_precision, precision
 from .base import Base
-from .coordinates import Spec, PadSpec, CoordSpec
+from .coordinates import Spec, PadSpec, CoordSpec, BaseCoordTransform, CoordTransform
 from .psfs import PSF
 from .wavefronts import Wavefront
 from . import layers

@@ -87,6 +87,48 @@ class DetectorOp(ctypes.Structure):
     ]
 
 
+DLX_TF_TRANSLATION = 1
+DLX_TF_SHEAR = 2
+DLX_TF_COMPRESSION = 4
+DLX_TF_ROTATION = 8
+
+DLX_AP_CIRCLE = 1
+DLX_AP_SQUARE = 2
+DLX_AP_RECTANGLE = 3
+DLX_AP_REG_POLYGON = 4
+DLX_AP_SPIDER = 5
+
+
+class CoordTransformC(ctypes.Structure):
+    """``dlx_coord_transform``."""
+
+    _fields_ = [
+        ("flags", c_int32),
+        ("reserved_", c_int32),
+        ("translation", c_double * 2),
+        ("shear", c_double * 2),
+        ("compression", c_double * 2),
+        ("rotation", c_double),
+    ]
+
+
+class SoftAperture(ctypes.Structure):
+    """``dlx_soft_aperture``."""
+
+    _fields_ = [
+        ("kind", c_int32),
+        ("occulting", c_int32),
+        ("nsides", c_int32),
+        ("n_angles", c_int32),
+        ("n_transforms", c_int32),
+        ("group", c_int32),
+        ("size", c_double * 2),
+        ("softening", c_double),
+        ("angles_deg", c_double * 16),
+        ("transforms", CoordTransformC * 3),
+    ]
+
+
 SIGNATURES = {
     "dlx_version": (c_char_p, []),
     "dlx_last_error": (c_char_p, []),
@@ -120,6 +162,13 @@ SIGNATURES = {
                                      c_void_p, c_size_t, c_void_p]),
     "dlx_detector_vjp": (c_int32, [c_int32, c_int32, _fp, POINTER(DetectorOp), c_int32, _fp,
                                    _fp, c_void_p, c_size_t, c_void_p]),
+    "dlx_soft_apertures_workspace_size": (c_size_t, [c_int32, c_int32]),
+    "dlx_soft_apertures": (c_int32, [c_int32, c_double, c_double, c_int32,
+                                     POINTER(SoftAperture), _fp, c_void_p, c_size_t,
+                                     c_void_p]),
+    "dlx_aperture_basis": (c_int32, [c_int32, c_double, c_double, c_int32,
+                                     POINTER(CoordTransformC), c_double, c_int32, c_int32,
+                                     POINTER(c_int32), _fp, c_void_p]),
     "dlx_mft_batched_workspace_size": (c_size_t, [c_int32, POINTER(MftInstance), c_int32]),
     "dlx_mft_batched_psf": (c_int32, [c_int32, POINTER(MftInstance), c_int32,
                                       POINTER(c_void_p), c_void_p, c_size_t, c_void_p]),

@@ -1,4 +1,4 @@
-"""Coordinate / sampling specifications (coordinates.py:23-157)."""
+"""Coordinate / sampling specifications and transformations (coordinates.py:23-316)."""
 
 from __future__ import annotations
 
@@ -6,7 +6,7 @@ import numpy as np
 
 from .base import Base
 
-__all__ = ["Spec", "PadSpec", "CoordSpec"]
+__all__ = ["Spec", "PadSpec", "CoordSpec", "BaseCoordTransform", "CoordTransform"]
 
 
 class Spec(Base):
@@ -48,3 +48,73 @@ class CoordSpec(Spec):
         if self.d is None:
             raise ValueError("d must be specified to calculate extent.")
         return self.c - (self.n / 2) * self.d, self.c + (self.n / 2) * self.d
+
+
+class BaseCoordTransform(Base):
+    """Abstract coordinate transformation (coordinates.py:160-227)."""
+
+    def calculate(self, npix: int, diameter: float):
+        from .utils.coordinates import pixel_coords
+        return self(pixel_coords(npix, diameter))
+
+    def __call__(self, coords):  # pragma: no cover - abstract
+        raise NotImplementedError
+
+    def apply(self, coords):
+        return self(coords)
+
+
+class CoordTransform(BaseCoordTransform):
+    """Translation, then shear, compression and rotation (coordinates.py:230-316).
+
+    The dynamic apertures do not call ``__call__``: they hand the parameters
+    to the HIP kernels (``_struct``), which apply the same steps per pixel;
+    ``__call__`` transforms a (2, n, n) device coordinate grid for direct use.
+    """
+
+    def __init__(self, translation=None, rotation=None, compression=None, shear=None):
+        def check(v, shape, name):
+            if v is None:
+                return None
+            v = np.asarray(v, dtype=float)
+            if v.shape != shape:
+                raise ValueError(f"{name} must have shape {shape}.")
+            return v
+        self.translation = check(translation, (2,), "translation")
+        self.rotation = check(rotation, (), "rotation")
+        self.compression = check(compression, (2,), "compression")
+        self.shear = check(shear, (2,), "shear")
+
+    def __call__(self, coords):
+        from .utils import coordinates as uc
+        if self.translation is not None:
+            coords = uc.translate_coords(coords, self.translation)
+        if self.shear is not None:
+            coords = uc.shear_coords(coords, self.shear)
+        if self.compression is not None:
+            coords = uc.compress_coords(coords, self.compression)
+        if self.rotation is not None:
+            coords = uc.rotate_coords(coords, float(self.rotation))
+        return coords
+
+    def _struct(self):
+        """The ``dlx_coord_transform`` of this transformation."""
+        from . import _lib
+        t = _lib.CoordTransformC()
+        if self.translation is not None:
+            t.flags |= _lib.DLX_TF_TRANSLATION
+            t.translation[:] = [float(v) for v in self.translation]
+        if self.shear is not None:
+            t.flags |= _lib.DLX_TF_SHEAR
+            t.shear[:] = [float(v) for v in self.shear]
+        if self.compression is not None:
+            t.flags |= _lib.DLX_TF_COMPRESSION
+            t.compression[:] = [float(v) for v in self.compression]
+        if self.rotation is not None:
+            t.flags |= _lib.DLX_TF_ROTATION
+            t.rotation = float(self.rotation)
+        return t
+
+    def _key(self):
+        return tuple(None if v is None else tuple(np.atleast_1d(v).tolist())
+                     for v in (self.translation, self.shear, self.compression, self.rotation))

@@ -87,6 +87,202 @@ aperture_kernel(int npix, int os, double d_sub, double r_prim, double r_sec, dou
   if (support) support[p] = any ? 1.f : 0.f;
 }
 
+// ---------------------------------------------------------------------------
+// Dynamic soft apertures (layers/apertures.py:155-641, utils/geometry.py)
+// ---------------------------------------------------------------------------
+
+constexpr int kMaxApTransforms = 3;
+constexpr int kMaxApEdges = 64;
+constexpr int kMaxAps = 16;
+
+struct TfDev {
+  int flags;
+  double tx, ty, sx, sy, cx, cy, rc, rs;   // rc, rs: cos(-r), sin(-r)
+};
+
+struct TfChain {
+  int n;
+  TfDev t[kMaxApTransforms];
+};
+
+// Pixel (i, j) of the grid through the chain.  Shear reads the transposed
+// plane, so the chain is carried for (i, j) and (j, i) together: P and Q.
+__device__ __forceinline__ void tf_point(const TfChain& ch, int i, int j, double c, double d,
+                                         double h, double& x, double& y) {
+  double px = c + (j - h) * d, py = c + (i - h) * d;   // pixel_coords, 'xy' order
+  double qx = c + (i - h) * d, qy = c + (j - h) * d;   // the same at (j, i)
+  for (int k = 0; k < ch.n; ++k) {
+    const TfDev& t = ch.t[k];
+    if (t.flags & DLX_TF_TRANSLATION) {
+      px -= t.tx; py -= t.ty;
+      qx -= t.tx; qy -= t.ty;
+    }
+    if (t.flags & DLX_TF_SHEAR) {
+      const double npx = px + qx * t.sx, npy = py + qy * t.sy;
+      const double nqx = qx + px * t.sx, nqy = qy + py * t.sy;
+      px = npx; py = npy; qx = nqx; qy = nqy;
+    }
+    if (t.flags & DLX_TF_COMPRESSION) {
+      px *= t.cx; py *= t.cy;
+      qx *= t.cx; qy *= t.cy;
+    }
+    if (t.flags & DLX_TF_ROTATION) {
+      const double npx = t.rc * px + t.rs * py, npy = -t.rs * px + t.rc * py;
+      const double nqx = t.rc * qx + t.rs * qy, nqy = -t.rs * qx + t.rc * qy;
+      px = npx; py = npy; qx = nqx; qy = nqy;
+    }
+  }
+  x = px;
+  y = py;
+}
+
+struct ApDev {
+  int kind, occulting, nedges, nang;
+  double s0, s1, clip;
+  double ca[16], sa[16];                             // spider arm rotations
+  double ea[kMaxApEdges], eb[kMaxApEdges], ec[kMaxApEdges];   // polygon edges a x + b y + c
+  TfChain ch;
+};
+
+// Clipped soft distance (positive inside, negated when occulting) and the
+// per-block minimum and maximum of it.
+__global__ void __launch_bounds__(256)
+soft_distance_kernel(ApDev ap, int npix, double dx, double c, double* __restrict__ dist,
+                     double2* __restrict__ part) {
+  __shared__ double smin[256], smax[256];
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long total = (long)npix * npix;
+  double v = 0.0;
+  const bool in = p < total;
+  if (in) {
+    const int i = (int)(p / npix), j = (int)(p - (long)i * npix);
+    double x, y;
+    tf_point(ap.ch, i, j, c, dx, (npix - 1) / 2.0, x, y);
+    double dist_edge;   // the reference's *_distance: negative inside
+    if (ap.kind == DLX_AP_CIRCLE) {
+      dist_edge = hypot(x, y) - ap.s0;
+    } else if (ap.kind == DLX_AP_SQUARE) {
+      dist_edge = fmax(fabs(x), fabs(y)) - ap.s0 / 2.0;
+    } else if (ap.kind == DLX_AP_RECTANGLE) {
+      dist_edge = fmax(fabs(x) - ap.s0 / 2.0, fabs(y) - ap.s1 / 2.0);
+    } else if (ap.kind == DLX_AP_REG_POLYGON) {
+      dist_edge = -INFINITY;
+      for (int e = 0; e < ap.nedges; ++e)
+        dist_edge = fmax(dist_edge, ap.ea[e] * x + ap.eb[e] * y + ap.ec[e]);
+    } else {   // spider: -(min over arms of the arm distance) is positive on an arm
+      double m = INFINITY;
+      for (int a = 0; a < ap.nang; ++a) {
+        const double xr = ap.ca[a] * x + ap.sa[a] * y;
+        const double yr = -ap.sa[a] * x + ap.ca[a] * y;
+        m = fmin(m, fmax(fabs(xr) - ap.s0 / 2.0, yr));
+      }
+      dist_edge = m;
+    }
+    v = -dist_edge;
+    if (ap.occulting) v = -v;
+    v = fmin(fmax(v, -ap.clip), ap.clip);
+    dist[p] = v;
+  }
+  smin[threadIdx.x] = in ? v : INFINITY;
+  smax[threadIdx.x] = in ? v : -INFINITY;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(smin[0], smax[0]);
+}
+
+struct ApGroups {
+  int nap, nblk;
+  int group[kMaxAps];
+};
+
+// soften (utils/geometry.py:49-92) of every aperture, combined as
+// sum over groups of products.  Each block first reduces the per-block
+// extrema of every aperture (a fixed order: deterministic).
+__global__ void __launch_bounds__(256)
+soft_combine_kernel(ApGroups g, long total, const double* __restrict__ dist,
+                    const double2* __restrict__ part, float* __restrict__ out) {
+  __shared__ double smin[256], smax[256];
+  __shared__ double amin[kMaxAps], amax[kMaxAps];
+  for (int a = 0; a < g.nap; ++a) {
+    double mn = INFINITY, mx = -INFINITY;
+    for (int b = threadIdx.x; b < g.nblk; b += 256) {
+      const double2 q = part[(long)a * g.nblk + b];
+      mn = fmin(mn, q.x);
+      mx = fmax(mx, q.y);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+        smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+        smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      amin[a] = smin[0];
+      amax[a] = smax[0];
+    }
+    __syncthreads();
+  }
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= total) return;
+  double sum = 0.0, prod = 1.0;
+  for (int a = 0; a < g.nap; ++a) {
+    const double d = dist[(long)a * total + p];
+    const double mn = amin[a], mx = amax[a];
+    const double t = mx == mn ? (d > 0.0 ? 1.0 : 0.0) : (d - mn) / (mx - mn);
+    prod *= t;
+    if (a + 1 == g.nap || g.group[a + 1] != g.group[a]) {
+      sum += prod;
+      prod = 1.0;
+    }
+  }
+  out[p] = (float)sum;
+}
+
+// Zernike (nsides == 0) or n-sided polike basis on transformed coordinates
+// divided by the aperture extent (utils/zernikes.py:257-294, :352-397).
+__global__ void __launch_bounds__(256)
+aperture_basis_kernel(ZernChunk z, int nmodes, int npix, double dx, double c, TfChain ch,
+                      double inv_extent, int nsides, float* __restrict__ out) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long total = (long)npix * npix;
+  if (p >= total) return;
+  const int i = (int)(p / npix), j = (int)(p - (long)i * npix);
+  double x, y;
+  tf_point(ch, i, j, c, dx, (npix - 1) / 2.0, x, y);
+  x *= inv_extent;
+  y *= inv_extent;
+  double scale = 1.0;
+  if (nsides >= 3) {   // 1/r_alpha * Z(coords / r_alpha)
+    const double alpha = M_PI / nsides;
+    const double phi = atan2(y, x) + alpha;
+    const double wedge = floor((phi + alpha) / (2.0 * alpha));
+    const double u = phi - wedge * (2.0 * alpha);
+    const double r_alpha = cos(alpha) / cos(u);
+    x /= r_alpha;
+    y /= r_alpha;
+    scale = 1.0 / r_alpha;
+  }
+  const double rho = hypot(x, y), theta = atan2(y, x);
+  const double ap = rho <= 1.0 ? 1.0 : 0.0;
+  for (int q = 0; q < nmodes; ++q) {
+    double radial = 0.0;
+    for (int k = 0; k < z.nterms[q]; ++k) radial += z.c[q][k] * pow(rho, (double)(z.n[q] - 2 * k));
+    const int am = z.m[q] < 0 ? -z.m[q] : z.m[q];
+    const double trig = z.m[q] >= 0 ? cos(am * theta) : sin(am * theta);
+    out[(long)q * total + p] = (float)(scale * ap * radial * (z.norm[q] * trig));
+  }
+}
+
 void noll(int j, int& n, int& m) {
   n = (int)std::ceil(-0.5 + std::sqrt(1.0 + 8.0 * j) / 2.0) - 1;
   const double smallest = n * (n + 1) / 2.0 + 1.0;
@@ -101,6 +297,29 @@ double fact(double x) {
   if (x == 0.0) return 1.0;
   if (x < 0 && x == std::floor(x)) return INFINITY;
   return std::exp(std::lgamma(x + 1.0));
+}
+
+// Radial coefficients, orders and normalisations of nq Noll modes
+// (zernike_factors / eval_azimuthal, utils/zernikes.py:122-204).
+int zern_chunk(const int32_t* js, int nq, ZernChunk& z) {
+  for (int q = 0; q < nq; ++q) {
+    const int j = js[q];
+    DLX_CHECK_ARG(j >= 1, "The Zernike index must be greater than 0.");
+    int n, m;
+    noll(j, n, m);
+    const int nt = (n - m) / 2 + 1;  // k = arange((n - m) // 2 + 1), signed m
+    DLX_CHECK_ARG(nt <= kMaxTerms, "Zernike index %d is beyond the supported order", j);
+    z.n[q] = n;
+    z.m[q] = m;
+    z.nterms[q] = nt;
+    for (int k = 0; k < nt; ++k) {
+      const double sign = (k & 1) ? -1.0 : 1.0;
+      z.c[q][k] = sign * fact(std::fabs((double)(n - k))) / fact(k) /
+                  fact((double)((n + m) / 2 - k)) / fact((double)((n - m) / 2 - k));
+    }
+    z.norm[q] = std::sqrt((double)n + 1.0) * (m != 0 ? std::sqrt(2.0) : 1.0);
+  }
+  return DLX_OK;
 }
 
 }  // namespace
@@ -131,23 +350,8 @@ extern "C" int dlx_zernike_basis(int32_t nmodes, const int32_t* js, int64_t npix
   for (int q0 = 0; q0 < nmodes; q0 += kModesPerLaunch) {
     const int nq = nmodes - q0 < kModesPerLaunch ? nmodes - q0 : kModesPerLaunch;
     ZernChunk z{};
-    for (int q = 0; q < nq; ++q) {
-      const int j = js[q0 + q];
-      DLX_CHECK_ARG(j >= 1, "The Zernike index must be greater than 0.");
-      int n, m;
-      noll(j, n, m);
-      const int nt = (n - m) / 2 + 1;  // k = arange((n - m) // 2 + 1), signed m
-      DLX_CHECK_ARG(nt <= kMaxTerms, "Zernike index %d is beyond the supported order", j);
-      z.n[q] = n;
-      z.m[q] = m;
-      z.nterms[q] = nt;
-      for (int k = 0; k < nt; ++k) {
-        const double sign = (k & 1) ? -1.0 : 1.0;
-        z.c[q][k] = sign * fact(std::fabs((double)(n - k))) / fact(k) /
-                    fact((double)((n + m) / 2 - k)) / fact((double)((n - m) / 2 - k));
-      }
-      z.norm[q] = std::sqrt((double)n + 1.0) * (m != 0 ? std::sqrt(2.0) : 1.0);
-    }
+    const int st = zern_chunk(js + q0, nq, z);
+    if (st) return st;
     hipLaunchKernelGGL(zernike_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, z,
                        nq, npix, x, y, 2.0 / diameter, out + (long)q0 * npix);
     DLX_HIP(hipGetLastError());
@@ -182,5 +386,173 @@ extern "C" int dlx_circular_aperture(int32_t npix, double diameter, int32_t over
                      secondary_diameter > 0 ? secondary_diameter / 2.0 : 0.0, spider_width / 2.0,
                      sp, transmission, support);
   DLX_HIP(hipGetLastError());
+  return DLX_OK;
+}
+
+namespace dlx {
+namespace {
+
+int build_chain(int n, const dlx_coord_transform* t, TfChain& ch) {
+  DLX_CHECK_ARG(n >= 0 && n <= kMaxApTransforms, "at most %d coordinate transforms, got %d",
+                kMaxApTransforms, n);
+  DLX_CHECK_ARG(n == 0 || t, "null transforms");
+  ch.n = n;
+  for (int k = 0; k < n; ++k) {
+    TfDev& d = ch.t[k];
+    d.flags = t[k].flags;
+    d.tx = t[k].translation[0];
+    d.ty = t[k].translation[1];
+    d.sx = t[k].shear[0];
+    d.sy = t[k].shear[1];
+    d.cx = t[k].compression[0];
+    d.cy = t[k].compression[1];
+    d.rc = std::cos(-t[k].rotation);
+    d.rs = std::sin(-t[k].rotation);
+  }
+  return DLX_OK;
+}
+
+// Edges of the regular polygon as lines a x + b y + c (reg_polygon_edges and
+// line_distance, utils/geometry.py:487-579): vertices at angles
+// linspace(-pi, pi, n, endpoint=False) on radius rmax, slope
+// m = nandiv(dy, dx) (0 where dx == 0), horizontal rule for m == 0.
+void polygon_edges(int n, double rmax, ApDev& a) {
+  a.nedges = n;
+  for (int e = 0; e < n; ++e) {
+    const int f = (e + 1) % n;
+    const double ti = -M_PI + 2.0 * M_PI * e / n, tj = -M_PI + 2.0 * M_PI * f / n;
+    const double xi = rmax * std::cos(ti), yi = rmax * std::sin(ti);
+    const double xj = rmax * std::cos(tj), yj = rmax * std::sin(tj);
+    const double den = xj - xi;
+    const double m = den == 0.0 ? 0.0 : (yj - yi) / den;
+    auto sgn = [](double v) { return v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0); };
+    if (m == 0.0) {   // -sign(yi) * (yi - y)
+      const double s = sgn(yi);
+      a.ea[e] = 0.0;
+      a.eb[e] = s;
+      a.ec[e] = -s * yi;
+    } else {          // -sign(yi - m xi) * (m x - y + yi - m xi) / sqrt(m^2 + 1)
+      const double s = sgn(yi - m * xi), r = std::sqrt(m * m + 1.0);
+      a.ea[e] = -s * m / r;
+      a.eb[e] = s / r;
+      a.ec[e] = -s * (yi - m * xi) / r;
+    }
+  }
+}
+
+}  // namespace
+}  // namespace dlx
+
+extern "C" size_t dlx_soft_apertures_workspace_size(int32_t npix, int32_t n) {
+  if (npix <= 0 || n <= 0) return 0;
+  const long total = (long)npix * npix;
+  const long nblk = (total + 255) / 256;
+  return align_up((size_t)n * total * sizeof(double)) +
+         align_up((size_t)n * nblk * sizeof(double2));
+}
+
+extern "C" int dlx_soft_apertures(int32_t npix, double pixel_scale, double center, int32_t n,
+                                 const dlx_soft_aperture* aps, float* out, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(npix > 0 && npix <= 46340, "npix must be in [1, 46340], got %d", npix);
+  DLX_CHECK_ARG(n >= 1 && n <= kMaxAps, "between 1 and %d apertures, got %d", kMaxAps, n);
+  DLX_CHECK_ARG(aps && out, "null argument to dlx_soft_apertures");
+  DLX_CHECK_ARG(std::isfinite(pixel_scale) && pixel_scale > 0, "pixel_scale must be > 0");
+  ApGroups g{};
+  g.nap = n;
+  std::vector<ApDev> dev(n);
+  for (int a = 0; a < n; ++a) {
+    const dlx_soft_aperture& s = aps[a];
+    ApDev& d = dev[a];
+    d = ApDev{};
+    d.kind = s.kind;
+    d.occulting = s.occulting != 0;
+    d.s0 = s.size[0];
+    d.s1 = s.size[1];
+    DLX_CHECK_ARG(s.softening > 0, "softening must be greater than 0.");
+    d.clip = pixel_scale * s.softening / 2.0;
+    int st = build_chain(s.n_transforms, s.transforms, d.ch);
+    if (st) return st;
+    switch (s.kind) {
+      case DLX_AP_CIRCLE:
+      case DLX_AP_SQUARE:
+      case DLX_AP_RECTANGLE:
+        break;
+      case DLX_AP_REG_POLYGON:
+        DLX_CHECK_ARG(s.nsides >= 3 && s.nsides <= kMaxApEdges,
+                      "a regular polygon needs 3 to %d sides, got %d", kMaxApEdges, s.nsides);
+        polygon_edges(s.nsides, s.size[0], d);
+        break;
+      case DLX_AP_SPIDER:
+        DLX_CHECK_ARG(s.n_angles >= 1 && s.n_angles <= 16, "a spider needs 1 to 16 arms, got %d",
+                      s.n_angles);
+        d.nang = s.n_angles;
+        for (int k = 0; k < s.n_angles; ++k) {   // rotate_coords by deg2rad(angle)
+          const double r = s.angles_deg[k] * (M_PI / 180.0);
+          d.ca[k] = std::cos(-r);
+          d.sa[k] = std::sin(-r);
+        }
+        break;
+      default:
+        DLX_CHECK_ARG(false, "unknown aperture kind %d", s.kind);
+    }
+    g.group[a] = s.group;
+    if (a > 0) {
+      // groups consecutive: a group index, once left, does not come back
+      for (int b = 0; b + 1 < a; ++b)
+        DLX_CHECK_ARG(!(aps[b].group == s.group && aps[a - 1].group != s.group),
+                      "apertures of one group must be consecutive");
+    }
+  }
+  const long total = (long)npix * npix;
+  const int nblk = (int)((total + 255) / 256);
+  g.nblk = nblk;
+  const size_t need = dlx_soft_apertures_workspace_size(npix, n);
+  if (workspace_bytes < need || !workspace) {
+    set_error("dlx_soft_apertures workspace too small: need %zu bytes, got %zu", need,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  double* dist = (double*)workspace;
+  double2* part = (double2*)((char*)workspace + align_up((size_t)n * total * sizeof(double)));
+  hipStream_t st = (hipStream_t)stream;
+  for (int a = 0; a < n; ++a) {
+    hipLaunchKernelGGL(soft_distance_kernel, dim3(nblk), dim3(256), 0, st, dev[a], npix,
+                       pixel_scale, center, dist + (long)a * total, part + (long)a * nblk);
+    DLX_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(soft_combine_kernel, dim3(nblk), dim3(256), 0, st, g, total, dist, part,
+                     out);
+  DLX_HIP(hipGetLastError());
+  return DLX_OK;
+}
+
+extern "C" int dlx_aperture_basis(int32_t npix, double pixel_scale, double center,
+                                  int32_t n_transforms, const dlx_coord_transform* transforms,
+                                  double extent, int32_t nsides, int32_t nmodes,
+                                  const int32_t* js, float* out, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(npix > 0 && npix <= 46340, "npix must be in [1, 46340], got %d", npix);
+  DLX_CHECK_ARG(nmodes >= 0, "negative mode count");
+  if (nmodes == 0) return DLX_OK;
+  DLX_CHECK_ARG(js && out, "null argument to dlx_aperture_basis");
+  DLX_CHECK_ARG(std::isfinite(extent) && extent != 0.0, "extent must be non-zero");
+  DLX_CHECK_ARG(nsides == 0 || nsides >= 3, "nsides must be zero or greater than two.");
+  TfChain ch{};
+  int st = build_chain(n_transforms, transforms, ch);
+  if (st) return st;
+  const long total = (long)npix * npix;
+  hipStream_t s = (hipStream_t)stream;
+  for (int q0 = 0; q0 < nmodes; q0 += kModesPerLaunch) {
+    const int nq = nmodes - q0 < kModesPerLaunch ? nmodes - q0 : kModesPerLaunch;
+    ZernChunk z{};
+    st = zern_chunk(js + q0, nq, z);
+    if (st) return st;
+    hipLaunchKernelGGL(aperture_basis_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       s, z, nq, npix, pixel_scale, center, ch, 1.0 / extent, nsides,
+                       out + (long)q0 * total);
+    DLX_HIP(hipGetLastError());
+  }
   return DLX_OK;
 }

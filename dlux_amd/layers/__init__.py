@@ -6,3 +6,6 @@ from .optics import Optic, BasisOptic
 from .propagators import Propagator, FFT, MFT, ASMPropagator
 from .detector_layers import (DetectorLayer, ApplyPixelResponse, ApplyJitter, ApplySaturation,
                               AddConstant, Downsample)
+from .apertures import (ApertureLayer, BaseDynamicAperture, DynamicAperture, CircularAperture,
+                        SquareAperture, RectangularAperture, RegPolyAperture, Spider,
+                        AberratedAperture, CompositeAperture, CompoundAperture, MultiAperture)

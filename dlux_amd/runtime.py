@@ -542,6 +542,44 @@ def mft_batched_psf(instances, n_outputs: int | None = None, outputs=None,
 
 
 # ---------------------------------------------------------------------------
+# Dynamic apertures (dlx_soft_apertures / dlx_aperture_basis)
+# ---------------------------------------------------------------------------
+
+
+def soft_apertures(npix: int, pixel_scale: float, apertures, center: float = 0.0,
+                   out: torch.Tensor | None = None) -> torch.Tensor:
+    """Transmission (npix x npix float32) of a list of ``_lib.SoftAperture``
+    descriptors: sum over groups of the product over each group."""
+    n = len(apertures)
+    dev = device()
+    lib = _lib.load()
+    arr = (_lib.SoftAperture * max(n, 1))(*apertures)
+    if out is None:
+        out = torch.empty((npix, npix), dtype=torch.float32, device=dev)
+    ws = workspace(lib.dlx_soft_apertures_workspace_size(int(npix), n), dev)
+    st = lib.dlx_soft_apertures(int(npix), float(pixel_scale), float(center), n, arr,
+                                out.data_ptr(), ws.data_ptr(), ws.numel(), stream_handle())
+    _lib.check(st, "dlx_soft_apertures")
+    return out
+
+
+def aperture_basis(npix: int, pixel_scale: float, js, transforms=(), extent: float = 1.0,
+                   nsides: int = 0, center: float = 0.0) -> torch.Tensor:
+    """(len(js), npix, npix) Zernike / polike basis on transformed coordinates
+    divided by `extent` (dlx_aperture_basis)."""
+    js = [int(j) for j in np.atleast_1d(np.asarray(js))]
+    dev = device()
+    tfs = (_lib.CoordTransformC * max(len(transforms), 1))(*transforms)
+    jarr = (ctypes.c_int32 * max(len(js), 1))(*js)
+    out = torch.empty((len(js), npix, npix), dtype=torch.float32, device=dev)
+    st = _lib.load().dlx_aperture_basis(int(npix), float(pixel_scale), float(center),
+                                        len(transforms), tfs, float(extent), int(nsides),
+                                        len(js), jarr, out.data_ptr(), stream_handle())
+    _lib.check(st, "dlx_aperture_basis")
+    return out
+
+
+# ---------------------------------------------------------------------------
 # Detector chain (dlx_detector_apply / dlx_detector_vjp)
 # ---------------------------------------------------------------------------
 

@@ -73,6 +73,25 @@ def polar2cart(coordinates: torch.Tensor) -> torch.Tensor:
     return torch.stack([r * torch.cos(phi), r * torch.sin(phi)])
 
 
+def translate_coords(coords, translation):
+    """coords - translation per axis (utils/coordinates.py:23-39)."""
+    t = torch.as_tensor(np.asarray(translation, np.float32), device=coords.device)
+    return coords - t[:, None, None]
+
+
+def compress_coords(coords, compress):
+    """coords * compress per axis (utils/coordinates.py:42-58)."""
+    c = torch.as_tensor(np.asarray(compress, np.float32), device=coords.device)
+    return coords * c[:, None, None]
+
+
+def shear_coords(coords, shear):
+    """coords + transpose(coords) * shear per axis, with each coordinate plane
+    transposed (utils/coordinates.py:61-78)."""
+    s = torch.as_tensor(np.asarray(shear, np.float32), device=coords.device)
+    return coords + coords.transpose(1, 2) * s[:, None, None]
+
+
 def rotate_coords(coords: torch.Tensor, rotation: float) -> torch.Tensor:
     """utils/coordinates.py:81-101."""
     x, y = coords

@@ -268,6 +268,90 @@ int dlx_circular_aperture(int32_t npix, double diameter, int32_t oversample,
                           float* support, void* stream);
 
 /*
+ * Coordinate transformation of a dynamic aperture (CoordTransform,
+ * coordinates.py:230-316): translation, then shear, compression and rotation,
+ * each applied when its flag bit is set.  Shear follows
+ * utils/coordinates.py:61-78: x += shear_x * x^T and y += shear_y * y^T, where
+ * ^T transposes the coordinate plane (pixel (i, j) reads pixel (j, i)).
+ * Rotation is rotate_coords (utils/coordinates.py:81-101), in radians.
+ */
+enum dlx_transform_flags {
+  DLX_TF_TRANSLATION = 1,
+  DLX_TF_SHEAR = 2,
+  DLX_TF_COMPRESSION = 4,
+  DLX_TF_ROTATION = 8
+};
+
+typedef struct dlx_coord_transform {
+  int32_t flags;
+  int32_t reserved_;
+  double translation[2];
+  double shear[2];
+  double compression[2];
+  double rotation;
+} dlx_coord_transform;
+
+/*
+ * Soft-edged dynamic aperture (layers/apertures.py:155-641, utils/geometry.py:
+ * 229-579): the signed distance to the edge (positive inside; inverted when
+ * occulting), clipped to +-clip with clip = pixel_scale * softening / 2, then
+ * rescaled by its own minimum and maximum over the grid (soften,
+ * utils/geometry.py:49-92; a constant field gives the hard support d > 0).
+ *   DLX_AP_CIRCLE       size[0] = radius        (CircularAperture)
+ *   DLX_AP_SQUARE       size[0] = width         (SquareAperture)
+ *   DLX_AP_RECTANGLE    size = width, height    (RectangularAperture)
+ *   DLX_AP_REG_POLYGON  size[0] = rmax, nsides  (RegPolyAperture, 3..64 sides)
+ *   DLX_AP_SPIDER       size[0] = arm width, angles_deg[n_angles] (Spider, <= 16 arms)
+ * The coordinates are the pixel centres c + (k - (npix-1)/2) * pixel_scale
+ * ('xy' order), put through transforms[0 .. n_transforms) in order (an
+ * enclosing composite aperture's transformation first).
+ */
+enum dlx_aperture_kind {
+  DLX_AP_CIRCLE = 1,
+  DLX_AP_SQUARE = 2,
+  DLX_AP_RECTANGLE = 3,
+  DLX_AP_REG_POLYGON = 4,
+  DLX_AP_SPIDER = 5
+};
+
+typedef struct dlx_soft_aperture {
+  int32_t kind;
+  int32_t occulting;
+  int32_t nsides;
+  int32_t n_angles;
+  int32_t n_transforms;       /* 0..3 */
+  int32_t group;              /* see dlx_soft_apertures                         */
+  double size[2];
+  double softening;           /* pixels, > 0                                    */
+  double angles_deg[16];
+  dlx_coord_transform transforms[3];
+} dlx_soft_aperture;
+
+/*
+ * Transmission of n (<= 16) soft apertures on an npix x npix grid:
+ *   out = sum over groups g of prod over the apertures of g of T_a
+ * (apertures of one group consecutive).  One group is a CompoundAperture
+ * (product, layers/apertures.py:1005-1069); one aperture per group is a
+ * MultiAperture (sum, :1072-1117); a MultiAperture of CompoundApertures is a
+ * group per compound.  (The typedef above describes one aperture.)
+ */
+size_t dlx_soft_apertures_workspace_size(int32_t npix, int32_t n);
+int dlx_soft_apertures(int32_t npix, double pixel_scale, double center, int32_t n,
+                      const dlx_soft_aperture* apertures, float* out, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/*
+ * Zernike basis of an aberrated dynamic aperture (AberratedAperture.calc_basis,
+ * layers/apertures.py:735-752; DynamicZernikeBasis, polynomials.py:85-115): the
+ * pixel coordinates put through the transforms, divided by `extent`, then
+ * Z_j (nsides == 0, utils/zernikes.py:257-294) or the n-sided polike
+ * (nsides >= 3, :352-397) on the unit disk.  out: nmodes x npix x npix float32.
+ */
+int dlx_aperture_basis(int32_t npix, double pixel_scale, double center, int32_t n_transforms,
+                       const dlx_coord_transform* transforms, double extent, int32_t nsides,
+                       int32_t nmodes, const int32_t* js, float* out, void* stream);
+
+/*
  * Centred, zero-padded 2-D FFT of `batch` N x N complex64 fields to M x M
  * (M - N even, M a power of two up to 8192):
  *   forward: out = fftshift(fft2(ifftshift(pad_M(in)))) / M

@@ -23,6 +23,7 @@ Parity status per function (all pinned unless marked):
   * poly_psf_grad (adjoint gradient)                             pinned (FD)
   * gaussian / jitter_kernel / convolve_same / detector           pinned
   * detector_vjp                                                 pinned (FD, adjoint identity)
+  * coord_transform / soften / *_distance / soft_aperture / polike  pinned
   * asm_prop (angular spectrum)                                  parity unpinned
       -- its conventions live in abcdLux (absent); only textbook invariants
          (z=0 identity, unitarity, z1+z2 composition) are checked.
@@ -651,3 +652,118 @@ def detector_vjp(psf, layers, g):
             t = float(layer[1])
             g = g * np.where(x < t, 1.0, np.where(x == t, 0.5, 0.0))
     return g
+
+
+# ---------------------------------------------------------------------------
+# Dynamic apertures  (src/dLux/layers/apertures.py, src/dLux/utils/geometry.py,
+# src/dLux/coordinates.py, src/dLux/utils/coordinates.py, utils/zernikes.py)
+# ---------------------------------------------------------------------------
+
+
+def coord_transform(coords, translation=None, rotation=None, compression=None, shear=None):
+    """CoordTransform.__call__: translation, shear, compression, rotation, in
+    that order (coordinates.py:306-316).  Shear adds each coordinate plane's
+    transpose times the factor (utils/coordinates.py:61-78)."""
+    coords = np.asarray(coords, np.float64)
+    if translation is not None:
+        coords = coords - np.asarray(translation, float)[:, None, None]
+    if shear is not None:
+        coords = coords + np.transpose(coords, (0, 2, 1)) * np.asarray(shear, float)[:, None, None]
+    if compression is not None:
+        coords = coords * np.asarray(compression, float)[:, None, None]
+    if rotation is not None:
+        coords = rotate_coords(coords, float(rotation))
+    return coords
+
+
+def soften(distances, clip_dist, invert=False):
+    """Clip to +-clip_dist and rescale by the extrema; a constant field gives
+    the support d > 0 (utils/geometry.py:49-92)."""
+    d = np.asarray(distances, np.float64)
+    if invert:
+        d = -d
+    d = np.clip(d, -clip_dist, clip_dist)
+    lo, hi = d.min(), d.max()
+    if hi == lo:
+        return (d > 0).astype(float)
+    return (d - lo) / (hi - lo)
+
+
+def circ_distance(coords, radius):
+    """utils/geometry.py:402-418."""
+    return cart2polar(coords)[0] - radius
+
+
+def square_distance(coords, width):
+    """utils/geometry.py:421-437."""
+    return np.max(np.abs(coords), axis=0) - width / 2
+
+
+def rectangle_distance(coords, width, height):
+    """utils/geometry.py:440-460."""
+    return np.maximum(np.abs(coords[0]) - width / 2, np.abs(coords[1]) - height / 2)
+
+
+def spider_distance(coords, width, angle):
+    """One arm: max(|x'| - w/2, y') after rotating by the arm angle
+    (utils/geometry.py:463-484)."""
+    x, y = rotate_coords(coords, np.deg2rad(angle))
+    return np.maximum(np.abs(x) - width / 2, y)
+
+
+def reg_polygon_distance(coords, nsides, radius):
+    """Max over edges of the signed line distance; edge slopes by nandiv (0
+    where the edge is vertical) and the horizontal rule for slope 0
+    (utils/geometry.py:487-579)."""
+    t = np.linspace(-np.pi, np.pi, nsides, endpoint=False)
+    x, y = radius * np.cos(t), radius * np.sin(t)
+    j = np.roll(np.arange(nsides), -1)
+    out = None
+    for i in range(nsides):
+        den = x[j[i]] - x[i]
+        m = 0.0 if den == 0 else (y[j[i]] - y[i]) / den
+        if m == 0:
+            val = -np.sign(y[i]) * (y[i] - coords[1])
+        else:
+            dist = (m * coords[0] - coords[1] + y[i] - m * x[i]) / np.sqrt(m ** 2 + 1)
+            val = -np.sign(y[i] - m * x[i]) * dist
+        out = val if out is None else np.maximum(out, val)
+    return out
+
+
+def soft_aperture(kind, coords, pixel_scale, softening=1.0, occulting=False, transforms=(),
+                  **shape):
+    """DynamicAperture.transmission for kind in circle / square / rectangle /
+    reg_polygon / spider (layers/apertures.py:299-641): transforms applied in
+    order, clip = pixel_scale * softening / 2, distances negated (positive
+    inside) then softened, inverted when occulting."""
+    for t in transforms:
+        coords = coord_transform(coords, **t)
+    clip = pixel_scale * softening / 2
+    if kind == "circle":
+        d = -circ_distance(coords, shape["radius"])
+    elif kind == "square":
+        d = -square_distance(coords, shape["width"])
+    elif kind == "rectangle":
+        d = -rectangle_distance(coords, shape["width"], shape["height"])
+    elif kind == "reg_polygon":
+        d = -reg_polygon_distance(coords, shape["nsides"], shape["rmax"])
+    elif kind == "spider":
+        d = -np.min([spider_distance(coords, shape["width"], a)
+                     for a in np.atleast_1d(shape["angles"])], axis=0)
+    else:
+        raise ValueError(kind)
+    return soften(d, clip, occulting)
+
+
+def polike(nsides, j, coordinates, diameter=2.0):
+    """Zernike on an n-sided aperture: (1/r_a) Z_j(coords / r_a) with
+    r_a = cos(pi/n) / cos(u), u the angle within the wedge (utils/zernikes.py:318-349)."""
+    coords = np.asarray(coordinates, np.float64) / (diameter / 2)
+    theta = cart2polar(coords)[1]
+    alpha = np.pi / nsides
+    phi = theta + alpha
+    wedge = np.floor((phi + alpha) / (2.0 * alpha))
+    u = phi - wedge * (2 * alpha)
+    r_alpha = np.cos(alpha) / np.cos(u)
+    return 1 / r_alpha * zernike(j, coords / r_alpha)

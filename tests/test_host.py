@@ -381,3 +381,80 @@ def test_resolved_source_construction():
         prs.model(None, return_wf=True)
     with pytest.raises(NotImplementedError):
         src._point_terms()
+
+
+def test_dynamic_aperture_construction():
+    """tests/layers/test_apertures.py:56-245 (construction, attributes, errors)."""
+    import dlux_amd as dl
+    from dlux_amd import layers as L
+    ap = L.CircularAperture(0.1, dl.CoordTransform(translation=[1.0, 1.0]), False, 1.0, True)
+    assert ap.softness.shape == () and ap.radius.shape == ()
+    assert np.allclose(ap.translation, [1.0, 1.0])
+    with pytest.raises(AttributeError):
+        ap.not_an_attr
+    with pytest.raises(TypeError):
+        L.CircularAperture(0.1, transformation=2)
+    with pytest.raises(ValueError):
+        L.CircularAperture(0.1, softening=0.0)
+    assert L.CircularAperture(0.1).nsides == 0 and np.isclose(L.CircularAperture(0.1).extent, 0.1)
+    sq = L.SquareAperture(0.2)
+    assert sq.nsides == 4 and np.isclose(sq.extent, np.sqrt(2) * 0.2)
+    rect = L.RectangularAperture(0.2, 0.4)
+    assert rect.height.shape == () and np.isclose(rect.extent, np.hypot(0.1, 0.2))
+    poly = L.RegPolyAperture(6, 0.1)
+    assert poly.nsides == 6 and poly.rmax.shape == ()
+    sp = L.Spider(0.1, np.linspace(0, 360, 6, endpoint=False))
+    assert sp.occulting and sp.width.shape == ()
+    with pytest.raises(TypeError):
+        sp.extent
+    with pytest.raises(TypeError):
+        sp.nsides
+    with pytest.raises(TypeError):
+        L.AberratedAperture(L.CircularAperture(0.1, occulting=True), [1, 2])
+    with pytest.raises(TypeError):
+        L.AberratedAperture(L.Spider(0.1, [0.0], occulting=False), [1, 2])
+    with pytest.raises(TypeError):
+        L.AberratedAperture(None, [1, 2])
+    with pytest.raises(NotImplementedError):
+        L.AberratedAperture(L.CircularAperture(0.1), [1, 2]).calculate()
+    with pytest.raises(ValueError, match="effect"):
+        L.AberratedAperture(L.CircularAperture(0.1), [1, 2], effect="invalid")
+    ab = L.AberratedAperture(L.CircularAperture(1.0), [1, 2, 3, 4])
+    comp = L.CompoundAperture([L.CircularAperture(0.1), ab])
+    assert isinstance(comp.CircularAperture, L.CircularAperture)
+    with pytest.raises(AttributeError):
+        comp.not_an_attr
+    with pytest.raises(TypeError):
+        L.CompoundAperture([ab, ab])
+    multi = L.MultiAperture([comp, L.CircularAperture(0.05)])
+    assert len(multi._aberrated_apertures()) == 1
+    assert dl.CoordTransform()._struct().flags == 0
+    with pytest.raises(ValueError, match="translation"):
+        dl.CoordTransform(translation=[1.0])
+
+
+def test_aperture_abi_validation_without_gpu(lib):
+    from dlux_amd import _lib
+    aps = (_lib.SoftAperture * 2)()
+    aps[0].kind, aps[0].softening, aps[0].size[0] = _lib.DLX_AP_CIRCLE, 1.0, 0.3
+    aps[1].kind, aps[1].softening, aps[1].nsides = _lib.DLX_AP_REG_POLYGON, 1.0, 2
+    st = lib.dlx_soft_apertures(64, 1 / 64, 0.0, 2, aps, 4096, None, 0, None)
+    assert st == 1 and b"3 to 64 sides" in lib.dlx_last_error()
+    aps[1].nsides, aps[1].softening = 6, 0.0
+    st = lib.dlx_soft_apertures(64, 1 / 64, 0.0, 2, aps, 4096, None, 0, None)
+    assert st == 1 and b"softening" in lib.dlx_last_error()
+    aps[1].softening, aps[1].kind = 1.0, 9
+    st = lib.dlx_soft_apertures(64, 1 / 64, 0.0, 2, aps, 4096, None, 0, None)
+    assert st == 1 and b"unknown aperture kind" in lib.dlx_last_error()
+    aps[1].kind = _lib.DLX_AP_SPIDER
+    assert lib.dlx_soft_apertures(64, 1 / 64, 0.0, 2, aps, 4096, None, 0, None) == 1
+    aps[1].n_angles = 3
+    need = lib.dlx_soft_apertures_workspace_size(64, 2)
+    assert need >= 2 * 64 * 64 * 8
+    st = lib.dlx_soft_apertures(64, 1 / 64, 0.0, 2, aps, 4096, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+    js = (ctypes.c_int32 * 2)(1, 2)
+    st = lib.dlx_aperture_basis(64, 1 / 64, 0.0, 0, None, 1.0, 2, 2, js, 4096, None)
+    assert st == 1 and b"nsides" in lib.dlx_last_error()
+    st = lib.dlx_aperture_basis(64, 1 / 64, 0.0, 4, None, 1.0, 0, 2, js, 4096, None)
+    assert st == 1 and b"at most 3" in lib.dlx_last_error()

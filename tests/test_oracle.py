@@ -421,3 +421,69 @@ def test_detector_vjp_adjoint_identity_and_differences():
         e[idx] = h
         fd = np.sum((orc.detector(x + e, chain) - orc.detector(x - e, chain)) * g) / (2 * h)
         assert abs(fd - vjp[idx]) < 1e-7
+
+
+# ---------------------------------------------------------------- dynamic apertures
+def test_soften_closed_form_and_constant_field():
+    """A circle well inside the grid: min = -clip, max = +clip, so the
+    softened transmission is clip(1/2 - circ_distance / (2 clip), 0, 1); a
+    circle covering the whole grid gives the hard support (all ones)."""
+    c = orc.pixel_coords(128, 1.0)
+    dx = 1.0 / 128
+    t = orc.soft_aperture("circle", c, dx, softening=3.0, radius=0.3)
+    clip = dx * 3.0 / 2
+    ref = np.clip(0.5 - orc.circ_distance(c, 0.3) / (2 * clip), 0, 1)
+    assert np.max(np.abs(t - ref)) < 1e-12
+    big = orc.soft_aperture("circle", c, dx, softening=1.0, radius=5.0)
+    assert np.array_equal(big, np.ones_like(big))
+    occ = orc.soft_aperture("circle", c, dx, softening=3.0, occulting=True, radius=0.3)
+    assert np.max(np.abs(occ - (1 - ref))) < 1e-12
+
+
+def test_polygon_square_rectangle_identities():
+    """Square == rectangle with equal sides; the 4-gon (vertices on the axes)
+    == the square of side rmax*sqrt(2) turned by 45 degrees; a 64-gon is
+    within rmax (1 - cos(pi/64)) of the circle."""
+    c = orc.pixel_coords(96, 2.0)
+    assert np.allclose(orc.square_distance(c, 0.9), orc.rectangle_distance(c, 0.9, 0.9))
+    diamond = orc.reg_polygon_distance(c, 4, 0.7)
+    turned = orc.square_distance(orc.rotate_coords(c, np.pi / 4), 0.7 * np.sqrt(2))
+    assert np.max(np.abs(diamond - turned)) < 1e-12
+    # the inscribed 64-gon lies inside the circle, by at most rmax (1 - cos(pi/64))
+    poly = orc.reg_polygon_distance(c, 64, 0.8)
+    gap = (orc.circ_distance(c, 0.8) - poly)[np.hypot(*c) <= 0.8]
+    assert np.all(gap <= 1e-12) and np.all(gap >= -0.8 * (1 - np.cos(np.pi / 64)) - 1e-12)
+
+
+def test_coord_transform_closed_forms():
+    """Integer-pixel translation shifts the transmission; shear (s, 0) on a
+    centred square grid is x' = x + s y (the transposed x plane is y);
+    compression (2, 1) turns the circle into an ellipse."""
+    n, dx = 64, 1.0 / 64
+    c = orc.pixel_coords(n, 1.0)
+    base = orc.soft_aperture("circle", c, dx, softening=2.0, radius=0.2)
+    moved = orc.soft_aperture("circle", c, dx, softening=2.0, radius=0.2,
+                              transforms=[{"translation": [3 * dx, -5 * dx]}])
+    assert np.allclose(moved[:n - 8, 3:], base[5:n - 3, :n - 3], atol=1e-9)
+    sheared = orc.coord_transform(c, shear=[0.3, 0.0])
+    assert np.allclose(sheared[0], c[0] + 0.3 * c[1]) and np.allclose(sheared[1], c[1])
+    ell = orc.soft_aperture("circle", c, dx, softening=0.5, radius=0.3,
+                            transforms=[{"compression": [2.0, 1.0]}])
+    r_ell = np.hypot(2 * c[0], c[1])
+    assert np.all(ell[r_ell < 0.3 - 2 * dx] == 1) and np.all(ell[r_ell > 0.3 + 2 * dx] == 0)
+
+
+def test_polike_support_and_limit():
+    """The n-sided polike piston is 1/r_a on the regular n-gon of
+    circumradius 1: its support is that polygon (the hexagon as built by
+    reg_polygon_distance; the triangle turned by pi/3); with many sides the
+    polike basis tends to the Zernike basis."""
+    c = orc.pixel_coords(128, 2.0)
+    for nsides, rot in ((6, 0.0), (3, np.pi / 3)):
+        sup = orc.polike(nsides, 1, c) != 0
+        poly = orc.reg_polygon_distance(orc.rotate_coords(c, rot), nsides, 1.0) < 0
+        assert np.array_equal(sup, poly)
+    inner = np.hypot(*c) < 0.95
+    for j in (2, 4, 7):   # r_a = cos(pi/n)/cos(u) differs from 1 by < 5e-6 at n = 1000
+        diff = orc.polike(1000, j, c) - orc.zernike(j, c)
+        assert np.max(np.abs(diff[inner])) < 5e-4
