@@ -47,6 +47,7 @@ def _parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-detector", action="store_true", help="skip the detector-chain leg")
+    ap.add_argument("--no-aperture", action="store_true", help="skip the dynamic-aperture leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient leg")
     ap.add_argument("--calibrate", action="store_true",
@@ -274,6 +275,62 @@ def _detector_leg(torch, steps, check, warmup=3):
     return res
 
 
+def _aperture_leg(torch, steps, warmup=3):
+    """Dynamic aperture: a 512^2 pupil from an aberrated circular aperture (36
+    Zernikes on its own transformed coordinates) and a 3-arm spider, whose
+    translation and rotation change every step, so the soft transmission and
+    the basis are regenerated on the device each step; then the 64-lambda
+    split-BF16x3 PSF of the workload above.  Also times the regeneration
+    alone (transmission + basis)."""
+    import dlux_amd as dl
+    from dlux_amd import layers, runtime
+    from dlux_amd.utils import units
+    n, n_out, nwl = 512, 256, 64
+    wls = np.linspace(0.8e-6, 1.2e-6, nwl)
+    js = list(range(2, 38))
+    coeffs = torch.as_tensor(np.random.default_rng(1).normal(0, 30e-9, len(js)),
+                             dtype=torch.float32, device="cuda")
+
+    def system(k):
+        tf = dl.CoordTransform(translation=[1e-4 * (k % 7), -1e-4 * (k % 5)],
+                               rotation=1e-3 * k)
+        pupil = layers.AberratedAperture(layers.CircularAperture(0.5, tf, normalise=True),
+                                         js, coeffs)
+        spider = layers.Spider(8.0 / n, [0.0, 120.0, 240.0], tf)
+        return dl.AngularOpticalSystem(wf_npixels=n, diameter=1.0,
+                                       layers=[("pupil", pupil), ("spider", spider)],
+                                       psf_npixels=n_out,
+                                       psf_pixel_scale=float(units.rad2arcsec(2.5e-7)))
+    with runtime.precision("bf16x3"):
+        for k in range(warmup):
+            system(k).propagate(wls)
+        torch.cuda.synchronize()
+        beg, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        systems = [system(warmup + k) for k in range(steps)]
+        beg.record()
+        for s in systems:
+            s.propagate(wls)
+        end.record()
+        torch.cuda.synchronize()
+    ms = beg.elapsed_time(end) / steps
+    # regeneration alone
+    gens = [system(1000 + k) for k in range(steps)]
+    torch.cuda.synchronize()
+    beg.record()
+    for s in gens:
+        ab = s.layers["pupil"]
+        ab.aperture._grid_transmission(n, 1.0 / n)
+        ab._grid_basis(n, 1.0 / n)
+        s.layers["spider"]._grid_transmission(n, 1.0 / n)
+    end.record()
+    torch.cuda.synchronize()
+    ms_gen = beg.elapsed_time(end) / steps
+    return {"workload": "dynamic aperture: aberrated circle (36 Zernikes) + 3-arm spider, "
+                        "moved every step; 512^2 -> 256^2 x 64 lambda, split-BF16x3",
+            "psf_per_s": 1e3 / ms, "ms_per_step": ms, "regenerate_ms": ms_gen,
+            "regenerate_note": "soft transmissions (two launches each) + 36-mode basis"}
+
+
 def _serving_leg(torch, rates, duration):
     """Poisson request stream (75% PSF, 20% gradient, 5% ASM) at each rate."""
     from dlux_amd import serving
@@ -363,6 +420,8 @@ def main():
     print("[bench] detector leg", file=sys.stderr, flush=True)
     detector = None if args.no_detector else _detector_leg(torch, max(args.steps // 5, 10),
                                                            not args.no_cpu)
+    print("[bench] dynamic-aperture leg", file=sys.stderr, flush=True)
+    aperture = None if args.no_aperture else _aperture_leg(torch, max(args.steps // 5, 10))
     print("[bench] serving leg", file=sys.stderr, flush=True)
     serve = None if args.no_serving else _serving_leg(torch, args.serving_rates,
                                                       args.serving_seconds)
@@ -419,6 +478,7 @@ def main():
         "batched": batched,
         "asm": asm,
         "detector": detector,
+        "aperture": aperture,
         "serving": serve,
     }
     print(json.dumps(line))

@@ -18,7 +18,33 @@ struct ZernChunk {
   int n[kModesPerLaunch], m[kModesPerLaunch], nterms[kModesPerLaunch];
   double c[kModesPerLaunch][kMaxTerms];
   double norm[kModesPerLaunch];
+  int kmax[kModesPerLaunch];      // terms with a finite coefficient: k <= (n - |m|)/2
+  int nan0[kModesPerLaunch];      // a zero-coefficient term has a negative power of rho
 };
+
+// Z_q(rho, theta) with cos(theta) = ct, sin(theta) = st: the radial sum as
+// rho^|m| times a Horner polynomial in rho^2 over the finite-coefficient terms
+// (the others have coefficient 0 from 1/Gamma(<= 0); at rho == 0 a negative
+// power makes them 0 * inf = NaN, kept via nan0), and cos/sin(|m| theta) by
+// complex powers of (ct + i st).
+__device__ __forceinline__ double zern_eval(const ZernChunk& z, int q, double rho, double ct,
+                                            double st) {
+  const int am = z.m[q] < 0 ? -z.m[q] : z.m[q];
+  const double u = rho * rho;
+  double acc = z.c[q][0];
+  for (int k = 1; k <= z.kmax[q]; ++k) acc = acc * u + z.c[q][k];
+  double rp = 1.0, cr = 1.0, ci = 0.0;
+  for (int k = 0; k < am; ++k) {
+    rp *= rho;
+    const double t = cr * ct - ci * st;
+    ci = cr * st + ci * ct;
+    cr = t;
+  }
+  double radial = acc * rp;
+  if (z.nan0[q] && rho == 0.0) radial = NAN;
+  const double trig = z.m[q] >= 0 ? cr : ci;
+  return radial * (z.norm[q] * trig);
+}
 
 // Z_j = [rho <= 1] * sum_k c_k rho^(n - 2k) * norm * trig(|m| theta)
 // (utils/zernikes.py:153-204, :227-257).  Terms whose coefficient is zero
@@ -30,15 +56,11 @@ zernike_kernel(ZernChunk z, int nmodes, long npix, const float* __restrict__ xs,
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npix) return;
   const double x = (double)xs[p] * inv_r, y = (double)ys[p] * inv_r;
-  const double rho = hypot(x, y), theta = atan2(y, x);
+  const double rho = hypot(x, y);
+  const double ct = rho > 0.0 ? x / rho : 1.0, st = rho > 0.0 ? y / rho : 0.0;   // atan2(0, 0) = 0
   const double ap = rho <= 1.0 ? 1.0 : 0.0;
-  for (int q = 0; q < nmodes; ++q) {
-    double radial = 0.0;
-    for (int k = 0; k < z.nterms[q]; ++k) radial += z.c[q][k] * pow(rho, (double)(z.n[q] - 2 * k));
-    const int am = z.m[q] < 0 ? -z.m[q] : z.m[q];
-    const double trig = z.m[q] >= 0 ? cos(am * theta) : sin(am * theta);
-    out[(long)q * npix + p] = (float)(ap * radial * (z.norm[q] * trig));
-  }
+  for (int q = 0; q < nmodes; ++q)
+    out[(long)q * npix + p] = (float)(ap * zern_eval(z, q, rho, ct, st));
 }
 
 // Static circular aperture (utils/apertures.py:153-242): oversampled hard
@@ -272,15 +294,11 @@ aperture_basis_kernel(ZernChunk z, int nmodes, int npix, double dx, double c, Tf
     y /= r_alpha;
     scale = 1.0 / r_alpha;
   }
-  const double rho = hypot(x, y), theta = atan2(y, x);
+  const double rho = hypot(x, y);
+  const double ct = rho > 0.0 ? x / rho : 1.0, st = rho > 0.0 ? y / rho : 0.0;
   const double ap = rho <= 1.0 ? 1.0 : 0.0;
-  for (int q = 0; q < nmodes; ++q) {
-    double radial = 0.0;
-    for (int k = 0; k < z.nterms[q]; ++k) radial += z.c[q][k] * pow(rho, (double)(z.n[q] - 2 * k));
-    const int am = z.m[q] < 0 ? -z.m[q] : z.m[q];
-    const double trig = z.m[q] >= 0 ? cos(am * theta) : sin(am * theta);
-    out[(long)q * total + p] = (float)(scale * ap * radial * (z.norm[q] * trig));
-  }
+  for (int q = 0; q < nmodes; ++q)
+    out[(long)q * total + p] = (float)(scale * ap * zern_eval(z, q, rho, ct, st));
 }
 
 void noll(int j, int& n, int& m) {
@@ -312,11 +330,18 @@ int zern_chunk(const int32_t* js, int nq, ZernChunk& z) {
     z.n[q] = n;
     z.m[q] = m;
     z.nterms[q] = nt;
+    z.nan0[q] = 0;
     for (int k = 0; k < nt; ++k) {
       const double sign = (k & 1) ? -1.0 : 1.0;
       z.c[q][k] = sign * fact(std::fabs((double)(n - k))) / fact(k) /
                   fact((double)((n + m) / 2 - k)) / fact((double)((n - m) / 2 - k));
+      if (z.c[q][k] == 0.0 && n - 2 * k < 0) z.nan0[q] = 1;
     }
+    // the finite-coefficient terms are k = 0 .. (n - |m|)/2, powers rho^(n - 2k) >= rho^|m|
+    z.kmax[q] = (n - (m < 0 ? -m : m)) / 2;
+    for (int k = 0; k <= z.kmax[q]; ++k)
+      DLX_CHECK_ARG(std::isfinite(z.c[q][k]) && z.c[q][k] != 0.0,
+                    "Zernike index %d: unexpected radial coefficient", j);
     z.norm[q] = std::sqrt((double)n + 1.0) * (m != 0 ? std::sqrt(2.0) : 1.0);
   }
   return DLX_OK;
