@@ -150,7 +150,7 @@ SIGNATURES = {
     "dlx_psf_poly_vjp_workspace_size": (c_size_t, [POINTER(MftGeom), c_int32, c_int32]),
     "dlx_psf_poly_vjp": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, c_double, c_double,
                                    c_double, c_double, c_double, _fp, _fp, _fp, c_double, _fp,
-                                   _fp, _fp, c_int32, c_void_p, c_size_t, c_void_p]),
+                                   _fp, _fp, _fp, c_int32, c_void_p, c_size_t, c_void_p]),
     "dlx_project_basis_workspace_size": (c_size_t, [c_int32, c_int64, c_int32]),
     "dlx_project_basis": (c_int32, [c_int32, c_int64, c_int32, _fp, _fp, _fp, c_void_p,
                                     c_size_t, c_void_p]),
@@ -166,6 +166,10 @@ SIGNATURES = {
     "dlx_soft_apertures": (c_int32, [c_int32, c_double, c_double, c_int32,
                                      POINTER(SoftAperture), _fp, c_void_p, c_size_t,
                                      c_void_p]),
+    "dlx_soft_apertures_vjp_workspace_size": (c_size_t, [c_int32, c_int32]),
+    "dlx_soft_apertures_vjp": (c_int32, [c_int32, c_double, c_double, c_int32,
+                                         POINTER(SoftAperture), _fp, _fp, c_void_p, c_size_t,
+                                         c_void_p]),
     "dlx_aperture_basis": (c_int32, [c_int32, c_double, c_double, c_int32,
                                      POINTER(CoordTransformC), c_double, c_int32, c_int32,
                                      POINTER(c_int32), _fp, c_void_p]),

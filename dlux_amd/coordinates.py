@@ -76,6 +76,10 @@ class CoordTransform(BaseCoordTransform):
         def check(v, shape, name):
             if v is None:
                 return None
+            if _is_tensor(v):   # kept as given, so gradients can reach it
+                if tuple(v.shape) != shape:
+                    raise ValueError(f"{name} must have shape {shape}.")
+                return v
             v = np.asarray(v, dtype=float)
             if v.shape != shape:
                 raise ValueError(f"{name} must have shape {shape}.")
@@ -103,18 +107,35 @@ class CoordTransform(BaseCoordTransform):
         t = _lib.CoordTransformC()
         if self.translation is not None:
             t.flags |= _lib.DLX_TF_TRANSLATION
-            t.translation[:] = [float(v) for v in self.translation]
+            t.translation[:] = list(_floats(self.translation))
         if self.shear is not None:
             t.flags |= _lib.DLX_TF_SHEAR
-            t.shear[:] = [float(v) for v in self.shear]
+            t.shear[:] = list(_floats(self.shear))
         if self.compression is not None:
             t.flags |= _lib.DLX_TF_COMPRESSION
-            t.compression[:] = [float(v) for v in self.compression]
+            t.compression[:] = list(_floats(self.compression))
         if self.rotation is not None:
             t.flags |= _lib.DLX_TF_ROTATION
-            t.rotation = float(self.rotation)
+            t.rotation = _floats(self.rotation)[0]
         return t
 
     def _key(self):
-        return tuple(None if v is None else tuple(np.atleast_1d(v).tolist())
+        return tuple(None if v is None else _floats(v)
                      for v in (self.translation, self.shear, self.compression, self.rotation))
+
+    # (parameter slot, attribute, element) of the transform's gradient slots
+    # in the dynamic-aperture geometry VJP
+    _SLOTS = ((2, "translation", 0), (3, "translation", 1), (4, "shear", 0), (5, "shear", 1),
+              (6, "compression", 0), (7, "compression", 1), (8, "rotation", None))
+
+
+def _is_tensor(v) -> bool:
+    return type(v).__module__.startswith("torch") and hasattr(v, "requires_grad")
+
+
+def _floats(v) -> tuple:
+    """The values of a parameter (numpy array, number or torch tensor) as a
+    tuple of Python floats."""
+    if _is_tensor(v):
+        return tuple(v.detach().double().cpu().reshape(-1).tolist())
+    return tuple(float(x) for x in np.atleast_1d(np.asarray(v, float)).reshape(-1))

@@ -216,7 +216,7 @@ int mft_split4(const dlx_mft_geom& g, int batch, const uint8_t* P, const uint8_t
 // adjoint matrices (used when the cache is bypassed), the stage-1 intermediate
 // (U^T, later V^T), Ebar, and the per-wavelength gradient planes.
 struct VjpLayout {
-  size_t P, Wf, Wa, T, Eb, H, total;
+  size_t P, U, Wf, Wa, T, Eb, H, H2, total;
 };
 VjpLayout vjp_layout(const dlx_mft_geom& g, int c, bool split) {
   VjpLayout L;
@@ -227,7 +227,11 @@ VjpLayout vjp_layout(const dlx_mft_geom& g, int c, bool split) {
   L.T = align_up((size_t)c * (t1 > t2 ? t1 : t2));
   L.Eb = align_up((size_t)c * mat_bytes(g.n_out, g.n_out, split));
   L.H = align_up((size_t)c * g.n_in * g.n_in * sizeof(float));
-  L.total = L.P + L.Wf + L.Wa + L.T + L.Eb + L.H;
+  // the amplitude cotangent: phasors U = P / A (P at unit amplitude) and the
+  // Re(conj(U) Pbar) planes
+  L.U = L.P;
+  L.H2 = L.H;
+  L.total = L.P + L.U + L.Wf + L.Wa + L.T + L.Eb + L.H + L.H2;
   return L;
 }
 
@@ -484,13 +488,14 @@ int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavele
                      const double* weight, double pixel_scale_in, double pixel_scale_out,
                      double center, double tilt_x, double tilt_y, const float* amp,
                      const float* opd, const float* phase, double scale, const float* dpsf,
-                     float* grad_opd, float* grad_phase, int32_t precision, void* workspace,
-                     size_t workspace_bytes, void* stream) {
+                     float* grad_opd, float* grad_phase, float* grad_amp, int32_t precision,
+                     void* workspace, size_t workspace_bytes, void* stream) {
   clear_error();
   if (int st = check_geom(geom)) return st;
   DLX_CHECK_ARG(nwl > 0, "nwl must be > 0, got %d", nwl);
   DLX_CHECK_ARG(wavelength && weight && dpsf, "null argument to dlx_psf_poly_vjp");
-  DLX_CHECK_ARG(grad_opd || grad_phase, "dlx_psf_poly_vjp needs grad_opd or grad_phase");
+  DLX_CHECK_ARG(grad_opd || grad_phase || grad_amp,
+                "dlx_psf_poly_vjp needs grad_opd, grad_phase or grad_amp");
   DLX_CHECK_ARG(precision == DLX_PREC_FP32 || precision == DLX_PREC_BF16X3,
                 "unknown precision %d", precision);
   DLX_CHECK_ARG(std::isfinite(pixel_scale_in) && pixel_scale_in != 0.0 &&
@@ -519,6 +524,8 @@ int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavele
   uint8_t* T = Wa_ws + L.Wa;
   uint8_t* Eb = T + L.T;
   float* H = (float*)(Eb + L.Eb);
+  float* H2 = grad_amp ? (float*)((uint8_t*)H + L.H) : nullptr;
+  uint8_t* U = (uint8_t*)H + L.H + L.H2;
   const long nin = g.n_in, nout = g.n_out;
   const double x0 = center - (g.n_in - 1) / 2.0 * pixel_scale_in;
 
@@ -547,6 +554,9 @@ int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavele
       for (int i = 0; i < nb; ++i) pc.inv_wl[i] = 1.0 / wavelength[l0 + i];
       DLX_HIP(launch_pupil_split4(pc, nb, g.n_in, pixel_scale_in, x0, tilt_x, tilt_y, amp, opd,
                                   phase, (float)scale, P, pb, s));
+      if (grad_amp)
+        DLX_HIP(launch_pupil_split4(pc, nb, g.n_in, pixel_scale_in, x0, tilt_x, tilt_y, nullptr,
+                                    opd, phase, (float)scale, U, pb, s));
       // forward stage 1: U^T = (P W^T)^T, split4 rows n_out
       GemmSplit g1{};
       g1.A = P; g1.a_row_bytes = rb_in; g1.a_batch_bytes = pb;
@@ -582,12 +592,18 @@ int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavele
       g4.M = g.n_in; g4.N = g.n_in; g4.K = g.n_out; g4.batch = nb; g4.epilogue = EPI_GRAD;
       for (int i = 0; i < nb; ++i) g4.alpha[i] = alpha[i];
       g4.P = P; g4.p_ld = rb_in; g4.p_batch = pb;
+      g4.C2 = H2; g4.U = U;
       DLX_HIP(launch_cgemm_split4(g4, s));
     } else {
       const long pst = nin * nin, wst = nout * nin, est = nout * nout;
       if (int st = dlx_pupil_field(g.n_in, nb, wavelength + l0, pixel_scale_in, center, tilt_x,
                                    tilt_y, amp, opd, phase, scale, nullptr, P, stream))
         return st;
+      if (grad_amp)
+        if (int st = dlx_pupil_field(g.n_in, nb, wavelength + l0, pixel_scale_in, center,
+                                     tilt_x, tilt_y, nullptr, opd, phase, scale, nullptr, U,
+                                     stream))
+          return st;
       GemmBatch g1{};
       g1.A = (const float2*)P; g1.lda = nin; g1.sA = pst;
       g1.B = (const float2*)Wf; g1.ldb = nin; g1.sB = wst;
@@ -621,11 +637,15 @@ int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavele
       g4.epilogue = EPI_GRAD; g4.precision = DLX_PREC_FP32;
       for (int i = 0; i < nb; ++i) g4.alpha[i] = alpha[i];
       g4.P = P; g4.p_ld = nin; g4.p_batch = pst;
+      g4.C2 = H2; g4.U = U;
       DLX_HIP(launch_cgemm_nt(g4, s));
     }
     if (int st = cache_release(hf, s)) return st;
     if (int st = cache_release(ha, s)) return st;
-    DLX_HIP(launch_grad_reduce(gc, nb, nin * nin, H, l0 > 0, grad_opd, grad_phase, s));
+    if (grad_opd || grad_phase)
+      DLX_HIP(launch_grad_reduce(gc, nb, nin * nin, H, l0 > 0, grad_opd, grad_phase, s));
+    if (grad_amp)   // sum over wavelengths of Re(conj(U) Pbar) = dL/dA
+      DLX_HIP(launch_grad_reduce(gc, nb, nin * nin, H2, l0 > 0, nullptr, grad_amp, s));
   }
   return DLX_OK;
 }

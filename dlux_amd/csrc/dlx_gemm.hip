@@ -139,7 +139,13 @@ cgemm_nt_fp32(GemmBatch g) {
             ((float2*)g.C)[(long)b * g.sC + (long)m * g.ldc + n] = make_float2(gm * re, gm * im);
           } else if (EPI == EPI_GRAD) {
             const float2 pv = ((const float2*)g.P)[(long)b * g.p_batch + (long)m * g.p_ld + n];
-            ((float*)g.C)[(long)b * g.sC + (long)m * g.ldc + n] = pv.x * im - pv.y * re;
+            const long o = (long)b * g.sC + (long)m * g.ldc + n;
+            ((float*)g.C)[o] = pv.x * im - pv.y * re;
+            if (g.C2) {
+              const float2 uv =
+                  ((const float2*)g.U)[(long)b * g.p_batch + (long)m * g.p_ld + n];
+              g.C2[o] = uv.x * re + uv.y * im;
+            }
           } else {
             ((float*)g.C)[(long)b * g.sC + (long)m * g.ldc + n] = re * re + im * im;
           }

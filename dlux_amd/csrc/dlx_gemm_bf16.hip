@@ -141,7 +141,16 @@ __device__ __forceinline__ void epilogue_split(const GemmSplit& g, int b, int m0
                                 __uint_as_float((uint32_t)*(const uint16_t*)(pr + 16) << 16);
               const float pim = __uint_as_float((uint32_t)*(const uint16_t*)(pr + 32) << 16) +
                                 __uint_as_float((uint32_t)*(const uint16_t*)(pr + 48) << 16);
-              ((float*)g.C)[(long)b * g.c_batch + (long)m * g.ldc + n] = pre * im - pim * re;
+              const long o = (long)b * g.c_batch + (long)m * g.ldc + n;
+              ((float*)g.C)[o] = pre * im - pim * re;
+              if (g.C2) {   // the phasor U in P's layout
+                const uint8_t* ur = (const uint8_t*)g.U + (pr - (const uint8_t*)g.P);
+                const float ure = __uint_as_float((uint32_t)*(const uint16_t*)(ur) << 16) +
+                                  __uint_as_float((uint32_t)*(const uint16_t*)(ur + 16) << 16);
+                const float uim = __uint_as_float((uint32_t)*(const uint16_t*)(ur + 32) << 16) +
+                                  __uint_as_float((uint32_t)*(const uint16_t*)(ur + 48) << 16);
+                g.C2[o] = ure * re + uim * im;
+              }
             } else {
               ((float*)g.C)[(long)b * g.c_batch + (long)m * g.ldc + n] = re * re + im * im;
             }

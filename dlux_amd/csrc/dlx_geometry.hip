@@ -228,7 +228,8 @@ struct ApGroups {
 // extrema of every aperture (a fixed order: deterministic).
 __global__ void __launch_bounds__(256)
 soft_combine_kernel(ApGroups g, long total, const double* __restrict__ dist,
-                    const double2* __restrict__ part, float* __restrict__ out) {
+                    const double2* __restrict__ part, float* __restrict__ out,
+                    double* __restrict__ tplanes, double2* __restrict__ ext) {
   __shared__ double smin[256], smax[256];
   __shared__ double amin[kMaxAps], amax[kMaxAps];
   for (int a = 0; a < g.nap; ++a) {
@@ -251,6 +252,7 @@ soft_combine_kernel(ApGroups g, long total, const double* __restrict__ dist,
     if (threadIdx.x == 0) {
       amin[a] = smin[0];
       amax[a] = smax[0];
+      if (ext && blockIdx.x == 0) ext[a] = make_double2(smin[0], smax[0]);
     }
     __syncthreads();
   }
@@ -261,13 +263,14 @@ soft_combine_kernel(ApGroups g, long total, const double* __restrict__ dist,
     const double d = dist[(long)a * total + p];
     const double mn = amin[a], mx = amax[a];
     const double t = mx == mn ? (d > 0.0 ? 1.0 : 0.0) : (d - mn) / (mx - mn);
+    if (tplanes) tplanes[(long)a * total + p] = t;
     prod *= t;
     if (a + 1 == g.nap || g.group[a + 1] != g.group[a]) {
       sum += prod;
       prod = 1.0;
     }
   }
-  out[p] = (float)sum;
+  if (out) out[p] = (float)sum;
 }
 
 // Zernike (nsides == 0) or n-sided polike basis on transformed coordinates
@@ -299,6 +302,208 @@ aperture_basis_kernel(ZernChunk z, int nmodes, int npix, double dx, double c, Tf
   const double ap = rho <= 1.0 ? 1.0 : 0.0;
   for (int q = 0; q < nmodes; ++q)
     out[(long)q * total + p] = (float)(scale * ap * zern_eval(z, q, rho, ct, st));
+}
+
+// ---- geometry gradients of the soft apertures ----------------------------
+// Parameters per aperture, in this order: size[0], size[1], then the last
+// transform of its chain: translation x, y, shear x, y, compression x, y,
+// rotation.  The per-pixel derivative of the clipped soft distance dc is
+// carried forward through that transform and the edge distance; soften's
+// extrema enter through JAX's rule for min/max (the derivative averaged
+// over the tied pixels), so
+//   dT/dth = (ddc - dmn)/R - (dc - mn)(dmx - dmn)/R^2,  R = mx - mn.
+constexpr int kGeoParams = 9;
+constexpr int kVjpSums = 3 * kGeoParams + 4;   // S1[9], Smn[9], Smx[9], S0, S2, cmn, cmx
+
+// dist and its gradient for one pixel: returns the clipped soft distance dc
+// and d(dc)/d(theta) in dd[9] (zero where clipped).
+__device__ void soft_point_grad(const ApDev& ap, int i, int j, int npix, double dx, double c,
+                                double& dc, double (&dd)[kGeoParams]) {
+  const double h = (npix - 1) / 2.0;
+  for (int k = 0; k < kGeoParams; ++k) dd[k] = 0.0;
+  // every transform but the last, without derivatives
+  TfChain pre = ap.ch;
+  const int own = pre.n > 0 ? pre.n - 1 : -1;
+  pre.n = own > 0 ? own : 0;
+  double px = c + (j - h) * dx, py = c + (i - h) * dx;
+  double qx = c + (i - h) * dx, qy = c + (j - h) * dx;
+  {
+    double x, y;
+    tf_point(pre, i, j, c, dx, h, x, y);
+    double xq, yq;
+    tf_point(pre, j, i, c, dx, h, xq, yq);
+    px = x; py = y; qx = xq; qy = yq;
+  }
+  // Jacobians of (x, y) with respect to the 9 parameters (sizes: 0)
+  double jx[kGeoParams] = {0}, jy[kGeoParams] = {0};
+  double jqx[kGeoParams] = {0}, jqy[kGeoParams] = {0};
+  if (own >= 0) {
+    const TfDev& t = ap.ch.t[own];
+    if (t.flags & DLX_TF_TRANSLATION) {
+      px -= t.tx; py -= t.ty; qx -= t.tx; qy -= t.ty;
+      jx[2] = -1.0; jy[3] = -1.0; jqx[2] = -1.0; jqy[3] = -1.0;
+    }
+    if (t.flags & DLX_TF_SHEAR) {
+      const double npx = px + qx * t.sx, npy = py + qy * t.sy;
+      const double nqx = qx + px * t.sx, nqy = qy + py * t.sy;
+      double njx[kGeoParams], njy[kGeoParams], njqx[kGeoParams], njqy[kGeoParams];
+      for (int k = 0; k < kGeoParams; ++k) {
+        njx[k] = jx[k] + jqx[k] * t.sx;
+        njy[k] = jy[k] + jqy[k] * t.sy;
+        njqx[k] = jqx[k] + jx[k] * t.sx;
+        njqy[k] = jqy[k] + jy[k] * t.sy;
+      }
+      njx[4] += qx; njy[5] += qy; njqx[4] += px; njqy[5] += py;
+      for (int k = 0; k < kGeoParams; ++k) {
+        jx[k] = njx[k]; jy[k] = njy[k]; jqx[k] = njqx[k]; jqy[k] = njqy[k];
+      }
+      px = npx; py = npy; qx = nqx; qy = nqy;
+    }
+    if (t.flags & DLX_TF_COMPRESSION) {
+      for (int k = 0; k < kGeoParams; ++k) {
+        jx[k] *= t.cx; jy[k] *= t.cy;
+      }
+      jx[6] += px; jy[7] += py;
+      px *= t.cx; py *= t.cy;
+    }
+    if (t.flags & DLX_TF_ROTATION) {
+      for (int k = 0; k < kGeoParams; ++k) {
+        const double a = jx[k], b = jy[k];
+        jx[k] = t.rc * a + t.rs * b;
+        jy[k] = -t.rs * a + t.rc * b;
+      }
+      jx[8] += t.rs * px - t.rc * py;
+      jy[8] += t.rc * px + t.rs * py;
+      const double npx = t.rc * px + t.rs * py, npy = -t.rs * px + t.rc * py;
+      px = npx; py = npy;
+    }
+  }
+  const double x = px, y = py;
+  // edge distance (negative inside) and its partials
+  double dist, gx = 0.0, gy = 0.0, g0 = 0.0, g1 = 0.0;
+  if (ap.kind == DLX_AP_CIRCLE) {
+    const double r = hypot(x, y);
+    dist = r - ap.s0;
+    if (r > 0.0) { gx = x / r; gy = y / r; }
+    g0 = -1.0;
+  } else if (ap.kind == DLX_AP_SQUARE) {
+    if (fabs(x) >= fabs(y)) { dist = fabs(x) - ap.s0 / 2.0; gx = x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); }
+    else { dist = fabs(y) - ap.s0 / 2.0; gy = y > 0 ? 1.0 : (y < 0 ? -1.0 : 0.0); }
+    g0 = -0.5;
+  } else if (ap.kind == DLX_AP_RECTANGLE) {
+    const double ax = fabs(x) - ap.s0 / 2.0, ay = fabs(y) - ap.s1 / 2.0;
+    if (ax >= ay) { dist = ax; gx = x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); g0 = -0.5; }
+    else { dist = ay; gy = y > 0 ? 1.0 : (y < 0 ? -1.0 : 0.0); g1 = -0.5; }
+  } else if (ap.kind == DLX_AP_REG_POLYGON) {
+    int best = 0;
+    dist = -INFINITY;
+    for (int e = 0; e < ap.nedges; ++e) {
+      const double v = ap.ea[e] * x + ap.eb[e] * y + ap.ec[e];
+      if (v > dist) { dist = v; best = e; }
+    }
+    gx = ap.ea[best]; gy = ap.eb[best];
+    g0 = ap.s0 != 0.0 ? ap.ec[best] / ap.s0 : 0.0;   // c scales with rmax
+  } else {   // spider: min over arms of max(|x'| - w/2, y')
+    dist = INFINITY;
+    for (int a = 0; a < ap.nang; ++a) {
+      const double xr = ap.ca[a] * x + ap.sa[a] * y, yr = -ap.sa[a] * x + ap.ca[a] * y;
+      const double bx = fabs(xr) - ap.s0 / 2.0;
+      const double v = fmax(bx, yr);
+      if (v < dist) {
+        dist = v;
+        if (bx >= yr) {
+          const double sg = xr > 0 ? 1.0 : (xr < 0 ? -1.0 : 0.0);
+          gx = sg * ap.ca[a]; gy = sg * ap.sa[a]; g0 = -0.5;
+        } else {
+          gx = -ap.sa[a]; gy = ap.ca[a]; g0 = 0.0;
+        }
+      }
+    }
+  }
+  const double sgn = ap.occulting ? 1.0 : -1.0;   // d = -dist, or +dist when occulting
+  const double d = sgn * dist;
+  dc = fmin(fmax(d, -ap.clip), ap.clip);
+  if (d > -ap.clip && d < ap.clip) {
+    dd[0] = sgn * g0;
+    dd[1] = sgn * g1;
+    for (int k = 2; k < kGeoParams; ++k) dd[k] = sgn * (gx * jx[k] + gy * jy[k]);
+  }
+}
+
+// Per-block partial sums for one aperture a: with gT = dL/dT_a = dL/dout times
+// the other members of its group,
+//   S1[k] = sum gT ddc_k,  Smn[k] / Smx[k] = sum of ddc_k over pixels at the
+//   minimum / maximum, S0 = sum gT, S2 = sum gT (dc - mn), cmn / cmx = counts.
+__global__ void __launch_bounds__(256)
+soft_vjp_kernel(ApDev ap, int a, ApGroups g, int npix, double dx, double c,
+                const float* __restrict__ dout, const double* __restrict__ tplanes,
+                const double2* __restrict__ ext, double* __restrict__ part) {
+  __shared__ double red[kVjpSums][4];
+  const long total = (long)npix * npix;
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  double v[kVjpSums];
+  for (int k = 0; k < kVjpSums; ++k) v[k] = 0.0;
+  if (p < total) {
+    const int i = (int)(p / npix), j = (int)(p - (long)i * npix);
+    double dc, dd[kGeoParams];
+    soft_point_grad(ap, i, j, npix, dx, c, dc, dd);
+    const double mn = ext[a].x, mx = ext[a].y;
+    double gt = dout[p];
+    for (int b = 0; b < g.nap; ++b)
+      if (b != a && g.group[b] == g.group[a]) gt *= tplanes[(long)b * total + p];
+    for (int k = 0; k < kGeoParams; ++k) v[k] = gt * dd[k];
+    if (dc == mn) {
+      for (int k = 0; k < kGeoParams; ++k) v[kGeoParams + k] = dd[k];
+      v[3 * kGeoParams + 2] = 1.0;
+    }
+    if (dc == mx) {
+      for (int k = 0; k < kGeoParams; ++k) v[2 * kGeoParams + k] = dd[k];
+      v[3 * kGeoParams + 3] = 1.0;
+    }
+    v[3 * kGeoParams] = gt;
+    v[3 * kGeoParams + 1] = gt * (dc - mn);
+  }
+  // wave sums, then the 4 waves of the block
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  for (int k = 0; k < kVjpSums; ++k) {
+    double x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[k][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < kVjpSums) {
+    const int k = threadIdx.x;
+    part[(long)blockIdx.x * kVjpSums + k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+  }
+}
+
+// One block per aperture: fixed-order sum of the block partials, then
+//   dL/dth_k = S1_k / R - dmn_k S0 / R - (dmx_k - dmn_k) S2 / R^2
+// with dmn_k = Smn_k / cmn, dmx_k = Smx_k / cmx (0 when the field is constant).
+__global__ void __launch_bounds__(64)
+soft_vjp_final_kernel(int nblk, const double* __restrict__ part, const double2* __restrict__ ext,
+                      float* __restrict__ out) {
+  const int a = blockIdx.x;
+  const int k = threadIdx.x;
+  const double* pa = part + (long)a * nblk * kVjpSums;
+  __shared__ double tot[kVjpSums];
+  if (k < kVjpSums) {
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) acc += pa[(long)b * kVjpSums + k];
+    tot[k] = acc;
+  }
+  __syncthreads();
+  if (k >= kGeoParams) return;
+  const double mn = ext[a].x, mx = ext[a].y, R = mx - mn;
+  double gk = 0.0;
+  if (R != 0.0) {
+    const double cmn = tot[3 * kGeoParams + 2], cmx = tot[3 * kGeoParams + 3];
+    const double dmn = cmn > 0 ? tot[kGeoParams + k] / cmn : 0.0;
+    const double dmx = cmx > 0 ? tot[2 * kGeoParams + k] / cmx : 0.0;
+    const double S0 = tot[3 * kGeoParams], S2 = tot[3 * kGeoParams + 1];
+    gk = tot[k] / R - dmn * S0 / R - (dmx - dmn) * S2 / (R * R);
+  }
+  out[a * kGeoParams + k] = (float)gk;
 }
 
 void noll(int j, int& n, int& m) {
@@ -468,29 +673,22 @@ void polygon_edges(int n, double rmax, ApDev& a) {
 }  // namespace
 }  // namespace dlx
 
-extern "C" size_t dlx_soft_apertures_workspace_size(int32_t npix, int32_t n) {
-  if (npix <= 0 || n <= 0) return 0;
-  const long total = (long)npix * npix;
-  const long nblk = (total + 255) / 256;
-  return align_up((size_t)n * total * sizeof(double)) +
-         align_up((size_t)n * nblk * sizeof(double2));
-}
+namespace dlx {
+namespace {
 
-extern "C" int dlx_soft_apertures(int32_t npix, double pixel_scale, double center, int32_t n,
-                                 const dlx_soft_aperture* aps, float* out, void* workspace,
-                                 size_t workspace_bytes, void* stream) {
-  clear_error();
+// Device descriptors and groups of n soft apertures (validated).
+int build_apertures(int npix, double pixel_scale, int n, const dlx_soft_aperture* aps,
+                    std::vector<ApDev>& dev, ApGroups& g) {
   DLX_CHECK_ARG(npix > 0 && npix <= 46340, "npix must be in [1, 46340], got %d", npix);
   DLX_CHECK_ARG(n >= 1 && n <= kMaxAps, "between 1 and %d apertures, got %d", kMaxAps, n);
-  DLX_CHECK_ARG(aps && out, "null argument to dlx_soft_apertures");
+  DLX_CHECK_ARG(aps, "null aperture descriptors");
   DLX_CHECK_ARG(std::isfinite(pixel_scale) && pixel_scale > 0, "pixel_scale must be > 0");
-  ApGroups g{};
+  g = ApGroups{};
   g.nap = n;
-  std::vector<ApDev> dev(n);
+  dev.assign(n, ApDev{});
   for (int a = 0; a < n; ++a) {
     const dlx_soft_aperture& s = aps[a];
     ApDev& d = dev[a];
-    d = ApDev{};
     d.kind = s.kind;
     d.occulting = s.occulting != 0;
     d.s0 = s.size[0];
@@ -523,32 +721,121 @@ extern "C" int dlx_soft_apertures(int32_t npix, double pixel_scale, double cente
         DLX_CHECK_ARG(false, "unknown aperture kind %d", s.kind);
     }
     g.group[a] = s.group;
-    if (a > 0) {
-      // groups consecutive: a group index, once left, does not come back
-      for (int b = 0; b + 1 < a; ++b)
-        DLX_CHECK_ARG(!(aps[b].group == s.group && aps[a - 1].group != s.group),
-                      "apertures of one group must be consecutive");
-    }
+    // groups consecutive: a group index, once left, does not come back
+    for (int b = 0; b + 1 < a; ++b)
+      DLX_CHECK_ARG(!(aps[b].group == s.group && aps[a - 1].group != s.group),
+                    "apertures of one group must be consecutive");
   }
+  return DLX_OK;
+}
+
+struct SoftLayout {
+  size_t dist, part, tplanes, ext, vpart, total;
+};
+
+SoftLayout soft_layout(int npix, int n, bool vjp) {
+  SoftLayout L{};
+  const long total = (long)npix * npix;
+  const long nblk = (total + 255) / 256;
+  L.dist = 0;
+  L.part = align_up((size_t)n * total * sizeof(double));
+  L.tplanes = L.part + align_up((size_t)n * nblk * sizeof(double2));
+  L.total = L.tplanes;
+  if (vjp) {
+    L.ext = L.tplanes + align_up((size_t)n * total * sizeof(double));
+    L.vpart = L.ext + align_up((size_t)n * sizeof(double2));
+    L.total = L.vpart + align_up((size_t)n * nblk * kVjpSums * sizeof(double));
+  }
+  return L;
+}
+
+// The distance passes of every aperture, then the combination (and, for the
+// VJP, the per-aperture transmissions and extrema).
+int soft_forward(const std::vector<ApDev>& dev, const ApGroups& g, int npix, double pixel_scale,
+                 double center, float* out, uint8_t* ws, const SoftLayout& L, bool vjp,
+                 hipStream_t st) {
   const long total = (long)npix * npix;
   const int nblk = (int)((total + 255) / 256);
-  g.nblk = nblk;
-  const size_t need = dlx_soft_apertures_workspace_size(npix, n);
-  if (workspace_bytes < need || !workspace) {
-    set_error("dlx_soft_apertures workspace too small: need %zu bytes, got %zu", need,
-              workspace_bytes);
-    return DLX_ERR_WORKSPACE;
-  }
-  double* dist = (double*)workspace;
-  double2* part = (double2*)((char*)workspace + align_up((size_t)n * total * sizeof(double)));
-  hipStream_t st = (hipStream_t)stream;
-  for (int a = 0; a < n; ++a) {
+  double* dist = (double*)(ws + L.dist);
+  double2* part = (double2*)(ws + L.part);
+  for (int a = 0; a < g.nap; ++a) {
     hipLaunchKernelGGL(soft_distance_kernel, dim3(nblk), dim3(256), 0, st, dev[a], npix,
                        pixel_scale, center, dist + (long)a * total, part + (long)a * nblk);
     DLX_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(soft_combine_kernel, dim3(nblk), dim3(256), 0, st, g, total, dist, part,
-                     out);
+  ApGroups gg = g;
+  gg.nblk = nblk;
+  hipLaunchKernelGGL(soft_combine_kernel, dim3(nblk), dim3(256), 0, st, gg, total, dist, part,
+                     out, vjp ? (double*)(ws + L.tplanes) : (double*)nullptr,
+                     vjp ? (double2*)(ws + L.ext) : (double2*)nullptr);
+  DLX_HIP(hipGetLastError());
+  return DLX_OK;
+}
+
+}  // namespace
+}  // namespace dlx
+
+extern "C" size_t dlx_soft_apertures_workspace_size(int32_t npix, int32_t n) {
+  if (npix <= 0 || n <= 0) return 0;
+  return soft_layout(npix, n, false).total;
+}
+
+extern "C" int dlx_soft_apertures(int32_t npix, double pixel_scale, double center, int32_t n,
+                                 const dlx_soft_aperture* aps, float* out, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+  clear_error();
+  std::vector<ApDev> dev;
+  ApGroups g;
+  int st = build_apertures(npix, pixel_scale, n, aps, dev, g);
+  if (st) return st;
+  DLX_CHECK_ARG(out, "null argument to dlx_soft_apertures");
+  const SoftLayout L = soft_layout(npix, n, false);
+  if (workspace_bytes < L.total || !workspace) {
+    set_error("dlx_soft_apertures workspace too small: need %zu bytes, got %zu", L.total,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  return soft_forward(dev, g, npix, pixel_scale, center, out, (uint8_t*)workspace, L, false,
+                      (hipStream_t)stream);
+}
+
+extern "C" size_t dlx_soft_apertures_vjp_workspace_size(int32_t npix, int32_t n) {
+  if (npix <= 0 || n <= 0) return 0;
+  return soft_layout(npix, n, true).total;
+}
+
+extern "C" int dlx_soft_apertures_vjp(int32_t npix, double pixel_scale, double center,
+                                     int32_t n, const dlx_soft_aperture* aps, const float* dout,
+                                     float* grads, void* workspace, size_t workspace_bytes,
+                                     void* stream) {
+  clear_error();
+  std::vector<ApDev> dev;
+  ApGroups g;
+  int st = build_apertures(npix, pixel_scale, n, aps, dev, g);
+  if (st) return st;
+  DLX_CHECK_ARG(dout && grads, "null argument to dlx_soft_apertures_vjp");
+  const SoftLayout L = soft_layout(npix, n, true);
+  if (workspace_bytes < L.total || !workspace) {
+    set_error("dlx_soft_apertures_vjp workspace too small: need %zu bytes, got %zu", L.total,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  uint8_t* ws = (uint8_t*)workspace;
+  hipStream_t s = (hipStream_t)stream;
+  if ((st = soft_forward(dev, g, npix, pixel_scale, center, nullptr, ws, L, true, s))) return st;
+  const long total = (long)npix * npix;
+  const int nblk = (int)((total + 255) / 256);
+  ApGroups gg = g;
+  gg.nblk = nblk;
+  double* vpart = (double*)(ws + L.vpart);
+  for (int a = 0; a < n; ++a) {
+    hipLaunchKernelGGL(soft_vjp_kernel, dim3(nblk), dim3(256), 0, s, dev[a], a, gg, npix,
+                       pixel_scale, center, dout, (const double*)(ws + L.tplanes),
+                       (const double2*)(ws + L.ext), vpart + (long)a * nblk * kVjpSums);
+    DLX_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(soft_vjp_final_kernel, dim3(n), dim3(64), 0, s, nblk, vpart,
+                     (const double2*)(ws + L.ext), grads);
   DLX_HIP(hipGetLastError());
   return DLX_OK;
 }

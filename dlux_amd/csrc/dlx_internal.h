@@ -95,7 +95,9 @@ enum Epilogue {
   EPI_VJP = 4,          // C(m,n) = 2 G(m,n) alpha acc (complex64, ldc)
   EPI_VJP_SPLIT_T = 5,  // as EPI_SPLIT_T with value 2 G(n,m) alpha acc (split4 rows n, k = m)
   EPI_GRAD = 6,         // C(m,n) = Im(conj(P(m,n)) alpha acc) (float, ldc); P complex64
-                        // (fp32 kernels, p_ld elements) or split4 (p_ld bytes per row)
+                        // (fp32 kernels, p_ld elements) or split4 (p_ld bytes per row);
+                        // C2 (if set, same layout as C) = Re(conj(U(m,n)) alpha acc),
+                        // U in P's layout
 };
 
 struct GemmBatch {
@@ -109,6 +111,8 @@ struct GemmBatch {
   float alpha[kMaxBatch];   // per-instance scale of the result
   const float* G; long ldg; long g_batch;          // EPI_VJP
   const void* P; long p_ld; long p_batch;           // EPI_GRAD
+  float* C2;                                        // EPI_GRAD real part (optional)
+  const void* U;                                    // EPI_GRAD: phasor for C2, P's layout
 };
 
 hipError_t launch_cgemm_nt(const GemmBatch& g, hipStream_t s);
@@ -125,6 +129,8 @@ struct GemmSplit {
   float alpha[kMaxBatch];
   const float* G; long ldg; long g_batch;          // EPI_VJP_SPLIT_T
   const void* P; long p_ld; long p_batch;           // EPI_GRAD (split4, bytes)
+  float* C2;                                        // EPI_GRAD real part (optional)
+  const void* U;                                    // EPI_GRAD: phasor for C2, P's layout
 };
 
 hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s);
@@ -189,6 +195,7 @@ hipError_t launch_sum_planes(long npix, int nplanes, const float* planes, int ac
 
 // Backward reductions (dlx_grad.hip).  partial holds nbasis * nsets * project_parts(npix)
 // floats.
+// g[p] /= amp[p] where amp[p] > 0, else 0 (the amplitude cotangent from Re(conj(P) Pbar)).
 hipError_t launch_grad_reduce(const GradChunk& c, int nb, long npix, const float* h,
                               int accumulate, float* g_opd, float* g_phase, hipStream_t s);
 int project_parts(long npix);

@@ -11,8 +11,13 @@ and one whose aperture moves regenerates them on the device, never on the host.
 
 All of them lower into the fused pupil plan (amplitude, normalisation and OPD
 terms), so an optical system built from them keeps the single dlx_psf_poly
-path.  Gradients flow to AberratedAperture coefficients; gradients with
-respect to aperture geometry (radius, transformation, ...) are not implemented.
+path.  Gradients flow to AberratedAperture coefficients and, through
+dlx_soft_apertures_vjp, to the geometry of the soft apertures: a size
+(radius, width, height, rmax) or a parameter of the innermost transformation
+that applies to an aperture, given as a torch tensor that requires grad.
+Gradients with respect to softening, spider angles, an outer transformation
+under an aperture's own, or the geometry of an aberrated aperture's basis
+raise NotImplementedError.
 """
 
 from __future__ import annotations
@@ -21,7 +26,7 @@ import numpy as np
 import torch
 
 from .. import _lib, runtime
-from ..coordinates import BaseCoordTransform
+from ..coordinates import BaseCoordTransform, CoordTransform, _floats, _is_tensor
 from .optical_layers import OpticalLayer
 
 __all__ = ["ApertureLayer", "BaseDynamicAperture", "DynamicAperture", "CircularAperture",
@@ -38,6 +43,33 @@ def _grid_of(coords):
     x = c[0, 0]
     centre = float(0.5 * (x[0] + x[-1]))
     return int(c.shape[-1]), centre
+
+
+def _param(v):
+    """A geometric parameter: a torch tensor is kept (gradients can reach it),
+    anything else becomes a float64 numpy array."""
+    return v if _is_tensor(v) else np.asarray(v, float)
+
+
+def _tracked(v) -> bool:
+    return _is_tensor(v) and v.requires_grad
+
+
+def _grad_sources(items):
+    """(tensors, map) for the geometry VJP: the distinct tracked tensors and,
+    per (aperture, slot) that feeds one, (aperture, slot, tensor index,
+    element or None)."""
+    params, smap = [], []
+    for a, (_, _, src) in enumerate(items):
+        for slot, t, elem in src:
+            for i, q in enumerate(params):
+                if q is t:
+                    break
+            else:
+                params.append(t)
+                i = len(params) - 1
+            smap.append((a, slot, i, elem))
+    return params, smap
 
 
 class ApertureLayer(OpticalLayer):
@@ -64,10 +96,14 @@ class ApertureLayer(OpticalLayer):
         return value
 
     def _grid_transmission(self, npix: int, pixel_scale: float, centre: float = 0.0):
-        descs = self._descs(())
+        items = self._descs(())
+        params, smap = _grad_sources(items)
+        if params and torch.is_grad_enabled():
+            return runtime.soft_apertures_tracked(npix, pixel_scale, _grouped(items), centre,
+                                                  params, smap)
         key = ("t", npix, float(pixel_scale), float(centre), self._param_key())
         return self._cache(key, lambda: runtime.soft_apertures(
-            npix, pixel_scale, _grouped(descs), centre))
+            npix, pixel_scale, _grouped(items), centre))
 
     def _param_key(self):
         raise NotImplementedError
@@ -76,7 +112,7 @@ class ApertureLayer(OpticalLayer):
 def _grouped(descs):
     """Descriptors with consecutive integer group ids (same key -> same group)."""
     out, ids = [], {}
-    for key, d in descs:
+    for key, d, _ in descs:
         d.group = ids.setdefault(key, len(ids))
         out.append(d)
     return out
@@ -103,8 +139,8 @@ class BaseDynamicAperture(ApertureLayer):
         raise AttributeError(f"{type(self).__name__} has no attribute {key!r}")
 
     def _transforms(self, outer):
-        return tuple(outer) + ((self.transformation._struct(),)
-                               if self.transformation is not None else ())
+        return tuple(outer) + ((self.transformation,) if self.transformation is not None
+                               else ())
 
     def _tkey(self):
         return None if self.transformation is None else self.transformation._key()
@@ -136,8 +172,8 @@ class DynamicAperture(BaseDynamicAperture):
                  normalise: bool = False):
         super().__init__(transformation=transformation, normalise=normalise)
         self.occulting = bool(occulting)
-        self.softness = np.asarray(softening, float)
-        if self.softness <= 0:
+        self.softness = _param(softening)
+        if _floats(self.softness)[0] <= 0:
             raise ValueError("softening must be greater than 0.")
 
     def _fill(self, d):  # pragma: no cover - per shape
@@ -147,18 +183,41 @@ class DynamicAperture(BaseDynamicAperture):
         d = _lib.SoftAperture()
         d.kind = self._kind
         d.occulting = int(self.occulting)
-        d.softening = float(self.softness)
+        d.softening = _floats(self.softness)[0]
         tfs = self._transforms(transforms)
         if len(tfs) > 3:
             raise NotImplementedError("at most three nested coordinate transformations")
         d.n_transforms = len(tfs)
         for k, t in enumerate(tfs):
-            d.transforms[k] = t
+            d.transforms[k] = t._struct()
         self._fill(d)
-        return [(id(self) if group is None else group, d)]
+        return [(id(self) if group is None else group, d, self._sources(tfs))]
+
+    def _sizes(self):
+        """(slot, parameter) of the sizes the geometry VJP differentiates."""
+        return ()
+
+    def _sources(self, tfs):
+        """(slot, tensor, element) of the tracked parameters of this aperture."""
+        if torch.is_grad_enabled():
+            if _tracked(self.softness):
+                raise NotImplementedError("gradients with respect to softening are not "
+                                          "implemented")
+            for t in tfs[:-1]:
+                if any(_tracked(getattr(t, f, None)) for _, f, _ in CoordTransform._SLOTS):
+                    raise NotImplementedError(
+                        "gradients with respect to a transformation are implemented for the "
+                        "innermost transformation that applies to an aperture only")
+        src = [(slot, v, None) for slot, v in self._sizes() if _tracked(v)]
+        if tfs and isinstance(tfs[-1], CoordTransform):
+            for slot, f, elem in CoordTransform._SLOTS:
+                v = getattr(tfs[-1], f)
+                if _tracked(v):
+                    src.append((slot, v, elem))
+        return src
 
     def _param_key(self):
-        return (type(self).__name__, self.occulting, float(self.softness), self._tkey(),
+        return (type(self).__name__, self.occulting, _floats(self.softness), self._tkey(),
                 self._shape_key())
 
 
@@ -170,13 +229,16 @@ class CircularAperture(DynamicAperture):
     def __init__(self, radius, transformation=None, occulting: bool = False, softening=1.0,
                  normalise: bool = False):
         super().__init__(transformation, occulting, softening, normalise)
-        self.radius = np.asarray(radius, float)
+        self.radius = _param(radius)
 
     def _fill(self, d):
-        d.size[0] = float(self.radius)
+        d.size[0] = _floats(self.radius)[0]
+
+    def _sizes(self):
+        return ((0, self.radius),)
 
     def _shape_key(self):
-        return (float(self.radius),)
+        return _floats(self.radius)
 
     @property
     def extent(self):
@@ -195,13 +257,16 @@ class SquareAperture(DynamicAperture):
     def __init__(self, width, transformation=None, occulting: bool = False, softening=1.0,
                  normalise: bool = False):
         super().__init__(transformation, occulting, softening, normalise)
-        self.width = np.asarray(width, float)
+        self.width = _param(width)
 
     def _fill(self, d):
-        d.size[0] = float(self.width)
+        d.size[0] = _floats(self.width)[0]
+
+    def _sizes(self):
+        return ((0, self.width),)
 
     def _shape_key(self):
-        return (float(self.width),)
+        return _floats(self.width)
 
     @property
     def extent(self):
@@ -220,19 +285,22 @@ class RectangularAperture(DynamicAperture):
     def __init__(self, height, width, transformation=None, occulting: bool = False,
                  softening=1.0, normalise: bool = False):
         super().__init__(transformation, occulting, softening, normalise)
-        self.height = np.asarray(height, float)
-        self.width = np.asarray(width, float)
+        self.height = _param(height)
+        self.width = _param(width)
 
     def _fill(self, d):
-        d.size[0] = float(self.width)
-        d.size[1] = float(self.height)
+        d.size[0] = _floats(self.width)[0]
+        d.size[1] = _floats(self.height)[0]
+
+    def _sizes(self):
+        return ((0, self.width), (1, self.height))
 
     def _shape_key(self):
-        return (float(self.width), float(self.height))
+        return _floats(self.width) + _floats(self.height)
 
     @property
     def extent(self):
-        return np.hypot(self.height / 2.0, self.width / 2.0)
+        return (self.height ** 2 + self.width ** 2) ** 0.5 / 2.0
 
     @property
     def nsides(self) -> int:
@@ -249,14 +317,17 @@ class RegPolyAperture(DynamicAperture):
                  softening=1.0, normalise: bool = False):
         super().__init__(transformation, occulting, softening, normalise)
         self._nsides = int(nsides)
-        self.rmax = np.asarray(rmax, float)
+        self.rmax = _param(rmax)
 
     def _fill(self, d):
-        d.size[0] = float(self.rmax)
+        d.size[0] = _floats(self.rmax)[0]
         d.nsides = self._nsides
 
+    def _sizes(self):
+        return ((0, self.rmax),)
+
     def _shape_key(self):
-        return (float(self.rmax), self._nsides)
+        return _floats(self.rmax) + (self._nsides,)
 
     @property
     def extent(self):
@@ -276,20 +347,26 @@ class Spider(DynamicAperture):
     def __init__(self, width, angles, transformation=None, occulting: bool = True,
                  softening=1.0, normalise: bool = False):
         super().__init__(transformation, occulting, softening, normalise)
-        self.width = np.asarray(width, float)
-        self.angles = np.asarray(angles, float)
+        self.width = _param(width)
+        self.angles = _param(angles)
 
     def _fill(self, d):
-        angles = np.atleast_1d(self.angles)
-        if not 1 <= angles.size <= 16:
-            raise ValueError(f"a spider needs 1 to 16 arms, got {angles.size}")
-        d.size[0] = float(self.width)
-        d.n_angles = angles.size
+        angles = _floats(self.angles)
+        if not 1 <= len(angles) <= 16:
+            raise ValueError(f"a spider needs 1 to 16 arms, got {len(angles)}")
+        d.size[0] = _floats(self.width)[0]
+        d.n_angles = len(angles)
         for k, a in enumerate(angles):
-            d.angles_deg[k] = float(a)
+            d.angles_deg[k] = a
+
+    def _sizes(self):
+        if torch.is_grad_enabled() and _tracked(self.angles):
+            raise NotImplementedError("gradients with respect to spider angles are not "
+                                      "implemented")
+        return ((0, self.width),)
 
     def _shape_key(self):
-        return (float(self.width), tuple(np.atleast_1d(self.angles).tolist()))
+        return _floats(self.width) + (None,) + _floats(self.angles)
 
     @property
     def extent(self):
@@ -346,15 +423,23 @@ class AberratedAperture(ApertureLayer):
         return self.aperture._param_key()
 
     def _grid_basis(self, npix, pixel_scale, centre=0.0, outer=()):
-        tfs = tuple(outer) + ((self.aperture.transformation._struct(),)
-                              if self.aperture.transformation is not None else ())
+        if torch.is_grad_enabled() and (
+                any(_tracked(v) for _, v in self.aperture._sizes())
+                or any(_tracked(getattr(t, f, None)) for t in
+                       tuple(outer) + (self.aperture.transformation,) if t is not None
+                       for _, f, _ in CoordTransform._SLOTS)):
+            raise NotImplementedError("gradients with respect to the geometry of an "
+                                      "aberrated aperture's basis are not implemented")
+        tfs = tuple(t._struct() for t in outer) + (
+            (self.aperture.transformation._struct(),)
+            if self.aperture.transformation is not None else ())
         key = ("b", npix, float(pixel_scale), float(centre), self.aperture._param_key(),
-               tuple(self.noll_inds), tuple(bytes(t) for t in outer))
+               tuple(self.noll_inds), tuple(bytes(t) for t in tfs))
         cached = self.__dict__.get("_basis")
         if cached is not None and cached[0] == key:
             return cached[1]
         basis = runtime.aperture_basis(npix, pixel_scale, self.noll_inds, tfs,
-                                       float(self.aperture.extent), self.aperture.nsides,
+                                       _floats(self.aperture.extent)[0], self.aperture.nsides,
                                        centre)
         self.__dict__["_basis"] = (key, basis)
         return basis
@@ -436,7 +521,7 @@ class CompositeAperture(BaseDynamicAperture):
         (CompositeAperture.calc_basis, layers/apertures.py:911-929), the basis
         coordinates carry only the outermost composite's transformation and
         the member aperture's own, not those of intermediate composites."""
-        outer = tuple(outer) or ((self.transformation._struct(),)
+        outer = tuple(outer) or ((self.transformation,)
                                  if self.transformation is not None else ())
         total = None
         for ap in self._aberrated_apertures():

@@ -128,7 +128,11 @@ class PupilPlan:
         # copies of a system (Base.set) share the cache and hit it while their
         # transmission arrays are the same, unmodified tensors.
         key = self._amp_key()
-        if key in amp_cache:
+        # an amplitude that carries an autograd graph is rebuilt every call
+        tracked = torch.is_grad_enabled() and any(
+            op[0] == "mul" and isinstance(op[1], torch.Tensor) and op[1].requires_grad
+            for op in self.amp_ops)
+        if not tracked and key in amp_cache:
             self.amp, self.scale, _ = amp_cache[key]
         else:
             amp, scale = None, 1.0 / self.n ** 2
@@ -145,9 +149,10 @@ class PupilPlan:
                     ref = torch.sum(e2) if op[1] == "power" else torch.max(e2)
                     amp = (a.double() * (scale * torch.sqrt(op[2] / ref))).float()
                     scale = 1.0
-            amp_cache.clear()
-            refs = [op[1] for op in self.amp_ops if op[0] == "mul"]
-            amp_cache[key] = (amp, scale, refs)
+            if not tracked:
+                amp_cache.clear()
+                refs = [op[1] for op in self.amp_ops if op[0] == "mul"]
+                amp_cache[key] = (amp, scale, refs)
             self.amp, self.scale = amp, scale
         for attr, terms in (("opd", self.opd_terms), ("phase", self.phase_terms)):
             acc = None

@@ -332,7 +332,8 @@ def _psf_poly_impl(p: dict, amp_t, opd_t, ph_t, out=None, ws=None) -> torch.Tens
     return out
 
 
-def _psf_poly_vjp_impl(p: dict, amp_t, opd_t, ph_t, dpsf, need_opd=True, need_phase=False):
+def _psf_poly_vjp_impl(p: dict, amp_t, opd_t, ph_t, dpsf, need_opd=True, need_phase=False,
+                      need_amp=False):
     lib = _lib.load()
     dev = p["dev"]
     n = p["n_in"]
@@ -340,6 +341,7 @@ def _psf_poly_vjp_impl(p: dict, amp_t, opd_t, ph_t, dpsf, need_opd=True, need_ph
     ws = workspace(need, dev)
     g_opd = torch.empty((n, n), dtype=torch.float32, device=dev) if need_opd else None
     g_ph = torch.empty((n, n), dtype=torch.float32, device=dev) if need_phase else None
+    g_amp = torch.empty((n, n), dtype=torch.float32, device=dev) if need_amp else None
     d = as_float(dpsf, dev)
     if tuple(d.shape) != (p["n_out"], p["n_out"]):
         raise ValueError(f"dpsf must have shape {(p['n_out'], p['n_out'])}, "
@@ -347,9 +349,11 @@ def _psf_poly_vjp_impl(p: dict, amp_t, opd_t, ph_t, dpsf, need_opd=True, need_ph
     st = lib.dlx_psf_poly_vjp(p["g"], p["wl"].size, _lib.dptr(p["wl"]), _lib.dptr(p["w"]),
                               p["dx"], p["pout"], p["center"], p["tilt"][0], p["tilt"][1],
                               _ptr(amp_t), _ptr(opd_t), _ptr(ph_t), p["scale"], d.data_ptr(),
-                              _ptr(g_opd), _ptr(g_ph), p["prec"], ws.data_ptr(), ws.numel(),
-                              stream_handle())
+                              _ptr(g_opd), _ptr(g_ph), _ptr(g_amp), p["prec"], ws.data_ptr(),
+                              ws.numel(), stream_handle())
     _lib.check(st, "dlx_psf_poly_vjp")
+    if need_amp:
+        return g_opd, g_ph, g_amp
     return g_opd, g_ph
 
 
@@ -365,10 +369,16 @@ class _PsfPolyFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dpsf):
         opd_t, ph_t, amp_t = ctx.saved_tensors
-        g_opd, g_ph = _psf_poly_vjp_impl(ctx.p, amp_t, opd_t, ph_t, dpsf.contiguous(),
-                                         need_opd=ctx.needs_input_grad[0],
-                                         need_phase=ctx.needs_input_grad[1])
-        return g_opd, g_ph, None, None
+        need_amp = bool(ctx.needs_input_grad[2])
+        out = _psf_poly_vjp_impl(ctx.p, amp_t, opd_t, ph_t, dpsf.contiguous(),
+                                 need_opd=ctx.needs_input_grad[0],
+                                 need_phase=ctx.needs_input_grad[1], need_amp=need_amp)
+        g_amp = None
+        if need_amp:
+            g_opd, g_ph, g_amp = out   # dL/d amp at fixed scale (P already carries it)
+        else:
+            g_opd, g_ph = out
+        return g_opd, g_ph, g_amp, None
 
 
 def psf_poly(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: float,
@@ -378,9 +388,9 @@ def psf_poly(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: float,
              ws: torch.Tensor | None = None) -> torch.Tensor:
     """Fused polychromatic PSF sum_l w_l |MFT_l(P_l)|^2 -> (n_out, n_out) float32.
 
-    Differentiable (torch autograd) with respect to opd and phase: the
-    backward is dlx_psf_poly_vjp.  Gradients with respect to amp are not
-    implemented and raise.
+    Differentiable (torch autograd) with respect to opd, phase and amp: the
+    backward is dlx_psf_poly_vjp (dL/d amp at the fixed scale, from the pupil
+    phasor, so it is exact where amp == 0 too).
     """
     wl, w, g, prec = _poly_args(n_in, n_out, wavelengths, weights, focal_length, inverse,
                                 precision)
@@ -391,9 +401,6 @@ def psf_poly(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: float,
              tilt=(float(tilt[0]), float(tilt[1])), scale=float(scale))
     if torch.is_grad_enabled() and any(t is not None and t.requires_grad
                                        for t in (amp_t, opd_t, ph_t)):
-        if amp_t is not None and amp_t.requires_grad:
-            raise NotImplementedError("psf_poly: gradient with respect to the amplitude is not "
-                                      "implemented on the fused path")
         if out is not None:
             raise ValueError("psf_poly: out= cannot be used when gradients are recorded")
         return _PsfPolyFn.apply(opd_t, ph_t, amp_t, p)
@@ -404,8 +411,9 @@ def psf_poly_vjp(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: fl
                  pixel_scale_out: float, dpsf, amp=None, opd=None, phase=None, scale=1.0,
                  tilt=(0.0, 0.0), center=0.0, focal_length=None, inverse=False,
                  precision: str | None = None, need_opd: bool = True,
-                 need_phase: bool = False):
-    """(dL/dOPD, dL/dphase) of the fused PSF for dL/dPSF = dpsf (either may be None)."""
+                 need_phase: bool = False, need_amp: bool = False):
+    """(dL/dOPD, dL/dphase) of the fused PSF for dL/dPSF = dpsf (either may be
+    None), and dL/d amp (at the fixed scale) third when ``need_amp``."""
     wl, w, g, prec = _poly_args(n_in, n_out, wavelengths, weights, focal_length, inverse,
                                 precision)
     dev = device()
@@ -413,7 +421,7 @@ def psf_poly_vjp(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: fl
     p = dict(dev=dev, g=g, wl=wl, w=w, prec=prec, n_in=int(n_in), n_out=int(n_out),
              dx=float(pixel_scale_in), pout=float(pixel_scale_out), center=float(center),
              tilt=(float(tilt[0]), float(tilt[1])), scale=float(scale))
-    return _psf_poly_vjp_impl(p, amp_t, opd_t, ph_t, dpsf, need_opd, need_phase)
+    return _psf_poly_vjp_impl(p, amp_t, opd_t, ph_t, dpsf, need_opd, need_phase, need_amp)
 
 
 class MftBatch:
@@ -561,6 +569,66 @@ def soft_apertures(npix: int, pixel_scale: float, apertures, center: float = 0.0
                                 out.data_ptr(), ws.data_ptr(), ws.numel(), stream_handle())
     _lib.check(st, "dlx_soft_apertures")
     return out
+
+
+def soft_apertures_vjp(npix: int, pixel_scale: float, apertures, dout: torch.Tensor,
+                       center: float = 0.0) -> torch.Tensor:
+    """(n, 9) float32 dL/dtheta of each aperture's geometry parameters (size
+    0, size 1, and translation x, y, shear x, y, compression x, y, rotation of
+    the last transform of its chain) given dL/dtransmission
+    (dlx_soft_apertures_vjp)."""
+    n = len(apertures)
+    dev = device()
+    lib = _lib.load()
+    dout = dout.detach()
+    if dout.dtype != torch.float32 or not dout.is_contiguous() or dout.device != dev:
+        dout = dout.to(device=dev, dtype=torch.float32).contiguous()
+    if tuple(dout.shape) != (npix, npix):
+        raise ValueError(f"dout must be ({npix}, {npix}), got {tuple(dout.shape)}")
+    arr = (_lib.SoftAperture * max(n, 1))(*apertures)
+    grads = torch.empty((max(n, 1), 9), dtype=torch.float32, device=dev)
+    ws = workspace(lib.dlx_soft_apertures_vjp_workspace_size(int(npix), n), dev)
+    st = lib.dlx_soft_apertures_vjp(int(npix), float(pixel_scale), float(center), n, arr,
+                                    dout.data_ptr(), grads.data_ptr(), ws.data_ptr(),
+                                    ws.numel(), stream_handle())
+    _lib.check(st, "dlx_soft_apertures_vjp")
+    return grads[:n]
+
+
+class _SoftAperturesFn(torch.autograd.Function):
+    """Transmission of soft apertures as a function of their tracked geometry
+    tensors; the backward is dlx_soft_apertures_vjp."""
+
+    @staticmethod
+    def forward(ctx, npix, pixel_scale, center, apertures, smap, *params):
+        ctx.meta = (npix, pixel_scale, center, apertures, smap)
+        ctx.shapes = [(p.shape, p.dtype, p.device) for p in params]
+        return soft_apertures(npix, pixel_scale, apertures, center)
+
+    @staticmethod
+    def backward(ctx, gout):
+        npix, pixel_scale, center, apertures, smap = ctx.meta
+        g = soft_apertures_vjp(npix, pixel_scale, apertures, gout, center).double().cpu()
+        grads = [None] * len(ctx.shapes)
+        for a, slot, i, elem in smap:
+            shape, dtype, dev = ctx.shapes[i]
+            part = torch.zeros(shape, dtype=torch.float64)
+            if elem is None:
+                part.fill_(float(g[a, slot]))
+            else:
+                part.reshape(-1)[elem] = g[a, slot]
+            grads[i] = part if grads[i] is None else grads[i] + part
+        grads = [None if q is None else q.to(device=ctx.shapes[i][2], dtype=ctx.shapes[i][1])
+                 for i, q in enumerate(grads)]
+        return (None, None, None, None, None) + tuple(grads)
+
+
+def soft_apertures_tracked(npix: int, pixel_scale: float, apertures, center, params, smap):
+    """soft_apertures whose result carries gradients to the geometry tensors
+    ``params``; ``smap`` lists (aperture, parameter slot, tensor index,
+    element or None) for each slot that feeds a tensor."""
+    return _SoftAperturesFn.apply(int(npix), float(pixel_scale), float(center),
+                                  list(apertures), list(smap), *params)
 
 
 def aperture_basis(npix: int, pixel_scale: float, js, transforms=(), extent: float = 1.0,

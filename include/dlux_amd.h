@@ -167,10 +167,14 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
  * N_out x N_out), writes
  *   grad_opd[p]   = sum_l (2 pi/lambda_l) Im(conj(P_l[p]) Pbar_l[p])
  *   grad_phase[p] = sum_l Im(conj(P_l[p]) Pbar_l[p])
+ *   grad_amp[p]   = sum_l Re(conj(U_l[p]) Pbar_l[p]),  U_l = P_l / amp (P_l at unit
+ *                   amplitude, so defined where amp[p] == 0 too)
  * with Pbar_l = sqrt(w_l) c_l conj(Y_l) (2 dpsf * E_l) X_l^H, the conjugate-
- * transpose contractions of the forward MFT (either output may be NULL, not
- * both).  The per-wavelength fields are recomputed, not stored: the work is
- * four contractions per wavelength.  Arguments as dlx_psf_poly.
+ * transpose contractions of the forward MFT (any output may be NULL, not all).
+ * grad_amp is dL/d amp at fixed scale; the cotangent of a power normalisation
+ * folded into amp is left to the caller's chain rule.  The per-wavelength
+ * fields are recomputed, not stored: the work is four contractions per
+ * wavelength (and one more pupil build for grad_amp).  Arguments as dlx_psf_poly.
  * Replaces the reverse-mode derivative JAX builds of the same chain
  * (jax.grad through OpticalSystem.propagate; MFT at utils/propagation.py:178-256,
  * phase/OPD application at wavefronts.py:332-367, |E|^2 at wavefronts.py:268-283).
@@ -182,7 +186,7 @@ int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavele
                      double pixel_scale_out, double center, double tilt_x,
                      double tilt_y, const float* amp, const float* opd,
                      const float* phase, double scale, const float* dpsf,
-                     float* grad_opd, float* grad_phase, int32_t precision,
+                     float* grad_opd, float* grad_phase, float* grad_amp, int32_t precision,
                      void* workspace, size_t workspace_bytes, void* stream);
 
 /*
@@ -339,6 +343,22 @@ size_t dlx_soft_apertures_workspace_size(int32_t npix, int32_t n);
 int dlx_soft_apertures(int32_t npix, double pixel_scale, double center, int32_t n,
                       const dlx_soft_aperture* apertures, float* out, void* workspace,
                       size_t workspace_bytes, void* stream);
+
+/*
+ * Vector-Jacobian product of dlx_soft_apertures with respect to each
+ * aperture's geometry: given dout = dL/dout (npix x npix float32), writes
+ * grads[a * 9 + k] = dL/dtheta_k for theta = (size[0], size[1], and the last
+ * transform of its chain: translation x, y, shear x, y, compression x, y,
+ * rotation); parameters a transform does not have get 0.  soften's extrema
+ * contribute through the average over the pixels that attain them (the rule
+ * JAX applies to min / max).  Replaces the reverse-mode derivative JAX builds
+ * through DynamicAperture.transmission (layers/apertures.py:299-641) and soften
+ * (utils/geometry.py:49-92).
+ */
+size_t dlx_soft_apertures_vjp_workspace_size(int32_t npix, int32_t n);
+int dlx_soft_apertures_vjp(int32_t npix, double pixel_scale, double center, int32_t n,
+                           const dlx_soft_aperture* apertures, const float* dout, float* grads,
+                           void* workspace, size_t workspace_bytes, void* stream);
 
 /*
  * Zernike basis of an aberrated dynamic aperture (AberratedAperture.calc_basis,

@@ -186,3 +186,182 @@ def test_compound_normalises_after_aberrations(dl):
     plain = L.CircularAperture(0.1)(wf)
     amp = L.AberratedAperture(L.CircularAperture(0.1), [1, 2, 3, 4], effect="amplitude")(wf)
     assert torch.allclose(amp.amplitude, plain.amplitude)
+
+
+# --- geometry gradients -------------------------------------------------------
+
+_SLOT_FIELDS = [("size", 0), ("size", 1), ("translation", 0), ("translation", 1), ("shear", 0),
+                ("shear", 1), ("compression", 0), ("compression", 1), ("rotation", None)]
+
+
+def _oracle_of(kind, shape, tf):
+    """Oracle transmission as a function of the 9 parameter slots."""
+    def f(theta, coords, dx, softening, occulting):
+        sh = dict(shape)
+        if kind == "circle":
+            sh["radius"] = theta[0]
+        elif kind in ("square", "spider"):
+            sh["width"] = theta[0]
+        elif kind == "rectangle":
+            sh["width"], sh["height"] = theta[0], theta[1]
+        elif kind == "reg_polygon":
+            sh["rmax"] = theta[0]
+        t = {}
+        if "translation" in tf:
+            t["translation"] = [theta[2], theta[3]]
+        if "shear" in tf:
+            t["shear"] = [theta[4], theta[5]]
+        if "compression" in tf:
+            t["compression"] = [theta[6], theta[7]]
+        if "rotation" in tf:
+            t["rotation"] = theta[8]
+        return orc.soft_aperture(kind, coords, dx, softening=softening, occulting=occulting,
+                                 transforms=[t] if t else [], **sh)
+    return f
+
+
+def _theta0(kind, shape, tf):
+    th = np.zeros(9)
+    th[0] = {"circle": shape.get("radius"), "square": shape.get("width"),
+             "spider": shape.get("width"), "rectangle": shape.get("width"),
+             "reg_polygon": shape.get("rmax")}[kind]
+    if kind == "rectangle":
+        th[1] = shape["height"]
+    for k, (name, e) in enumerate(_SLOT_FIELDS[2:], start=2):
+        if name in tf:
+            th[k] = tf[name] if e is None else tf[name][e]
+    return th
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_soft_aperture_geometry_vjp_vs_fd(dl, case):
+    """dlx_soft_apertures_vjp against central differences of the oracle
+    transmission, L = sum(dout * T) for a random dout; every slot the
+    aperture has.  Bound 1e-5 of the largest gradient."""
+    from dlux_amd import layers as L, runtime
+    kind, ap, shape, occ = _cases(dl, L)[case]
+    print(f"[aperture vjp] {kind} occulting={occ}", flush=True)
+    n, dx = 128, 1.0 / 128
+    coords = orc.pixel_coords(n, 1.0)
+    tf = TF if ap.transformation is not None and ap.transformation.translation is not None \
+        else {}
+    descs = [d for _, d, _ in ap._descs(())]
+    dout = np.random.default_rng(case).normal(size=(n, n))
+    g = runtime.soft_apertures_vjp(n, dx, descs, torch.as_tensor(dout, device="cuda")) \
+        .double().cpu().numpy()[0]
+    f = _oracle_of(kind, shape, tf)
+    th0 = _theta0(kind, shape, tf)
+    slots = [0] + ([1] if kind == "rectangle" else []) + (list(range(2, 9)) if tf else [])
+    ref = np.zeros(9)
+    for k in slots:
+        h = 1e-7 * max(1.0, abs(th0[k]))
+        tp, tm = th0.copy(), th0.copy()
+        tp[k] += h
+        tm[k] -= h
+        ref[k] = np.sum(dout * (f(tp, coords, dx, float(ap.softness), occ)
+                                - f(tm, coords, dx, float(ap.softness), occ))) / (2 * h)
+    err = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
+    print(f"  grad {np.array2string(g[slots], precision=4)}\n  err {err:.2e}")
+    assert err < 1e-5
+    others = [k for k in range(9) if k not in slots]
+    assert np.all(g[others] == 0)
+
+
+def test_compound_geometry_vjp_vs_fd(dl):
+    """Geometry gradients through a CompoundAperture (product) inside a
+    MultiAperture (sum): the other members of a group enter the cotangent."""
+    print("[aperture vjp] compound / multi", flush=True)
+    from dlux_amd import layers as L, runtime
+    n, dx = 96, 1.0 / 96
+    c = orc.pixel_coords(n, 1.0)
+    inner = dict(translation=[-0.1, 0.05], rotation=0.2)
+    ap = L.MultiAperture([
+        L.CompoundAperture([L.CircularAperture(0.3, dl.CoordTransform(**inner)),
+                            L.Spider(0.02, [30.0, 150.0], dl.CoordTransform(**inner))]),
+        L.SquareAperture(0.2, dl.CoordTransform(translation=[0.25, 0.2])),
+    ])
+    from dlux_amd.layers.apertures import _grouped
+    descs = _grouped(ap._descs(()))
+    dout = np.random.default_rng(5).normal(size=(n, n))
+    g = runtime.soft_apertures_vjp(n, dx, descs, torch.as_tensor(dout, device="cuda")) \
+        .double().cpu().numpy()
+    assert len(descs) == 3
+
+    def T(r, w, s, t0):
+        tf_in = dict(translation=[t0, 0.05], rotation=0.2)
+        return (orc.soft_aperture("circle", c, dx, transforms=[tf_in], radius=r)
+                * orc.soft_aperture("spider", c, dx, occulting=True, transforms=[tf_in],
+                                    width=w, angles=[30.0, 150.0])
+                + orc.soft_aperture("square", c, dx, transforms=[dict(translation=[0.25, 0.2])],
+                                    width=s))
+    base = [0.3, 0.02, 0.2, -0.1]
+    ref = []
+    for k in range(4):
+        h = 1e-7
+        p, m = list(base), list(base)
+        p[k] += h
+        m[k] -= h
+        ref.append(np.sum(dout * (T(*p) - T(*m))) / (2 * h))
+    # translation x of the circle and spider's transforms both feed t0
+    got = [g[0, 0], g[1, 0], g[2, 0], g[0, 2] + g[1, 2]]
+    err = np.max(np.abs(np.array(got) - ref)) / np.max(np.abs(ref))
+    print(f"  err {err:.2e}")
+    assert err < 1e-5
+
+
+def test_geometry_gradients_through_psf(dl):
+    """A radius and a translation given as tensors that require grad: the
+    gradient of a PSF loss through the fused pupil path (amplitude cotangent
+    of dlx_psf_poly_vjp, then dlx_soft_apertures_vjp) against central
+    differences of the oracle PSF on the oracle's transmission.  Bound 1e-3
+    of the largest (FP32 PSF path; FD in FP64)."""
+    print("[aperture vjp] through the PSF", flush=True)
+    from dlux_amd import layers as L
+    from dlux_amd.utils import units
+    n, n_out, p = 128, 48, 2.5e-7
+    radius = torch.tensor(0.45, dtype=torch.float64, requires_grad=True)
+    shift = torch.tensor([0.02, -0.01], dtype=torch.float64, requires_grad=True)
+    ap = L.CircularAperture(radius, dl.CoordTransform(translation=shift, rotation=0.0),
+                            softening=2.0)
+    sp = L.Spider(0.015, [0.0, 120.0, 240.0])
+    optics = dl.AngularOpticalSystem(
+        wf_npixels=n, diameter=1.0, layers=[("pupil", ap), ("spider", sp)],
+        psf_npixels=n_out, psf_pixel_scale=float(units.rad2arcsec(p)))
+    wls = np.array([0.9e-6, 1.1e-6])
+    psf = optics.propagate(wls)
+    c = orc.pixel_coords(n, 1.0)
+    dx = 1.0 / n
+    spider = orc.soft_aperture("spider", c, dx, occulting=True, width=0.015,
+                               angles=[0.0, 120.0, 240.0])
+    w = np.full(2, 0.5)
+
+    def model(r, tx, ty):
+        amp = orc.soft_aperture("circle", c, dx, softening=2.0,
+                                transforms=[dict(translation=[tx, ty], rotation=0.0)],
+                                radius=r) * spider
+        return orc.poly_psf(amp, np.zeros((1, n, n)), np.zeros(1), wls, w, 1.0, n_out, p,
+                            normalise_power=False)
+    ref = model(0.45, 0.02, -0.01)
+    assert float(np.max(np.abs(psf.detach().double().cpu().numpy() - ref)) / ref.max()) < 1e-5
+    wgt = np.random.default_rng(9).normal(size=(n_out, n_out))
+    loss = torch.sum(psf * torch.as_tensor(wgt, device=psf.device, dtype=psf.dtype)) * 1e3
+    loss.backward()
+    got = np.array([float(radius.grad), float(shift.grad[0]), float(shift.grad[1])])
+    fd = []
+    for k in range(3):
+        h = 1e-7   # below the 8e-7 by which one pixel clears the clip bound (a kink)
+        a, b = [0.45, 0.02, -0.01], [0.45, 0.02, -0.01]
+        a[k] += h
+        b[k] -= h
+        fd.append(1e3 * np.sum(wgt * (model(*a) - model(*b))) / (2 * h))
+    fd = np.array(fd)
+    err = np.max(np.abs(got - fd)) / np.max(np.abs(fd))
+    print(f"  got {got}  fd {fd}  err {err:.2e}")
+    assert err < 1e-3
+    # gradients that are not implemented say so
+    ab = L.AberratedAperture(L.CircularAperture(radius), [2, 3])
+    with pytest.raises(NotImplementedError):
+        ab._grid_basis(32, 1 / 32)
+    with pytest.raises(NotImplementedError):
+        L.CircularAperture(0.4, softening=torch.tensor(1.0, requires_grad=True)) \
+            ._grid_transmission(32, 1 / 32)

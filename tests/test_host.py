@@ -115,10 +115,10 @@ def test_batched_and_vjp_abi_validation_without_gpu(lib):
     one = _lib.dptr([1e-6])
     w = _lib.dptr([1.0])
     st = lib.dlx_psf_poly_vjp(g, 1, one, w, 1 / 64, 2.5e-7, 0.0, 0.0, 0.0, None, None, None,
-                              1.0, 4096, None, None, 0, None, 0, None)
-    assert st == 1 and b"grad_opd or grad_phase" in lib.dlx_last_error()
+                              1.0, 4096, None, None, None, 0, None, 0, None)
+    assert st == 1 and b"grad_opd, grad_phase or grad_amp" in lib.dlx_last_error()
     st = lib.dlx_psf_poly_vjp(g, 1, one, w, 1 / 64, 2.5e-7, 0.0, 0.0, 0.0, None, None, None,
-                              1.0, 4096, 8192, None, 0, None, 0, None)
+                              1.0, 4096, 8192, None, None, 0, None, 0, None)
     assert st == 3 and b"workspace" in lib.dlx_last_error()
     assert lib.dlx_psf_poly_vjp_workspace_size(g, 1, 1) > lib.dlx_psf_poly_workspace_size(g, 1, 1)
     assert lib.dlx_project_basis_workspace_size(36, 512 * 512, 1) >= 36 * 4
