@@ -9,6 +9,9 @@ optical-system / layer / propagator API:
 """
 
 from . import runtime, utils
+from . import parametric, polynomials
+from .parametric import Parametric, ParametricBasis, ExplicitBasis, ImplicitBasis, CoordBasis
+from .polynomials import DynamicZernike, ZernikeBasis, DynamicZernikeBasis
 from .runtime import set_precision, get_precision, precision
 from .base import Base
 from .coordinates import Spec, PadSpec, CoordSpec, BaseCoordTransform, CoordTransform

@@ -642,20 +642,36 @@ int build_chain(int n, const dlx_coord_transform* t, TfChain& ch) {
   return DLX_OK;
 }
 
+// The vertex angles of linspace(-pi, pi, n, endpoint=False): k * step + start,
+// multiplied and added as two roundings.
+double polygon_angle(int k, int n) {
+  const double step = (M_PI - (-M_PI)) / n;
+  volatile double t = k * step;
+  return t + (-M_PI);
+}
+
 // Edges of the regular polygon as lines a x + b y + c (reg_polygon_edges and
-// line_distance, utils/geometry.py:487-579): vertices at angles
-// linspace(-pi, pi, n, endpoint=False) on radius rmax, slope
-// m = nandiv(dy, dx) (0 where dx == 0), horizontal rule for m == 0.
+// line_distance, utils/geometry.py:487-579): vertices at those angles on
+// radius rmax, slope m = nandiv(dy, dx) (+inf where dx == 0, nandiv's
+// default fill, utils/math.py:228-250): the vertical rule for m = inf, the
+// horizontal rule for m == 0, the general line otherwise.
 void polygon_edges(int n, double rmax, ApDev& a) {
   a.nedges = n;
   for (int e = 0; e < n; ++e) {
     const int f = (e + 1) % n;
-    const double ti = -M_PI + 2.0 * M_PI * e / n, tj = -M_PI + 2.0 * M_PI * f / n;
+    const double ti = polygon_angle(e, n), tj = polygon_angle(f, n);
     const double xi = rmax * std::cos(ti), yi = rmax * std::sin(ti);
     const double xj = rmax * std::cos(tj), yj = rmax * std::sin(tj);
     const double den = xj - xi;
-    const double m = den == 0.0 ? 0.0 : (yj - yi) / den;
     auto sgn = [](double v) { return v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0); };
+    if (den == 0.0) {   // -sign(xi) * (xi - x)
+      const double s = sgn(xi);
+      a.ea[e] = s;
+      a.eb[e] = 0.0;
+      a.ec[e] = -s * xi;
+      continue;
+    }
+    const double m = (yj - yi) / den;
     if (m == 0.0) {   // -sign(yi) * (yi - y)
       const double s = sgn(yi);
       a.ea[e] = 0.0;

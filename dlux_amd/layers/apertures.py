@@ -26,6 +26,7 @@ import numpy as np
 import torch
 
 from .. import _lib, runtime
+from ..polynomials import DynamicZernikeBasis
 from ..coordinates import BaseCoordTransform, CoordTransform, _floats, _is_tensor
 from .optical_layers import OpticalLayer
 
@@ -404,7 +405,17 @@ class AberratedAperture(ApertureLayer):
                 raise ValueError("Zernike indices must be greater than zero.")
         if coefficients is None:
             coefficients = np.zeros(len(self.noll_inds), np.float32)
-        self.coefficients = coefficients
+        self.aberrations = DynamicZernikeBasis(js=self.noll_inds, coefficients=coefficients,
+                                               nsides=aperture.nsides)
+
+    @property
+    def coefficients(self):
+        """The coefficients of the Zernike parameterisation."""
+        return self.aberrations.coefficients
+
+    @coefficients.setter
+    def coefficients(self, value):
+        self.aberrations = self.aberrations.set("coefficients", value)
 
     def calculate(self):
         raise NotImplementedError(

@@ -9,7 +9,8 @@ import torch
 
 from .. import runtime
 
-__all__ = ["eval_basis", "factorial", "nandiv", "triangular_number", "gaussian"]
+__all__ = ["eval_basis", "solve_basis", "factorial", "nandiv", "triangular_number",
+           "gaussian"]
 
 
 def eval_basis(basis, coefficients) -> torch.Tensor:
@@ -18,6 +19,25 @@ def eval_basis(basis, coefficients) -> torch.Tensor:
     utils/math.py:177-196 (HIP kernel dlx_eval_basis).
     """
     return runtime.eval_basis(basis, coefficients)
+
+
+def solve_basis(array, basis):
+    """Least-squares coefficients of ``array`` over ``basis`` (its trailing
+    dimensions match array's), shape basis.shape[:-array.ndim]
+    (utils/math.py:199-225).  A host FP64 solve: fitting, not the hot path."""
+    a = array.detach().double().cpu().numpy() if isinstance(array, torch.Tensor) \
+        else np.asarray(array, float)
+    b = basis.detach().double().cpu().numpy() if isinstance(basis, torch.Tensor) \
+        else np.asarray(basis, float)
+    out_shape = b.shape[-a.ndim:] if a.ndim else ()
+    if out_shape != a.shape:
+        raise ValueError(
+            "The trailing dimensions of basis must match the shape of array, "
+            f"received {out_shape} and {a.shape}."
+        )
+    matrix = b.reshape((-1, a.size)).T
+    lead = b.shape[:-a.ndim] if a.ndim else b.shape
+    return np.linalg.lstsq(matrix, a.ravel(), rcond=None)[0].reshape(lead)
 
 
 def factorial(n: float) -> float:
@@ -34,7 +54,7 @@ def triangular_number(n: int):
     return n * (n + 1) / 2
 
 
-def nandiv(a, b, fill=0.0):
+def nandiv(a, b, fill=float("inf")):
     """a / b with `fill` where b == 0.  utils/math.py:228-249."""
     a = torch.as_tensor(a)
     b = torch.as_tensor(b, device=a.device)

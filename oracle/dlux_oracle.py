@@ -712,17 +712,20 @@ def spider_distance(coords, width, angle):
 
 
 def reg_polygon_distance(coords, nsides, radius):
-    """Max over edges of the signed line distance; edge slopes by nandiv (0
-    where the edge is vertical) and the horizontal rule for slope 0
-    (utils/geometry.py:487-579)."""
+    """Max over edges of the signed line distance; edge slopes by nandiv
+    (+inf where the edge is vertical: its default fill), the vertical rule for
+    an infinite slope and the horizontal rule for slope 0
+    (utils/geometry.py:487-579, utils/math.py:228-250)."""
     t = np.linspace(-np.pi, np.pi, nsides, endpoint=False)
     x, y = radius * np.cos(t), radius * np.sin(t)
     j = np.roll(np.arange(nsides), -1)
     out = None
     for i in range(nsides):
         den = x[j[i]] - x[i]
-        m = 0.0 if den == 0 else (y[j[i]] - y[i]) / den
-        if m == 0:
+        m = np.inf if den == 0 else (y[j[i]] - y[i]) / den
+        if m == np.inf:
+            val = -np.sign(x[i]) * (x[i] - coords[0])
+        elif m == 0:
             val = -np.sign(y[i]) * (y[i] - coords[1])
         else:
             dist = (m * coords[0] - coords[1] + y[i] - m * x[i]) / np.sqrt(m ** 2 + 1)

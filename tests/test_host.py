@@ -458,3 +458,102 @@ def test_aperture_abi_validation_without_gpu(lib):
     assert st == 1 and b"nsides" in lib.dlx_last_error()
     st = lib.dlx_aperture_basis(64, 1 / 64, 0.0, 4, None, 1.0, 0, 2, js, 4096, None)
     assert st == 1 and b"at most 3" in lib.dlx_last_error()
+
+
+def test_soft_apertures_vjp_abi_validation_without_gpu(lib):
+    from dlux_amd import _lib
+    aps = (_lib.SoftAperture * 1)()
+    aps[0].kind, aps[0].softening, aps[0].size[0] = _lib.DLX_AP_CIRCLE, 1.0, 0.3
+    need = lib.dlx_soft_apertures_vjp_workspace_size(64, 1)
+    assert need > lib.dlx_soft_apertures_workspace_size(64, 1)
+    st = lib.dlx_soft_apertures_vjp(64, 1 / 64, 0.0, 1, aps, None, 4096, None, 0, None)
+    assert st == 1 and b"null argument" in lib.dlx_last_error()
+    st = lib.dlx_soft_apertures_vjp(64, 1 / 64, 0.0, 1, aps, 4096, 4096, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+    aps[0].softening = -1.0
+    st = lib.dlx_soft_apertures_vjp(64, 1 / 64, 0.0, 1, aps, 4096, 4096, None, need, None)
+    assert st == 1 and b"softening" in lib.dlx_last_error()
+
+
+def test_geometry_gradient_sources():
+    """Which tensors the geometry VJP feeds: sizes and the innermost
+    transformation of each aperture; the rest raise NotImplementedError."""
+    torch = pytest.importorskip("torch")
+    import dlux_amd as dl
+    from dlux_amd import layers as L
+    from dlux_amd.layers.apertures import _grad_sources
+    r = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+    t = torch.tensor([0.01, 0.02], dtype=torch.float64, requires_grad=True)
+    rot = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+    ap = L.CircularAperture(r, dl.CoordTransform(translation=t, rotation=rot))
+    d = ap._descs(())
+    assert d[0][1].size[0] == pytest.approx(0.3) and d[0][1].transforms[0].rotation == \
+        pytest.approx(0.1)
+    params, smap = _grad_sources(d)
+    assert len(params) == 3
+    assert sorted(s[1] for s in smap) == [0, 2, 3, 8]
+    # a member without its own transform differentiates the enclosing one
+    comp = L.CompoundAperture([L.CircularAperture(0.4), L.SquareAperture(0.2)],
+                              transformation=dl.CoordTransform(translation=t))
+    params, smap = _grad_sources(comp._descs(()))
+    assert len(params) == 1 and [(a, s) for a, s, _, _ in smap] == [(0, 2), (0, 3), (1, 2),
+                                                                    (1, 3)]
+    # ... but not under a member's own transformation
+    comp = L.CompoundAperture([L.CircularAperture(0.4, dl.CoordTransform(rotation=0.2))],
+                              transformation=dl.CoordTransform(translation=t))
+    with pytest.raises(NotImplementedError):
+        comp._descs(())
+    with torch.no_grad():
+        comp._descs(())
+    with pytest.raises(NotImplementedError):
+        L.Spider(0.02, torch.tensor([0.0, 90.0], requires_grad=True))._descs(())
+    # keys are plain floats whatever the parameter type
+    assert ap._param_key() == L.CircularAperture(
+        0.3, dl.CoordTransform(translation=[0.01, 0.02], rotation=0.1))._param_key()
+
+
+def test_polynomial_construction_and_errors():
+    """tests/test_polynomials.py construction and error cases that need no
+    evaluation."""
+    import dlux_amd as dl
+    import dlux_amd.utils as dlu
+    z = dl.DynamicZernike(4)
+    assert z.j == 4 and z.name == dlu.zernike_name(4) == "Defocus" and (z.n, z.m) == (2, 0)
+    with pytest.raises(ValueError, match="greater than 0"):
+        dl.DynamicZernike(0)
+    assert dlu.radial_orders_to_indices([2]) == [4, 5, 6]
+    assert dlu.radial_orders_to_indices([0, 1]) == [1, 2, 3]
+    assert dlu.zernike_name(36) == "Heptafoil Y" and dlu.zernike_name(37) == "Zernike 37"
+    coords = np.zeros((2, 8, 8))
+    with pytest.raises(ValueError, match="exactly one"):
+        dl.ZernikeBasis(coords)
+    with pytest.raises(ValueError, match="exactly one"):
+        dl.ZernikeBasis(coords, js=[1], radial_orders=[0])
+    with pytest.raises(ValueError, match="At least one"):
+        dl.ZernikeBasis(coords, js=[])
+    with pytest.raises(ValueError, match="greater than zero"):
+        dl.ZernikeBasis(coords, js=[0])
+    with pytest.raises(ValueError, match="non-negative"):
+        dl.ZernikeBasis(coords, radial_orders=[-1])
+    b = dl.DynamicZernikeBasis(radial_orders=[2])
+    assert b.coefficient_shape == (3,) and b.js == [4, 5, 6]
+    with pytest.raises(ValueError, match="wavefront or coordinates"):
+        dl.DynamicZernikeBasis(js=[1]).evaluate()
+    for nsides in (-1, 1, 2):
+        with pytest.raises(ValueError, match="zero or greater"):
+            dl.DynamicZernikeBasis(js=[1], nsides=nsides)
+    with pytest.raises(ValueError, match="Coefficient shape"):
+        dl.DynamicZernikeBasis(js=[4, 5], coefficients=np.ones(3))
+    # solve_basis: least squares over the trailing dimensions
+    rng = np.random.default_rng(0)
+    basis = rng.normal(size=(3, 5, 5))
+    c = np.array([1.0, -2.0, 0.5])
+    assert np.allclose(dlu.solve_basis(np.tensordot(c, basis, 1), basis), c)
+    with pytest.raises(ValueError, match="trailing dimensions"):
+        dlu.solve_basis(np.ones((4, 4)), basis)
+    # AberratedAperture keeps its Zernikes as a DynamicZernikeBasis
+    from dlux_amd import layers as L
+    ab = L.AberratedAperture(L.RegPolyAperture(6, 0.4), [2, 3], np.array([1.0, 2.0]))
+    assert isinstance(ab.aberrations, dl.DynamicZernikeBasis) and ab.aberrations.nsides == 6
+    assert np.allclose(ab.set("coefficients", np.array([3.0, 4.0])).coefficients, [3, 4])
+    assert np.allclose(ab.coefficients, [1, 2])

@@ -455,6 +455,22 @@ def test_polygon_square_rectangle_identities():
     assert np.all(gap <= 1e-12) and np.all(gap >= -0.8 * (1 - np.cos(np.pi / 64)) - 1e-12)
 
 
+def test_polygon_distance_closed_form():
+    """For every n, the edge distance of the regular n-gon with vertices at
+    angles -pi + 2 pi k / n on radius r is max_k (x cos phi_k + y sin phi_k)
+    - r cos(pi / n), phi_k = -pi + (2k + 1) pi / n (the outward edge
+    normals).  Odd n >= 5 have an edge whose end points share x exactly:
+    nandiv's infinite slope, the vertical rule."""
+    c = orc.pixel_coords(101, 2.4)
+    for n in range(3, 10):
+        r = 0.9
+        phi = -np.pi + (2 * np.arange(n) + 1) * np.pi / n
+        ref = np.max(c[0][None] * np.cos(phi)[:, None, None]
+                     + c[1][None] * np.sin(phi)[:, None, None], axis=0) - r * np.cos(np.pi / n)
+        got = orc.reg_polygon_distance(c, n, r)
+        assert np.max(np.abs(got - ref)) < 1e-9, n
+
+
 def test_coord_transform_closed_forms():
     """Integer-pixel translation shifts the transmission; shear (s, 0) on a
     centred square grid is x' = x + s y (the transposed x plane is y);
