@@ -166,6 +166,14 @@ SIGNATURES = {
     "dlx_soft_apertures": (c_int32, [c_int32, c_double, c_double, c_int32,
                                      POINTER(SoftAperture), _fp, c_void_p, c_size_t,
                                      c_void_p]),
+    "dlx_lct_workspace_size": (c_size_t, [c_int32, c_int32, c_int32, c_int32]),
+    "dlx_lct": (c_int32, [c_int32, c_int32, c_int32, POINTER(c_double), POINTER(c_double),
+                          c_double, c_double, c_double, c_double, _fp, _fp, c_int32, c_void_p,
+                          c_size_t, c_void_p]),
+    "dlx_lct_fft_workspace_size": (c_size_t, [c_int32, c_int32, c_int32]),
+    "dlx_lct_fft": (c_int32, [c_int32, c_int32, c_int32, POINTER(c_double), POINTER(c_double),
+                              c_double, c_double, c_double, _fp, _fp, POINTER(c_double),
+                              c_void_p, c_size_t, c_void_p]),
     "dlx_soft_apertures_vjp_workspace_size": (c_size_t, [c_int32, c_int32]),
     "dlx_soft_apertures_vjp": (c_int32, [c_int32, c_double, c_double, c_int32,
                                          POINTER(SoftAperture), _fp, _fp, c_void_p, c_size_t,

@@ -332,6 +332,157 @@ int dlx_mft(const dlx_mft_geom* geom, int32_t batch, const void* in, void* out,
   return cache_release(hy, s);
 }
 
+size_t dlx_lct_workspace_size(int32_t n_in, int32_t n_out, int32_t batch, int32_t precision) {
+  if (n_in <= 0 || n_out <= 0 || batch <= 0) return 0;
+  const bool split = precision == DLX_PREC_BF16X3;
+  size_t total = 2 * align_up((size_t)batch * mat_bytes(n_out, n_in, split));  // K, U^T
+  if (split) total += align_up((size_t)batch * mat_bytes(n_in, n_in, true));
+  return total;
+}
+
+int dlx_lct(int32_t n_in, int32_t n_out, int32_t batch, const double* abcd,
+            const double* wavelength, double pixel_scale_in, double center_in,
+            double pixel_scale_out, double center_out, const void* in, void* out,
+            int32_t precision, void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(n_in > 0 && n_in <= 16384 && n_out > 0 && n_out <= 16384,
+                "n_in and n_out must be in [1, 16384], got %d and %d", n_in, n_out);
+  DLX_CHECK_ARG(batch >= 0 && batch <= 65535, "batch must be in [0, 65535], got %d", batch);
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(abcd && wavelength && in && out, "null argument to dlx_lct");
+  DLX_CHECK_ARG(precision == DLX_PREC_FP32 || precision == DLX_PREC_BF16X3,
+                "unknown precision %d", precision);
+  const double A = abcd[0], B = abcd[1], D = abcd[3];
+  DLX_CHECK_ARG(std::isfinite(A) && std::isfinite(D) && std::isfinite(B) && B != 0.0,
+                "the ABCD matrix needs finite A and D and a finite, non-zero B");
+  DLX_CHECK_ARG(std::isfinite(pixel_scale_in) && pixel_scale_in > 0 &&
+                    std::isfinite(pixel_scale_out) && pixel_scale_out > 0,
+                "pixel scales must be > 0");
+  DLX_CHECK_ARG(std::isfinite(center_in) && std::isfinite(center_out), "centres must be finite");
+  for (int b = 0; b < batch; ++b)
+    DLX_CHECK_ARG(std::isfinite(wavelength[b]) && wavelength[b] > 0,
+                  "wavelength[%d] must be > 0", b);
+  const size_t need = dlx_lct_workspace_size(n_in, n_out, batch, precision);
+  if (!workspace || workspace_bytes < need) {
+    set_error("dlx_lct workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const bool split = precision == DLX_PREC_BF16X3;
+  const size_t mat = align_up((size_t)batch * mat_bytes(n_out, n_in, split));
+  uint8_t* K_ws = (uint8_t*)workspace;
+  uint8_t* Ut = K_ws + mat;
+  // complex64 kernels; in the split form staged in the intermediate's space
+  float2* Kf = split ? (float2*)Ut : (float2*)K_ws;
+  const long kst = (long)n_out * n_in;
+  const double x0i = center_in - (n_in - 1) / 2.0 * pixel_scale_in;
+  const double x0o = center_out - (n_out - 1) / 2.0 * pixel_scale_out;
+  std::vector<double> alpha(batch);
+  for (int b0 = 0; b0 < batch; b0 += LctMatChunk::kMax) {
+    const int nb = batch - b0 < LctMatChunk::kMax ? batch - b0 : LctMatChunk::kMax;
+    LctMatChunk c;
+    for (int i = 0; i < nb; ++i) {
+      const double lb = wavelength[b0 + i] * B;
+      c.ci2[i] = A / (2.0 * lb);
+      c.co2[i] = D / (2.0 * lb);
+      c.cx[i] = -1.0 / lb;
+      c.c0[i] = B > 0 ? -0.125 : 0.125;   // the phase of 1 / sqrt(i B), per axis
+      alpha[b0 + i] = pixel_scale_in * pixel_scale_out / (wavelength[b0 + i] * std::fabs(B));
+    }
+    DLX_HIP(launch_lct_matrix(c, nb, n_in, n_out, x0i, pixel_scale_in, x0o, pixel_scale_out,
+                              Kf + (long)b0 * kst, kst, s));
+  }
+  dlx_mft_geom g{};
+  g.n_in = n_in;
+  g.n_out = n_out;
+  if (split) {
+    DLX_HIP(launch_to_split4(Kf, batch, n_out, n_in, kst, K_ws,
+                             (long)mat_bytes(n_out, n_in, true), s));
+    uint8_t* X = K_ws + 2 * mat;
+    DLX_HIP(launch_to_split4((const float2*)in, batch, n_in, n_in, (long)n_in * n_in, X,
+                             (long)mat_bytes(n_in, n_in, true), s));
+    return mft_split4(g, batch, X, K_ws, K_ws, Ut, alpha.data(), out, EPI_STORE, s, nullptr,
+                      nullptr);
+  }
+  return mft_fp32(g, batch, (const float2*)in, Kf, Kf, (float2*)Ut, alpha.data(), out,
+                  EPI_STORE, s, false);
+}
+
+size_t dlx_lct_fft_workspace_size(int32_t n, int32_t m, int32_t batch) {
+  if (n <= 0 || m <= 0 || batch <= 0) return 0;
+  return align_up((size_t)batch * n * n * sizeof(float2)) + dlx_fft2_workspace_size(n, m, batch);
+}
+
+int dlx_lct_fft(int32_t n, int32_t m, int32_t batch, const double* abcd,
+                const double* wavelength, double pixel_scale_in, double center_in,
+                double center_out, const void* in, void* out, double* pixel_scale_out,
+                void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(n > 0 && m >= n && ((m - n) % 2) == 0,
+                "need 0 < n <= m with m - n even, got n = %d, m = %d", n, m);
+  DLX_CHECK_ARG(batch >= 0 && batch <= 65535, "batch must be in [0, 65535], got %d", batch);
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(abcd && wavelength && in && out, "null argument to dlx_lct_fft");
+  const double A = abcd[0], B = abcd[1], D = abcd[3];
+  DLX_CHECK_ARG(std::isfinite(A) && std::isfinite(D) && std::isfinite(B) && B != 0.0,
+                "the ABCD matrix needs finite A and D and a finite, non-zero B");
+  DLX_CHECK_ARG(std::isfinite(pixel_scale_in) && pixel_scale_in > 0, "pixel_scale_in must be > 0");
+  DLX_CHECK_ARG(std::isfinite(center_in) && std::isfinite(center_out), "centres must be finite");
+  for (int b = 0; b < batch; ++b)
+    DLX_CHECK_ARG(std::isfinite(wavelength[b]) && wavelength[b] > 0,
+                  "wavelength[%d] must be > 0", b);
+  const size_t need = dlx_lct_fft_workspace_size(n, m, batch);
+  if (!workspace || workspace_bytes < need) {
+    set_error("dlx_lct_fft workspace too small: need %zu bytes, got %zu", need,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const double sg = B > 0 ? 1.0 : -1.0, dx = pixel_scale_in, M = m;
+  // the centred FFT pairs offsets from index m / 2; the grids are symmetric
+  // about (m - 1) / 2, de apart
+  const double de = (m / 2) - (m - 1) / 2.0;
+  float2* C1 = (float2*)workspace;
+  uint8_t* fws = (uint8_t*)workspace + align_up((size_t)batch * n * n * sizeof(float2));
+  const long nn = (long)n * n, mm = (long)m * m;
+  // input chirp (A term), with the linear terms that centre the output on center_out
+  for (int b0 = 0; b0 < batch; b0 += LctChirpChunk::kMax) {
+    const int nb = batch - b0 < LctChirpChunk::kMax ? batch - b0 : LctChirpChunk::kMax;
+    LctChirpChunk c;
+    for (int i = 0; i < nb; ++i) {
+      const double lb = wavelength[b0 + i] * B;
+      c.q2[i] = A / (2.0 * lb);
+      c.q1[i] = -center_out / lb - sg * de / (M * dx);
+      c.q0[i] = center_out * center_in / lb + sg * de * center_in / (M * dx) + sg * de * de / M;
+      c.x0[i] = center_in - (n - 1) / 2.0 * dx;
+      c.dx[i] = dx;
+    }
+    DLX_HIP(launch_lct_chirp(c, nb, n, (const float2*)in + (long)b0 * nn, C1 + (long)b0 * nn,
+                             s));
+  }
+  if (int st = dlx_fft2(n, m, batch, sg < 0, C1, out, fws, dlx_fft2_workspace_size(n, m, batch),
+                        stream))
+    return st;
+  // output chirp (D term), the output-side linear terms and the constant phase
+  for (int b0 = 0; b0 < batch; b0 += LctChirpChunk::kMax) {
+    const int nb = batch - b0 < LctChirpChunk::kMax ? batch - b0 : LctChirpChunk::kMax;
+    LctChirpChunk c;
+    for (int i = 0; i < nb; ++i) {
+      const double lb = wavelength[b0 + i] * B;
+      const double d_out = wavelength[b0 + i] * std::fabs(B) / (M * dx);
+      if (pixel_scale_out) pixel_scale_out[b0 + i] = d_out;
+      c.q2[i] = D / (2.0 * lb);
+      c.q1[i] = -center_in / lb - sg * de / (M * d_out);
+      c.q0[i] = sg * de * center_out / (M * d_out) - sg / 8.0;
+      c.x0[i] = center_out - (m - 1) / 2.0 * d_out;
+      c.dx[i] = d_out;
+    }
+    DLX_HIP(launch_lct_chirp(c, nb, m, (const float2*)out + (long)b0 * mm,
+                             (float2*)out + (long)b0 * mm, s));
+  }
+  return DLX_OK;
+}
+
 int dlx_pupil_field(int32_t n, int32_t batch, const double* wavelength, double pixel_scale,
                     double center, double tilt_x, double tilt_y, const float* amp,
                     const float* opd, const float* phase, double scale, const void* in,

@@ -171,6 +171,21 @@ struct GradChunk {
 
 hipError_t launch_dft_chunk(const DftChunk& c, int nb, int n_in, int n_out, int inverse,
                             float2* out, long bstride, hipStream_t s);
+
+// Linear canonical transforms (dlx_lct.hip): per-instance phase polynomials in turns.
+struct LctMatChunk {
+  static constexpr int kMax = 48;
+  double ci2[kMax], co2[kMax], cx[kMax], c0[kMax];
+};
+struct LctChirpChunk {
+  static constexpr int kMax = 48;
+  double q2[kMax], q1[kMax], q0[kMax], x0[kMax], dx[kMax];   // grid x_k = x0 + k dx
+};
+hipError_t launch_lct_matrix(const LctMatChunk& c, int nb, int n_in, int n_out, double x0_in,
+                             double dx_in, double x0_out, double dx_out, float2* out,
+                             long bstride, hipStream_t s);
+hipError_t launch_lct_chirp(const LctChirpChunk& c, int nb, int n, const float2* in,
+                            float2* out, hipStream_t s);
 hipError_t launch_pupil_chunk(const PupilChunk& c, int nb, int n, double dx, double x0,
                               double tx, double ty, const float* amp, const float* opd,
                               const float* phase, float scale, const float2* in,
@@ -195,7 +210,6 @@ hipError_t launch_sum_planes(long npix, int nplanes, const float* planes, int ac
 
 // Backward reductions (dlx_grad.hip).  partial holds nbasis * nsets * project_parts(npix)
 // floats.
-// g[p] /= amp[p] where amp[p] > 0, else 0 (the amplitude cotangent from Re(conj(P) Pbar)).
 hipError_t launch_grad_reduce(const GradChunk& c, int nb, long npix, const float* h,
                               int accumulate, float* g_opd, float* g_phase, hipStream_t s);
 int project_parts(long npix);

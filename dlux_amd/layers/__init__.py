@@ -3,7 +3,9 @@
 from .optical_layers import (OpticalLayer, TransmissiveLayer, AberratedLayer, BasisLayer,
                              Tilt, Normalise)
 from .optics import Optic, BasisOptic
-from .propagators import Propagator, FFT, MFT, ASMPropagator
+from .propagators import (Propagator, FFT, MFT, ASMPropagator, ABCDElement, ABCDFreeSpace,
+                          ABCDLens, ABCDMirror, ABCDConjugatePlane, ABCDPropagator,
+                          MFTPropagator, FFTPropagator, Fraunhofer, Fresnel)
 from .detector_layers import (DetectorLayer, ApplyPixelResponse, ApplyJitter, ApplySaturation,
                               AddConstant, Downsample)
 from .apertures import (ApertureLayer, BaseDynamicAperture, DynamicAperture, CircularAperture,

@@ -836,6 +836,56 @@ def asm_prop(fields, wavelengths, pixel_scale: float, distance: float, m: int) -
     return out[0] if squeeze else out
 
 
+def lct(fields, wavelengths, abcd, pixel_scale_in: float, center_in: float, n_out: int,
+        pixel_scale_out: float, center_out: float = 0.0,
+        precision: str | None = None) -> torch.Tensor:
+    """Collins-integral LCT of (B, N, N) fields to (B, n_out, n_out) on the
+    given output grid (dlx_lct)."""
+    f = as_complex(fields)
+    squeeze = f.ndim == 2
+    if squeeze:
+        f = f[None]
+    b, n, _ = f.shape
+    wl = _bcast(wavelengths, b)
+    m = np.asarray(abcd, float).reshape(-1)
+    if m.size != 4:
+        raise ValueError("abcd must be a 2 x 2 matrix")
+    prec = _prec_code(precision)
+    lib = _lib.load()
+    ws = workspace(lib.dlx_lct_workspace_size(n, int(n_out), b, prec), f.device)
+    out = torch.empty((b, int(n_out), int(n_out)), dtype=torch.complex64, device=f.device)
+    st = lib.dlx_lct(n, int(n_out), b, _lib.dptr(m), _lib.dptr(wl), float(pixel_scale_in),
+                     float(center_in), float(pixel_scale_out), float(center_out), f.data_ptr(),
+                     out.data_ptr(), prec, ws.data_ptr(), ws.numel(), stream_handle())
+    _lib.check(st, "dlx_lct")
+    return out[0] if squeeze else out
+
+
+def lct_fft(fields, wavelengths, abcd, pixel_scale_in: float, center_in: float, m: int,
+            center_out: float = 0.0):
+    """FFT-grid LCT of (B, N, N) fields to (B, m, m) (dlx_lct_fft); returns
+    (fields, output pixel scale per field)."""
+    f = as_complex(fields)
+    squeeze = f.ndim == 2
+    if squeeze:
+        f = f[None]
+    b, n, _ = f.shape
+    wl = _bcast(wavelengths, b)
+    mat = np.asarray(abcd, float).reshape(-1)
+    if mat.size != 4:
+        raise ValueError("abcd must be a 2 x 2 matrix")
+    lib = _lib.load()
+    ws = workspace(lib.dlx_lct_fft_workspace_size(n, int(m), b), f.device)
+    out = torch.empty((b, int(m), int(m)), dtype=torch.complex64, device=f.device)
+    ps = _lib.dptr([0.0] * b)
+    st = lib.dlx_lct_fft(n, int(m), b, _lib.dptr(mat), _lib.dptr(wl), float(pixel_scale_in),
+                         float(center_in), float(center_out), f.data_ptr(), out.data_ptr(), ps,
+                         ws.data_ptr(), ws.numel(), stream_handle())
+    _lib.check(st, "dlx_lct_fft")
+    pss = np.array(ps[:b], float)
+    return (out[0], pss[0]) if squeeze else (out, pss)
+
+
 def profile_enable(enable: bool = True) -> None:
     """Start (and reset) or stop live event timing of the MFT contraction launches."""
     _lib.check(_lib.load().dlx_profile_enable(int(bool(enable))), "dlx_profile_enable")

@@ -399,6 +399,40 @@ int dlx_asm_prop(int32_t n, int32_t m, int32_t batch, const double* wavelength,
                  void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * Linear canonical transform (Collins integral) of `batch` N_in x N_in complex64
+ * fields for the ABCD matrix abcd = {A, B, C, D} (row-major; B != 0), per axis
+ *   u_out(x') = sqrt(dx dx' / (i lambda B)) sum_x u_in(x)
+ *               exp(i pi / (lambda B) (A x^2 - 2 x x' + D x'^2)),
+ * on x = center_in + (k - (N_in-1)/2) pixel_scale_in and
+ * x' = center_out + (l - (N_out-1)/2) pixel_scale_out; the 2-D field maps as
+ * K u K^T with two complex GEMMs (the MFT's), precision as for dlx_mft.
+ * wavelength[b] per field.  Replaces abcdLux.lct.lct_prop as called by
+ * MFTPropagator (layers/abcd_propagators.py:245-297); abcdLux is not
+ * available, so this follows the textbook Collins integral (the sign, the
+ * 1/i phase and the power-preserving sqrt(dx dx') scale are this library's
+ * convention).
+ */
+size_t dlx_lct_workspace_size(int32_t n_in, int32_t n_out, int32_t batch, int32_t precision);
+int dlx_lct(int32_t n_in, int32_t n_out, int32_t batch, const double* abcd,
+            const double* wavelength, double pixel_scale_in, double center_in,
+            double pixel_scale_out, double center_out, const void* in, void* out,
+            int32_t precision, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * The same transform on the FFT grid: M x M outputs (M - N even, M a power
+ * of two up to 8192) at pixel_scale_out[b] = lambda_b |B| / (M pixel_scale_in)
+ * (written if non-NULL), centred on center_out; computed as an input chirp,
+ * the centred padded FFT of dlx_fft2 (inverse when B < 0) and an output chirp.
+ * Replaces abcdLux.lct.lct_prop_fft as called by FFTPropagator
+ * (layers/abcd_propagators.py:300-383).
+ */
+size_t dlx_lct_fft_workspace_size(int32_t n, int32_t m, int32_t batch);
+int dlx_lct_fft(int32_t n, int32_t m, int32_t batch, const double* abcd,
+                const double* wavelength, double pixel_scale_in, double center_in,
+                double center_out, const void* in, void* out, double* pixel_scale_out,
+                void* workspace, size_t workspace_bytes, void* stream);
+
+/*
  * Detector chain: the reference's detector layers applied in order to
  * `batch` float32 images (n x n each, stride n*n):
  *   DLX_DET_PIXEL_RESPONSE  img *= map     (map: device f32, size x size, size =

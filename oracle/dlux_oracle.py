@@ -513,6 +513,87 @@ def asm_prop(u_in, dx, wavelength, z, npad):
     return np.fft.fftshift(np.fft.ifft2(U * H))
 
 
+# ---------------------------------------------------------------------------
+# Linear canonical transforms  (layers/abcd_propagators.py:23-383; the
+# transforms themselves live in abcdLux, absent: restated from the Collins
+# integral -- parity unpinned against abcdLux, pinned by closed forms)
+# ---------------------------------------------------------------------------
+
+
+def abcd_free_space(z):
+    return np.array([[1.0, float(z)], [0.0, 1.0]])
+
+
+def abcd_lens(f):
+    return np.array([[1.0, 0.0], [-1.0 / float(f), 1.0]])
+
+
+def abcd_mirror(radius):
+    return np.array([[1.0, 0.0], [-2.0 / float(radius), 1.0]])
+
+
+def abcd_conjugate_plane(f):
+    return np.array([[0.0, float(f)], [-1.0 / float(f), 0.0]])
+
+
+def compose_abcd(mats):
+    """Elements in propagation order: M = M_n ... M_2 M_1."""
+    out = np.eye(2)
+    for m in mats:
+        out = np.asarray(m, float) @ out
+    return out
+
+
+def lct_kernel(wavelength, xs_in, xs_out, abcd):
+    """1-D Collins kernel K[l, k] = sqrt(dx dx' / (i lambda B))
+    exp(i pi / (lambda B) (A x_k^2 - 2 x_k x'_l + D x'_l^2)) (B != 0)."""
+    A, B, D = abcd[0][0], abcd[0][1], abcd[1][1]
+    x = np.asarray(xs_in, np.float64)[None, :]
+    xp = np.asarray(xs_out, np.float64)[:, None]
+    dx = xs_in[1] - xs_in[0] if len(xs_in) > 1 else 1.0
+    dxp = xs_out[1] - xs_out[0] if len(xs_out) > 1 else 1.0
+    pref = np.sqrt(abs(dx * dxp) / (wavelength * abs(B))) * np.exp(-1j * np.pi / 4 * np.sign(B))
+    return pref * np.exp(1j * np.pi / (wavelength * B) * (A * x ** 2 - 2 * x * xp + D * xp ** 2))
+
+
+def lct(u_in, wavelength, xs_in, xs_out, abcd):
+    """The 2-D separable LCT of a field sampled on xs_in (both axes) to xs_out:
+    u_out = K u_in K^T (the Collins integral as a Riemann sum, prefactor
+    dx dx' / (i lambda B))."""
+    K = lct_kernel(wavelength, xs_in, xs_out, abcd)
+    return K @ np.asarray(u_in, np.complex128) @ K.T
+
+
+def lct_fft(u_in, wavelength, dx, c_in, c_out, npad, abcd):
+    """The same transform sampled on the FFT grid of npad points,
+    d_out = lambda |B| / (npad dx), centred on c_out, computed as chirp ->
+    centred padded FFT -> chirp.  Returns (field, d_out)."""
+    A, B, D = abcd[0][0], abcd[0][1], abcd[1][1]
+    n = u_in.shape[-1]
+    sg = np.sign(B)
+    lb = wavelength * B
+    d_out = wavelength * abs(B) / (npad * dx)
+    xs = spec_xs(n, dx, c_in)
+    xo = spec_xs(npad, d_out, c_out)
+    # the centred FFT pairs offsets from index npad // 2; the grids are
+    # symmetric about (npad - 1) / 2: delta apart
+    de = npad // 2 - (npad - 1) / 2
+    t_in = (A / (2 * lb) * xs ** 2 + (-c_out / lb - sg * de / (npad * dx)) * xs
+            + c_out * c_in / lb + sg * de * c_in / (npad * dx) + sg * de ** 2 / npad)
+    t_out = (D / (2 * lb) * xo ** 2 + (-c_in / lb - sg * de / (npad * d_out)) * xo
+             + c_in * c_out / lb + sg * de * c_out / (npad * d_out) + sg * de ** 2 / npad
+             - c_in * c_out / lb - sg * de ** 2 / npad - sg / 8)
+    q_in = np.exp(2j * np.pi * t_in)
+    q_out = np.exp(2j * np.pi * t_out)
+    u = np.asarray(u_in, np.complex128) * q_in[:, None] * q_in[None, :]
+    u = pad_to(u, npad)
+    if sg > 0:
+        F = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(u))) / npad
+    else:
+        F = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(u))) * npad
+    return F * q_out[:, None] * q_out[None, :], d_out
+
+
 def propagate_fft(phasor, wavelength, pixel_scale, center=0.0, pad=2, focal_length=None,
                   spec_c=None, inverse=False):
     """Wavefront.propagate_FFT with an optional output centre.  wavefronts.py:654-727.

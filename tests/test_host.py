@@ -557,3 +557,54 @@ def test_polynomial_construction_and_errors():
     assert isinstance(ab.aberrations, dl.DynamicZernikeBasis) and ab.aberrations.nsides == 6
     assert np.allclose(ab.set("coefficients", np.array([3.0, 4.0])).coefficients, [3, 4])
     assert np.allclose(ab.coefficients, [1, 2])
+
+
+def test_abcd_elements_and_propagators_host():
+    """tests/layers/test_abcd_propagators.py construction cases."""
+    import dlux_amd as dl
+    from dlux_amd import layers as L
+    from dlux_amd.coordinates import CoordSpec
+    els = [L.ABCDFreeSpace(1.0), L.ABCDLens(2.0), L.ABCDMirror(3.0), L.ABCDConjugatePlane(4.0)]
+    for el in els:
+        assert el.abcd.shape == (2, 2) and abs(np.linalg.det(el.abcd) - 1) < 1e-12
+    assert np.allclose(L.ABCDFreeSpace(1.5).abcd, [[1, 1.5], [0, 1]])
+    assert np.allclose(L.ABCDLens(2.0).abcd, [[1, 0], [-0.5, 1]])
+    assert np.allclose(L.ABCDMirror(3.0).abcd, [[1, 0], [-2 / 3, 1]])
+    assert np.allclose(L.ABCDConjugatePlane(4.0).abcd, [[0, 4], [-0.25, 0]])
+    # f - lens - f is the conjugate plane
+    prop = L.MFTPropagator([L.ABCDFreeSpace(2.0), L.ABCDLens(2.0), ("back", L.ABCDFreeSpace(2.0))],
+                           CoordSpec(n=8, d=1 / 32, c=0.0))
+    assert np.allclose(prop.abcd, L.ABCDConjugatePlane(2.0).abcd)
+    assert prop.n == 8 and np.allclose(prop.focal_length, 2.0)
+    assert isinstance(prop.ABCDLens, L.ABCDLens) and isinstance(prop.back, L.ABCDFreeSpace)
+    with pytest.raises(AttributeError):
+        prop.not_an_attr
+    with pytest.raises(TypeError):
+        L.MFTPropagator([L.ABCDFreeSpace(1.0), 3.0], CoordSpec(n=8, d=1.0, c=0.0))
+    with pytest.raises(ValueError, match="can not specify d"):
+        L.FFTPropagator([L.ABCDFreeSpace(1.0)], CoordSpec(n=8, d=1.0, c=0.0))
+    with pytest.raises(NotImplementedError):
+        L.Fraunhofer()
+    with pytest.raises(NotImplementedError):
+        L.Fresnel()
+    assert dl.layers.ABCDPropagator is L.ABCDPropagator
+
+
+def test_lct_abi_validation_without_gpu(lib):
+    abcd = (ctypes.c_double * 4)(1.0, 0.0, 0.0, 1.0)
+    wl = (ctypes.c_double * 1)(1e-6)
+    st = lib.dlx_lct(16, 8, 1, abcd, wl, 1e-3, 0.0, 1e-3, 0.0, 4096, 4096, 0, None, 0, None)
+    assert st == 1 and b"non-zero B" in lib.dlx_last_error()
+    abcd[1] = 0.5
+    st = lib.dlx_lct(16, 8, 1, abcd, wl, 1e-3, 0.0, 1e-3, 0.0, 4096, 4096, 0, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+    assert lib.dlx_lct_workspace_size(16, 8, 1, 0) >= 2 * 8 * 16 * 8
+    assert lib.dlx_lct_workspace_size(16, 8, 1, 1) > lib.dlx_lct_workspace_size(16, 8, 1, 0)
+    st = lib.dlx_lct(16, 8, 1, abcd, wl, -1e-3, 0.0, 1e-3, 0.0, 4096, 4096, 0, None, 0, None)
+    assert st == 1 and b"pixel scales" in lib.dlx_last_error()
+    st = lib.dlx_lct_fft(16, 31, 1, abcd, wl, 1e-3, 0.0, 0.0, 4096, 4096, None, None, 0, None)
+    assert st == 1 and b"m - n even" in lib.dlx_last_error()
+    need = lib.dlx_lct_fft_workspace_size(16, 32, 1)
+    assert need >= 16 * 16 * 8 + 32 * 32 * 8
+    st = lib.dlx_lct_fft(16, 32, 1, abcd, wl, 1e-3, 0.0, 0.0, 4096, 4096, None, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()

@@ -503,3 +503,87 @@ def test_polike_support_and_limit():
     for j in (2, 4, 7):   # r_a = cos(pi/n)/cos(u) differs from 1 by < 5e-6 at n = 1000
         diff = orc.polike(1000, j, c) - orc.zernike(j, c)
         assert np.max(np.abs(diff[inner])) < 5e-4
+
+
+# --- linear canonical transforms (abcdLux absent: closed-form pins) ---------
+
+
+def _gaussian_lct_closed_form(xs_out, wavelength, q1, kappa, M):
+    """LCT of exp(i pi x^2 / (lambda q1) + 2 pi i kappa x) per axis, from the
+    Gaussian integral int exp(i pi a x^2 / lambda - 2 pi i s x) dx =
+    sqrt(i lambda / a) exp(-i pi lambda s^2 / a), a = A / B + 1 / q1,
+    s = x' / (lambda B) - kappa (principal roots, Im a > 0)."""
+    (A, B), (C, D) = M
+    a = A / B + 1 / q1
+    xp = np.asarray(xs_out)
+    s = xp / (wavelength * B) - kappa
+    amp = np.sqrt(1 / (1j * wavelength * B)) * np.sqrt(1j * wavelength / a)
+    return amp * np.exp(-1j * np.pi * wavelength * s ** 2 / a
+                        + 1j * np.pi * D * xp ** 2 / (wavelength * B))
+
+
+@pytest.mark.parametrize("case", ["free", "free_back", "lens_free", "mirror", "conjugate"])
+def test_lct_gaussian_beam_abcd_law(case):
+    """A sampled, tilted Gaussian beam through the Collins sum equals the
+    closed form, scaled by dx'/dx for the power-preserving sampling; the grids
+    are off-centre to pin the centring, the tilt pins the sign of the cross
+    term.  Untilted, the closed form's exponent is the ABCD law
+    q2 = (A q1 + B) / (C q1 + D)."""
+    wl, w0 = 1e-6, 0.5e-3
+    zr = np.pi * w0 ** 2 / wl
+    q1 = -1j * zr                       # exp(i pi x^2 / (lambda q1)) = exp(-x^2 / w0^2)
+    kappa = 200.0                       # tilt of lambda kappa = 0.2 mrad
+    M = {"free": orc.compose_abcd([orc.abcd_free_space(0.5)]),
+         "free_back": orc.compose_abcd([orc.abcd_free_space(-0.4)]),
+         "lens_free": orc.compose_abcd([orc.abcd_free_space(0.3), orc.abcd_lens(0.8),
+                                        orc.abcd_free_space(0.2)]),
+         "mirror": orc.compose_abcd([orc.abcd_free_space(0.2), orc.abcd_mirror(1.5),
+                                     orc.abcd_free_space(0.25)]),
+         "conjugate": orc.abcd_conjugate_plane(0.6)}[case]
+    assert abs(np.linalg.det(M) - 1) < 1e-12
+    q2 = (M[0, 0] * q1 + M[0, 1]) / (M[1, 0] * q1 + M[1, 1])
+    n, dx = 128, 40e-6
+    xs = orc.spec_xs(n, dx, 3 * dx)
+    w_out = np.sqrt(wl / (np.pi * np.imag(1 / q2)))
+    x_c = wl * kappa * M[0, 1]          # where the tilted beam lands (A x + B theta, x = 0)
+    dxo = 8 * w_out / 110
+    xo = orc.spec_xs(120, dxo, x_c - 5 * dxo)
+    g = np.exp(1j * np.pi * xs ** 2 / (wl * q1) + 2j * np.pi * kappa * xs)
+    u = g[:, None] * g[None, :]
+    out = orc.lct(u, wl, xs, xo, M)
+    h = _gaussian_lct_closed_form(xo, wl, q1, kappa, M) * np.sqrt(dxo / dx)
+    ref = h[:, None] * h[None, :]
+    assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) < 1e-9
+    # power is conserved (to the tails the +-4 w_out window leaves out)
+    assert abs(np.sum(np.abs(out) ** 2) / np.sum(np.abs(u) ** 2) - 1) < 1e-8
+    # untilted: the ABCD law
+    h0 = _gaussian_lct_closed_form(xo, wl, q1, 0.0, M)
+    abcd_law = np.exp(1j * np.pi * xo ** 2 / (wl * q2))
+    ratio = h0 / abcd_law
+    assert np.allclose(ratio, ratio[0], rtol=1e-9)
+
+
+def test_lct_conjugate_plane_is_the_mft():
+    """[[0, f], [-1/f, 0]]: the LCT is the MFT with focal length f times 1/i
+    (utils/propagation.py:178-256 is pinned above)."""
+    rng = np.random.default_rng(2)
+    n, n_out, dx, dxo, wl, f = 24, 20, 1e-3 / 24, 2e-6, 0.9e-6, 3.0
+    u = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    out = orc.lct(u, wl, orc.spec_xs(n, dx), orc.spec_xs(n_out, dxo),
+                  orc.abcd_conjugate_plane(f))
+    ref = orc.mft(u, wl, dx, n_out, dxo, focal_length=f) / 1j
+    assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("n,m", [(16, 32), (15, 31), (16, 64)])
+def test_lct_fft_form_equals_collins_sum(n, m):
+    """The chirp -> centred FFT -> chirp form equals the Collins sum on the
+    FFT grid, for B of either sign and off-centre grids."""
+    rng = np.random.default_rng(n + m)
+    for M in (orc.compose_abcd([orc.abcd_free_space(0.3), orc.abcd_lens(0.5)]),
+              orc.abcd_conjugate_plane(2.0), orc.abcd_free_space(-0.4)):
+        for cin, cout in ((0.0, 0.0), (1e-4, -3e-4)):
+            u = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+            F, d_out = orc.lct_fft(u, 1e-6, 1e-4, cin, cout, m, M)
+            ref = orc.lct(u, 1e-6, orc.spec_xs(n, 1e-4, cin), orc.spec_xs(m, d_out, cout), M)
+            assert np.max(np.abs(F - ref)) / np.max(np.abs(ref)) < 1e-12
