@@ -48,6 +48,7 @@ def _parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-detector", action="store_true", help="skip the detector-chain leg")
     ap.add_argument("--no-aperture", action="store_true", help="skip the dynamic-aperture leg")
+    ap.add_argument("--no-lct", action="store_true", help="skip the LCT leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient leg")
     ap.add_argument("--calibrate", action="store_true",
@@ -331,6 +332,64 @@ def _aperture_leg(torch, steps, warmup=3):
             "regenerate_note": "soft transmissions (two launches each) + 36-mode basis"}
 
 
+def _lct_leg(torch, steps, check, warmup=3):
+    """Linear canonical transforms of a lens + free-space system (B = 0.5 m):
+    the Collins integral of 64 wavelengths of a 512^2 field to 256^2 on the
+    MFT's split-BF16x3 GEMMs (dlx_lct), and the FFT form of one 1024^2 field
+    on a 2048^2 grid (dlx_lct_fft).  Algorithmic work: 8 (N_out N_in^2 +
+    N_out^2 N_in) FLOPs per wavelength for the matrix form; two FFT passes
+    reading and writing M^2 complex64 values plus the two chirp passes for
+    the FFT form."""
+    from dlux_amd import runtime
+    abcd = np.array([[1.0, 0.0], [-1.0 / 0.8, 1.0]])
+    abcd = np.array([[1.0, 0.2], [0.0, 1.0]]) @ abcd @ np.array([[1.0, 0.3], [0.0, 1.0]])
+    n, n_out, nwl, dx, dxo = 512, 256, 64, 20e-6, 25e-6
+    wls = np.linspace(0.8e-6, 1.2e-6, nwl)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    f = torch.polar(torch.ones((nwl, n, n), device="cuda"),
+                    torch.rand((nwl, n, n), generator=g, device="cuda")).contiguous()
+    ev = lambda: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    for _ in range(warmup):
+        out = runtime.lct(f, wls, abcd, dx, 0.0, n_out, dxo, 0.0, precision="bf16x3")
+    torch.cuda.synchronize()
+    beg, end = ev()
+    beg.record()
+    for _ in range(steps):
+        out = runtime.lct(f, wls, abcd, dx, 0.0, n_out, dxo, 0.0, precision="bf16x3")
+    end.record()
+    torch.cuda.synchronize()
+    ms = beg.elapsed_time(end) / steps
+    flops = nwl * 8.0 * (n_out * n * n + n_out * n_out * n)
+    err = None
+    if check:
+        from oracle import dlux_oracle as orc
+        u0 = f[0].cpu().numpy().astype(np.complex128)
+        ref = orc.lct(u0, wls[0], orc.spec_xs(n, dx), orc.spec_xs(n_out, dxo), abcd)
+        err = float(np.max(np.abs(out[0].cpu().numpy() - ref)) / np.max(np.abs(ref)))
+    # FFT form
+    nf, m = 1024, 2048
+    f1 = torch.polar(torch.ones((nf, nf), device="cuda"),
+                     torch.rand((nf, nf), generator=g, device="cuda")).contiguous()
+    for _ in range(warmup):
+        runtime.lct_fft(f1, [1e-6], abcd, 5e-6, 0.0, m, 0.0)
+    torch.cuda.synchronize()
+    beg.record()
+    for _ in range(steps):
+        runtime.lct_fft(f1, [1e-6], abcd, 5e-6, 0.0, m, 0.0)
+    end.record()
+    torch.cuda.synchronize()
+    ms_fft = beg.elapsed_time(end) / steps
+    fft_bytes = 2 * 2 * 8 * m * m + 2 * 8 * nf * nf + 2 * 8 * m * m
+    return {"workload": "lct: lens f = 0.8 m between 0.3 m and 0.2 m of free space; "
+                        "64 lambda x 512^2 -> 256^2 (split-BF16x3 GEMMs)",
+            "ms_per_call": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+            "frac_of_bf16_peak": flops / (ms * 1e-3) / 1e12 / 2500.0,
+            "max_rel_err_vs_fp64": err,
+            "fft_workload": "1024^2 -> 2048^2 FFT grid, one wavelength",
+            "fft_ms_per_call": ms_fft,
+            "fft_achieved_GBps": fft_bytes / (ms_fft * 1e-3) / 1e9}
+
+
 def _serving_leg(torch, rates, duration):
     """Poisson request stream (75% PSF, 20% gradient, 5% ASM) at each rate."""
     from dlux_amd import serving
@@ -422,6 +481,8 @@ def main():
                                                            not args.no_cpu)
     print("[bench] dynamic-aperture leg", file=sys.stderr, flush=True)
     aperture = None if args.no_aperture else _aperture_leg(torch, max(args.steps // 5, 10))
+    print("[bench] lct leg", file=sys.stderr, flush=True)
+    lct = None if args.no_lct else _lct_leg(torch, max(args.steps // 10, 5), not args.no_cpu)
     print("[bench] serving leg", file=sys.stderr, flush=True)
     serve = None if args.no_serving else _serving_leg(torch, args.serving_rates,
                                                       args.serving_seconds)
@@ -479,6 +540,7 @@ def main():
         "asm": asm,
         "detector": detector,
         "aperture": aperture,
+        "lct": lct,
         "serving": serve,
     }
     print(json.dumps(line))

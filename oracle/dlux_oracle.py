@@ -27,6 +27,10 @@ Parity status per function (all pinned unless marked):
   * asm_prop (angular spectrum)                                  parity unpinned
       -- its conventions live in abcdLux (absent); only textbook invariants
          (z=0 identity, unitarity, z1+z2 composition) are checked.
+  * abcd_* / compose_abcd / lct / lct_fft (Collins integral)      parity unpinned
+      -- against abcdLux (absent); pinned by closed forms: a tilted Gaussian
+         beam (the ABCD law), the conjugate plane as the MFT, the FFT form as
+         the Collins sum on the FFT grid.
 """
 
 from __future__ import annotations
