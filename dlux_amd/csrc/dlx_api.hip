@@ -372,9 +372,8 @@ int dlx_lct(int32_t n_in, int32_t n_out, int32_t batch, const double* abcd,
   const size_t mat = align_up((size_t)batch * mat_bytes(n_out, n_in, split));
   uint8_t* K_ws = (uint8_t*)workspace;
   uint8_t* Ut = K_ws + mat;
-  // complex64 kernels; in the split form staged in the intermediate's space
-  float2* Kf = split ? (float2*)Ut : (float2*)K_ws;
   const long kst = (long)n_out * n_in;
+  const long kbytes = (long)mat_bytes(n_out, n_in, split);
   const double x0i = center_in - (n_in - 1) / 2.0 * pixel_scale_in;
   const double x0o = center_out - (n_out - 1) / 2.0 * pixel_scale_out;
   std::vector<double> alpha(batch);
@@ -389,23 +388,25 @@ int dlx_lct(int32_t n_in, int32_t n_out, int32_t batch, const double* abcd,
       c.c0[i] = B > 0 ? -0.125 : 0.125;   // the phase of 1 / sqrt(i B), per axis
       alpha[b0 + i] = pixel_scale_in * pixel_scale_out / (wavelength[b0 + i] * std::fabs(B));
     }
-    DLX_HIP(launch_lct_matrix(c, nb, n_in, n_out, x0i, pixel_scale_in, x0o, pixel_scale_out,
-                              Kf + (long)b0 * kst, kst, s));
+    if (split)
+      DLX_HIP(launch_lct_matrix_split4(c, nb, n_in, n_out, x0i, pixel_scale_in, x0o,
+                                       pixel_scale_out, K_ws + (long)b0 * kbytes, kbytes, s));
+    else
+      DLX_HIP(launch_lct_matrix(c, nb, n_in, n_out, x0i, pixel_scale_in, x0o, pixel_scale_out,
+                                (float2*)K_ws + (long)b0 * kst, kst, s));
   }
   dlx_mft_geom g{};
   g.n_in = n_in;
   g.n_out = n_out;
   if (split) {
-    DLX_HIP(launch_to_split4(Kf, batch, n_out, n_in, kst, K_ws,
-                             (long)mat_bytes(n_out, n_in, true), s));
     uint8_t* X = K_ws + 2 * mat;
     DLX_HIP(launch_to_split4((const float2*)in, batch, n_in, n_in, (long)n_in * n_in, X,
                              (long)mat_bytes(n_in, n_in, true), s));
     return mft_split4(g, batch, X, K_ws, K_ws, Ut, alpha.data(), out, EPI_STORE, s, nullptr,
                       nullptr);
   }
-  return mft_fp32(g, batch, (const float2*)in, Kf, Kf, (float2*)Ut, alpha.data(), out,
-                  EPI_STORE, s, false);
+  return mft_fp32(g, batch, (const float2*)in, (const float2*)K_ws, (const float2*)K_ws,
+                  (float2*)Ut, alpha.data(), out, EPI_STORE, s, false);
 }
 
 size_t dlx_lct_fft_workspace_size(int32_t n, int32_t m, int32_t batch) {

@@ -184,6 +184,9 @@ struct LctChirpChunk {
 hipError_t launch_lct_matrix(const LctMatChunk& c, int nb, int n_in, int n_out, double x0_in,
                              double dx_in, double x0_out, double dx_out, float2* out,
                              long bstride, hipStream_t s);
+hipError_t launch_lct_matrix_split4(const LctMatChunk& c, int nb, int n_in, int n_out,
+                                    double x0_in, double dx_in, double x0_out, double dx_out,
+                                    uint8_t* out, long batch_bytes, hipStream_t s);
 hipError_t launch_lct_chirp(const LctChirpChunk& c, int nb, int n, const float2* in,
                             float2* out, hipStream_t s);
 hipError_t launch_pupil_chunk(const PupilChunk& c, int nb, int n, double dx, double x0,

@@ -8,8 +8,21 @@
 // so a 2-D field maps as u_out = K u_in K^T.  All phases are accumulated in
 // FP64 turns and reduced to [-1/2, 1/2] before the hardware sine / cosine.
 #include "dlx_internal.h"
+#include "dlx_split.h"
 
 namespace dlx {
+
+__device__ __forceinline__ float2 lct_element(const LctMatChunk& c, int b, double x,
+                                              double xp) {
+  // each term reduced on its own: |ci2 x^2| can reach 1e6 turns
+  double t1 = c.ci2[b] * x * x, t2 = c.co2[b] * xp * xp, t3 = c.cx[b] * x * xp;
+  t1 -= rint(t1);
+  t2 -= rint(t2);
+  t3 -= rint(t3);
+  double t = t1 + t2 + t3 + c.c0[b];
+  t -= rint(t);
+  return make_float2(__builtin_amdgcn_cosf((float)t), __builtin_amdgcn_sinf((float)t));
+}
 
 // K[b][l][k] = exp(2 pi i (ci2 x_k^2 + co2 x'_l^2 + cx x_k x'_l + c0)),
 // x_k = x0_in + k dx_in, x'_l = x0_out + l dx_out; stored N_out x N_in.
@@ -21,15 +34,40 @@ lct_matrix_kernel(LctMatChunk c, int n_in, int n_out, double x0_in, double dx_in
   const int b = blockIdx.z;
   if (k >= n_in) return;
   const double x = x0_in + (double)k * dx_in, xp = x0_out + (double)l * dx_out;
-  // each term reduced on its own: |ci2 x^2| can reach 1e6 turns
-  double t1 = c.ci2[b] * x * x, t2 = c.co2[b] * xp * xp, t3 = c.cx[b] * x * xp;
-  t1 -= rint(t1);
-  t2 -= rint(t2);
-  t3 -= rint(t3);
-  double t = t1 + t2 + t3 + c.c0[b];
-  t -= rint(t);
-  const float s = __builtin_amdgcn_sinf((float)t), co = __builtin_amdgcn_cosf((float)t);
-  out[(long)b * bstride + (long)l * n_in + k] = make_float2(co, s);
+  out[(long)b * bstride + (long)l * n_in + k] = lct_element(c, b, x, xp);
+}
+
+// The same matrices written in split4 (rows l, k = n_in): a quad of lanes per
+// 8-element chunk, quad-cooperative 16-byte stores; padding k is zero.
+__global__ void __launch_bounds__(256)
+lct_matrix_split4_kernel(LctMatChunk c, int n_in, int n_out, double x0_in, double dx_in,
+                         double x0_out, double dx_out, uint8_t* __restrict__ out,
+                         long batch_bytes) {
+  const long kc = split4_chunks(n_in);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (t >= (long)n_out * kc * 4) return;   // whole quads exit together
+  const int q = (int)(t & 3);
+  const long cidx = t >> 2;
+  const int l = (int)(cidx / kc), ch = (int)(cidx - (long)l * kc);
+  const double xp = x0_out + (double)l * dx_out;
+  float2 v[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int k = ch * 8 + 2 * q + e;
+    v[e] = k < n_in ? lct_element(c, b, x0_in + (double)k * dx_in, xp) : make_float2(0.f, 0.f);
+  }
+  store_chunk_quad(out + (long)b * batch_bytes, l, split4_row_bytes(n_in), ch, q, v[0], v[1]);
+}
+
+hipError_t launch_lct_matrix_split4(const LctMatChunk& c, int nb, int n_in, int n_out,
+                                    double x0_in, double dx_in, double x0_out, double dx_out,
+                                    uint8_t* out, long batch_bytes, hipStream_t s) {
+  const long work = (long)n_out * split4_chunks(n_in) * 4;
+  hipLaunchKernelGGL(lct_matrix_split4_kernel, dim3((unsigned)((work + 255) / 256), nb),
+                     dim3(256), 0, s, c, n_in, n_out, x0_in, dx_in, x0_out, dx_out, out,
+                     batch_bytes);
+  return hipGetLastError();
 }
 
 hipError_t launch_lct_matrix(const LctMatChunk& c, int nb, int n_in, int n_out, double x0_in,
