@@ -468,8 +468,10 @@ def main():
                          f"(numpy BLAS, {threads} threads)"}
 
     print("[bench] gradient leg", file=sys.stderr, flush=True)
+    # the step is host-bound: a longer warmup lets the host side settle (the first
+    # few dozen steps of a fresh process run up to 1.5x slower)
     grad = None if args.no_grad else _grad_leg(torch, runtime, workloads, "bf16x3",
-                                                max(args.steps // 5, 5), 3, not args.no_cpu)
+                                                max(args.steps // 5, 5), 30, not args.no_cpu)
     print("[bench] uneven batch leg", file=sys.stderr, flush=True)
     batched = None if args.no_batch else _batch_leg(torch, runtime, workloads,
                                                     max(args.steps // 20, 3), 2)
