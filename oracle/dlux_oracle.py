@@ -25,8 +25,9 @@ Parity status per function (all pinned unless marked):
   * detector_vjp                                                 pinned (FD, adjoint identity)
   * coord_transform / soften / *_distance / soft_aperture / polike  pinned
   * asm_prop (angular spectrum)                                  parity unpinned
-      -- its conventions live in abcdLux (absent); only textbook invariants
-         (z=0 identity, unitarity, z1+z2 composition) are checked.
+      -- its conventions live in abcdLux (absent); checked by textbook
+         invariants (z=0 identity, unitarity, z1+z2 composition) and the
+         paraxial Gaussian-beam closed form.
   * abcd_* / compose_abcd / lct / lct_fft (Collins integral)      parity unpinned
       -- against abcdLux (absent); pinned by closed forms: a tilted Gaussian
          beam (the ABCD law), the conjugate plane as the MFT, the FFT form as
