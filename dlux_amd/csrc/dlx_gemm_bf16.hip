@@ -515,17 +515,18 @@ __device__ __forceinline__ void deep_mainloop_impl(uint8_t* lds, const uint8_t* 
   for (int s = 0; s < nst; ++s) {
     __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of stage s have landed
     __syncthreads();                 // every wave's have; stage s-1 reads retired
-    if (s + 1 < nst) issue(s + 1);
     const uint8_t* la = lds + (s & 1) * kBuf;
     const uint8_t* lb = la + BM * kRowDeep;
     u32x4_t fa[TM][4], fb[TN][4];
-#pragma unroll
-    for (int i = 0; i < TM; ++i) {
+    // fragments in the order they are consumed: A row block 0, all of B, the
+    // rest of A, so the first row's products wait only for their own reads
+    auto read_a = [&](int i) {
       const int r = wm + 32 * i + lr;
 #pragma unroll
       for (int p = 0; p < 4; ++p)
         fa[i][p] = *(const u32x4_t*)(la + r * kRowDeep + (((lk << 2) | p) ^ ((r >> 1) & 7)) * 16);
-    }
+    };
+    read_a(0);
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       const int r = wn + 32 * j + lr;
@@ -533,11 +534,18 @@ __device__ __forceinline__ void deep_mainloop_impl(uint8_t* lds, const uint8_t* 
       for (int p = 0; p < 4; ++p)
         fb[j][p] = *(const u32x4_t*)(lb + r * kRowDeep + (((lk << 2) | p) ^ ((r >> 1) & 7)) * 16);
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 1; i < TM; ++i) read_a(i);
+    // the next stage's loads go out behind this stage's LDS reads (measured
+    // 1.5-2 % faster than issuing them first at 512 -> 256)
+    if (s + 1 < nst) issue(s + 1);
     // raise this wave's issue priority while it feeds the matrix cores, so the
     // SIMD's other wave does its loads and LDS reads in the gaps
     if (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
+      __builtin_amdgcn_sched_barrier(0);
       const bf16x8_t arh = as_bf(fa[i][0]), arl = as_bf(fa[i][1]);
       const bf16x8_t aih = as_bf(fa[i][2]), ail = as_bf(fa[i][3]);
       const bf16x8_t naih = as_bf(neg_bf(fa[i][2])), nail = as_bf(neg_bf(fa[i][3]));
@@ -562,6 +570,7 @@ __device__ __forceinline__ void deep_mainloop_impl(uint8_t* lds, const uint8_t* 
         accI[i][j] = im;
       }
     }
+    __builtin_amdgcn_sched_barrier(0);
     if (PRIO) __builtin_amdgcn_s_setprio(0);
   }
 }
