@@ -878,12 +878,18 @@ hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s) {
     return hipGetLastError();
   }
   // 128 x 128 tiles: the occupancy-2 kernel when they give two workgroups per
-  // CU (256 CUs), else the one-workgroup-per-CU direct-to-LDS kernel; 64 x 64
-  // tiles for small problems.
+  // CU (256 CUs); else 128 x 64 tiles on it when those do (7 % faster than
+  // one 128 x 128 workgroup per CU for the 64-lambda 256^2 output stage); else
+  // the one-workgroup-per-CU direct-to-LDS kernel; 64 x 64 tiles for small
+  // problems.
   const long tiles128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch;
   int mode = staging_mode();
   if (mode == 3) mode = 0;
-  if (mode == 0 && tiles128 < 512) mode = 1;
+  if (mode == 0 && tiles128 < 512) {
+    const long tiles_2x1 = (long)((g.M + 127) / 128) * ((g.N + 63) / 64) * g.batch;
+    if (g.M > 64 && g.N > 64 && tiles_2x1 >= 512) return dispatch_split<2, 1>(g, s, 0);
+    mode = 1;
+  }
   if (g.M <= 64 || g.N <= 64 || tiles128 < 128) return dispatch_split<1, 1>(g, s, mode);
   return dispatch_split<2, 2>(g, s, mode);
 }
