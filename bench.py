@@ -34,7 +34,7 @@ ERR_BOUND = 1e-5
 # FETCH_SIZE / WRITE_SIZE runs scaled by the same counters on a streaming kernel of
 # known size (profiles/mft_poly_512to256_64wl_hbm_traffic.md).  The counters cannot
 # be read from inside this process.
-TRAFFIC_BYTES = {"bf16x3": 269.0e6}
+TRAFFIC_BYTES = {"bf16x3": 273.2e6}
 
 
 def _parse():

@@ -377,36 +377,54 @@ int dlx_lct(int32_t n_in, int32_t n_out, int32_t batch, const double* abcd,
   const double x0i = center_in - (n_in - 1) / 2.0 * pixel_scale_in;
   const double x0o = center_out - (n_out - 1) / 2.0 * pixel_scale_out;
   std::vector<double> alpha(batch);
-  for (int b0 = 0; b0 < batch; b0 += LctMatChunk::kMax) {
-    const int nb = batch - b0 < LctMatChunk::kMax ? batch - b0 : LctMatChunk::kMax;
-    LctMatChunk c;
-    for (int i = 0; i < nb; ++i) {
-      const double lb = wavelength[b0 + i] * B;
-      c.ci2[i] = A / (2.0 * lb);
-      c.co2[i] = D / (2.0 * lb);
-      c.cx[i] = -1.0 / lb;
-      c.c0[i] = B > 0 ? -0.125 : 0.125;   // the phase of 1 / sqrt(i B), per axis
-      alpha[b0 + i] = pixel_scale_in * pixel_scale_out / (wavelength[b0 + i] * std::fabs(B));
+  for (int b = 0; b < batch; ++b)
+    alpha[b] = pixel_scale_in * pixel_scale_out / (wavelength[b] * std::fabs(B));
+  // the kernels depend only on the grids, the ABCD matrix and lambda: keep them
+  // resident in the device cache across calls (generated on a miss)
+  std::vector<double> key = {1001.0, (double)n_in, (double)n_out, (double)batch,
+                             split ? 1.0 : 0.0, A, B, D, pixel_scale_in, center_in,
+                             pixel_scale_out, center_out};
+  key.insert(key.end(), wavelength, wavelength + batch);
+  void *cached = nullptr, *handle = nullptr;
+  bool fresh = true;
+  if (int st = cache_acquire_raw(key, (size_t)batch * kbytes, s, &cached, &handle, &fresh))
+    return st;
+  uint8_t* Kp = cached ? (uint8_t*)cached : K_ws;
+  if (fresh) {
+    for (int b0 = 0; b0 < batch; b0 += LctMatChunk::kMax) {
+      const int nb = batch - b0 < LctMatChunk::kMax ? batch - b0 : LctMatChunk::kMax;
+      LctMatChunk c;
+      for (int i = 0; i < nb; ++i) {
+        const double lb = wavelength[b0 + i] * B;
+        c.ci2[i] = A / (2.0 * lb);
+        c.co2[i] = D / (2.0 * lb);
+        c.cx[i] = -1.0 / lb;
+        c.c0[i] = B > 0 ? -0.125 : 0.125;   // the phase of 1 / sqrt(i B), per axis
+      }
+      if (split)
+        DLX_HIP(launch_lct_matrix_split4(c, nb, n_in, n_out, x0i, pixel_scale_in, x0o,
+                                         pixel_scale_out, Kp + (long)b0 * kbytes, kbytes, s));
+      else
+        DLX_HIP(launch_lct_matrix(c, nb, n_in, n_out, x0i, pixel_scale_in, x0o,
+                                  pixel_scale_out, (float2*)Kp + (long)b0 * kst, kst, s));
     }
-    if (split)
-      DLX_HIP(launch_lct_matrix_split4(c, nb, n_in, n_out, x0i, pixel_scale_in, x0o,
-                                       pixel_scale_out, K_ws + (long)b0 * kbytes, kbytes, s));
-    else
-      DLX_HIP(launch_lct_matrix(c, nb, n_in, n_out, x0i, pixel_scale_in, x0o, pixel_scale_out,
-                                (float2*)K_ws + (long)b0 * kst, kst, s));
+    if (int st = cache_ready(handle, s)) return st;
   }
   dlx_mft_geom g{};
   g.n_in = n_in;
   g.n_out = n_out;
+  int st;
   if (split) {
     uint8_t* X = K_ws + 2 * mat;
     DLX_HIP(launch_to_split4((const float2*)in, batch, n_in, n_in, (long)n_in * n_in, X,
                              (long)mat_bytes(n_in, n_in, true), s));
-    return mft_split4(g, batch, X, K_ws, K_ws, Ut, alpha.data(), out, EPI_STORE, s, nullptr,
-                      nullptr);
-  }
-  return mft_fp32(g, batch, (const float2*)in, (const float2*)K_ws, (const float2*)K_ws,
+    st = mft_split4(g, batch, X, Kp, Kp, Ut, alpha.data(), out, EPI_STORE, s, nullptr, nullptr);
+  } else {
+    st = mft_fp32(g, batch, (const float2*)in, (const float2*)Kp, (const float2*)Kp,
                   (float2*)Ut, alpha.data(), out, EPI_STORE, s, false);
+  }
+  if (int r = cache_release(handle, s)) return r;
+  return st;
 }
 
 size_t dlx_lct_fft_workspace_size(int32_t n, int32_t m, int32_t batch) {

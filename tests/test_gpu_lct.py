@@ -48,8 +48,12 @@ def test_lct_vs_oracle(rt, case, prec):
     dx, dxo, c_in, c_out = 40e-6, 35e-6, 2e-5, -7e-5
     wls = np.array([0.9e-6, 1.0e-6, 1.2e-6])
     u = rng.normal(size=(nb, n_in, n_in)) + 1j * rng.normal(size=(nb, n_in, n_in))
-    got = rt.lct(torch.as_tensor(u.astype(np.complex64), device="cuda"), wls, _MATS[case],
-                 dx, c_in, n_out, dxo, c_out, precision=prec).cpu().numpy()
+    ut = torch.as_tensor(u.astype(np.complex64), device="cuda")
+    # three calls: generated, then cached on the second sighting, then a hit
+    runs = [rt.lct(ut, wls, _MATS[case], dx, c_in, n_out, dxo, c_out, precision=prec)
+            for _ in range(3)]
+    assert all(torch.equal(runs[0], r) for r in runs[1:])
+    got = runs[0].cpu().numpy()
     tol = {"fp32": 2e-5, "bf16x3": 5e-5}[prec]
     for b in range(nb):
         ref = orc.lct(u[b], wls[b], orc.spec_xs(n_in, dx, c_in), orc.spec_xs(n_out, dxo, c_out),
