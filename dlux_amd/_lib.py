@@ -162,6 +162,11 @@ SIGNATURES = {
                                      c_void_p, c_size_t, c_void_p]),
     "dlx_detector_vjp": (c_int32, [c_int32, c_int32, _fp, POINTER(DetectorOp), c_int32, _fp,
                                    _fp, c_void_p, c_size_t, c_void_p]),
+    "dlx_detector_vjp_params_workspace_size": (c_size_t, [c_int32, c_int32, POINTER(DetectorOp),
+                                                          c_int32]),
+    "dlx_detector_vjp_params": (c_int32, [c_int32, c_int32, _fp, POINTER(DetectorOp), c_int32,
+                                          _fp, _fp, POINTER(c_void_p), c_void_p, c_size_t,
+                                          c_void_p]),
     "dlx_soft_apertures_workspace_size": (c_size_t, [c_int32, c_int32]),
     "dlx_soft_apertures": (c_int32, [c_int32, c_double, c_double, c_int32,
                                      POINTER(SoftAperture), _fp, c_void_p, c_size_t,

@@ -40,6 +40,7 @@ struct EwList {
   int kind[kMaxEw];
   float value[kMaxEw];
   const float* map[kMaxEw];
+  int op[kMaxEw];   // index of the op in the caller's list
 };
 
 // v after the ops of l; p indexes the pixel within its image (maps are per image).
@@ -558,6 +559,93 @@ det_ew_grad_kernel(const float* g, const float* __restrict__ r, int npix, EwList
   out[i] = g[i] * ew_deriv(post, r[i], p);
 }
 
+// Parameter cotangents of one elementwise list.  Per pixel, the values entering
+// each op are recomputed from v0 (the value entering the list) and the
+// cotangent gout (after the list) is walked back through the ops:
+//   map m:        dL/dm[p] += v * g,          g *= m[p]
+//   saturate t:   dL/dt    += g [v > t] (1/2 at v == t, jnp.minimum's tie rule)
+//   add c:        dL/dc    += g
+// Maps sum over the batch in the thread (fixed order); scalars leave one
+// partial per block and slot for det_param_final_kernel.  With din set, the
+// cotangent entering the list is stored too.
+struct ParamOut {
+  float* map[kMaxEw];
+};
+
+__global__ void __launch_bounds__(256)
+det_param_grad_kernel(EwList l, const float* __restrict__ v0, const float* __restrict__ gout,
+                      int npix, int batch, float* __restrict__ din, ParamOut po,
+                      double* __restrict__ part) {
+  __shared__ double red[kMaxEw][4];
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  double mg[kMaxEw], sc[kMaxEw];
+#pragma unroll
+  for (int j = 0; j < kMaxEw; ++j) { mg[j] = 0.0; sc[j] = 0.0; }
+  if (p < npix) {
+    for (int b = 0; b < batch; ++b) {
+      const long i = (long)b * npix + p;
+      float v[kMaxEw + 1];
+      v[0] = v0[i];
+#pragma unroll
+      for (int j = 0; j < kMaxEw; ++j) {
+        if (j >= l.n) break;
+        const int k = l.kind[j];
+        const float x = v[j];
+        v[j + 1] = k == DLX_DET_PIXEL_RESPONSE ? x * l.map[j][p]
+                   : k == DLX_DET_SATURATE     ? (x > l.value[j] ? l.value[j] : x)
+                                               : x + l.value[j];
+      }
+      float g = gout[i];
+#pragma unroll
+      for (int j = kMaxEw - 1; j >= 0; --j) {
+        if (j >= l.n) continue;
+        const int k = l.kind[j];
+        const float x = v[j];
+        if (k == DLX_DET_PIXEL_RESPONSE) {
+          mg[j] += (double)x * g;
+          g *= l.map[j][p];
+        } else if (k == DLX_DET_SATURATE) {
+          const float t = l.value[j];
+          sc[j] += (double)g * (x > t ? 1.0 : (x == t ? 0.5 : 0.0));
+          g *= x < t ? 1.f : (x == t ? 0.5f : 0.f);
+        } else {
+          sc[j] += (double)g;
+        }
+      }
+      if (din) din[i] = g;
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxEw; ++j)
+      if (j < l.n && po.map[j]) po.map[j][p] = (float)mg[j];
+  }
+  // block sums of the scalar slots: wave shuffles, then the 4 waves in order
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kMaxEw; ++j) {
+    double x = sc[j];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[j][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxEw) {
+    const int j = threadIdx.x;
+    part[(long)blockIdx.x * kMaxEw + j] = red[j][0] + red[j][1] + red[j][2] + red[j][3];
+  }
+}
+
+struct ScalarOut {
+  float* out[kMaxEw];
+};
+
+__global__ void __launch_bounds__(64)
+det_param_final_kernel(const double* __restrict__ part, int nblk, ScalarOut so) {
+  const int j = threadIdx.x;
+  if (j >= kMaxEw || !so.out[j]) return;
+  double acc = 0.0;
+  for (int b = 0; b < nblk; ++b) acc += part[(long)b * kMaxEw + j];
+  so.out[j][0] = (float)acc;
+}
+
 struct Pass {
   int conv;        // 1: convolve, 0: block sum (k = 1: elementwise only)
   int n_in, n_out;
@@ -568,11 +656,12 @@ struct Pass {
   EwList pre, post;
 };
 
-bool ew_push(EwList& l, const dlx_detector_op& op) {
+bool ew_push(EwList& l, const dlx_detector_op& op, int index) {
   if (l.n == kMaxEw) return false;
   l.kind[l.n] = op.kind;
   l.value[l.n] = (float)op.value;
   l.map[l.n] = op.data;
+  l.op[l.n] = index;
   ++l.n;
   return true;
 }
@@ -603,14 +692,14 @@ int plan(int n, const dlx_detector_op* ops, int nops, std::vector<Pass>& passes)
       case DLX_DET_SATURATE:
       case DLX_DET_ADD_CONSTANT: {
         EwList& target = passes.empty() ? pending : passes.back().post;
-        if (!ew_push(target, op)) {
+        if (!ew_push(target, op, i)) {
           Pass p = make_pass(0, cur);
           if (passes.empty()) {
             p.pre = pending;
             pending = EwList{};
           }
           passes.push_back(p);
-          ew_push(passes.back().post, op);
+          ew_push(passes.back().post, op, i);
         }
         break;
       }
@@ -826,27 +915,72 @@ int dlx_detector_apply(int32_t n, int32_t batch, const float* in, const dlx_dete
   return DLX_OK;
 }
 
-int dlx_detector_vjp(int32_t n, int32_t batch, const float* in, const dlx_detector_op* ops,
-                     int32_t nops, const float* dout, float* din, void* workspace,
-                     size_t workspace_bytes, void* stream) {
-  clear_error();
+}  // extern "C"
+
+namespace {
+
+// Whether any op of l has a parameter cotangent requested.
+bool wants(const EwList& l, float* const* grads) {
+  if (!grads) return false;
+  for (int j = 0; j < l.n; ++j)
+    if (grads[l.op[j]]) return true;
+  return false;
+}
+
+size_t param_extra(const std::vector<Pass>& passes, int batch) {
+  int side_max = 0;
+  for (const Pass& p : passes) side_max = std::max(side_max, std::max(p.n_in, p.n_out));
+  const size_t nblk = ((size_t)side_max * side_max + 255) / 256;
+  return img_bytes(passes[0].n_in, batch) + align_up(nblk * kMaxEw * sizeof(double));
+}
+
+int param_grads(const EwList& l, float* const* grads, const float* v0, const float* gout,
+                int side, int batch, float* din, double* part, hipStream_t s) {
+  ParamOut po{};
+  ScalarOut so{};
+  for (int j = 0; j < l.n; ++j) {
+    float* gp = grads[l.op[j]];
+    if (l.kind[j] == DLX_DET_PIXEL_RESPONSE) po.map[j] = gp;
+    else so.out[j] = gp;
+  }
+  const int npix = side * side;
+  const int nblk = (npix + 255) / 256;
+  hipLaunchKernelGGL(det_param_grad_kernel, dim3(nblk), dim3(256), 0, s, l, v0, gout, npix,
+                     batch, din, po, part);
+  if (int st = launch_status("det_param_grad_kernel")) return st;
+  hipLaunchKernelGGL(det_param_final_kernel, dim3(1), dim3(64), 0, s, (const double*)part, nblk,
+                     so);
+  return launch_status("det_param_final_kernel");
+}
+
+int detector_vjp_impl(int32_t n, int32_t batch, const float* in, const dlx_detector_op* ops,
+                      int32_t nops, const float* dout, float* din, float* const* grads,
+                      void* workspace, size_t workspace_bytes, void* stream, const char* name) {
   std::vector<Pass> passes;
   int st = plan(n, ops, nops, passes);
   if (st) return st;
   DLX_CHECK_ARG(batch >= 0, "negative detector batch");
+  if (grads)
+    for (int k = 0; k < nops; ++k)
+      DLX_CHECK_ARG(!grads[k] || ops[k].kind == DLX_DET_PIXEL_RESPONSE ||
+                        ops[k].kind == DLX_DET_SATURATE || ops[k].kind == DLX_DET_ADD_CONSTANT,
+                    "op %d: parameter gradients are implemented for pixel responses, "
+                    "saturation and constants only", k);
   if (batch == 0) return DLX_OK;
-  DLX_CHECK_ARG(in && dout && din, "null image pointer to dlx_detector_vjp");
+  DLX_CHECK_ARG(in && dout && din, "null image pointer to %s", name);
   if ((st = check_grid(passes, batch))) return st;
   const Layout L = layout(passes, batch, true);
-  if (workspace_bytes < L.total || !workspace) {
-    set_error("dlx_detector_vjp workspace too small: need %zu bytes, got %zu", L.total,
-              workspace_bytes);
+  const size_t need = L.total + (grads ? param_extra(passes, batch) : 0);
+  if (workspace_bytes < need || !workspace) {
+    set_error("%s workspace too small: need %zu bytes, got %zu", name, need, workspace_bytes);
     return DLX_ERR_WORKSPACE;
   }
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
   const size_t np = passes.size();
   float* g[2] = {(float*)(ws + L.g_off[0]), (float*)(ws + L.g_off[1])};
+  float* T = grads ? (float*)(ws + L.total) : nullptr;
+  double* part = grads ? (double*)(ws + L.total + img_bytes(passes[0].n_in, batch)) : nullptr;
   auto x_in = [&](size_t i) -> const float* {
     return i == 0 ? in : (const float*)(ws + L.x_off[i]);
   };
@@ -864,6 +998,10 @@ int dlx_detector_vjp(int32_t n, int32_t batch, const float* in, const dlx_detect
     const float* gr = cur;
     int gr_idx = cur_idx;
     if (p.post.n > 0) {
+      if (wants(p.post, grads) &&
+          (st = param_grads(p.post, grads, (const float*)(ws + L.r_off[ii]), cur, p.n_out,
+                            batch, nullptr, part, s)))
+        return st;
       gr_idx = cur_idx < 0 ? 0 : cur_idx;
       const int npix = p.n_out * p.n_out;
       hipLaunchKernelGGL(det_ew_grad_kernel, dim3((npix + 255) / 256, batch), dim3(256), 0, s,
@@ -873,6 +1011,11 @@ int dlx_detector_vjp(int32_t n, int32_t batch, const float* in, const dlx_detect
     }
     const int dst_idx = gr_idx == 0 ? 1 : 0;
     float* dst = ii == 0 ? din : g[dst_idx];
+    // a pre list with parameter cotangents: the adjoint without it into T,
+    // then one pass for the list's parameters and the cotangent entering it
+    const bool pre_params = ii == 0 && wants(p.pre, grads);
+    const EwList pre = pre_params ? EwList{} : p.pre;
+    float* adst = pre_params ? T : dst;
     if (p.conv) {
       const int rpt = conv_rpt(p.K, p.sep);
       const int tiles_x = (p.n_in + kTileX - 1) / kTileX;
@@ -882,23 +1025,55 @@ int dlx_detector_vjp(int32_t n, int32_t batch, const float* in, const dlx_detect
       const dim3 grid(tiles_x * tiles_y, batch);
       if (rpt == 2)
         hipLaunchKernelGGL(det_corr_kernel<2>, grid, dim3(kThreads), lds, s, gr, p.n_in, tiles_x,
-                           p.K, lo, p.w, 0, p.sep, EwList{}, EwList{}, x_in(ii), p.pre,
-                           (float*)nullptr, dst);
+                           p.K, lo, p.w, 0, p.sep, EwList{}, EwList{}, x_in(ii), pre,
+                           (float*)nullptr, adst);
       else
         hipLaunchKernelGGL(det_corr_kernel<1>, grid, dim3(kThreads), lds, s, gr, p.n_in, tiles_x,
-                           p.K, lo, p.w, 0, p.sep, EwList{}, EwList{}, x_in(ii), p.pre,
-                           (float*)nullptr, dst);
+                           p.K, lo, p.w, 0, p.sep, EwList{}, EwList{}, x_in(ii), pre,
+                           (float*)nullptr, adst);
       if ((st = launch_status("det_corr_kernel"))) return st;
     } else {
       const int npix = p.n_in * p.n_in;
       hipLaunchKernelGGL(det_up_kernel, dim3((npix + 255) / 256, batch), dim3(256), 0, s, gr,
-                         p.n_in, p.k, p.n_out, x_in(ii), p.pre, dst);
+                         p.n_in, p.k, p.n_out, x_in(ii), pre, adst);
       if ((st = launch_status("det_up_kernel"))) return st;
     }
+    if (pre_params &&
+        (st = param_grads(p.pre, grads, in, T, p.n_in, batch, dst, part, s)))
+      return st;
     cur = dst;
     cur_idx = dst_idx;
   }
   return DLX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dlx_detector_vjp(int32_t n, int32_t batch, const float* in, const dlx_detector_op* ops,
+                     int32_t nops, const float* dout, float* din, void* workspace,
+                     size_t workspace_bytes, void* stream) {
+  clear_error();
+  return detector_vjp_impl(n, batch, in, ops, nops, dout, din, nullptr, workspace,
+                           workspace_bytes, stream, "dlx_detector_vjp");
+}
+
+size_t dlx_detector_vjp_params_workspace_size(int32_t n, int32_t batch,
+                                              const dlx_detector_op* ops, int32_t nops) {
+  std::vector<Pass> passes;
+  if (batch < 0 || plan(n, ops, nops, passes)) return 0;
+  return layout(passes, batch, true).total + param_extra(passes, batch);
+}
+
+int dlx_detector_vjp_params(int32_t n, int32_t batch, const float* in,
+                            const dlx_detector_op* ops, int32_t nops, const float* dout,
+                            float* din, float* const* grads, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(grads, "null parameter-gradient list to dlx_detector_vjp_params");
+  return detector_vjp_impl(n, batch, in, ops, nops, dout, din, grads, workspace,
+                           workspace_bytes, stream, "dlx_detector_vjp_params");
 }
 
 }  // extern "C"

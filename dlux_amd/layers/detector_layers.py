@@ -56,6 +56,20 @@ class DetectorLayer(Base):
         return t
 
 
+def _scalar(v):
+    """A scalar layer parameter: a torch tensor is kept (gradients can reach
+    it), anything else becomes a float64 numpy scalar."""
+    return v if isinstance(v, torch.Tensor) else np.asarray(v, float)
+
+
+def _op_scalar(v):
+    """The op argument of a scalar parameter: the tensor itself when it
+    requires grad, else a Python float."""
+    if isinstance(v, torch.Tensor):
+        return v if v.requires_grad else float(v.detach().cpu())
+    return float(v)
+
+
 class ApplyPixelResponse(DetectorLayer):
     """psf * pixel_response (detector_layers.py:100-131)."""
 
@@ -69,8 +83,9 @@ class ApplyPixelResponse(DetectorLayer):
 
     def _det_op(self):
         resp = self.pixel_response
-        if isinstance(resp, torch.Tensor) and resp.is_cuda:
-            return ("pixel_response", resp)
+        if isinstance(resp, torch.Tensor) and (resp.is_cuda or resp.requires_grad):
+            # a tracked host tensor is uploaded on every call (its graph is per call)
+            return ("pixel_response", resp if resp.is_cuda else runtime.as_float(resp))
         return ("pixel_response", self._device((resp,), lambda: runtime.as_float(resp)))
 
     def __call__(self, psf: PSF) -> PSF:
@@ -122,10 +137,10 @@ class ApplySaturation(DetectorLayer):
     """minimum(psf, threshold) (detector_layers.py:202-229)."""
 
     def __init__(self, threshold):
-        self.threshold = np.asarray(threshold, float)
+        self.threshold = _scalar(threshold)
 
     def _det_op(self):
-        return ("saturate", float(self.threshold))
+        return ("saturate", _op_scalar(self.threshold))
 
     def __call__(self, psf: PSF) -> PSF:
         return self._run(psf)
@@ -135,10 +150,10 @@ class AddConstant(DetectorLayer):
     """psf + value (detector_layers.py:232-259)."""
 
     def __init__(self, value):
-        self.value = np.asarray(value, float)
+        self.value = _scalar(value)
 
     def _det_op(self):
-        return ("add", float(self.value))
+        return ("add", _op_scalar(self.value))
 
     def __call__(self, psf: PSF) -> PSF:
         return self._run(psf)

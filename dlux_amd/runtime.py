@@ -691,7 +691,8 @@ def _detector_ops(ops, dev):
         elif kind == "downsample":
             op.size = int(arg)
         else:
-            op.value = float(np.asarray(arg, np.float32))
+            op.value = float(arg.detach().float().cpu()) if isinstance(arg, torch.Tensor) \
+                else float(np.asarray(arg, np.float32))
     return arr, keep
 
 
@@ -726,7 +727,9 @@ def _detector_impl(images, ops) -> torch.Tensor:
     return out
 
 
-def _detector_vjp_impl(images, ops, dout) -> torch.Tensor:
+def _detector_vjp_impl(images, ops, dout, param_ops=()):
+    """d/dimages (and, for the op indices in param_ops, d/d each op's map or
+    value) of the detector chain for the cotangent dout."""
     dev = device()
     x = as_float(images, dev).contiguous()
     g = as_float(dout, dev).contiguous()
@@ -742,39 +745,84 @@ def _detector_vjp_impl(images, ops, dout) -> torch.Tensor:
         raise ValueError(f"cotangent shape {tuple(g.shape)} != output shape "
                          f"{lead + (side.value, side.value)}")
     din = torch.empty_like(x)
-    ws = workspace(lib.dlx_detector_workspace_size(n, batch, arr, len(ops), 1), dev)
-    st = lib.dlx_detector_vjp(n, batch, x.data_ptr(), arr, len(ops), g.data_ptr(),
-                              din.data_ptr(), ws.data_ptr(), ws.numel(), stream_handle())
-    _lib.check(st, "dlx_detector_vjp")
+    if not param_ops:
+        ws = workspace(lib.dlx_detector_workspace_size(n, batch, arr, len(ops), 1), dev)
+        st = lib.dlx_detector_vjp(n, batch, x.data_ptr(), arr, len(ops), g.data_ptr(),
+                                  din.data_ptr(), ws.data_ptr(), ws.numel(), stream_handle())
+        _lib.check(st, "dlx_detector_vjp")
+        del keep
+        return din
+    grads = {}
+    ptrs = (ctypes.c_void_p * max(len(ops), 1))()
+    for k in param_ops:
+        kind = ops[k][0]
+        if kind == "pixel_response":
+            t = torch.empty((arr[k].size, arr[k].size), dtype=torch.float32, device=dev)
+        elif kind in ("saturate", "add"):
+            t = torch.empty((), dtype=torch.float32, device=dev)
+        else:
+            raise NotImplementedError(
+                f"gradients with respect to the {kind} array are not implemented")
+        grads[k] = t
+        ptrs[k] = t.data_ptr()
+    ws = workspace(lib.dlx_detector_vjp_params_workspace_size(n, batch, arr, len(ops)), dev)
+    st = lib.dlx_detector_vjp_params(n, batch, x.data_ptr(), arr, len(ops), g.data_ptr(),
+                                     din.data_ptr(), ptrs, ws.data_ptr(), ws.numel(),
+                                     stream_handle())
+    _lib.check(st, "dlx_detector_vjp_params")
     del keep
-    return din
+    return din, grads
+
+
+def _tracked_param(arg) -> bool:
+    return isinstance(arg, torch.Tensor) and arg.requires_grad
 
 
 class _DetectorFn(torch.autograd.Function):
+    """The detector chain as a function of the images and of the tracked op
+    parameters (pixel-response maps, saturation levels, constants) listed in
+    pidx; the backward is dlx_detector_vjp(_params)."""
+
     @staticmethod
-    def forward(ctx, images, ops):
+    def forward(ctx, images, ops, pidx, *params):
         ctx.ops = ops
+        ctx.pidx = pidx
         ctx.save_for_backward(images)
         return _detector_impl(images, ops)
 
     @staticmethod
     def backward(ctx, g):
         (images,) = ctx.saved_tensors
-        return _detector_vjp_impl(images, ctx.ops, g), None
+        want = [k for k, need in zip(ctx.pidx, ctx.needs_input_grad[3:]) if need]
+        if not want:
+            return _detector_vjp_impl(images, ctx.ops, g), None, None, *([None] * len(ctx.pidx))
+        din, grads = _detector_vjp_impl(images, ctx.ops, g, want)
+        out = []
+        for k in ctx.pidx:
+            gk = grads.get(k)
+            if gk is not None:
+                ref = ctx.ops[k][1]
+                gk = gk.to(device=ref.device, dtype=ref.dtype).reshape(ref.shape)
+            out.append(gk)
+        return (din if ctx.needs_input_grad[0] else None), None, None, *out
 
 
 def detector(images, ops) -> torch.Tensor:
     """The detector chain `ops` applied to float32 images (..., n, n), as one
-    dlx_detector_apply call; differentiable in `images` (dlx_detector_vjp)."""
-    for kind, arg in ops:
+    dlx_detector_apply call; differentiable in `images` and in pixel-response
+    maps, saturation levels and constants given as tensors that require grad
+    (dlx_detector_vjp / dlx_detector_vjp_params)."""
+    pidx = []
+    for k, (kind, arg) in enumerate(ops):
         args = arg if isinstance(arg, (tuple, list)) else (arg,)
-        if torch.is_grad_enabled() and any(isinstance(a, torch.Tensor) and a.requires_grad
-                                           for a in args):
-            raise NotImplementedError(
-                f"gradients with respect to the {kind} array are not implemented")
-    if (torch.is_grad_enabled() and isinstance(images, torch.Tensor)
-            and images.requires_grad):
-        return _DetectorFn.apply(images, list(ops))
+        if torch.is_grad_enabled() and any(_tracked_param(a) for a in args):
+            if kind not in ("pixel_response", "saturate", "add"):
+                raise NotImplementedError(
+                    f"gradients with respect to the {kind} array are not implemented")
+            pidx.append(k)
+    track_images = isinstance(images, torch.Tensor) and images.requires_grad
+    if torch.is_grad_enabled() and (track_images or pidx):
+        return _DetectorFn.apply(images, list(ops), pidx, *[ops[k][1] for k in pidx])
     return _detector_impl(images, ops)
 
 

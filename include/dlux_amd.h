@@ -499,6 +499,24 @@ int dlx_detector_vjp(int32_t n, int32_t batch, const float* in, const dlx_detect
                      size_t workspace_bytes, void* stream);
 
 /*
+ * dlx_detector_vjp plus the cotangents of the ops' parameters: grads[k] (a
+ * host array of nops device pointers; NULL = not wanted) receives, for a
+ * DLX_DET_PIXEL_RESPONSE op, dL/dmap (size x size floats, summed over the
+ * batch) and, for DLX_DET_SATURATE / DLX_DET_ADD_CONSTANT, dL/dvalue (one
+ * float; the saturation level takes jnp.minimum's 1/2 at ties).  Convolution
+ * kernels and downsample factors have none (a non-NULL entry is an error).
+ * Replaces the reverse-mode derivative JAX builds for the detector layers'
+ * arrays (ApplyPixelResponse.pixel_response, ApplySaturation.saturation,
+ * AddConstant.value; layers/detector_layers.py:100-260).
+ */
+size_t dlx_detector_vjp_params_workspace_size(int32_t n, int32_t batch,
+                                              const dlx_detector_op* ops, int32_t nops);
+int dlx_detector_vjp_params(int32_t n, int32_t batch, const float* in,
+                            const dlx_detector_op* ops, int32_t nops, const float* dout,
+                            float* din, float* const* grads, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
+/*
  * Transfer-matrix cache (no counterpart in the reference: dLux rebuilds the
  * matrices inside every traced call).  dlx_mft, dlx_psf_poly,
  * dlx_psf_poly_vjp and dlx_mft_batched_psf keep the matrices of each

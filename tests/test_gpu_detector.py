@@ -281,3 +281,67 @@ def test_resolved_sources_vs_oracle(dl):
     scene = dl.Scene([("disk", prs), ("star", dl.PointSource(wls, position=-pos, flux=1.5))])
     ref3 = ref2 + star(wls, np.full(2, 0.5) * 1.5, -pos)
     assert _err(scene.model(optics), ref3) < PSF_TOL
+
+
+@pytest.mark.parametrize("where", ["pre", "post"])
+def test_detector_parameter_gradients_vs_fd(dl, where):
+    """dlx_detector_vjp_params through torch autograd: the cotangents of two
+    pixel-response maps (one entering a convolution -- the load-side list --
+    or applied after it, and one after a downsample), a saturation level and a
+    constant, against central differences of the oracle chain (directional
+    for the maps); the image cotangent against the oracle VJP.  Bound 2e-5
+    relative (float32 sums over the image and batch)."""
+    print(f"[detector] parameter gradients, map {where} the convolution", flush=True)
+    rng = np.random.default_rng(71 if where == "pre" else 72)
+    n, batch, k = 48, 3, 4
+    x = rng.uniform(0.0, 1.0, (batch, n, n))
+    kern = rng.uniform(0.0, 1.0, (5, 5))
+    kern /= kern.sum()
+    m1 = rng.uniform(0.8, 1.2, (n, n))
+    m2 = rng.uniform(0.8, 1.2, (n // k, n // k))
+    c = 0.37
+
+    def chain_of(m1_, m2_, t_, c_):
+        head = ([("pixel_response", m1_), ("convolve", kern)] if where == "pre"
+                else [("convolve", kern), ("pixel_response", m1_)])
+        return head + [("downsample", k), ("pixel_response", m2_), ("saturate", t_),
+                       ("add", c_)]
+    pre_sat = _oracle(x, chain_of(m1, m2, 1e30, 0.0)[:-2])
+    t = float(np.float32(_threshold_in_gap(pre_sat, 0.7)))
+    w = rng.normal(size=(batch, n // k, n // k))
+
+    def loss(m1_, m2_, t_, c_):
+        return float(np.sum(w * _oracle(x, chain_of(m1_, m2_, t_, c_))))
+
+    dev = "cuda"
+    f32 = lambda a: torch.tensor(np.asarray(a, np.float32), device=dev, requires_grad=True)
+    xt, m1t, m2t, tt, ct = f32(x), f32(m1), f32(m2), f32(t), f32(c)
+    chain_t = chain_of(m1t, m2t, tt, ct)
+    chain_t[1 if where == "pre" else 0] = ("convolve", torch.as_tensor(
+        kern.astype(np.float32), device=dev))
+    out = dl.runtime.detector(xt, chain_t)
+    assert _err(out, _oracle(x, chain_of(m1, m2, t, c))) < CHAIN_TOL
+    (out * torch.as_tensor(w.astype(np.float32), device=dev)).sum().backward()
+    h = 1e-4
+    V1, V2 = rng.normal(size=m1.shape), rng.normal(size=m2.shape)
+    fd = {
+        "m1": (loss(m1 + h * V1, m2, t, c) - loss(m1 - h * V1, m2, t, c)) / (2 * h),
+        "m2": (loss(m1, m2 + h * V2, t, c) - loss(m1, m2 - h * V2, t, c)) / (2 * h),
+        "t": (loss(m1, m2, t + h * t, c) - loss(m1, m2, t - h * t, c)) / (2 * h * t),
+        "c": (loss(m1, m2, t, c + h) - loss(m1, m2, t, c - h)) / (2 * h),
+    }
+    got = {
+        "m1": float(np.sum(m1t.grad.double().cpu().numpy() * V1)),
+        "m2": float(np.sum(m2t.grad.double().cpu().numpy() * V2)),
+        "t": float(tt.grad), "c": float(ct.grad),
+    }
+    for key in fd:
+        err = abs(got[key] - fd[key]) / max(abs(fd[key]), 1e-12)
+        print(f"  d/d{key}: got {got[key]:.6e} fd {fd[key]:.6e} err {err:.2e}")
+        assert err < 2e-5, key
+    assert abs(got["c"] - np.sum(w)) < 2e-5 * abs(np.sum(w)) + 1e-4
+    vref = np.stack([orc.detector_vjp(xi, chain_of(m1, m2, t, c), wi) for xi, wi in zip(x, w)])
+    assert _err(xt.grad, vref) < CHAIN_TOL
+    # a convolution kernel that requires grad is refused
+    with pytest.raises(NotImplementedError):
+        dl.runtime.detector(xt, [("convolve", f32(kern))])

@@ -608,3 +608,21 @@ def test_lct_abi_validation_without_gpu(lib):
     assert need >= 16 * 16 * 8 + 32 * 32 * 8
     st = lib.dlx_lct_fft(16, 32, 1, abcd, wl, 1e-3, 0.0, 0.0, 4096, 4096, None, None, 0, None)
     assert st == 3 and b"workspace" in lib.dlx_last_error()
+
+
+def test_detector_param_vjp_abi_validation_without_gpu(lib):
+    from dlux_amd import _lib
+    ops = (_lib.DetectorOp * 3)()
+    ops[0].kind, ops[0].size, ops[0].data = _lib.DLX_DET_CONVOLVE, 3, 4096
+    ops[1].kind, ops[1].size = _lib.DLX_DET_DOWNSAMPLE, 2
+    ops[2].kind, ops[2].value = _lib.DLX_DET_ADD_CONSTANT, 1.0
+    ptrs = (ctypes.c_void_p * 3)(4096, None, None)
+    st = lib.dlx_detector_vjp_params(16, 1, 4096, ops, 3, 4096, 4096, ptrs, None, 0, None)
+    assert st == 1 and b"parameter gradients are implemented" in lib.dlx_last_error()
+    ptrs[0], ptrs[2] = None, 8192
+    need = lib.dlx_detector_vjp_params_workspace_size(16, 1, ops, 3)
+    assert need > lib.dlx_detector_workspace_size(16, 1, ops, 3, 1)
+    st = lib.dlx_detector_vjp_params(16, 1, 4096, ops, 3, 4096, 4096, ptrs, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+    st = lib.dlx_detector_vjp_params(16, 1, 4096, ops, 3, 4096, 4096, None, None, 0, None)
+    assert st == 1 and b"null parameter-gradient list" in lib.dlx_last_error()
