@@ -637,6 +637,87 @@ struct ScalarOut {
   float* out[kMaxEw];
 };
 
+// Cotangent of a convolution kernel: dL/dK[a][b] = sum over outputs i of
+// g[i] * x[i + c - (a, b)] (x as the convolution saw it: load-side ops
+// applied, zero outside the image; c = (K - 1) / 2 per axis).  One workgroup
+// per 64 x 16 output tile and image stages the tile's x window and g in LDS
+// and leaves one FP64 partial per tap; det_kgrad_final_kernel sums them.
+__global__ void __launch_bounds__(256)
+det_kgrad_kernel(const float* __restrict__ x, const float* __restrict__ g, int n, int tiles_x,
+                 int K, EwList pre, double* __restrict__ part) {
+  extern __shared__ float sm[];
+  const int W = kTileX + K - 1, H = kTileY + K - 1;
+  float* xs = sm;
+  float* gs = sm + H * W;
+  const int tile = blockIdx.x;
+  const int tx0 = (tile % tiles_x) * kTileX, ty0 = (tile / tiles_x) * kTileY;
+  const int c = (K - 1) / 2;
+  const long img = (long)blockIdx.y * n * n;
+  for (int e = threadIdx.x; e < H * W; e += 256) {
+    const int r = e / W, q = e - r * W;
+    const int y = ty0 + c - (K - 1) + r, xx = tx0 + c - (K - 1) + q;
+    float v = 0.f;
+    if (y >= 0 && y < n && xx >= 0 && xx < n) {
+      const long p = (long)y * n + xx;
+      v = ew_apply(pre, x[img + p], p);
+    }
+    xs[e] = v;
+  }
+  for (int e = threadIdx.x; e < kTileY * kTileX; e += 256) {
+    const int r = e / kTileX, q = e - r * kTileX;
+    const int y = ty0 + r, xx = tx0 + q;
+    gs[e] = (y < n && xx < n) ? g[img + (long)y * n + xx] : 0.f;
+  }
+  __syncthreads();
+  const long base = ((long)blockIdx.y * gridDim.x + tile) * K * K;
+  for (int t = threadIdx.x; t < K * K; t += 256) {
+    const int a = t / K, bb = t - a * K;
+    double acc = 0.0;
+    for (int iy = 0; iy < kTileY; ++iy) {
+      const float* xr = xs + (iy + K - 1 - a) * W + (K - 1 - bb);
+      const float* gr = gs + iy * kTileX;
+      float sum = 0.f;
+      for (int ix = 0; ix < kTileX; ++ix) sum += gr[ix] * xr[ix];
+      acc += sum;
+    }
+    part[base + t] = acc;
+  }
+}
+
+// One workgroup per tap: the fixed-order sum of its nparts partials.
+__global__ void __launch_bounds__(256)
+det_kgrad_final_kernel(const double* __restrict__ part, long nparts, int K2,
+                       float* __restrict__ out) {
+  __shared__ double red[256];
+  const int t = blockIdx.x;
+  double acc = 0.0;
+  for (long w = threadIdx.x; w < nparts; w += 256) acc += part[w * K2 + t];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[t] = (float)red[0];
+}
+
+// Separable kernels K[a][b] = wy[a] wx[b]: dL/dwy[a] = sum_b dK[a][b] wx[b],
+// dL/dwx[b] = sum_a dK[a][b] wy[a] (out: the K column then the K row values).
+__global__ void __launch_bounds__(128)
+det_kgrad_sep_kernel(const float* __restrict__ dK, int K, const float* __restrict__ w,
+                     float* __restrict__ out) {
+  for (int t = threadIdx.x; t < 2 * K; t += 128) {
+    double acc = 0.0;
+    if (t < K) {
+      for (int b = 0; b < K; ++b) acc += (double)dK[t * K + b] * w[K + b];
+    } else {
+      const int b = t - K;
+      for (int a = 0; a < K; ++a) acc += (double)dK[a * K + b] * w[a];
+    }
+    out[t] = (float)acc;
+  }
+}
+
 __global__ void __launch_bounds__(64)
 det_param_final_kernel(const double* __restrict__ part, int nblk, ScalarOut so) {
   const int j = threadIdx.x;
@@ -647,6 +728,7 @@ det_param_final_kernel(const double* __restrict__ part, int nblk, ScalarOut so) 
 }
 
 struct Pass {
+  int op_index;    // the convolve op's index in the caller's list
   int conv;        // 1: convolve, 0: block sum (k = 1: elementwise only)
   int n_in, n_out;
   int K;           // convolve kernel side
@@ -709,6 +791,7 @@ int plan(int n, const dlx_detector_op* ops, int nops, std::vector<Pass>& passes)
         DLX_CHECK_ARG(op.size >= 1 && op.size <= kMaxKernel,
                       "convolution kernel side must be in [1, %d], got %d", kMaxKernel, op.size);
         Pass p = make_pass(1, cur);
+        p.op_index = i;
         p.K = op.size;
         p.sep = op.kind == DLX_DET_CONVOLVE_SEPARABLE;
         p.w = op.data;
@@ -927,11 +1010,51 @@ bool wants(const EwList& l, float* const* grads) {
   return false;
 }
 
+size_t kgrad_parts(const Pass& p, int batch) {
+  const size_t tiles = (size_t)((p.n_in + kTileX - 1) / kTileX) * ((p.n_in + kTileY - 1) / kTileY);
+  return tiles * batch * p.K * p.K;
+}
+
+size_t kgrad_bytes(const std::vector<Pass>& passes, int batch) {
+  size_t most = 0;
+  for (const Pass& p : passes)
+    if (p.conv)
+      most = std::max(most, align_up(kgrad_parts(p, batch) * sizeof(double)) +
+                                align_up((size_t)p.K * p.K * sizeof(float)));
+  return most;
+}
+
 size_t param_extra(const std::vector<Pass>& passes, int batch) {
   int side_max = 0;
   for (const Pass& p : passes) side_max = std::max(side_max, std::max(p.n_in, p.n_out));
   const size_t nblk = ((size_t)side_max * side_max + 255) / 256;
-  return img_bytes(passes[0].n_in, batch) + align_up(nblk * kMaxEw * sizeof(double));
+  return img_bytes(passes[0].n_in, batch) + align_up(nblk * kMaxEw * sizeof(double)) +
+         kgrad_bytes(passes, batch);
+}
+
+// dL/d(kernel) of convolve pass p (K x K) or of its separable factors (2K).
+int kernel_grad(const Pass& p, const float* x, const float* g, int batch, float* out,
+                uint8_t* ws, hipStream_t s) {
+  const int tiles_x = (p.n_in + kTileX - 1) / kTileX;
+  const int tiles_y = (p.n_in + kTileY - 1) / kTileY;
+  const long nparts = (long)tiles_x * tiles_y * batch;
+  double* part = (double*)ws;
+  float* dK = (float*)(ws + align_up(kgrad_parts(p, batch) * sizeof(double)));
+  const size_t lds = sizeof(float) * ((size_t)(kTileX + p.K - 1) * (kTileY + p.K - 1) +
+                                      kTileX * kTileY);
+  hipLaunchKernelGGL(det_kgrad_kernel, dim3(tiles_x * tiles_y, batch), dim3(256), lds, s, x, g,
+                     p.n_in, tiles_x, p.K, p.pre, part);
+  if (int st = launch_status("det_kgrad_kernel")) return st;
+  const int K2 = p.K * p.K;
+  hipLaunchKernelGGL(det_kgrad_final_kernel, dim3(K2), dim3(256), 0, s, (const double*)part,
+                     nparts, K2, p.sep ? dK : out);
+  if (int st = launch_status("det_kgrad_final_kernel")) return st;
+  if (p.sep) {
+    hipLaunchKernelGGL(det_kgrad_sep_kernel, dim3(1), dim3(128), 0, s, (const float*)dK, p.K,
+                       p.w, out);
+    if (int st = launch_status("det_kgrad_sep_kernel")) return st;
+  }
+  return DLX_OK;
 }
 
 int param_grads(const EwList& l, float* const* grads, const float* v0, const float* gout,
@@ -962,10 +1085,8 @@ int detector_vjp_impl(int32_t n, int32_t batch, const float* in, const dlx_detec
   DLX_CHECK_ARG(batch >= 0, "negative detector batch");
   if (grads)
     for (int k = 0; k < nops; ++k)
-      DLX_CHECK_ARG(!grads[k] || ops[k].kind == DLX_DET_PIXEL_RESPONSE ||
-                        ops[k].kind == DLX_DET_SATURATE || ops[k].kind == DLX_DET_ADD_CONSTANT,
-                    "op %d: parameter gradients are implemented for pixel responses, "
-                    "saturation and constants only", k);
+      DLX_CHECK_ARG(!grads[k] || ops[k].kind != DLX_DET_DOWNSAMPLE,
+                    "op %d: a downsample factor has no gradient", k);
   if (batch == 0) return DLX_OK;
   DLX_CHECK_ARG(in && dout && din, "null image pointer to %s", name);
   if ((st = check_grid(passes, batch))) return st;
@@ -981,6 +1102,13 @@ int detector_vjp_impl(int32_t n, int32_t batch, const float* in, const dlx_detec
   float* g[2] = {(float*)(ws + L.g_off[0]), (float*)(ws + L.g_off[1])};
   float* T = grads ? (float*)(ws + L.total) : nullptr;
   double* part = grads ? (double*)(ws + L.total + img_bytes(passes[0].n_in, batch)) : nullptr;
+  uint8_t* kws = nullptr;
+  if (grads) {
+    int side_max = 0;
+    for (const Pass& q : passes) side_max = std::max(side_max, std::max(q.n_in, q.n_out));
+    const size_t nblk = ((size_t)side_max * side_max + 255) / 256;
+    kws = (uint8_t*)part + align_up(nblk * kMaxEw * sizeof(double));
+  }
   auto x_in = [&](size_t i) -> const float* {
     return i == 0 ? in : (const float*)(ws + L.x_off[i]);
   };
@@ -1011,6 +1139,9 @@ int detector_vjp_impl(int32_t n, int32_t batch, const float* in, const dlx_detec
     }
     const int dst_idx = gr_idx == 0 ? 1 : 0;
     float* dst = ii == 0 ? din : g[dst_idx];
+    if (p.conv && grads && grads[p.op_index] &&
+        (st = kernel_grad(p, x_in(ii), gr, batch, grads[p.op_index], kws, s)))
+      return st;
     // a pre list with parameter cotangents: the adjoint without it into T,
     // then one pass for the list's parameters and the cotangent entering it
     const bool pre_params = ii == 0 && wants(p.pre, grads);

@@ -99,7 +99,7 @@ class ApplyJitter(DetectorLayer):
 
     def __init__(self, sigma, kernel_size: int = 9, oversample: int = 3):
         self.kernel_size = int(kernel_size)
-        self.sigma = np.asarray(sigma, float)
+        self.sigma = _scalar(sigma)
         self.oversample = int(oversample)
         if self.kernel_size <= 0:
             raise ValueError("kernel_size must be greater than 0.")
@@ -123,8 +123,22 @@ class ApplyJitter(DetectorLayer):
         g = g.reshape(n, f).sum(1)
         return (g / g.sum()).astype(np.float32)
 
+    def _factor_tracked(self) -> torch.Tensor:
+        """_factor as a torch expression of a sigma that requires grad (the
+        gradient reaches sigma through dlx_detector_vjp_params' kernel
+        cotangent)."""
+        s = self.sigma.to(runtime.device(), torch.float64)
+        n, f = self.kernel_size, self.oversample
+        xs = torch.linspace(-5.0, 5.0, n * f, dtype=torch.float64, device=s.device)
+        g = torch.exp(-0.5 * (xs / s) ** 2).reshape(n, f).sum(1)
+        return (g / g.sum()).float()
+
     def _det_op(self):
         # two 1-D passes (2K multiply-adds per pixel) instead of K^2
+        if isinstance(self.sigma, torch.Tensor) and self.sigma.requires_grad \
+                and torch.is_grad_enabled():
+            f = self._factor_tracked()
+            return ("convolve_separable", (f, f))
         key = (float(self.sigma), self.kernel_size, self.oversample)
         f = self._device(key, lambda: runtime.as_float(self._factor()))
         return ("convolve_separable", (f, f))

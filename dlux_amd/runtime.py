@@ -756,13 +756,14 @@ def _detector_vjp_impl(images, ops, dout, param_ops=()):
     ptrs = (ctypes.c_void_p * max(len(ops), 1))()
     for k in param_ops:
         kind = ops[k][0]
-        if kind == "pixel_response":
+        if kind in ("pixel_response", "convolve"):
             t = torch.empty((arr[k].size, arr[k].size), dtype=torch.float32, device=dev)
+        elif kind == "convolve_separable":
+            t = torch.empty((2 * arr[k].size,), dtype=torch.float32, device=dev)
         elif kind in ("saturate", "add"):
             t = torch.empty((), dtype=torch.float32, device=dev)
         else:
-            raise NotImplementedError(
-                f"gradients with respect to the {kind} array are not implemented")
+            raise NotImplementedError(f"a {kind} op has no parameter gradient")
         grads[k] = t
         ptrs[k] = t.data_ptr()
     ws = workspace(lib.dlx_detector_vjp_params_workspace_size(n, batch, arr, len(ops)), dev)
@@ -780,49 +781,60 @@ def _tracked_param(arg) -> bool:
 
 class _DetectorFn(torch.autograd.Function):
     """The detector chain as a function of the images and of the tracked op
-    parameters (pixel-response maps, saturation levels, constants) listed in
-    pidx; the backward is dlx_detector_vjp(_params)."""
+    parameters (response maps, convolution kernels or their separable factors,
+    saturation levels, constants) listed in pmap as (op index, position in
+    the op's tuple or None); the backward is dlx_detector_vjp(_params)."""
 
     @staticmethod
-    def forward(ctx, images, ops, pidx, *params):
+    def forward(ctx, images, ops, pmap, *params):
         ctx.ops = ops
-        ctx.pidx = pidx
+        ctx.pmap = pmap
         ctx.save_for_backward(images)
         return _detector_impl(images, ops)
 
     @staticmethod
     def backward(ctx, g):
         (images,) = ctx.saved_tensors
-        want = [k for k, need in zip(ctx.pidx, ctx.needs_input_grad[3:]) if need]
+        need = ctx.needs_input_grad[3:]
+        want = sorted({k for (k, _), nd in zip(ctx.pmap, need) if nd})
+        nones = [None] * len(ctx.pmap)
         if not want:
-            return _detector_vjp_impl(images, ctx.ops, g), None, None, *([None] * len(ctx.pidx))
+            return _detector_vjp_impl(images, ctx.ops, g), None, None, *nones
         din, grads = _detector_vjp_impl(images, ctx.ops, g, want)
         out = []
-        for k in ctx.pidx:
-            gk = grads.get(k)
-            if gk is not None:
-                ref = ctx.ops[k][1]
-                gk = gk.to(device=ref.device, dtype=ref.dtype).reshape(ref.shape)
-            out.append(gk)
+        for (k, j), nd in zip(ctx.pmap, need):
+            if not nd:
+                out.append(None)
+                continue
+            gk = grads[k]
+            arg = ctx.ops[k][1]
+            ref = arg if j is None else arg[j]
+            if j is not None:   # separable factors: the column then the row values
+                half = gk.numel() // 2
+                gk = gk[:half] if j == 0 else gk[half:]
+            out.append(gk.to(device=ref.device, dtype=ref.dtype).reshape(ref.shape))
         return (din if ctx.needs_input_grad[0] else None), None, None, *out
 
 
 def detector(images, ops) -> torch.Tensor:
     """The detector chain `ops` applied to float32 images (..., n, n), as one
-    dlx_detector_apply call; differentiable in `images` and in pixel-response
-    maps, saturation levels and constants given as tensors that require grad
+    dlx_detector_apply call; differentiable in `images` and in the ops'
+    arrays and values given as tensors that require grad -- response maps,
+    convolution kernels (or separable factors), saturation levels, constants
     (dlx_detector_vjp / dlx_detector_vjp_params)."""
-    pidx = []
-    for k, (kind, arg) in enumerate(ops):
-        args = arg if isinstance(arg, (tuple, list)) else (arg,)
-        if torch.is_grad_enabled() and any(_tracked_param(a) for a in args):
-            if kind not in ("pixel_response", "saturate", "add"):
-                raise NotImplementedError(
-                    f"gradients with respect to the {kind} array are not implemented")
-            pidx.append(k)
+    pmap, params = [], []
+    if torch.is_grad_enabled():
+        for k, (kind, arg) in enumerate(ops):
+            parts = list(enumerate(arg)) if isinstance(arg, (tuple, list)) else [(None, arg)]
+            for j, a in parts:
+                if _tracked_param(a):
+                    if kind == "downsample":
+                        raise NotImplementedError("a downsample factor has no gradient")
+                    pmap.append((k, j))
+                    params.append(a)
     track_images = isinstance(images, torch.Tensor) and images.requires_grad
-    if torch.is_grad_enabled() and (track_images or pidx):
-        return _DetectorFn.apply(images, list(ops), pidx, *[ops[k][1] for k in pidx])
+    if torch.is_grad_enabled() and (track_images or pmap):
+        return _DetectorFn.apply(images, list(ops), pmap, *params)
     return _detector_impl(images, ops)
 
 

@@ -500,14 +500,18 @@ int dlx_detector_vjp(int32_t n, int32_t batch, const float* in, const dlx_detect
 
 /*
  * dlx_detector_vjp plus the cotangents of the ops' parameters: grads[k] (a
- * host array of nops device pointers; NULL = not wanted) receives, for a
- * DLX_DET_PIXEL_RESPONSE op, dL/dmap (size x size floats, summed over the
- * batch) and, for DLX_DET_SATURATE / DLX_DET_ADD_CONSTANT, dL/dvalue (one
- * float; the saturation level takes jnp.minimum's 1/2 at ties).  Convolution
- * kernels and downsample factors have none (a non-NULL entry is an error).
+ * host array of nops device pointers; NULL = not wanted) receives, summed
+ * over the batch,
+ *   DLX_DET_PIXEL_RESPONSE      dL/dmap     size x size floats
+ *   DLX_DET_CONVOLVE            dL/dkernel  size x size floats
+ *   DLX_DET_CONVOLVE_SEPARABLE  dL/d(column factor) then dL/d(row factor), 2 size floats
+ *   DLX_DET_SATURATE, DLX_DET_ADD_CONSTANT  dL/dvalue, one float (the saturation
+ *                               level takes jnp.minimum's 1/2 at ties)
+ * A downsample factor has none (a non-NULL entry is an error).
  * Replaces the reverse-mode derivative JAX builds for the detector layers'
- * arrays (ApplyPixelResponse.pixel_response, ApplySaturation.saturation,
- * AddConstant.value; layers/detector_layers.py:100-260).
+ * arrays (ApplyPixelResponse.pixel_response, ApplyJitter's kernel,
+ * ApplySaturation.saturation, AddConstant.value;
+ * layers/detector_layers.py:100-260, psfs.py:93-110).
  */
 size_t dlx_detector_vjp_params_workspace_size(int32_t n, int32_t batch,
                                               const dlx_detector_op* ops, int32_t nops);

@@ -342,6 +342,66 @@ def test_detector_parameter_gradients_vs_fd(dl, where):
     assert abs(got["c"] - np.sum(w)) < 2e-5 * abs(np.sum(w)) + 1e-4
     vref = np.stack([orc.detector_vjp(xi, chain_of(m1, m2, t, c), wi) for xi, wi in zip(x, w)])
     assert _err(xt.grad, vref) < CHAIN_TOL
-    # a convolution kernel that requires grad is refused
-    with pytest.raises(NotImplementedError):
-        dl.runtime.detector(xt, [("convolve", f32(kern))])
+
+
+def test_detector_kernel_and_jitter_gradients_vs_fd(dl):
+    """Kernel cotangents of dlx_detector_vjp_params: a full convolution
+    kernel (directional central difference of the oracle chain), separable
+    factors, and ApplyJitter's sigma through its Gaussian factor.  Bound 2e-5
+    relative."""
+    print("[detector] kernel and jitter gradients", flush=True)
+    from dlux_amd import layers
+    rng = np.random.default_rng(81)
+    n, batch = 64, 2
+    x = rng.uniform(0.0, 1.0, (batch, n, n))
+    kern = rng.uniform(0.0, 1.0, (7, 7))
+    kern /= kern.sum()
+    w = rng.normal(size=(batch, n // 2, n // 2))
+    resp = rng.uniform(0.9, 1.1, (n, n))
+
+    def loss(chain):
+        return float(np.sum(w * _oracle(x, chain)))
+
+    def chain_of(k_):
+        return [("pixel_response", resp), ("convolve", k_), ("downsample", 2), ("add", 0.5)]
+    dev = "cuda"
+    xt = torch.as_tensor(x.astype(np.float32), device=dev)
+    wt = torch.as_tensor(w.astype(np.float32), device=dev)
+    kt = torch.tensor(kern.astype(np.float32), device=dev, requires_grad=True)
+    rt = torch.as_tensor(resp.astype(np.float32), device=dev)
+    out = dl.runtime.detector(xt, [("pixel_response", rt), ("convolve", kt), ("downsample", 2),
+                                   ("add", 0.5)])
+    (out * wt).sum().backward()
+    h, V = 1e-4, rng.normal(size=kern.shape)
+    fd = (loss(chain_of(kern + h * V)) - loss(chain_of(kern - h * V))) / (2 * h)
+    got = float(np.sum(kt.grad.double().cpu().numpy() * V))
+    print(f"  d/dkernel: got {got:.6e} fd {fd:.6e}")
+    assert abs(got - fd) < 2e-5 * abs(fd)
+    # separable factors
+    wy, wx = rng.uniform(0, 1, 5), rng.uniform(0, 1, 5)
+    wyt = torch.tensor(wy.astype(np.float32), device=dev, requires_grad=True)
+    wxt = torch.tensor(wx.astype(np.float32), device=dev, requires_grad=True)
+    out = dl.runtime.detector(xt, [("convolve_separable", (wyt, wxt)), ("downsample", 2)])
+    (out * wt).sum().backward()
+    Vy, Vx = rng.normal(size=5), rng.normal(size=5)
+
+    def lsep(a, b):
+        return loss([("convolve_separable", (a, b)), ("downsample", 2)])
+    fdy = (lsep(wy + h * Vy, wx) - lsep(wy - h * Vy, wx)) / (2 * h)
+    fdx = (lsep(wy, wx + h * Vx) - lsep(wy, wx - h * Vx)) / (2 * h)
+    gy = float(np.sum(wyt.grad.double().cpu().numpy() * Vy))
+    gx = float(np.sum(wxt.grad.double().cpu().numpy() * Vx))
+    print(f"  d/dwy: got {gy:.6e} fd {fdy:.6e}; d/dwx: got {gx:.6e} fd {fdx:.6e}")
+    assert abs(gy - fdy) < 2e-5 * abs(fdy) and abs(gx - fdx) < 2e-5 * abs(fdx)
+    # ApplyJitter's sigma through a LayeredDetector
+    sig = torch.tensor(0.8, dtype=torch.float64, requires_grad=True)
+    det = dl.LayeredDetector([layers.ApplyJitter(sig), layers.Downsample(2)])
+    img = det(dl.PSF(xt[0], 1.0))
+    (img * wt[0]).sum().backward()
+
+    def ljit(s_):
+        return float(np.sum(w[0] * orc.detector(x[0], [("convolve", orc.jitter_kernel(s_)),
+                                                       ("downsample", 2)])))
+    fds = (ljit(0.8 + 1e-5) - ljit(0.8 - 1e-5)) / 2e-5
+    print(f"  d/dsigma: got {float(sig.grad):.6e} fd {fds:.6e}")
+    assert abs(float(sig.grad) - fds) < 1e-4 * abs(fds)

@@ -616,10 +616,10 @@ def test_detector_param_vjp_abi_validation_without_gpu(lib):
     ops[0].kind, ops[0].size, ops[0].data = _lib.DLX_DET_CONVOLVE, 3, 4096
     ops[1].kind, ops[1].size = _lib.DLX_DET_DOWNSAMPLE, 2
     ops[2].kind, ops[2].value = _lib.DLX_DET_ADD_CONSTANT, 1.0
-    ptrs = (ctypes.c_void_p * 3)(4096, None, None)
+    ptrs = (ctypes.c_void_p * 3)(None, 4096, None)
     st = lib.dlx_detector_vjp_params(16, 1, 4096, ops, 3, 4096, 4096, ptrs, None, 0, None)
-    assert st == 1 and b"parameter gradients are implemented" in lib.dlx_last_error()
-    ptrs[0], ptrs[2] = None, 8192
+    assert st == 1 and b"downsample factor has no gradient" in lib.dlx_last_error()
+    ptrs[0], ptrs[1], ptrs[2] = 4096, None, 8192
     need = lib.dlx_detector_vjp_params_workspace_size(16, 1, ops, 3)
     assert need > lib.dlx_detector_workspace_size(16, 1, ops, 3, 1)
     st = lib.dlx_detector_vjp_params(16, 1, 4096, ops, 3, 4096, 4096, ptrs, None, 0, None)
