@@ -23,7 +23,7 @@ from .layers import (OpticalLayer, TransmissiveLayer, AberratedLayer, BasisLayer
 from .optical_systems import (BaseOpticalSystem, OpticalSystem, ParametricOpticalSystem,
                               LayeredOpticalSystem, AngularOpticalSystem,
                               CartesianOpticalSystem)
-from .spectra import Spectrum
+from .spectra import BaseSpectrum, SimpleSpectrum, Spectrum, PolySpectrum
 from .sources import (BaseSource, Source, PointSource, PointSources, BinarySource, ResolvedSource,
                       PointResolvedSource, Scene)
 from .detectors import BaseDetector, LayeredDetector

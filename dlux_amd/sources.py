@@ -17,7 +17,7 @@ import torch
 
 from .base import Base
 from .psfs import PSF
-from .spectra import Spectrum
+from .spectra import BaseSpectrum, Spectrum
 from .utils.source import fluxes_from_contrast, positions_from_sep
 
 __all__ = ["BaseSource", "Source", "PointSource", "PointSources", "BinarySource",
@@ -53,7 +53,7 @@ def _infer_n_wavelengths(wavelengths, spectrum) -> int:
             "wavelengths must be provided when weights is None and no spectrum "
             "is supplied."
         )
-    if not isinstance(spectrum, Spectrum):
+    if not isinstance(spectrum, BaseSpectrum):
         raise TypeError("spectrum must be a dLux BaseSpectrum instance from dLux.spectra.")
     return len(np.asarray(spectrum.wavelengths, dtype=float))
 
@@ -125,7 +125,7 @@ class Source(BaseSource):
 
     def __init__(self, wavelengths=None, weights=None, spectrum=None):
         if spectrum is not None:
-            if not isinstance(spectrum, Spectrum):
+            if not isinstance(spectrum, BaseSpectrum):
                 raise TypeError(
                     f"spectrum must be a dLux BaseSpectrum instance from "
                     f"dLux.spectra, "

@@ -1,4 +1,8 @@
-"""Spectra (spectra.py:20-117): wavelengths plus weights normalised to sum 1."""
+"""Spectra (spectra.py:12-178): wavelengths plus weights normalised to sum 1.
+
+Spectra are host data (a few dozen wavelengths and weights); the optical
+systems hand them to the device as per-wavelength kernel arguments.
+"""
 
 from __future__ import annotations
 
@@ -6,14 +10,30 @@ import numpy as np
 
 from .base import Base
 
-__all__ = ["Spectrum"]
+__all__ = ["BaseSpectrum", "SimpleSpectrum", "Spectrum", "PolySpectrum"]
 
 
-class Spectrum(Base):
+class BaseSpectrum(Base):
+    """Abstract spectrum (spectra.py:12-25)."""
+
+    def normalise(self):  # pragma: no cover - abstract
+        raise NotImplementedError
+
+
+class SimpleSpectrum(BaseSpectrum):
+    """A spectrum defined on a 1-D array of wavelengths (spectra.py:28-53)."""
+
+    def __init__(self, wavelengths):
+        self.wavelengths = np.asarray(wavelengths, float)
+        if self.wavelengths.ndim != 1:
+            raise ValueError("wavelengths must be a 1d array.")
+
+
+class Spectrum(SimpleSpectrum):
     """Wavelengths (m) and relative weights, normalised to sum 1 (spectra.py:56-117)."""
 
     def __init__(self, wavelengths, weights=None):
-        self.wavelengths = np.asarray(wavelengths, float)
+        super().__init__(wavelengths)
         if weights is None:
             weights = np.ones(self.wavelengths.shape) / self.wavelengths.shape[-1]
         weights = np.asarray(weights, float)
@@ -30,3 +50,25 @@ class Spectrum(Base):
     def normalise(self) -> "Spectrum":
         s = self.weights.sum(-1)[:, None] if self.weights.ndim == 2 else self.weights.sum()
         return self.divide("weights", s)
+
+
+class PolySpectrum(SimpleSpectrum):
+    """Weights from the polynomial c0 + c1 x + ... + cn x^n at each
+    wavelength, normalised to sum 1 (spectra.py:120-178)."""
+
+    def __init__(self, wavelengths, coefficients):
+        super().__init__(wavelengths)
+        self.coefficients = np.asarray(coefficients, float)
+        if self.coefficients.ndim != 1:
+            raise ValueError("Coefficients must be a 1d array.")
+
+    def _eval_weight(self, wavelength):
+        return sum(c * wavelength ** i for i, c in enumerate(self.coefficients))
+
+    @property
+    def weights(self) -> np.ndarray:
+        w = np.array([self._eval_weight(x) for x in self.wavelengths], float)
+        return w / w.sum()
+
+    def normalise(self) -> "PolySpectrum":
+        return self

@@ -626,3 +626,19 @@ def test_detector_param_vjp_abi_validation_without_gpu(lib):
     assert st == 3 and b"workspace" in lib.dlx_last_error()
     st = lib.dlx_detector_vjp_params(16, 1, 4096, ops, 3, 4096, 4096, None, None, 0, None)
     assert st == 1 and b"null parameter-gradient list" in lib.dlx_last_error()
+
+
+def test_spectra_classes():
+    """spectra.py: BaseSpectrum / SimpleSpectrum / Spectrum / PolySpectrum."""
+    import dlux_amd as dl
+    wl = np.linspace(1e-6, 2e-6, 5)
+    ps = dl.PolySpectrum(wl, [1.0, 2e5])
+    ref = 1.0 + 2e5 * wl
+    assert np.allclose(ps.weights, ref / ref.sum()) and ps.normalise() is ps
+    with pytest.raises(ValueError, match="1d"):
+        dl.PolySpectrum(wl, np.ones((2, 2)))
+    with pytest.raises(ValueError, match="1d array"):
+        dl.Spectrum(np.ones((2, 2)))
+    assert isinstance(ps, dl.BaseSpectrum) and isinstance(dl.Spectrum(wl), dl.SimpleSpectrum)
+    src = dl.PointSource(spectrum=ps)
+    assert np.allclose(src.spectrum.weights, ps.weights)
