@@ -299,6 +299,23 @@ class Wavefront(Base):
                             center=cs[0])
         return self.set(phasor=outs[0], pixel_scale=np.asarray(pss[0], float), center=cs[0])
 
+    # placeholders, as in the reference (wavefronts.py:814-832)
+    def propagate_fresnel(self):
+        """LCT-based MFT Fresnel propagation (not implemented in the reference either)."""
+        raise NotImplementedError()
+
+    def propagate_fresnel_fft(self):
+        """LCT-based FFT Fresnel propagation (not implemented in the reference either)."""
+        raise NotImplementedError()
+
+    def propagate_fraunhofer(self):
+        """Fraunhofer propagation via MFT (not implemented in the reference either)."""
+        raise NotImplementedError()
+
+    def propagate_fraunhofer_fft(self):
+        """Fraunhofer propagation via FFT (not implemented in the reference either)."""
+        raise NotImplementedError()
+
     def propagate_ASM(self, distance: float, npad: int) -> "Wavefront":
         """Angular-spectrum propagation to `distance`, padded to npad (no crop)."""
         d = float(np.ravel(self.pixel_scale)[0])
