@@ -7,7 +7,7 @@ if the library is missing every compute entry point raises.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_size_t, c_void_p
+from ctypes import c_float, POINTER, c_char_p, c_double, c_int32, c_int64, c_size_t, c_void_p
 from pathlib import Path
 
 import numpy as np
@@ -183,6 +183,11 @@ SIGNATURES = {
     "dlx_soft_apertures_vjp": (c_int32, [c_int32, c_double, c_double, c_int32,
                                          POINTER(SoftAperture), _fp, _fp, c_void_p, c_size_t,
                                          c_void_p]),
+    "dlx_interp_transform": (c_int32, [c_int32, c_double, c_double, c_int32, c_double, c_double,
+                                       c_int32, POINTER(CoordTransformC), c_int32, c_float,
+                                       c_int32, _fp, _fp, c_void_p]),
+    "dlx_interp_points": (c_int32, [c_int32, c_double, c_double, c_int64, _fp, _fp, c_int32,
+                                    c_float, c_int32, _fp, _fp, c_void_p]),
     "dlx_aperture_basis": (c_int32, [c_int32, c_double, c_double, c_int32,
                                      POINTER(CoordTransformC), c_double, c_int32, c_int32,
                                      POINTER(c_int32), _fp, c_void_p]),

@@ -304,6 +304,80 @@ aperture_basis_kernel(ZernChunk z, int nmodes, int npix, double dx, double c, Tf
     out[(long)q * total + p] = (float)(scale * ap * zern_eval(z, q, rho, ct, st));
 }
 
+// Resampling through a coordinate transformation: output pixel (i, j) of an
+// n_out grid (spacing d_out, centre c_out) is sent through the transform chain
+// and sampled from each n_in x n_in input image on its knot grid (spacing
+// d_in, centre c_in): bilinear (method 1) or nearest knot (method 0); points
+// outside the knots' span take `fill` (interpax.interp2d with extrap = fill,
+// utils/interpolation.py:13-50).
+__global__ void __launch_bounds__(256)
+interp_kernel(TfChain ch, int n_out, double d_out, double c_out, int n_in, double d_in,
+              double c_in, int method, float fill, int batch, const float* __restrict__ in,
+              float* __restrict__ out) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long total = (long)n_out * n_out;
+  if (p >= total) return;
+  const int i = (int)(p / n_out), j = (int)(p - (long)i * n_out);
+  double x, y;
+  tf_point(ch, i, j, c_out, d_out, (n_out - 1) / 2.0, x, y);
+  const double h = (n_in - 1) / 2.0;
+  const double u = (x - c_in) / d_in + h, v = (y - c_in) / d_in + h;   // column, row
+  const bool inside = u >= 0.0 && u <= n_in - 1 && v >= 0.0 && v <= n_in - 1;
+  const long nn = (long)n_in * n_in;
+  if (!inside) {
+    for (int b = 0; b < batch; ++b) out[(long)b * total + p] = fill;
+    return;
+  }
+  if (method == 0) {
+    const int c = min((int)rint(u), n_in - 1), r = min((int)rint(v), n_in - 1);
+    for (int b = 0; b < batch; ++b) out[(long)b * total + p] = in[(long)b * nn + (long)r * n_in + c];
+    return;
+  }
+  const int c0 = min((int)floor(u), n_in > 1 ? n_in - 2 : 0);
+  const int r0 = min((int)floor(v), n_in > 1 ? n_in - 2 : 0);
+  const int c1 = min(c0 + 1, n_in - 1), r1 = min(r0 + 1, n_in - 1);
+  const double tu = u - c0, tv = v - r0;
+  for (int b = 0; b < batch; ++b) {
+    const float* f = in + (long)b * nn;
+    const double f00 = f[(long)r0 * n_in + c0], f01 = f[(long)r0 * n_in + c1];
+    const double f10 = f[(long)r1 * n_in + c0], f11 = f[(long)r1 * n_in + c1];
+    const double top = f00 + (f01 - f00) * tu, bot = f10 + (f11 - f10) * tu;
+    out[(long)b * total + p] = (float)(top + (bot - top) * tv);
+  }
+}
+
+// The same sampling at explicit points (qx, qy: npts float64 coordinates).
+__global__ void __launch_bounds__(256)
+interp_points_kernel(long npts, const double* __restrict__ qx, const double* __restrict__ qy,
+                     int n_in, double d_in, double c_in, int method, float fill, int batch,
+                     const float* __restrict__ in, float* __restrict__ out) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npts) return;
+  const double h = (n_in - 1) / 2.0;
+  const double u = (qx[p] - c_in) / d_in + h, v = (qy[p] - c_in) / d_in + h;
+  const bool inside = u >= 0.0 && u <= n_in - 1 && v >= 0.0 && v <= n_in - 1;
+  const long nn = (long)n_in * n_in;
+  for (int b = 0; b < batch; ++b) {
+    const float* f = in + (long)b * nn;
+    float r;
+    if (!inside) {
+      r = fill;
+    } else if (method == 0) {
+      r = f[(long)min((int)rint(v), n_in - 1) * n_in + min((int)rint(u), n_in - 1)];
+    } else {
+      const int c0 = min((int)floor(u), n_in > 1 ? n_in - 2 : 0);
+      const int r0 = min((int)floor(v), n_in > 1 ? n_in - 2 : 0);
+      const int c1 = min(c0 + 1, n_in - 1), r1 = min(r0 + 1, n_in - 1);
+      const double tu = u - c0, tv = v - r0;
+      const double f00 = f[(long)r0 * n_in + c0], f01 = f[(long)r0 * n_in + c1];
+      const double f10 = f[(long)r1 * n_in + c0], f11 = f[(long)r1 * n_in + c1];
+      const double top = f00 + (f01 - f00) * tu, bot = f10 + (f11 - f10) * tu;
+      r = (float)(top + (bot - top) * tv);
+    }
+    out[(long)b * npts + p] = r;
+  }
+}
+
 // ---- geometry gradients of the soft apertures ----------------------------
 // Parameters per aperture, in this order: size[0], size[1], then the last
 // transform of its chain: translation x, y, shear x, y, compression x, y,
@@ -882,5 +956,50 @@ extern "C" int dlx_aperture_basis(int32_t npix, double pixel_scale, double cente
                        out + (long)q0 * total);
     DLX_HIP(hipGetLastError());
   }
+  return DLX_OK;
+}
+
+extern "C" int dlx_interp_transform(int32_t n_in, double pixel_scale_in, double center_in,
+                                    int32_t n_out, double pixel_scale_out, double center_out,
+                                    int32_t n_transforms, const dlx_coord_transform* transforms,
+                                    int32_t method, float fill, int32_t batch, const float* in,
+                                    float* out, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(n_in > 0 && n_in <= 46340 && n_out > 0 && n_out <= 46340,
+                "n_in and n_out must be in [1, 46340], got %d and %d", n_in, n_out);
+  DLX_CHECK_ARG(batch >= 0 && batch <= 65535, "batch must be in [0, 65535], got %d", batch);
+  DLX_CHECK_ARG(method == 0 || method == 1, "method must be 0 (nearest) or 1 (linear), got %d",
+                method);
+  DLX_CHECK_ARG(std::isfinite(pixel_scale_in) && pixel_scale_in > 0 &&
+                    std::isfinite(pixel_scale_out) && pixel_scale_out > 0,
+                "pixel scales must be > 0");
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(in && out, "null argument to dlx_interp_transform");
+  TfChain ch{};
+  if (int st = build_chain(n_transforms, transforms, ch)) return st;
+  const long total = (long)n_out * n_out;
+  hipLaunchKernelGGL(interp_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, ch, n_out, pixel_scale_out, center_out, n_in,
+                     pixel_scale_in, center_in, method, fill, batch, in, out);
+  DLX_HIP(hipGetLastError());
+  return DLX_OK;
+}
+
+extern "C" int dlx_interp_points(int32_t n_in, double pixel_scale_in, double center_in,
+                                 int64_t npts, const double* qx, const double* qy,
+                                 int32_t method, float fill, int32_t batch, const float* in,
+                                 float* out, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(n_in > 0 && n_in <= 46340, "n_in must be in [1, 46340], got %d", n_in);
+  DLX_CHECK_ARG(npts >= 0 && batch >= 0 && batch <= 65535, "bad point count or batch");
+  DLX_CHECK_ARG(method == 0 || method == 1, "method must be 0 (nearest) or 1 (linear), got %d",
+                method);
+  DLX_CHECK_ARG(std::isfinite(pixel_scale_in) && pixel_scale_in > 0, "pixel scale must be > 0");
+  if (npts == 0 || batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(qx && qy && in && out, "null argument to dlx_interp_points");
+  hipLaunchKernelGGL(interp_points_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (long)npts, qx, qy, n_in, pixel_scale_in, center_in,
+                     method, fill, batch, in, out);
+  DLX_HIP(hipGetLastError());
   return DLX_OK;
 }

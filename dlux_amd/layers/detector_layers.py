@@ -16,8 +16,8 @@ from ..base import Base
 from ..psfs import PSF
 from ..utils.math import gaussian
 
-__all__ = ["DetectorLayer", "ApplyPixelResponse", "ApplyJitter", "ApplySaturation",
-           "AddConstant", "Downsample"]
+__all__ = ["DetectorLayer", "ApplyInterpolation", "ApplyPixelResponse", "ApplyJitter",
+           "ApplySaturation", "AddConstant", "Downsample"]
 
 
 class DetectorLayer(Base):
@@ -68,6 +68,22 @@ def _op_scalar(v):
     if isinstance(v, torch.Tensor):
         return v if v.requires_grad else float(v.detach().cpu())
     return float(v)
+
+
+class ApplyInterpolation(DetectorLayer):
+    """Resample the PSF through a coordinate transformation
+    (detector_layers.py:68-97); not fused into the detector chain."""
+
+    def __init__(self, transformation, method: str = "linear", fill: float = 0.0):
+        from ..coordinates import BaseCoordTransform
+        if not isinstance(transformation, BaseCoordTransform):
+            raise TypeError("transformation must be a BaseCoordTransform.")
+        self.transformation = transformation
+        self.method = str(method)
+        self.fill = np.asarray(fill, dtype=float)
+
+    def __call__(self, psf: PSF) -> PSF:
+        return psf.interpolate(self.transformation, method=self.method, fill=float(self.fill))
 
 
 class ApplyPixelResponse(DetectorLayer):

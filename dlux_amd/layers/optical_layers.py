@@ -17,8 +17,8 @@ from ..base import Base
 from ..wavefronts import Wavefront
 from ..utils.math import eval_basis as _eval_basis
 
-__all__ = ["OpticalLayer", "TransmissiveLayer", "AberratedLayer", "BasisLayer", "Tilt",
-           "Normalise"]
+__all__ = ["OpticalLayer", "Interpolate", "TransmissiveLayer", "AberratedLayer", "BasisLayer",
+           "Tilt", "Normalise"]
 
 
 def _as_float(value):
@@ -43,6 +43,25 @@ class OpticalLayer(Base):
     # Layers that cannot be folded into a fused plan return False.
     def _lower(self, plan) -> bool:
         return False
+
+
+class Interpolate(OpticalLayer):
+    """Resample the wavefront through a coordinate transformation
+    (optical_layers.py:108-141)."""
+
+    def __init__(self, transformation, method: str = "linear", complex: bool = True,
+                 fill: float = 0.0):
+        from ..coordinates import BaseCoordTransform
+        if not isinstance(transformation, BaseCoordTransform):
+            raise TypeError("transformation must be a BaseCoordTransform.")
+        self.transformation = transformation
+        self.method = str(method)
+        self.complex = bool(complex)
+        self.fill = np.asarray(fill, dtype=float)
+
+    def __call__(self, wavefront: Wavefront) -> Wavefront:
+        return wavefront.interpolate(self.transformation, method=self.method,
+                                     complex=self.complex, fill=float(self.fill))
 
 
 class TransmissiveLayer(OpticalLayer):

@@ -52,6 +52,22 @@ class PSF(Base):
         from . import runtime
         return self.set(data=runtime.detector(self.data, [("saturate", float(value))]))
 
+    def rotate(self, angle: float, method: str = "linear") -> "PSF":
+        """The PSF rotated by `angle` radians about its centre (psfs.py:112-128)."""
+        from .utils.interpolation import rotate
+        return self.set(data=rotate(self.data, angle, method))
+
+    def interpolate(self, transformation, method: str = "linear", fill: float = 0.0) -> "PSF":
+        """The PSF sampled at transformation(pixel_coords(n, n * pixel_scale))
+        (psfs.py:130-157)."""
+        from . import runtime
+        from .coordinates import BaseCoordTransform
+        if not isinstance(transformation, BaseCoordTransform):
+            raise TypeError("transformation must be a BaseCoordTransform.")
+        d = float(np.mean(self.pixel_scale))
+        return self.set(data=runtime.interp_transform(
+            self.data, d, 0.0, self.npixels, d, 0.0, [transformation._struct()], method, fill))
+
     def resize(self, npixels: int) -> "PSF":
         """psfs.py:159-173."""
         from .utils.array_ops import resize

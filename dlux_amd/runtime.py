@@ -648,6 +648,63 @@ def aperture_basis(npix: int, pixel_scale: float, js, transforms=(), extent: flo
 
 
 # ---------------------------------------------------------------------------
+# Resampling (dlx_interp_transform / dlx_interp_points)
+# ---------------------------------------------------------------------------
+
+_INTERP_METHODS = {"nearest": 0, "linear": 1}
+
+
+def _interp_method(method: str) -> int:
+    if method not in _INTERP_METHODS:
+        raise NotImplementedError(
+            f"interpolation method {method!r}: 'linear' and 'nearest' are implemented")
+    return _INTERP_METHODS[method]
+
+
+def interp_transform(images, pixel_scale_in: float, center_in: float, n_out: int,
+                     pixel_scale_out: float, center_out: float, transforms=(),
+                     method: str = "linear", fill: float = 0.0) -> torch.Tensor:
+    """Float32 images (..., n_in, n_in) resampled onto an n_out grid whose pixel
+    coordinates go through the ``_lib.CoordTransformC`` chain first."""
+    dev = device()
+    x = as_float(images, dev).contiguous()
+    if x.ndim < 2 or x.shape[-1] != x.shape[-2]:
+        raise ValueError(f"images must be (..., n, n), got {tuple(x.shape)}")
+    lead = tuple(x.shape[:-2])
+    batch = int(np.prod(lead)) if lead else 1
+    tfs = (_lib.CoordTransformC * max(len(transforms), 1))(*transforms)
+    out = torch.empty(lead + (int(n_out), int(n_out)), dtype=torch.float32, device=dev)
+    st = _lib.load().dlx_interp_transform(int(x.shape[-1]), float(pixel_scale_in),
+                                          float(center_in), int(n_out), float(pixel_scale_out),
+                                          float(center_out), len(transforms), tfs,
+                                          _interp_method(method), float(fill), batch,
+                                          x.data_ptr(), out.data_ptr(), stream_handle())
+    _lib.check(st, "dlx_interp_transform")
+    return out
+
+
+def interp_points(images, pixel_scale_in: float, center_in: float, xs, ys,
+                  method: str = "linear", fill: float = 0.0) -> torch.Tensor:
+    """Float32 images (..., n, n) sampled at the points (xs, ys) (any common
+    shape) -> (..., *xs.shape)."""
+    dev = device()
+    x = as_float(images, dev).contiguous()
+    lead = tuple(x.shape[:-2])
+    batch = int(np.prod(lead)) if lead else 1
+    qx = torch.as_tensor(xs, device=dev).to(torch.float64).contiguous()
+    qy = torch.as_tensor(ys, device=dev).to(torch.float64).contiguous()
+    if qx.shape != qy.shape:
+        raise ValueError("xs and ys must have the same shape")
+    out = torch.empty(lead + tuple(qx.shape), dtype=torch.float32, device=dev)
+    st = _lib.load().dlx_interp_points(int(x.shape[-1]), float(pixel_scale_in),
+                                       float(center_in), qx.numel(), qx.data_ptr(),
+                                       qy.data_ptr(), _interp_method(method), float(fill),
+                                       batch, x.data_ptr(), out.data_ptr(), stream_handle())
+    _lib.check(st, "dlx_interp_points")
+    return out
+
+
+# ---------------------------------------------------------------------------
 # Detector chain (dlx_detector_apply / dlx_detector_vjp)
 # ---------------------------------------------------------------------------
 

@@ -299,6 +299,45 @@ class Wavefront(Base):
                             center=cs[0])
         return self.set(phasor=outs[0], pixel_scale=np.asarray(pss[0], float), center=cs[0])
 
+    # --- resampling (wavefronts.py:442-566) ---------------------------------
+    def _fields(self, complex: bool):
+        """(2, ...) float planes: real and imaginary, or amplitude and phase."""
+        ph = self.phasor
+        if complex:
+            return torch.stack([ph.real, ph.imag]).float()
+        return torch.stack([torch.abs(ph), torch.angle(ph)]).float()
+
+    def _from_fields(self, f, complex: bool):
+        if complex:
+            return torch.complex(f[0], f[1])
+        return f[0] * torch.exp(1j * f[1].to(torch.float32))
+
+    def scale_to(self, npixels: int, pixel_scale, complex: bool = True) -> "Wavefront":
+        """Resampled to npixels at pixel_scale (wavefronts.py:442-483)."""
+        from .utils.interpolation import scale
+        pixel_scale = np.asarray(pixel_scale, float)
+        ratio = float(np.ravel(pixel_scale)[0]) / float(np.ravel(self.pixel_scale)[0])
+        f = scale(self._fields(complex), int(npixels), ratio)
+        return self.set(phasor=self._from_fields(f, complex), pixel_scale=pixel_scale)
+
+    def interpolate(self, transformation, method: str = "linear", complex: bool = True,
+                    fill: float = 0.0) -> "Wavefront":
+        """Sampled at transformation(self.coordinates()) (wavefronts.py:485-524)."""
+        from .coordinates import BaseCoordTransform
+        if not isinstance(transformation, BaseCoordTransform):
+            raise TypeError("transformation must be a BaseCoordTransform.")
+        d = float(np.ravel(self.pixel_scale)[0])
+        c = float(np.ravel(self.center)[0])
+        f = runtime.interp_transform(self._fields(complex), d, c, self.npixels, d, c,
+                                     [transformation._struct()], method, fill)
+        return self.set(phasor=self._from_fields(f, complex))
+
+    def rotate(self, angle, method: str = "linear", complex: bool = True) -> "Wavefront":
+        """Rotated by `angle` radians about the grid centre (wavefronts.py:526-566)."""
+        from .utils.interpolation import rotate
+        f = rotate(self._fields(complex), angle, method)
+        return self.set(phasor=self._from_fields(f, complex))
+
     # placeholders, as in the reference (wavefronts.py:814-832)
     def propagate_fresnel(self):
         """LCT-based MFT Fresnel propagation (not implemented in the reference either)."""

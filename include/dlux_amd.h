@@ -361,6 +361,33 @@ int dlx_soft_apertures_vjp(int32_t npix, double pixel_scale, double center, int3
                            void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * Resampling of `batch` float32 images (n_in x n_in, knots at center_in +
+ * (k - (n_in-1)/2) pixel_scale_in) onto an n_out x n_out grid whose pixel
+ * coordinates (center_out + (k - (n_out-1)/2) pixel_scale_out) are first sent
+ * through the transform chain: method 1 bilinear, 0 the nearest knot; points
+ * outside the knots' span take `fill`.  Replaces interpax.interp2d as called by
+ * dlu.interp / scale / rotate (utils/interpolation.py:13-108), hence
+ * Wavefront.interpolate / scale_to / rotate (wavefronts.py:442-566),
+ * PSF.rotate / interpolate (psfs.py:112-157), Interpolate and
+ * ApplyInterpolation (layers/optical_layers.py:108-141,
+ * layers/detector_layers.py:68-97).  interpax is not available, so this is the
+ * textbook bilinear rule.
+ */
+int dlx_interp_transform(int32_t n_in, double pixel_scale_in, double center_in, int32_t n_out,
+                         double pixel_scale_out, double center_out, int32_t n_transforms,
+                         const dlx_coord_transform* transforms, int32_t method, float fill,
+                         int32_t batch, const float* in, float* out, void* stream);
+
+/*
+ * The same sampling at npts explicit points (qx, qy: device float64) of each
+ * image: out[b][p].  Replaces interpax.interp2d as called by dlu.interp with
+ * arbitrary sample coordinates (utils/interpolation.py:13-50).
+ */
+int dlx_interp_points(int32_t n_in, double pixel_scale_in, double center_in, int64_t npts,
+                      const double* qx, const double* qy, int32_t method, float fill,
+                      int32_t batch, const float* in, float* out, void* stream);
+
+/*
  * Zernike basis of an aberrated dynamic aperture (AberratedAperture.calc_basis,
  * layers/apertures.py:735-752; DynamicZernikeBasis, polynomials.py:85-115): the
  * pixel coordinates put through the transforms, divided by `extent`, then

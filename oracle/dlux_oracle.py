@@ -28,6 +28,8 @@ Parity status per function (all pinned unless marked):
       -- its conventions live in abcdLux (absent); checked by textbook
          invariants (z=0 identity, unitarity, z1+z2 composition) and the
          paraxial Gaussian-beam closed form.
+  * interp_grid (bilinear / nearest resampling)                 pinned (scipy)
+      -- parity with interpax.interp2d (absent) unpinned beyond those rules.
   * abcd_* / compose_abcd / lct / lct_fft (Collins integral)      parity unpinned
       -- against abcdLux (absent); pinned by closed forms: a tilted Gaussian
          beam (the ABCD law), the conjugate plane as the MFT, the FFT form as
@@ -856,3 +858,31 @@ def polike(nsides, j, coordinates, diameter=2.0):
     u = phi - wedge * (2 * alpha)
     r_alpha = np.cos(alpha) / np.cos(u)
     return 1 / r_alpha * zernike(j, coords / r_alpha)
+
+
+# ---------------------------------------------------------------------------
+# Resampling  (utils/interpolation.py:13-108; interpax.interp2d is absent:
+# restated as the textbook bilinear / nearest rules, pinned against scipy)
+# ---------------------------------------------------------------------------
+
+
+def interp_grid(image, n, d, c, xs, ys, method="linear", fill=0.0):
+    """`image` (n x n, knots at c + (k - (n-1)/2) d on both axes) sampled at
+    (xs, ys); fill outside the knots' span."""
+    image = np.asarray(image, np.float64)
+    h = (n - 1) / 2.0
+    u = (np.asarray(xs, np.float64) - c) / d + h
+    v = (np.asarray(ys, np.float64) - c) / d + h
+    inside = (u >= 0) & (u <= n - 1) & (v >= 0) & (v <= n - 1)
+    uc, vc = np.clip(u, 0, n - 1), np.clip(v, 0, n - 1)
+    if method == "nearest":
+        out = image[np.rint(vc).astype(int), np.rint(uc).astype(int)]
+    else:
+        c0 = np.minimum(np.floor(uc).astype(int), max(n - 2, 0))
+        r0 = np.minimum(np.floor(vc).astype(int), max(n - 2, 0))
+        c1, r1 = np.minimum(c0 + 1, n - 1), np.minimum(r0 + 1, n - 1)
+        tu, tv = uc - c0, vc - r0
+        top = image[r0, c0] + (image[r0, c1] - image[r0, c0]) * tu
+        bot = image[r1, c0] + (image[r1, c1] - image[r1, c0]) * tu
+        out = top + (bot - top) * tv
+    return np.where(inside, out, fill)

@@ -605,3 +605,28 @@ def test_asm_gaussian_beam_paraxial_closed_form(z):
     h = _gaussian_lct_closed_form(orc.spec_xs(npad, dx), wl, q1, kappa, orc.abcd_free_space(z))
     ref = np.exp(2j * np.pi * z / wl) * h[:, None] * h[None, :]
     assert np.max(np.abs(out - ref)) / np.max(np.abs(ref)) < 1e-6
+
+
+def test_interp_grid_against_scipy_and_linear_functions():
+    """The bilinear / nearest restatement equals scipy's RegularGridInterpolator
+    (fill outside the knots) and reproduces a linear function exactly."""
+    from scipy.interpolate import RegularGridInterpolator
+    rng = np.random.default_rng(0)
+    n, d, c = 17, 0.3, 0.1
+    knots = orc.spec_xs(n, d, c)
+    img = rng.normal(size=(n, n))
+    xs, ys = rng.uniform(-3.5, 3.5, 500), rng.uniform(-3.5, 3.5, 500)
+    for method in ("linear", "nearest"):
+        ref = RegularGridInterpolator((knots, knots), img, method=method, bounds_error=False,
+                                      fill_value=-7.0)(np.stack([ys, xs], -1))
+        got = orc.interp_grid(img, n, d, c, xs, ys, method, fill=-7.0)
+        inside = np.abs(np.stack([xs, ys]) - c).max(0) < (n - 1) / 2 * d - 1e-9
+        if method == "nearest":   # ties (exactly half-way) may round either way; none here
+            assert np.allclose(got[inside], ref[inside])
+        else:
+            assert np.allclose(got, ref, atol=1e-12)
+    X, Y = np.meshgrid(knots, knots)
+    lin = 2.0 * X - 3.0 * Y + 0.5
+    got = orc.interp_grid(lin, n, d, c, xs, ys, "linear", fill=0.0)
+    inside = np.abs(np.stack([xs, ys]) - c).max(0) <= (n - 1) / 2 * d
+    assert np.allclose(got[inside], (2.0 * xs - 3.0 * ys + 0.5)[inside], atol=1e-12)
