@@ -14,7 +14,8 @@ from .parametric import Parametric, ParametricBasis, ExplicitBasis, ImplicitBasi
 from .polynomials import DynamicZernike, ZernikeBasis, DynamicZernikeBasis
 from .runtime import set_precision, get_precision, precision
 from .base import Base
-from .coordinates import Spec, PadSpec, CoordSpec, BaseCoordTransform, CoordTransform
+from .coordinates import (Spec, PadSpec, CoordSpec, BaseCoordTransform, CoordTransform,
+                          DistortedCoords)
 from .psfs import PSF
 from .wavefronts import Wavefront
 from . import layers

@@ -126,6 +126,7 @@ class SoftAperture(ctypes.Structure):
         ("softening", c_double),
         ("angles_deg", c_double * 16),
         ("transforms", CoordTransformC * 3),
+        ("coords", c_void_p),
     ]
 
 
@@ -189,8 +190,8 @@ SIGNATURES = {
     "dlx_interp_points": (c_int32, [c_int32, c_double, c_double, c_int64, _fp, _fp, c_int32,
                                     c_float, c_int32, _fp, _fp, c_void_p]),
     "dlx_aperture_basis": (c_int32, [c_int32, c_double, c_double, c_int32,
-                                     POINTER(CoordTransformC), c_double, c_int32, c_int32,
-                                     POINTER(c_int32), _fp, c_void_p]),
+                                     POINTER(CoordTransformC), c_void_p, c_double, c_int32,
+                                     c_int32, POINTER(c_int32), _fp, c_void_p]),
     "dlx_mft_batched_workspace_size": (c_size_t, [c_int32, POINTER(MftInstance), c_int32]),
     "dlx_mft_batched_psf": (c_int32, [c_int32, POINTER(MftInstance), c_int32,
                                       POINTER(c_void_p), c_void_p, c_size_t, c_void_p]),

@@ -6,7 +6,8 @@ import numpy as np
 
 from .base import Base
 
-__all__ = ["Spec", "PadSpec", "CoordSpec", "BaseCoordTransform", "CoordTransform"]
+__all__ = ["Spec", "PadSpec", "CoordSpec", "BaseCoordTransform", "CoordTransform",
+           "DistortedCoords"]
 
 
 class Spec(Base):
@@ -139,3 +140,66 @@ def _floats(v) -> tuple:
     if _is_tensor(v):
         return tuple(v.detach().double().cpu().reshape(-1).tolist())
     return tuple(float(x) for x in np.atleast_1d(np.asarray(v, float)).reshape(-1))
+
+
+class DistortedCoords(BaseCoordTransform):
+    """A 2-D polynomial distortion coords + sum_m c[:, m] x^px_m y^py_m over the
+    monomials of gen_powers(order + 1) without the constant
+    (coordinates.py:318-357).  On the device its coordinates are evaluated as
+    FP64 planes that the aperture and resampling kernels read in place of a
+    transform chain."""
+
+    def __init__(self, order: int = 1, distortion=None):
+        from .utils.coordinates import gen_powers
+        self.powers = np.array(gen_powers(int(order) + 1))[:, 1:]
+        if distortion is None:
+            distortion = np.zeros(self.powers.shape)
+        if not _is_tensor(distortion):
+            distortion = np.asarray(distortion, dtype=float)
+        if tuple(distortion.shape) != self.powers.shape:
+            raise ValueError("distortion shape must match powers shape.")
+        self.distortion = distortion
+
+    def __call__(self, coords):
+        from .utils.coordinates import distort_coords
+        return distort_coords(coords, self.distortion, self.powers)
+
+    def _key(self):
+        return ("distorted", tuple(self.powers.ravel().tolist()), _floats(self.distortion))
+
+
+def _chain_coords(tfs, npix: int, pixel_scale: float, centre: float, dev):
+    """The (2, npix, npix) float64 device planes of the pixel grid
+    centre + (k - (npix-1)/2) pixel_scale put through the transforms in order
+    (CoordTransform: translation, shear of the transposed planes, compression,
+    rotation; DistortedCoords: the polynomial)."""
+    import torch
+    xs = centre + (torch.arange(npix, dtype=torch.float64, device=dev) - (npix - 1) / 2.0) \
+        * pixel_scale
+    gy, gx = torch.meshgrid(xs, xs, indexing="ij")
+    c = torch.stack([gx, gy])
+    for t in tfs:
+        if isinstance(t, CoordTransform):
+            if t.translation is not None:
+                c = c - torch.tensor(_floats(t.translation), dtype=torch.float64,
+                                     device=dev)[:, None, None]
+            if t.shear is not None:
+                sh = torch.tensor(_floats(t.shear), dtype=torch.float64, device=dev)
+                c = c + c.transpose(1, 2) * sh[:, None, None]
+            if t.compression is not None:
+                c = c * torch.tensor(_floats(t.compression), dtype=torch.float64,
+                                     device=dev)[:, None, None]
+            if t.rotation is not None:
+                r = _floats(t.rotation)[0]
+                co, si = np.cos(-r), np.sin(-r)
+                c = torch.stack([co * c[0] + si * c[1], -si * c[0] + co * c[1]])
+        elif isinstance(t, DistortedCoords):
+            d = torch.tensor(np.asarray(_floats(t.distortion)).reshape(t.powers.shape),
+                             dtype=torch.float64, device=dev)
+            px = torch.as_tensor(t.powers[0], device=dev)
+            py = torch.as_tensor(t.powers[1], device=dev)
+            basis = c[0][None] ** px[:, None, None] * c[1][None] ** py[:, None, None]
+            c = c + torch.tensordot(d, basis, dims=([1], [0]))
+        else:
+            raise TypeError(f"unsupported coordinate transformation {type(t).__name__}")
+    return c.contiguous()

@@ -160,6 +160,7 @@ __device__ __forceinline__ void tf_point(const TfChain& ch, int i, int j, double
 
 struct ApDev {
   int kind, occulting, nedges, nang;
+  const double* xy;   // explicit coordinate planes (x then y), or NULL: the chain
   double s0, s1, clip;
   double ca[16], sa[16];                             // spider arm rotations
   double ea[kMaxApEdges], eb[kMaxApEdges], ec[kMaxApEdges];   // polygon edges a x + b y + c
@@ -179,7 +180,12 @@ soft_distance_kernel(ApDev ap, int npix, double dx, double c, double* __restrict
   if (in) {
     const int i = (int)(p / npix), j = (int)(p - (long)i * npix);
     double x, y;
-    tf_point(ap.ch, i, j, c, dx, (npix - 1) / 2.0, x, y);
+    if (ap.xy) {
+      x = ap.xy[p];
+      y = ap.xy[total + p];
+    } else {
+      tf_point(ap.ch, i, j, c, dx, (npix - 1) / 2.0, x, y);
+    }
     double dist_edge;   // the reference's *_distance: negative inside
     if (ap.kind == DLX_AP_CIRCLE) {
       dist_edge = hypot(x, y) - ap.s0;
@@ -277,13 +283,19 @@ soft_combine_kernel(ApGroups g, long total, const double* __restrict__ dist,
 // divided by the aperture extent (utils/zernikes.py:257-294, :352-397).
 __global__ void __launch_bounds__(256)
 aperture_basis_kernel(ZernChunk z, int nmodes, int npix, double dx, double c, TfChain ch,
-                      double inv_extent, int nsides, float* __restrict__ out) {
+                      const double* __restrict__ xy, double inv_extent, int nsides,
+                      float* __restrict__ out) {
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   const long total = (long)npix * npix;
   if (p >= total) return;
   const int i = (int)(p / npix), j = (int)(p - (long)i * npix);
   double x, y;
-  tf_point(ch, i, j, c, dx, (npix - 1) / 2.0, x, y);
+  if (xy) {
+    x = xy[p];
+    y = xy[total + p];
+  } else {
+    tf_point(ch, i, j, c, dx, (npix - 1) / 2.0, x, y);
+  }
   x *= inv_extent;
   y *= inv_extent;
   double scale = 1.0;
@@ -411,7 +423,11 @@ __device__ void soft_point_grad(const ApDev& ap, int i, int j, int npix, double 
   // Jacobians of (x, y) with respect to the 9 parameters (sizes: 0)
   double jx[kGeoParams] = {0}, jy[kGeoParams] = {0};
   double jqx[kGeoParams] = {0}, jqy[kGeoParams] = {0};
-  if (own >= 0) {
+  if (ap.xy) {   // explicit coordinates: no transform slots
+    const long total = (long)npix * npix, p = (long)i * npix + j;
+    px = ap.xy[p];
+    py = ap.xy[total + p];
+  } else if (own >= 0) {
     const TfDev& t = ap.ch.t[own];
     if (t.flags & DLX_TF_TRANSLATION) {
       px -= t.tx; py -= t.ty; qx -= t.tx; qy -= t.ty;
@@ -783,6 +799,7 @@ int build_apertures(int npix, double pixel_scale, int n, const dlx_soft_aperture
     d.occulting = s.occulting != 0;
     d.s0 = s.size[0];
     d.s1 = s.size[1];
+    d.xy = s.coords;
     DLX_CHECK_ARG(s.softening > 0, "softening must be greater than 0.");
     d.clip = pixel_scale * s.softening / 2.0;
     int st = build_chain(s.n_transforms, s.transforms, d.ch);
@@ -932,8 +949,9 @@ extern "C" int dlx_soft_apertures_vjp(int32_t npix, double pixel_scale, double c
 
 extern "C" int dlx_aperture_basis(int32_t npix, double pixel_scale, double center,
                                   int32_t n_transforms, const dlx_coord_transform* transforms,
-                                  double extent, int32_t nsides, int32_t nmodes,
-                                  const int32_t* js, float* out, void* stream) {
+                                  const double* coords, double extent, int32_t nsides,
+                                  int32_t nmodes, const int32_t* js, float* out,
+                                  void* stream) {
   clear_error();
   DLX_CHECK_ARG(npix > 0 && npix <= 46340, "npix must be in [1, 46340], got %d", npix);
   DLX_CHECK_ARG(nmodes >= 0, "negative mode count");
@@ -952,7 +970,7 @@ extern "C" int dlx_aperture_basis(int32_t npix, double pixel_scale, double cente
     st = zern_chunk(js + q0, nq, z);
     if (st) return st;
     hipLaunchKernelGGL(aperture_basis_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       s, z, nq, npix, pixel_scale, center, ch, 1.0 / extent, nsides,
+                       s, z, nq, npix, pixel_scale, center, ch, coords, 1.0 / extent, nsides,
                        out + (long)q0 * total);
     DLX_HIP(hipGetLastError());
   }

@@ -27,7 +27,8 @@ import torch
 
 from .. import _lib, runtime
 from ..polynomials import DynamicZernikeBasis
-from ..coordinates import BaseCoordTransform, CoordTransform, _floats, _is_tensor
+from ..coordinates import (BaseCoordTransform, CoordTransform, DistortedCoords, _chain_coords,
+                           _floats, _is_tensor)
 from .optical_layers import OpticalLayer
 
 __all__ = ["ApertureLayer", "BaseDynamicAperture", "DynamicAperture", "CircularAperture",
@@ -99,15 +100,32 @@ class ApertureLayer(OpticalLayer):
     def _grid_transmission(self, npix: int, pixel_scale: float, centre: float = 0.0):
         items = self._descs(())
         params, smap = _grad_sources(items)
+        keep = _explicit_coords(items, npix, pixel_scale, centre)
         if params and torch.is_grad_enabled():
-            return runtime.soft_apertures_tracked(npix, pixel_scale, _grouped(items), centre,
-                                                  params, smap)
-        key = ("t", npix, float(pixel_scale), float(centre), self._param_key())
-        return self._cache(key, lambda: runtime.soft_apertures(
-            npix, pixel_scale, _grouped(items), centre))
+            out = runtime.soft_apertures_tracked(npix, pixel_scale, _grouped(items), centre,
+                                                 params, smap, keep)
+        else:
+            key = ("t", npix, float(pixel_scale), float(centre), self._param_key())
+            out = self._cache(key, lambda: runtime.soft_apertures(
+                npix, pixel_scale, _grouped(items), centre))
+        del keep
+        return out
 
     def _param_key(self):
         raise NotImplementedError
+
+
+def _explicit_coords(items, npix, pixel_scale, centre):
+    """Descriptors whose chain holds a DistortedCoords get their coordinates as
+    FP64 device planes (returned, to keep them alive for the call)."""
+    keep = []
+    for _, d, _ in items:
+        chain = getattr(d, "_chain", None)
+        if chain is not None:
+            c = _chain_coords(chain, npix, float(pixel_scale), float(centre), runtime.device())
+            keep.append(c)
+            d.coords = c.data_ptr()
+    return keep
 
 
 def _grouped(descs):
@@ -186,11 +204,15 @@ class DynamicAperture(BaseDynamicAperture):
         d.occulting = int(self.occulting)
         d.softening = _floats(self.softness)[0]
         tfs = self._transforms(transforms)
-        if len(tfs) > 3:
-            raise NotImplementedError("at most three nested coordinate transformations")
-        d.n_transforms = len(tfs)
-        for k, t in enumerate(tfs):
-            d.transforms[k] = t._struct()
+        if all(isinstance(t, CoordTransform) for t in tfs):
+            if len(tfs) > 3:
+                raise NotImplementedError("at most three nested coordinate transformations")
+            d.n_transforms = len(tfs)
+            for k, t in enumerate(tfs):
+                d.transforms[k] = t._struct()
+        else:   # the kernels read the coordinates as planes (see _explicit_coords)
+            d.n_transforms = 0
+            d._chain = tfs
         self._fill(d)
         return [(id(self) if group is None else group, d, self._sources(tfs))]
 
@@ -210,6 +232,13 @@ class DynamicAperture(BaseDynamicAperture):
                         "gradients with respect to a transformation are implemented for the "
                         "innermost transformation that applies to an aperture only")
         src = [(slot, v, None) for slot, v in self._sizes() if _tracked(v)]
+        if any(isinstance(t, DistortedCoords) for t in tfs):
+            if torch.is_grad_enabled() and any(
+                    _tracked(getattr(t, f, None)) for t in tfs
+                    for f in ("distortion", "translation", "shear", "compression", "rotation")):
+                raise NotImplementedError("gradients with respect to a transformation chain "
+                                          "holding a DistortedCoords are not implemented")
+            return src
         if tfs and isinstance(tfs[-1], CoordTransform):
             for slot, f, elem in CoordTransform._SLOTS:
                 v = getattr(tfs[-1], f)
@@ -441,17 +470,21 @@ class AberratedAperture(ApertureLayer):
                        for _, f, _ in CoordTransform._SLOTS)):
             raise NotImplementedError("gradients with respect to the geometry of an "
                                       "aberrated aperture's basis are not implemented")
-        tfs = tuple(t._struct() for t in outer) + (
-            (self.aperture.transformation._struct(),)
-            if self.aperture.transformation is not None else ())
+        chain = tuple(outer) + ((self.aperture.transformation,)
+                                if self.aperture.transformation is not None else ())
+        explicit = not all(isinstance(t, CoordTransform) for t in chain)
+        tfs = () if explicit else tuple(t._struct() for t in chain)
         key = ("b", npix, float(pixel_scale), float(centre), self.aperture._param_key(),
-               tuple(self.noll_inds), tuple(bytes(t) for t in tfs))
+               tuple(self.noll_inds), tuple(bytes(t) for t in tfs),
+               tuple(t._key() for t in chain) if explicit else None)
         cached = self.__dict__.get("_basis")
         if cached is not None and cached[0] == key:
             return cached[1]
+        coords = _chain_coords(chain, npix, float(pixel_scale), float(centre),
+                               runtime.device()) if explicit else None
         basis = runtime.aperture_basis(npix, pixel_scale, self.noll_inds, tfs,
                                        _floats(self.aperture.extent)[0], self.aperture.nsides,
-                                       centre)
+                                       centre, coords=coords)
         self.__dict__["_basis"] = (key, basis)
         return basis
 

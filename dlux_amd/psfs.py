@@ -64,9 +64,14 @@ class PSF(Base):
         from .coordinates import BaseCoordTransform
         if not isinstance(transformation, BaseCoordTransform):
             raise TypeError("transformation must be a BaseCoordTransform.")
+        from .coordinates import CoordTransform, _chain_coords
         d = float(np.mean(self.pixel_scale))
-        return self.set(data=runtime.interp_transform(
-            self.data, d, 0.0, self.npixels, d, 0.0, [transformation._struct()], method, fill))
+        if isinstance(transformation, CoordTransform):
+            return self.set(data=runtime.interp_transform(
+                self.data, d, 0.0, self.npixels, d, 0.0, [transformation._struct()], method,
+                fill))
+        q = _chain_coords((transformation,), self.npixels, d, 0.0, runtime.device())
+        return self.set(data=runtime.interp_points(self.data, d, 0.0, q[0], q[1], method, fill))
 
     def resize(self, npixels: int) -> "PSF":
         """psfs.py:159-173."""

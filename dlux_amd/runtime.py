@@ -600,14 +600,16 @@ class _SoftAperturesFn(torch.autograd.Function):
     tensors; the backward is dlx_soft_apertures_vjp."""
 
     @staticmethod
-    def forward(ctx, npix, pixel_scale, center, apertures, smap, *params):
-        ctx.meta = (npix, pixel_scale, center, apertures, smap)
+    def forward(ctx, npix, pixel_scale, center, apertures, smap, keep, *params):
+        # keep: device tensors the descriptors point into (coordinate planes),
+        # held until the backward has run
+        ctx.meta = (npix, pixel_scale, center, apertures, smap, keep)
         ctx.shapes = [(p.shape, p.dtype, p.device) for p in params]
         return soft_apertures(npix, pixel_scale, apertures, center)
 
     @staticmethod
     def backward(ctx, gout):
-        npix, pixel_scale, center, apertures, smap = ctx.meta
+        npix, pixel_scale, center, apertures, smap, _ = ctx.meta
         g = soft_apertures_vjp(npix, pixel_scale, apertures, gout, center).double().cpu()
         grads = [None] * len(ctx.shapes)
         for a, slot, i, elem in smap:
@@ -620,29 +622,34 @@ class _SoftAperturesFn(torch.autograd.Function):
             grads[i] = part if grads[i] is None else grads[i] + part
         grads = [None if q is None else q.to(device=ctx.shapes[i][2], dtype=ctx.shapes[i][1])
                  for i, q in enumerate(grads)]
-        return (None, None, None, None, None) + tuple(grads)
+        return (None, None, None, None, None, None) + tuple(grads)
 
 
-def soft_apertures_tracked(npix: int, pixel_scale: float, apertures, center, params, smap):
+def soft_apertures_tracked(npix: int, pixel_scale: float, apertures, center, params, smap,
+                           keep=()):
     """soft_apertures whose result carries gradients to the geometry tensors
     ``params``; ``smap`` lists (aperture, parameter slot, tensor index,
-    element or None) for each slot that feeds a tensor."""
+    element or None) for each slot that feeds a tensor; ``keep`` holds the
+    device buffers the descriptors' ``coords`` point to until the backward."""
     return _SoftAperturesFn.apply(int(npix), float(pixel_scale), float(center),
-                                  list(apertures), list(smap), *params)
+                                  list(apertures), list(smap), tuple(keep), *params)
 
 
 def aperture_basis(npix: int, pixel_scale: float, js, transforms=(), extent: float = 1.0,
-                   nsides: int = 0, center: float = 0.0) -> torch.Tensor:
+                   nsides: int = 0, center: float = 0.0, coords=None) -> torch.Tensor:
     """(len(js), npix, npix) Zernike / polike basis on transformed coordinates
-    divided by `extent` (dlx_aperture_basis)."""
+    divided by `extent` (dlx_aperture_basis); `coords` (2, npix, npix) float64
+    device planes replace the transforms when given."""
     js = [int(j) for j in np.atleast_1d(np.asarray(js))]
     dev = device()
     tfs = (_lib.CoordTransformC * max(len(transforms), 1))(*transforms)
     jarr = (ctypes.c_int32 * max(len(js), 1))(*js)
     out = torch.empty((len(js), npix, npix), dtype=torch.float32, device=dev)
     st = _lib.load().dlx_aperture_basis(int(npix), float(pixel_scale), float(center),
-                                        len(transforms), tfs, float(extent), int(nsides),
-                                        len(js), jarr, out.data_ptr(), stream_handle())
+                                        len(transforms), tfs,
+                                        None if coords is None else coords.data_ptr(),
+                                        float(extent), int(nsides), len(js), jarr,
+                                        out.data_ptr(), stream_handle())
     _lib.check(st, "dlx_aperture_basis")
     return out
 

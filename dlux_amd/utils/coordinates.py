@@ -97,3 +97,34 @@ def rotate_coords(coords: torch.Tensor, rotation: float) -> torch.Tensor:
     x, y = coords
     c, s = float(np.cos(-rotation)), float(np.sin(-rotation))
     return torch.stack([c * x + s * y, -s * x + c * y])
+
+
+def gen_powers(degree: int):
+    """(x powers, y powers) of the 2-D monomials up to total degree - 1, in the
+    reference's order (utils/coordinates.py:104-133)."""
+    n = int(degree) * (int(degree) + 1) // 2
+    vals = np.arange(n)
+    tri = lambda k: k * (k + 1) // 2
+    ydiffs = np.repeat(tri(np.arange(degree)), np.arange(1, degree + 1))
+    ypows = vals - ydiffs
+    xdiffs = np.repeat(n - np.flip(tri(np.arange(1, degree + 1))), np.arange(degree, 0, -1))
+    xpows = np.flip(vals - xdiffs)
+    return np.array([xpows, ypows])
+
+
+def polynomial_basis(coords, pows):
+    """x^px y^py for each column of pows, over coords (2, ...)
+    (utils/coordinates.py:159-163)."""
+    pows = np.asarray(pows)
+    c = coords if isinstance(coords, torch.Tensor) else torch.as_tensor(np.asarray(coords))
+    p = torch.as_tensor(pows, device=c.device).reshape(pows.shape + (1,) * (c.ndim - 1))
+    return torch.prod(c[:, None] ** p, dim=0)
+
+
+def distort_coords(coords, coeffs, pows):
+    """coords + tensordot(coeffs, polynomial_basis(coords, pows))
+    (utils/coordinates.py:136-156)."""
+    c = coords if isinstance(coords, torch.Tensor) else torch.as_tensor(np.asarray(coords))
+    k = coeffs if isinstance(coeffs, torch.Tensor) else torch.as_tensor(np.asarray(coeffs))
+    k = k.to(device=c.device, dtype=c.dtype)
+    return c + torch.tensordot(k, polynomial_basis(c, pows).to(c.dtype), dims=([1], [0]))

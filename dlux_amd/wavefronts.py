@@ -326,10 +326,15 @@ class Wavefront(Base):
         from .coordinates import BaseCoordTransform
         if not isinstance(transformation, BaseCoordTransform):
             raise TypeError("transformation must be a BaseCoordTransform.")
+        from .coordinates import CoordTransform, _chain_coords
         d = float(np.ravel(self.pixel_scale)[0])
         c = float(np.ravel(self.center)[0])
-        f = runtime.interp_transform(self._fields(complex), d, c, self.npixels, d, c,
-                                     [transformation._struct()], method, fill)
+        if isinstance(transformation, CoordTransform):
+            f = runtime.interp_transform(self._fields(complex), d, c, self.npixels, d, c,
+                                         [transformation._struct()], method, fill)
+        else:
+            q = _chain_coords((transformation,), self.npixels, d, c, self.phasor.device)
+            f = runtime.interp_points(self._fields(complex), d, c, q[0], q[1], method, fill)
         return self.set(phasor=self._from_fields(f, complex))
 
     def rotate(self, angle, method: str = "linear", complex: bool = True) -> "Wavefront":

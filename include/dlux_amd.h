@@ -329,6 +329,9 @@ typedef struct dlx_soft_aperture {
   double softening;           /* pixels, > 0                                    */
   double angles_deg[16];
   dlx_coord_transform transforms[3];
+  const double* coords;       /* NULL, or device float64 planes x then y (2 x npix x npix)
+                                 of the transformed pixel coordinates, used instead of the
+                                 transform chain (a DistortedCoords, coordinates.py:318-357) */
 } dlx_soft_aperture;
 
 /*
@@ -393,10 +396,13 @@ int dlx_interp_points(int32_t n_in, double pixel_scale_in, double center_in, int
  * pixel coordinates put through the transforms, divided by `extent`, then
  * Z_j (nsides == 0, utils/zernikes.py:257-294) or the n-sided polike
  * (nsides >= 3, :352-397) on the unit disk.  out: nmodes x npix x npix float32.
+ * coords (may be NULL): device float64 planes x then y of the transformed pixel
+ * coordinates, used instead of the transforms (a DistortedCoords).
  */
 int dlx_aperture_basis(int32_t npix, double pixel_scale, double center, int32_t n_transforms,
-                       const dlx_coord_transform* transforms, double extent, int32_t nsides,
-                       int32_t nmodes, const int32_t* js, float* out, void* stream);
+                       const dlx_coord_transform* transforms, const double* coords, double extent,
+                       int32_t nsides, int32_t nmodes, const int32_t* js, float* out,
+                       void* stream);
 
 /*
  * Centred, zero-padded 2-D FFT of `batch` N x N complex64 fields to M x M

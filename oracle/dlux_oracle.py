@@ -23,7 +23,7 @@ Parity status per function (all pinned unless marked):
   * poly_psf_grad (adjoint gradient)                             pinned (FD)
   * gaussian / jitter_kernel / convolve_same / detector           pinned
   * detector_vjp                                                 pinned (FD, adjoint identity)
-  * coord_transform / soften / *_distance / soft_aperture / polike  pinned
+  * coord_transform / distort_coords / soften / *_distance / soft_aperture / polike  pinned
   * asm_prop (angular spectrum)                                  parity unpinned
       -- its conventions live in abcdLux (absent); checked by textbook
          invariants (z=0 identity, unitarity, z1+z2 composition) and the
@@ -748,7 +748,35 @@ def detector_vjp(psf, layers, g):
 # ---------------------------------------------------------------------------
 
 
-def coord_transform(coords, translation=None, rotation=None, compression=None, shear=None):
+def gen_powers(degree):
+    """(x powers, y powers) of the 2-D monomials (utils/coordinates.py:104-133)."""
+    n = degree * (degree + 1) // 2
+    vals = np.arange(n)
+    tri = lambda k: k * (k + 1) // 2
+    ypows = vals - np.repeat(tri(np.arange(degree)), np.arange(1, degree + 1))
+    xpows = np.flip(vals - np.repeat(n - np.flip(tri(np.arange(1, degree + 1))),
+                                     np.arange(degree, 0, -1)))
+    return np.array([xpows, ypows])
+
+
+def distort_coords(coords, coeffs, pows):
+    """coords + sum_m coeffs[:, m] x^px_m y^py_m (utils/coordinates.py:136-163)."""
+    coords = np.asarray(coords, np.float64)
+    pows = np.asarray(pows)
+    basis = np.prod(coords[:, None] ** pows.reshape(pows.shape + (1,) * (coords.ndim - 1)), 0)
+    return coords + np.tensordot(np.asarray(coeffs, np.float64), basis, axes=(-1, 0))
+
+
+def coord_transform(coords, translation=None, rotation=None, compression=None, shear=None,
+                    distortion=None, order=None):
+    """A CoordTransform, or with `distortion` (and `order`) a DistortedCoords
+    (coordinates.py:230-357)."""
+    if distortion is not None:
+        return distort_coords(coords, distortion, gen_powers(order + 1)[:, 1:])
+    return _coord_transform(coords, translation, rotation, compression, shear)
+
+
+def _coord_transform(coords, translation=None, rotation=None, compression=None, shear=None):
     """CoordTransform.__call__: translation, shear, compression, rotation, in
     that order (coordinates.py:306-316).  Shear adds each coordinate plane's
     transpose times the factor (utils/coordinates.py:61-78)."""

@@ -454,9 +454,9 @@ def test_aperture_abi_validation_without_gpu(lib):
     st = lib.dlx_soft_apertures(64, 1 / 64, 0.0, 2, aps, 4096, None, 0, None)
     assert st == 3 and b"workspace" in lib.dlx_last_error()
     js = (ctypes.c_int32 * 2)(1, 2)
-    st = lib.dlx_aperture_basis(64, 1 / 64, 0.0, 0, None, 1.0, 2, 2, js, 4096, None)
+    st = lib.dlx_aperture_basis(64, 1 / 64, 0.0, 0, None, None, 1.0, 2, 2, js, 4096, None)
     assert st == 1 and b"nsides" in lib.dlx_last_error()
-    st = lib.dlx_aperture_basis(64, 1 / 64, 0.0, 4, None, 1.0, 0, 2, js, 4096, None)
+    st = lib.dlx_aperture_basis(64, 1 / 64, 0.0, 4, None, None, 1.0, 0, 2, js, 4096, None)
     assert st == 1 and b"at most 3" in lib.dlx_last_error()
 
 
@@ -642,3 +642,31 @@ def test_spectra_classes():
     assert isinstance(ps, dl.BaseSpectrum) and isinstance(dl.Spectrum(wl), dl.SimpleSpectrum)
     src = dl.PointSource(spectrum=ps)
     assert np.allclose(src.spectrum.weights, ps.weights)
+
+
+def test_distorted_coords_host():
+    """DistortedCoords: shapes and errors; the FP64 coordinate planes that the
+    aperture and resampling kernels read (a CoordTransform then a distortion)
+    against the oracle's chain on the CPU."""
+    torch = pytest.importorskip("torch")
+    import dlux_amd as dl
+    from oracle import dlux_oracle as orc
+    from dlux_amd.coordinates import _chain_coords
+    assert dl.DistortedCoords(2).powers.tolist() == [[1, 0, 2, 1, 0], [0, 1, 0, 1, 2]]
+    with pytest.raises(ValueError):
+        dl.DistortedCoords(1, np.zeros((2, 5)))
+    D = np.array([[0.02, -0.01, 0.3, 0.1, -0.2], [0.01, 0.03, -0.1, 0.25, 0.15]])
+    tf = dict(translation=[0.03, -0.02], rotation=0.3, compression=[1.1, 0.9],
+              shear=[0.05, -0.02])
+    chain = (dl.CoordTransform(**tf), dl.DistortedCoords(2, D))
+    got = _chain_coords(chain, 24, 0.05, 0.01, "cpu").numpy()
+    xs = orc.spec_xs(24, 0.05, 0.01)
+    ref = np.array(np.meshgrid(xs, xs, indexing="xy"))
+    ref = orc.coord_transform(orc.coord_transform(ref, **tf), distortion=D, order=2)
+    assert np.max(np.abs(got - ref)) < 1e-14
+    # the host call form (float32 for the CoordTransform) matches the planes
+    c = torch.as_tensor(np.array(np.meshgrid(xs, xs, indexing="xy")))
+    out = chain[1](chain[0](c))
+    assert np.max(np.abs(np.asarray(out) - ref)) < 1e-6
+    with pytest.raises(TypeError):
+        _chain_coords((object(),), 4, 1.0, 0.0, "cpu")

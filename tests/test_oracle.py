@@ -630,3 +630,21 @@ def test_interp_grid_against_scipy_and_linear_functions():
     got = orc.interp_grid(lin, n, d, c, xs, ys, "linear", fill=0.0)
     inside = np.abs(np.stack([xs, ys]) - c).max(0) <= (n - 1) / 2 * d
     assert np.allclose(got[inside], (2.0 * xs - 3.0 * ys + 0.5)[inside], atol=1e-12)
+
+
+def test_distortion_closed_forms():
+    """DistortedCoords: gen_powers matches the monomial order (1, x, y, x^2, xy,
+    y^2, ...); a first-order distortion is the linear map coords + D coords;
+    zero distortion is the identity."""
+    assert orc.gen_powers(3).tolist() == [[0, 1, 0, 2, 1, 0], [0, 0, 1, 0, 1, 2]]
+    c = orc.pixel_coords(16, 2.0)
+    D = np.array([[0.1, -0.2], [0.05, 0.3]])
+    got = orc.coord_transform(c, distortion=D, order=1)
+    ref = c + np.einsum("ij,jyx->iyx", D, c)
+    assert np.allclose(got, ref, atol=1e-14)
+    assert np.array_equal(orc.coord_transform(c, distortion=np.zeros((2, 5)), order=2), c)
+    # second order: the x^2 coefficient of the x axis
+    D2 = np.zeros((2, 5))
+    D2[0, 2] = 0.5
+    got = orc.coord_transform(c, distortion=D2, order=2)
+    assert np.allclose(got[0], c[0] + 0.5 * c[0] ** 2) and np.allclose(got[1], c[1])
