@@ -193,6 +193,9 @@ SIGNATURES = {
                                      POINTER(CoordTransformC), c_void_p, c_double, c_int32,
                                      c_int32, POINTER(c_int32), _fp, c_void_p]),
     "dlx_mft_batched_workspace_size": (c_size_t, [c_int32, POINTER(MftInstance), c_int32]),
+    "dlx_batched_graphs": (c_int32, [c_int32]),
+    "dlx_batched_graph_stats": (c_int32, [POINTER(ctypes.c_uint64), POINTER(ctypes.c_uint64),
+                                          POINTER(c_int32)]),
     "dlx_mft_batched_psf": (c_int32, [c_int32, POINTER(MftInstance), c_int32,
                                       POINTER(c_void_p), c_void_p, c_size_t, c_void_p]),
     "dlx_noll_indices": (c_int32, [c_int32, POINTER(c_int32), POINTER(c_int32)]),

@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -414,6 +416,158 @@ int plan_batched(int n, const dlx_mft_instance* inst, int n_outputs, float* cons
   return DLX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// launches, eager or as a replayed graph
+// ---------------------------------------------------------------------------
+// Every launch reads its jobs and problems from the descriptors at the front of
+// the workspace; its grid and by-value arguments depend on the plan's shape
+// only.  So one captured graph serves every call with the same workspace and
+// the same shape, whatever the fields, wavelengths, weights, shifts, outputs
+// and matrix addresses (all uploaded before the launch).
+int launch_plan(const Plan& Q, uint8_t* ws, bool with_matrices, hipStream_t s) {
+  auto blocks = [](long rows) { return (unsigned)(rows < 16384 ? rows : 16384); };
+  if (Q.cvt_work > 0) {
+    hipLaunchKernelGGL(group_to_split4_kernel, dim3(blocks(Q.cvt_work)), dim3(256), 0, s,
+                       (const CvtJob*)(ws + Q.off_cvt), (int)Q.cvt.size(), Q.cvt_work);
+    if (int st = launch_status("group_to_split4_kernel")) return st;
+  }
+  if (with_matrices && Q.mat_work > 0) {
+    hipLaunchKernelGGL(group_dft_split4_kernel, dim3(blocks(Q.mat_work)), dim3(256), 0, s,
+                       (const MatJob*)(ws + Q.off_mat), (int)Q.mat.size(), Q.mat_work);
+    if (int st = launch_status("group_dft_split4_kernel")) return st;
+  }
+  auto grouped = [&](size_t off, const std::vector<GroupProblem>& probs, const int (&xcd)[9],
+                     int tile_n, int epi) -> int {
+    if (probs.empty()) return DLX_OK;
+    GroupLaunch l{};
+    l.probs = (const GroupProblem*)(ws + off);
+    l.nprobs = (int)probs.size();
+    l.tile_n = tile_n;
+    std::memcpy(l.xcd_start, xcd, sizeof(l.xcd_start));
+    DLX_HIP(launch_cgemm_split4_group(l, epi, s));
+    return DLX_OK;
+  };
+  const size_t n1 = Q.g1.size() * sizeof(GroupProblem), n2 = Q.g2.size() * sizeof(GroupProblem);
+  if (int st = grouped(Q.off_g1, Q.g1, Q.g1_xcd, 128, EPI_SPLIT_T)) return st;
+  if (int st = grouped(Q.off_g1 + n1, Q.g1n, Q.g1n_xcd, 64, EPI_SPLIT_T)) return st;
+  if (int st = grouped(Q.off_g2, Q.g2, Q.g2_xcd, 128, EPI_INTENSITY)) return st;
+  if (int st = grouped(Q.off_g2 + n2, Q.g2n, Q.g2n_xcd, 64, EPI_INTENSITY)) return st;
+  if (Q.sum.empty()) return DLX_OK;
+  long most = 0;
+  for (const SumJob& j : Q.sum) most = j.npix > most ? j.npix : most;
+  const unsigned gx = (unsigned)std::min<long>((most + 255) / 256, 256);
+  hipLaunchKernelGGL(group_sum_kernel, dim3(gx, (unsigned)Q.sum.size()), dim3(256), 0, s,
+                     (const SumJob*)(ws + Q.off_sum));
+  return launch_status("group_sum_kernel");
+}
+
+// Everything launch_plan's grids and by-value arguments depend on.
+std::vector<long> plan_shape(const Plan& Q, const uint8_t* ws, bool fresh, int dev) {
+  std::vector<long> k = {(long)dev, (long)(uintptr_t)ws, (long)fresh, (long)Q.off_mat,
+                         (long)Q.mat.size(), Q.mat_work, (long)Q.off_cvt, (long)Q.cvt.size(),
+                         Q.cvt_work,
+                         (long)Q.off_g1, (long)Q.off_g2, (long)Q.g1.size(), (long)Q.g1n.size(),
+                         (long)Q.g2.size(), (long)Q.g2n.size(), (long)Q.off_sum,
+                         (long)Q.sum.size()};
+  for (const int* x : {Q.g1_xcd, Q.g1n_xcd, Q.g2_xcd, Q.g2n_xcd}) k.insert(k.end(), x, x + 9);
+  for (const SumJob& j : Q.sum) k.push_back(j.npix);
+  return k;
+}
+
+struct GraphEntry {
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;     // recorded after the entry's latest launch
+  uint64_t tick = 0;
+};
+
+struct GraphCache {
+  std::mutex mu;
+  std::map<std::vector<long>, GraphEntry> map;
+  std::map<int, hipStream_t> capture;   // per device: a private stream to capture on
+  uint64_t tick = 0, captures = 0, replays = 0;
+  int enabled = -1;              // -1: not yet read from the environment
+  static constexpr size_t kMax = 64;
+};
+
+GraphCache& graphs() {
+  static GraphCache c;
+  return c;
+}
+
+int graphs_on(GraphCache& c) {
+  if (c.enabled < 0) {
+    // off by default: on ROCm 7.2 a launch of this graph took more host time
+    // than its direct launches (50-480 us against 50-380 us per call)
+    const char* e = std::getenv("DLUX_AMD_BATCHED_GRAPHS");
+    c.enabled = (e && e[0] == '1') ? 1 : 0;
+  }
+  return c.enabled;
+}
+
+void drop_entry(GraphEntry& e) {
+  if (e.done) {
+    hipEventSynchronize(e.done);   // the exec is destroyed after its last launch ran
+    hipEventDestroy(e.done);
+  }
+  if (e.exec) hipGraphExecDestroy(e.exec);
+  e = GraphEntry{};
+}
+
+// Launch the plan through the graph cache.  The first sighting of a shape runs
+// eagerly (*eager set); the second captures the launches on a private stream
+// (capture records and runs nothing), instantiates the graph and launches it
+// on `s`; later sightings launch it again.  The caller's stream may be the
+// null stream.
+int launch_plan_graph(const Plan& Q, uint8_t* ws, bool fresh, hipStream_t s, bool* eager) {
+  *eager = true;
+  GraphCache& c = graphs();
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (!graphs_on(c)) return DLX_OK;
+  int dev = 0;
+  DLX_HIP(hipGetDevice(&dev));
+  std::vector<long> key = plan_shape(Q, ws, fresh, dev);
+  auto it = c.map.find(key);
+  if (it == c.map.end()) {
+    if (c.map.size() >= GraphCache::kMax) {
+      auto old = c.map.begin();
+      for (auto q = c.map.begin(); q != c.map.end(); ++q)
+        if (q->second.tick < old->second.tick) old = q;
+      drop_entry(old->second);
+      c.map.erase(old);
+    }
+    c.map[key].tick = ++c.tick;
+    return DLX_OK;
+  }
+  GraphEntry& e = it->second;
+  e.tick = ++c.tick;
+  if (!e.exec) {
+    hipStream_t& cs = c.capture[dev];
+    if (!cs) DLX_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    DLX_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    const int st = launch_plan(Q, ws, fresh, cs);
+    const hipError_t end = hipStreamEndCapture(cs, &g);
+    if (st) {
+      if (end == hipSuccess && g) hipGraphDestroy(g);
+      return st;
+    }
+    DLX_HIP(end);
+    const hipError_t inst = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (inst != hipSuccess) {
+      e.exec = nullptr;
+      DLX_HIP(inst);
+    }
+    DLX_HIP(hipEventCreateWithFlags(&e.done, hipEventDisableTiming));
+    ++c.captures;
+  }
+  DLX_HIP(hipGraphLaunch(e.exec, s));
+  DLX_HIP(hipEventRecord(e.done, s));
+  ++c.replays;
+  *eager = false;
+  return DLX_OK;
+}
+
 template <typename T>
 void put(std::vector<uint8_t>& host, size_t off, const std::vector<T>& v) {
   if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
@@ -500,41 +654,44 @@ int dlx_mft_batched_psf(int32_t n, const dlx_mft_instance* instances, int32_t n_
   put(host, Q.off_sum, Q.sum);
   if (int st = upload(host.data(), host.size(), ws, s)) return st;
 
-  auto blocks = [](long rows) { return (unsigned)(rows < 16384 ? rows : 16384); };
-  hipLaunchKernelGGL(group_to_split4_kernel, dim3(blocks(Q.cvt_work)), dim3(256), 0, s,
-                     (const CvtJob*)(ws + Q.off_cvt), (int)Q.cvt.size(), Q.cvt_work);
-  if (int st = launch_status("group_to_split4_kernel")) return st;
-  if (fresh) {
-    hipLaunchKernelGGL(group_dft_split4_kernel, dim3(blocks(Q.mat_work)), dim3(256), 0, s,
-                       (const MatJob*)(ws + Q.off_mat), (int)Q.mat.size(), Q.mat_work);
-    if (int st = launch_status("group_dft_split4_kernel")) return st;
+  // the launches, as a replayed graph when this shape was seen before
+  bool eager = true;
+  if (cs == hipStreamCaptureStatusNone)
+    if (int st = launch_plan_graph(Q, ws, fresh, s, &eager)) return st;
+  if (eager)
+    if (int st = launch_plan(Q, ws, fresh, s)) return st;
+  if (fresh)
     if (int st = cache_ready(whandle, s)) return st;
-  }
-
-  auto grouped = [&](size_t off, const std::vector<GroupProblem>& probs, const int (&xcd)[9],
-                     int tile_n, int epi) -> int {
-    if (probs.empty()) return DLX_OK;
-    GroupLaunch l{};
-    l.probs = (const GroupProblem*)(ws + off);
-    l.nprobs = (int)probs.size();
-    l.tile_n = tile_n;
-    std::memcpy(l.xcd_start, xcd, sizeof(l.xcd_start));
-    DLX_HIP(launch_cgemm_split4_group(l, epi, s));
-    return DLX_OK;
-  };
-  const size_t n1 = Q.g1.size() * sizeof(GroupProblem), n2 = Q.g2.size() * sizeof(GroupProblem);
-  if (int st = grouped(Q.off_g1, Q.g1, Q.g1_xcd, 128, EPI_SPLIT_T)) return st;
-  if (int st = grouped(Q.off_g1 + n1, Q.g1n, Q.g1n_xcd, 64, EPI_SPLIT_T)) return st;
-  if (int st = grouped(Q.off_g2, Q.g2, Q.g2_xcd, 128, EPI_INTENSITY)) return st;
-  if (int st = grouped(Q.off_g2 + n2, Q.g2n, Q.g2n_xcd, 64, EPI_INTENSITY)) return st;
-
-  long most = 0;
-  for (const SumJob& j : Q.sum) most = j.npix > most ? j.npix : most;
-  const unsigned gx = (unsigned)std::min<long>((most + 255) / 256, 256);
-  hipLaunchKernelGGL(group_sum_kernel, dim3(gx, (unsigned)Q.sum.size()), dim3(256), 0, s,
-                     (const SumJob*)(ws + Q.off_sum));
-  if (int st = launch_status("group_sum_kernel")) return st;
   return cache_release(whandle, s);
+}
+
+int dlx_batched_graph_stats(uint64_t* captures, uint64_t* launches, int32_t* entries) {
+  clear_error();
+  GraphCache& c = graphs();
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (captures) *captures = c.captures;
+  if (launches) *launches = c.replays;
+  if (entries) {
+    int n = 0;
+    for (auto& kv : c.map) n += kv.second.exec != nullptr;
+    *entries = n;
+  }
+  return DLX_OK;
+}
+
+int dlx_batched_graphs(int32_t enable) {
+  clear_error();
+  GraphCache& c = graphs();
+  std::lock_guard<std::mutex> lock(c.mu);
+  const int prev = graphs_on(c);
+  if (enable >= 0) {
+    c.enabled = enable ? 1 : 0;
+    if (!c.enabled) {
+      for (auto& kv : c.map) drop_entry(kv.second);
+      c.map.clear();
+    }
+  }
+  return prev;
 }
 
 }  // extern "C"

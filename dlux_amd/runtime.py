@@ -1025,6 +1025,25 @@ def profile_read(stage: int) -> tuple[float, int]:
     return float(t.value), int(c.value)
 
 
+def batched_graphs(enable: bool | None = None) -> bool:
+    """Graph replay of the uneven batched launches (dlx_batched_graphs):
+    set it when `enable` is given; returns the previous setting."""
+    prev = _lib.load().dlx_batched_graphs(-1 if enable is None else int(bool(enable)))
+    if prev < 0:
+        _lib.check(prev, "dlx_batched_graphs")
+    return bool(prev)
+
+
+def batched_graph_stats() -> dict:
+    """Graphs captured, graph launches and graphs held by the uneven batched
+    path (dlx_batched_graph_stats)."""
+    import ctypes
+    c, l, e = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int32()
+    _lib.check(_lib.load().dlx_batched_graph_stats(ctypes.byref(c), ctypes.byref(l),
+                                                   ctypes.byref(e)), "dlx_batched_graph_stats")
+    return {"captures": int(c.value), "launches": int(l.value), "entries": int(e.value)}
+
+
 def cache_clear() -> None:
     """Free every cached transfer matrix (after their pending uses)."""
     _lib.check(_lib.load().dlx_cache_clear(), "dlx_cache_clear")

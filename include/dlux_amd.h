@@ -229,7 +229,12 @@ typedef struct dlx_mft_instance {
  * instances run in grouped persistent launches (split-BF16x3 contractions).
  * Instances that share a field pointer share its conversion.  `instances`
  * and `outputs` are HOST arrays (outputs holds device pointers).  Cannot be
- * captured into a graph (it uploads per-call descriptors).
+ * captured into a caller's graph (it uploads per-call descriptors); instead,
+ * when enabled, the launches themselves run as a HIP graph that the library captures (on a
+ * private stream) at the second call with the same workspace, batch shape
+ * (sizes, output grouping, shared fields) and matrix state (generated or
+ * resident), and launches on `stream` after uploading that call's
+ * descriptors (dlx_batched_graphs).
  * Replaces the vmapped source x wavelength propagation of
  * PointSources.model (sources.py:392-411) and OpticalSystem.propagate
  * (optical_systems.py:211-223) with MFT shifts (utils/propagation.py:178-256).
@@ -239,6 +244,19 @@ size_t dlx_mft_batched_workspace_size(int32_t n, const dlx_mft_instance* instanc
 int dlx_mft_batched_psf(int32_t n, const dlx_mft_instance* instances, int32_t n_outputs,
                         float* const* outputs, void* workspace, size_t workspace_bytes,
                         void* stream);
+
+/*
+ * Graph replay of dlx_mft_batched_psf's launches: enable = 1 on, 0 off (and
+ * every captured graph destroyed after its last launch ran), -1 query.
+ * Returns the previous setting.  Default off (environment
+ * DLUX_AMD_BATCHED_GRAPHS=1 turns it on): measured on ROCm 7.2, a launch of
+ * the captured graph cost more host time than the direct launches it replaces.  At most 64 graphs are kept,
+ * least recently used dropped first.  No counterpart in the reference (its
+ * launches are XLA's); for serving loops that repeat batch shapes.
+ */
+int dlx_batched_graphs(int32_t enable);
+/* Graphs captured and graph launches since the process started, and graphs held. */
+int dlx_batched_graph_stats(uint64_t* captures, uint64_t* launches, int32_t* entries);
 
 /*
  * Noll index j -> radial order n and signed azimuthal order m.

@@ -191,3 +191,93 @@ def test_batched_from_arrays_equals_dict_path(rt):
     with pytest.raises(ValueError, match="outside"):
         rt.MftBatch.from_arrays([stack], [0], 1e-6, 1.0 / n, 24, 2.5e-7, 1.0, 0,
                                 slice_index=nl)
+
+
+def test_batched_graph_replay_equals_eager(rt):
+    """The batched launches run as a captured HIP graph from the second call
+    with the same workspace and batch shape; every graph launch equals the
+    eager launches bit for bit (same kernels, same inputs), call after call,
+    with new field contents (matrices resident) or new shifts (matrices
+    generated inside the graph, cache off), and eager work and
+    synchronisations between the calls; on the default stream and on a side
+    stream."""
+    import contextlib
+    print("[batched] graph replay", flush=True)
+    rng = np.random.default_rng(21)
+    inst = _make(rng, [32, 64], 12, 3)
+    uniq = list({d["field"].data_ptr(): d["field"] for d in inst}.values())
+
+    def set_fields(k):
+        for q, f in enumerate(uniq):
+            g = _pupil(f.shape[0], 500 + 10 * k + q).astype(np.complex64)
+            f.copy_(torch.as_tensor(g, device=f.device))
+
+    def set_shifts(b, k):
+        r = np.random.default_rng(700 + k)
+        for i in range(b.n):
+            b.arr[i].shift_x = float(r.uniform(-3, 3))
+            b.arr[i].shift_y = float(r.uniform(-3, 3)) + 0.5   # never equal to shift_x
+
+    def run(b):
+        out = [None if o is None else o.clone() for o in b()]
+        x = torch.randn(256, 256, device="cuda")
+        (x @ x).sum().item()   # eager work and a synchronisation between calls
+        return out
+
+    def same(a, b):
+        return all((x is None and y is None) or torch.equal(x, y) for x, y in zip(a, b))
+
+    prev = rt.batched_graphs(None)
+    try:
+        for side in (False, True):
+            ctx = torch.cuda.stream(torch.cuda.Stream()) if side else contextlib.nullcontext()
+            with ctx:
+                b = rt.MftBatch(inst, n_outputs=3)
+                # new field contents, matrices resident after the first calls
+                rt.batched_graphs(False)
+                ref = []
+                for k in range(3):
+                    set_fields(k)
+                    ref.append(run(b))
+                rt.batched_graphs(True)
+                s0 = rt.batched_graph_stats()
+                for call in range(6):
+                    set_fields(call % 3)
+                    assert same(run(b), ref[call % 3]), f"side={side} call {call}"
+                s1 = rt.batched_graph_stats()
+                print(f"  side={side} fields: {s0} -> {s1}")
+                assert s1["captures"] == s0["captures"] + 1
+                assert s1["launches"] == s0["launches"] + 5
+                # new shifts, cache off: matrices generated in every call
+                rt.cache_set_budget(0)
+                try:
+                    rt.batched_graphs(False)
+                    ref = []
+                    for k in range(3):
+                        set_shifts(b, k)
+                        ref.append(run(b))
+                    rt.batched_graphs(True)
+                    s0 = rt.batched_graph_stats()
+                    for call in range(6):
+                        set_shifts(b, call % 3)
+                        assert same(run(b), ref[call % 3]), f"side={side} shifts call {call}"
+                    s1 = rt.batched_graph_stats()
+                finally:
+                    rt.cache_set_budget(16 << 30)
+                print(f"  side={side} shifts: {s0} -> {s1}")
+                assert s1["captures"] == s0["captures"] + 1
+                assert s1["launches"] == s0["launches"] + 5
+            torch.cuda.synchronize()
+        # the replayed results are the oracle's too
+        set_fields(0)
+        b = rt.MftBatch(inst, n_outputs=3)
+        for _ in range(3):
+            outs = b()
+        for d in inst:
+            d["field_np"] = d["field"].cpu().numpy().astype(np.complex128)
+        refo = _oracle_outputs(inst, 3)
+        for o in range(3):
+            if refo[o] is not None:
+                assert _peak_err(outs[o].cpu().numpy(), refo[o]) < TOL
+    finally:
+        rt.batched_graphs(prev)
