@@ -74,29 +74,32 @@ __device__ __forceinline__ uint32_t quad_xor(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xF, 0xF, false);
 }
 
-__device__ __forceinline__ uint32_t sel4(const uint32_t (&a)[4], int i) {
-  return (i & 2) ? ((i & 1) ? a[3] : a[2]) : ((i & 1) ? a[1] : a[0]);
+// a[i] for a lane-dependent i, as selects on values: an array indexed by a
+// lane-dependent value is placed in LDS by the compiler (a per-lane copy,
+// written and read back with bank conflicts)
+__device__ __forceinline__ uint32_t pick4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3,
+                                          int i) {
+  const uint32_t lo = (i & 1) ? a1 : a0, hi = (i & 1) ? a3 : a2;
+  return (i & 2) ? hi : lo;
 }
 
 __device__ __forceinline__ void store_chunk_quad(uint8_t* base, long row, long row_bytes,
                                                  long chunk, int q, float2 v0, float2 v1) {
-  uint32_t h0, l0, h1, l1, w[4];
+  uint32_t h0, l0, h1, l1;
   split_bf16(v0.x, h0, l0);
   split_bf16(v1.x, h1, l1);
-  w[0] = h0 | (h1 << 16);
-  w[1] = l0 | (l1 << 16);
+  const uint32_t w0 = h0 | (h1 << 16), w1 = l0 | (l1 << 16);
   split_bf16(v0.y, h0, l0);
   split_bf16(v1.y, h1, l1);
-  w[2] = h0 | (h1 << 16);
-  w[3] = l0 | (l1 << 16);
-  // g[m] = the word lane q ^ m holds for plane q
-  uint32_t g[4];
-  g[0] = sel4(w, q);
-  g[1] = quad_xor<1>(sel4(w, q ^ 1));
-  g[2] = quad_xor<2>(sel4(w, q ^ 2));
-  g[3] = quad_xor<3>(sel4(w, q ^ 3));
+  const uint32_t w2 = h0 | (h1 << 16), w3 = l0 | (l1 << 16);
+  // g_m = the word lane q ^ m holds for plane q
+  const uint32_t g0 = pick4(w0, w1, w2, w3, q);
+  const uint32_t g1 = quad_xor<1>(pick4(w0, w1, w2, w3, q ^ 1));
+  const uint32_t g2 = quad_xor<2>(pick4(w0, w1, w2, w3, q ^ 2));
+  const uint32_t g3 = quad_xor<3>(pick4(w0, w1, w2, w3, q ^ 3));
   // word j of plane q comes from lane j = q ^ m
-  const uint4 o = make_uint4(sel4(g, q), sel4(g, q ^ 1), sel4(g, q ^ 2), sel4(g, q ^ 3));
+  const uint4 o = make_uint4(pick4(g0, g1, g2, g3, q), pick4(g0, g1, g2, g3, q ^ 1),
+                             pick4(g0, g1, g2, g3, q ^ 2), pick4(g0, g1, g2, g3, q ^ 3));
   *(uint4*)(base + row * row_bytes + chunk * kChunkBytes + q * 16) = o;
 }
 
