@@ -189,6 +189,11 @@ SIGNATURES = {
                                        c_int32, _fp, _fp, c_void_p]),
     "dlx_interp_points": (c_int32, [c_int32, c_double, c_double, c_int64, _fp, _fp, c_int32,
                                     c_float, c_int32, _fp, _fp, c_void_p]),
+    "dlx_interp_transform_vjp": (c_int32, [c_int32, c_double, c_double, c_int32, c_double,
+                                           c_double, c_int32, POINTER(CoordTransformC), c_int32,
+                                           c_int32, _fp, _fp, _fp, c_void_p]),
+    "dlx_interp_points_vjp": (c_int32, [c_int32, c_double, c_double, c_int64, _fp, _fp,
+                                        c_int32, c_int32, _fp, _fp, _fp, c_void_p]),
     "dlx_aperture_basis": (c_int32, [c_int32, c_double, c_double, c_int32,
                                      POINTER(CoordTransformC), c_void_p, c_double, c_int32,
                                      c_int32, POINTER(c_int32), _fp, c_void_p]),

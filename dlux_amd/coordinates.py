@@ -168,6 +168,20 @@ class DistortedCoords(BaseCoordTransform):
         return ("distorted", tuple(self.powers.ravel().tolist()), _floats(self.distortion))
 
 
+def _untracked(tf, what: str) -> None:
+    """Raise when a parameter of `tf` is a tensor that requires grad while
+    gradients are recorded: the resampling kernels take the transformation
+    as constants."""
+    import torch
+    if not torch.is_grad_enabled():
+        return
+    for f in ("translation", "rotation", "compression", "shear", "distortion"):
+        v = getattr(tf, f, None)
+        if _is_tensor(v) and v.requires_grad:
+            raise NotImplementedError(f"gradients with respect to the transformation of {what} "
+                                      "are not implemented (the images' gradient is)")
+
+
 def _chain_coords(tfs, npix: int, pixel_scale: float, centre: float, dev):
     """The (2, npix, npix) float64 device planes of the pixel grid
     centre + (k - (npix-1)/2) pixel_scale put through the transforms in order

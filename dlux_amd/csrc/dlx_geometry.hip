@@ -3,6 +3,7 @@
 // apertures with a secondary obscuration and spider vanes.  One lane per
 // output pixel; FP64 arithmetic (these run once per optical system and feed
 // resident device buffers).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -387,6 +388,64 @@ interp_points_kernel(long npts, const double* __restrict__ qx, const double* __r
       r = (float)(top + (bot - top) * tv);
     }
     out[(long)b * npts + p] = r;
+  }
+}
+
+// ---- adjoint of the resampling (the images' cotangent) --------------------
+// Each output point adds its cotangent times its interpolation weights into
+// the knots it read: the transpose of the sampling matrix.  The sums go into
+// an FP64 workspace by atomics and are rounded to float32 once, so the result
+// does not depend on the order of the additions beyond FP64 rounding.  Points
+// outside the knots' span read no knot (their value is the constant fill).
+__global__ void __launch_bounds__(256) zero_f64_kernel(long n, double* __restrict__ a) {
+  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256)
+    a[k] = 0.0;
+}
+
+__global__ void __launch_bounds__(256)
+f64_to_f32_kernel(long n, const double* __restrict__ a, float* __restrict__ o) {
+  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256)
+    o[k] = (float)a[k];
+}
+
+// qx == NULL: point p is output pixel p of the n_out grid through the chain.
+__global__ void __launch_bounds__(256)
+interp_vjp_kernel(TfChain ch, int n_out, double d_out, double c_out, const double* __restrict__ qx,
+                  const double* __restrict__ qy, long npts, int n_in, double d_in, double c_in,
+                  int method, int batch, const float* __restrict__ dout,
+                  double* __restrict__ acc) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npts) return;
+  double x, y;
+  if (qx) {
+    x = qx[p];
+    y = qy[p];
+  } else {
+    const int i = (int)(p / n_out), j = (int)(p - (long)i * n_out);
+    tf_point(ch, i, j, c_out, d_out, (n_out - 1) / 2.0, x, y);
+  }
+  const double h = (n_in - 1) / 2.0;
+  const double u = (x - c_in) / d_in + h, v = (y - c_in) / d_in + h;
+  if (!(u >= 0.0 && u <= n_in - 1 && v >= 0.0 && v <= n_in - 1)) return;
+  const long nn = (long)n_in * n_in;
+  if (method == 0) {
+    const long k = (long)min((int)rint(v), n_in - 1) * n_in + min((int)rint(u), n_in - 1);
+    for (int b = 0; b < batch; ++b) atomicAdd(acc + (long)b * nn + k, (double)dout[(long)b * npts + p]);
+    return;
+  }
+  const int c0 = min((int)floor(u), n_in > 1 ? n_in - 2 : 0);
+  const int r0 = min((int)floor(v), n_in > 1 ? n_in - 2 : 0);
+  const int c1 = min(c0 + 1, n_in - 1), r1 = min(r0 + 1, n_in - 1);
+  const double tu = u - c0, tv = v - r0;
+  const double w00 = (1.0 - tu) * (1.0 - tv), w01 = tu * (1.0 - tv);
+  const double w10 = (1.0 - tu) * tv, w11 = tu * tv;
+  for (int b = 0; b < batch; ++b) {
+    const double g = dout[(long)b * npts + p];
+    double* a = acc + (long)b * nn;
+    atomicAdd(a + (long)r0 * n_in + c0, g * w00);
+    atomicAdd(a + (long)r0 * n_in + c1, g * w01);
+    atomicAdd(a + (long)r1 * n_in + c0, g * w10);
+    atomicAdd(a + (long)r1 * n_in + c1, g * w11);
   }
 }
 
@@ -1020,4 +1079,67 @@ extern "C" int dlx_interp_points(int32_t n_in, double pixel_scale_in, double cen
                      method, fill, batch, in, out);
   DLX_HIP(hipGetLastError());
   return DLX_OK;
+}
+
+namespace dlx {
+namespace {
+int interp_vjp_impl(const TfChain& ch, int n_out, double d_out, double c_out, const double* qx,
+                    const double* qy, long npts, int n_in, double d_in, double c_in, int method,
+                    int batch, const float* dout, float* din, double* ws, hipStream_t s) {
+  const long n = (long)batch * n_in * n_in;
+  const unsigned gz = (unsigned)std::min<long>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(zero_f64_kernel, dim3(gz), dim3(256), 0, s, n, ws);
+  DLX_HIP(hipGetLastError());
+  if (npts > 0) {
+    hipLaunchKernelGGL(interp_vjp_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s,
+                       ch, n_out, d_out, c_out, qx, qy, npts, n_in, d_in, c_in, method, batch,
+                       dout, ws);
+    DLX_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gz), dim3(256), 0, s, n, (const double*)ws, din);
+  DLX_HIP(hipGetLastError());
+  return DLX_OK;
+}
+}  // namespace
+}  // namespace dlx
+
+extern "C" int dlx_interp_transform_vjp(int32_t n_in, double pixel_scale_in, double center_in,
+                                        int32_t n_out, double pixel_scale_out,
+                                        double center_out, int32_t n_transforms,
+                                        const dlx_coord_transform* transforms, int32_t method,
+                                        int32_t batch, const float* dout, float* din,
+                                        double* workspace, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(n_in > 0 && n_in <= 46340 && n_out > 0 && n_out <= 46340,
+                "n_in and n_out must be in [1, 46340], got %d and %d", n_in, n_out);
+  DLX_CHECK_ARG(batch >= 0 && batch <= 65535, "batch must be in [0, 65535], got %d", batch);
+  DLX_CHECK_ARG(method == 0 || method == 1, "method must be 0 (nearest) or 1 (linear), got %d",
+                method);
+  DLX_CHECK_ARG(std::isfinite(pixel_scale_in) && pixel_scale_in > 0 &&
+                    std::isfinite(pixel_scale_out) && pixel_scale_out > 0,
+                "pixel scales must be > 0");
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(dout && din && workspace, "null argument to dlx_interp_transform_vjp");
+  TfChain ch{};
+  if (int st = build_chain(n_transforms, transforms, ch)) return st;
+  return interp_vjp_impl(ch, n_out, pixel_scale_out, center_out, nullptr, nullptr,
+                         (long)n_out * n_out, n_in, pixel_scale_in, center_in, method, batch,
+                         dout, din, workspace, (hipStream_t)stream);
+}
+
+extern "C" int dlx_interp_points_vjp(int32_t n_in, double pixel_scale_in, double center_in,
+                                     int64_t npts, const double* qx, const double* qy,
+                                     int32_t method, int32_t batch, const float* dout,
+                                     float* din, double* workspace, void* stream) {
+  clear_error();
+  DLX_CHECK_ARG(n_in > 0 && n_in <= 46340, "n_in must be in [1, 46340], got %d", n_in);
+  DLX_CHECK_ARG(npts >= 0 && batch >= 0 && batch <= 65535, "bad point count or batch");
+  DLX_CHECK_ARG(method == 0 || method == 1, "method must be 0 (nearest) or 1 (linear), got %d",
+                method);
+  DLX_CHECK_ARG(std::isfinite(pixel_scale_in) && pixel_scale_in > 0, "pixel scale must be > 0");
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(din && workspace && (npts == 0 || (qx && qy && dout)),
+                "null argument to dlx_interp_points_vjp");
+  return interp_vjp_impl(TfChain{}, 1, 1.0, 0.0, qx, qy, (long)npts, n_in, pixel_scale_in,
+                         center_in, method, batch, dout, din, workspace, (hipStream_t)stream);
 }

@@ -64,7 +64,8 @@ class PSF(Base):
         from .coordinates import BaseCoordTransform
         if not isinstance(transformation, BaseCoordTransform):
             raise TypeError("transformation must be a BaseCoordTransform.")
-        from .coordinates import CoordTransform, _chain_coords
+        from .coordinates import CoordTransform, _chain_coords, _untracked
+        _untracked(transformation, "PSF.interpolate")
         d = float(np.mean(self.pixel_scale))
         if isinstance(transformation, CoordTransform):
             return self.set(data=runtime.interp_transform(

@@ -278,19 +278,45 @@ def power(fields: torch.Tensor) -> torch.Tensor:
     return out if f.ndim == 3 else out[0]
 
 
-def intensity(fields: torch.Tensor, sum_batch: bool = False) -> torch.Tensor:
-    """|E|^2 of complex64 fields (optionally summed over the leading axis)."""
-    f = as_complex(fields)
+def _intensity(f: torch.Tensor, sum_batch: bool) -> torch.Tensor:
     if f.ndim == 2:
         f = f[None]
     b = f.shape[0]
     npix = f[0].numel()
+    f = f.contiguous()
     shape = tuple(f.shape[1:]) if sum_batch else tuple(f.shape)
     out = torch.empty(shape, dtype=torch.float32, device=f.device)
     st = _lib.load().dlx_intensity(npix, b, f.data_ptr(), int(sum_batch), out.data_ptr(),
                                    stream_handle())
     _lib.check(st, "dlx_intensity")
     return out
+
+
+class _IntensityFn(torch.autograd.Function):
+    """|E|^2 (summed over the leading axis or not); dL/dE = 2 E dL/dI (torch's
+    convention for a real function of a complex input)."""
+
+    @staticmethod
+    def forward(ctx, f, sum_batch):
+        ctx.save_for_backward(f)
+        ctx.sum_batch = sum_batch
+        return _intensity(f, sum_batch)
+
+    @staticmethod
+    def backward(ctx, g):
+        (f,) = ctx.saved_tensors
+        f3 = f if f.ndim == 3 else f[None]
+        g3 = g[None] if ctx.sum_batch else g.reshape(f3.shape)
+        return (2.0 * f3 * g3.to(f3.real.dtype)).reshape(f.shape), None
+
+
+def intensity(fields: torch.Tensor, sum_batch: bool = False) -> torch.Tensor:
+    """|E|^2 of complex64 fields (optionally summed over the leading axis).
+    Differentiable with respect to the fields."""
+    f = as_complex(fields)
+    if isinstance(fields, torch.Tensor) and fields.requires_grad and torch.is_grad_enabled():
+        return _IntensityFn.apply(f, bool(sum_batch))
+    return _intensity(f, sum_batch)
 
 
 def psf_poly_workspace(n_in, n_out, nwl, precision=None) -> int:
@@ -668,19 +694,18 @@ def _interp_method(method: str) -> int:
     return _INTERP_METHODS[method]
 
 
-def interp_transform(images, pixel_scale_in: float, center_in: float, n_out: int,
-                     pixel_scale_out: float, center_out: float, transforms=(),
-                     method: str = "linear", fill: float = 0.0) -> torch.Tensor:
-    """Float32 images (..., n_in, n_in) resampled onto an n_out grid whose pixel
-    coordinates go through the ``_lib.CoordTransformC`` chain first."""
-    dev = device()
+def _interp_prepare(images, dev):
     x = as_float(images, dev).contiguous()
     if x.ndim < 2 or x.shape[-1] != x.shape[-2]:
         raise ValueError(f"images must be (..., n, n), got {tuple(x.shape)}")
     lead = tuple(x.shape[:-2])
-    batch = int(np.prod(lead)) if lead else 1
+    return x, lead, (int(np.prod(lead)) if lead else 1)
+
+
+def _interp_transform_fwd(x, lead, batch, pixel_scale_in, center_in, n_out, pixel_scale_out,
+                          center_out, transforms, method, fill):
     tfs = (_lib.CoordTransformC * max(len(transforms), 1))(*transforms)
-    out = torch.empty(lead + (int(n_out), int(n_out)), dtype=torch.float32, device=dev)
+    out = torch.empty(lead + (int(n_out), int(n_out)), dtype=torch.float32, device=x.device)
     st = _lib.load().dlx_interp_transform(int(x.shape[-1]), float(pixel_scale_in),
                                           float(center_in), int(n_out), float(pixel_scale_out),
                                           float(center_out), len(transforms), tfs,
@@ -690,25 +715,93 @@ def interp_transform(images, pixel_scale_in: float, center_in: float, n_out: int
     return out
 
 
-def interp_points(images, pixel_scale_in: float, center_in: float, xs, ys,
-                  method: str = "linear", fill: float = 0.0) -> torch.Tensor:
-    """Float32 images (..., n, n) sampled at the points (xs, ys) (any common
-    shape) -> (..., *xs.shape)."""
-    dev = device()
-    x = as_float(images, dev).contiguous()
-    lead = tuple(x.shape[:-2])
-    batch = int(np.prod(lead)) if lead else 1
-    qx = torch.as_tensor(xs, device=dev).to(torch.float64).contiguous()
-    qy = torch.as_tensor(ys, device=dev).to(torch.float64).contiguous()
-    if qx.shape != qy.shape:
-        raise ValueError("xs and ys must have the same shape")
-    out = torch.empty(lead + tuple(qx.shape), dtype=torch.float32, device=dev)
+def _interp_points_fwd(x, lead, batch, pixel_scale_in, center_in, qx, qy, method, fill):
+    out = torch.empty(lead + tuple(qx.shape), dtype=torch.float32, device=x.device)
     st = _lib.load().dlx_interp_points(int(x.shape[-1]), float(pixel_scale_in),
                                        float(center_in), qx.numel(), qx.data_ptr(),
                                        qy.data_ptr(), _interp_method(method), float(fill),
                                        batch, x.data_ptr(), out.data_ptr(), stream_handle())
     _lib.check(st, "dlx_interp_points")
     return out
+
+
+class _InterpFn(torch.autograd.Function):
+    """Resampling as a function of the images; the backward is the transpose of
+    the sampling (dlx_interp_transform_vjp / dlx_interp_points_vjp)."""
+
+    @staticmethod
+    def forward(ctx, images, spec):
+        x, lead, batch = _interp_prepare(images, images.device)
+        ctx.meta = (tuple(x.shape), lead, batch, spec)
+        if spec[0] == "transform":
+            return _interp_transform_fwd(x, lead, batch, *spec[1:])
+        ctx.save_for_backward(spec[3], spec[4])
+        return _interp_points_fwd(x, lead, batch, spec[1], spec[2], spec[3], spec[4], *spec[5:])
+
+    @staticmethod
+    def backward(ctx, gout):
+        shape, lead, batch, spec = ctx.meta
+        n_in = shape[-1]
+        g = gout.to(torch.float32).contiguous()
+        din = torch.empty(shape, dtype=torch.float32, device=g.device)
+        ws = torch.empty(max(batch * n_in * n_in, 1), dtype=torch.float64, device=g.device)
+        lib = _lib.load()
+        if spec[0] == "transform":
+            _, d_in, c_in, n_out, d_out, c_out, transforms, method, _ = spec
+            tfs = (_lib.CoordTransformC * max(len(transforms), 1))(*transforms)
+            st = lib.dlx_interp_transform_vjp(n_in, float(d_in), float(c_in), int(n_out),
+                                              float(d_out), float(c_out), len(transforms), tfs,
+                                              _interp_method(method), batch, g.data_ptr(),
+                                              din.data_ptr(), ws.data_ptr(), stream_handle())
+            _lib.check(st, "dlx_interp_transform_vjp")
+        else:
+            qx, qy = ctx.saved_tensors
+            _, d_in, c_in, _, _, method, _ = spec
+            st = lib.dlx_interp_points_vjp(n_in, float(d_in), float(c_in), qx.numel(),
+                                           qx.data_ptr(), qy.data_ptr(), _interp_method(method),
+                                           batch, g.data_ptr(), din.data_ptr(), ws.data_ptr(),
+                                           stream_handle())
+            _lib.check(st, "dlx_interp_points_vjp")
+        return din, None
+
+
+def _tracks(images) -> bool:
+    return isinstance(images, torch.Tensor) and images.requires_grad and torch.is_grad_enabled()
+
+
+def interp_transform(images, pixel_scale_in: float, center_in: float, n_out: int,
+                     pixel_scale_out: float, center_out: float, transforms=(),
+                     method: str = "linear", fill: float = 0.0) -> torch.Tensor:
+    """Float32 images (..., n_in, n_in) resampled onto an n_out grid whose pixel
+    coordinates go through the ``_lib.CoordTransformC`` chain first.
+    Differentiable with respect to the images (dlx_interp_transform_vjp)."""
+    dev = device()
+    _interp_method(method)
+    if _tracks(images):
+        return _InterpFn.apply(images.to(dev), ("transform", pixel_scale_in, center_in, n_out,
+                                                pixel_scale_out, center_out, tuple(transforms),
+                                                method, fill))
+    x, lead, batch = _interp_prepare(images, dev)
+    return _interp_transform_fwd(x, lead, batch, pixel_scale_in, center_in, n_out,
+                                 pixel_scale_out, center_out, transforms, method, fill)
+
+
+def interp_points(images, pixel_scale_in: float, center_in: float, xs, ys,
+                  method: str = "linear", fill: float = 0.0) -> torch.Tensor:
+    """Float32 images (..., n, n) sampled at the points (xs, ys) (any common
+    shape) -> (..., *xs.shape).  Differentiable with respect to the images
+    (dlx_interp_points_vjp); the points are constants."""
+    dev = device()
+    _interp_method(method)
+    qx = torch.as_tensor(xs, device=dev).detach().to(torch.float64).contiguous()
+    qy = torch.as_tensor(ys, device=dev).detach().to(torch.float64).contiguous()
+    if qx.shape != qy.shape:
+        raise ValueError("xs and ys must have the same shape")
+    if _tracks(images):
+        return _InterpFn.apply(images.to(dev), ("points", pixel_scale_in, center_in, qx, qy,
+                                                method, fill))
+    x, lead, batch = _interp_prepare(images, dev)
+    return _interp_points_fwd(x, lead, batch, pixel_scale_in, center_in, qx, qy, method, fill)
 
 
 # ---------------------------------------------------------------------------

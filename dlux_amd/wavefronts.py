@@ -326,7 +326,8 @@ class Wavefront(Base):
         from .coordinates import BaseCoordTransform
         if not isinstance(transformation, BaseCoordTransform):
             raise TypeError("transformation must be a BaseCoordTransform.")
-        from .coordinates import CoordTransform, _chain_coords
+        from .coordinates import CoordTransform, _chain_coords, _untracked
+        _untracked(transformation, "Wavefront.interpolate")
         d = float(np.ravel(self.pixel_scale)[0])
         c = float(np.ravel(self.center)[0])
         if isinstance(transformation, CoordTransform):

@@ -409,6 +409,24 @@ int dlx_interp_points(int32_t n_in, double pixel_scale_in, double center_in, int
                       int32_t batch, const float* in, float* out, void* stream);
 
 /*
+ * Adjoints of dlx_interp_transform and dlx_interp_points with respect to the
+ * images: din (batch x n_in x n_in float32) = S^T dout, S the sampling
+ * matrix (bilinear weights or the nearest knot; points outside the knots'
+ * span contribute nothing).  workspace: batch x n_in x n_in doubles (the FP64
+ * sums, filled by a kernel; capturable).  Replaces the reverse-mode
+ * derivative JAX takes through interpax.interp2d in the resampling helpers
+ * (utils/interpolation.py:13-50).
+ */
+int dlx_interp_transform_vjp(int32_t n_in, double pixel_scale_in, double center_in,
+                             int32_t n_out, double pixel_scale_out, double center_out,
+                             int32_t n_transforms, const dlx_coord_transform* transforms,
+                             int32_t method, int32_t batch, const float* dout, float* din,
+                             double* workspace, void* stream);
+int dlx_interp_points_vjp(int32_t n_in, double pixel_scale_in, double center_in, int64_t npts,
+                          const double* qx, const double* qy, int32_t method, int32_t batch,
+                          const float* dout, float* din, double* workspace, void* stream);
+
+/*
  * Zernike basis of an aberrated dynamic aperture (AberratedAperture.calc_basis,
  * layers/apertures.py:735-752; DynamicZernikeBasis, polynomials.py:85-115): the
  * pixel coordinates put through the transforms, divided by `extent`, then

@@ -103,3 +103,120 @@ def test_resampling_api_vs_oracle(dl):
                                     dict(compression=[r, r]), "linear", 0.0))
     assert wf2.npixels == 24 and np.isclose(float(np.ravel(wf2.pixel_scale)[0]), 1.5 / n)
     assert _err(wf2.phasor.cpu().numpy(), ref) < TOL
+
+
+def _sampling_matrix(n_in, d_in, c_in, xs, ys, method):
+    """The oracle's sampling rule as a matrix: column k is the resampling of
+    the k-th unit image (fill 0)."""
+    cols = []
+    for k in range(n_in * n_in):
+        e = np.zeros(n_in * n_in)
+        e[k] = 1.0
+        cols.append(orc.interp_grid(e.reshape(n_in, n_in), n_in, d_in, c_in, xs, ys, method,
+                                    0.0).ravel())
+    return np.array(cols).T
+
+
+@pytest.mark.parametrize("method", ["linear", "nearest"])
+def test_resampling_gradient_is_the_transposed_sampling(dl, method):
+    """Gradients through the resampling (dlx_interp_transform_vjp,
+    dlx_interp_points_vjp): equal to S^T dout with S the oracle's sampling
+    matrix (1e-6 of the largest entry), for a transform chain and for
+    explicit points, over a batch; the dot-product identity <S x, y> =
+    <x, S^T y> at a larger size (1e-5 relative)."""
+    print(f"[interp vjp] {method}", flush=True)
+    from dlux_amd import runtime
+    rng = np.random.default_rng(8)
+    n_in, d_in, c_in, n_out, d_out = 9, 0.5, 0.1, 7, 0.6
+    tf = dl.CoordTransform(**TF)
+    xo = orc.spec_xs(n_out, d_out, 0.05)
+    q = orc.coord_transform(np.array(np.meshgrid(xo, xo, indexing="xy")), **TF)
+    S = _sampling_matrix(n_in, d_in, c_in, q[0], q[1], method)
+    x = torch.tensor(rng.normal(size=(2, n_in, n_in)), dtype=torch.float32, device="cuda",
+                     requires_grad=True)
+    y = rng.normal(size=(2, n_out, n_out))
+    out = runtime.interp_transform(x, d_in, c_in, n_out, d_out, 0.05, [tf._struct()], method, 0.0)
+    (out * torch.as_tensor(y, dtype=torch.float32, device="cuda")).sum().backward()
+    ref = np.stack([S.T @ y[b].ravel() for b in range(2)]).reshape(2, n_in, n_in)
+    err = np.max(np.abs(x.grad.double().cpu().numpy() - ref)) / np.max(np.abs(ref))
+    print(f"  transform: err {err:.2e}")
+    assert err < 1e-6
+    # explicit points (some outside the span), through the same rule
+    px = rng.uniform(-3, 3, size=(5, 6))
+    py = rng.uniform(-3, 3, size=(5, 6))
+    S = _sampling_matrix(n_in, d_in, c_in, px, py, method)
+    x.grad = None
+    out = runtime.interp_points(x, d_in, c_in, px, py, method, -1.0)
+    yp = rng.normal(size=(2, 5, 6))
+    (out * torch.as_tensor(yp, dtype=torch.float32, device="cuda")).sum().backward()
+    ref = np.stack([S.T @ yp[b].ravel() for b in range(2)]).reshape(2, n_in, n_in)
+    err = np.max(np.abs(x.grad.double().cpu().numpy() - ref)) / np.max(np.abs(ref))
+    print(f"  points: err {err:.2e}")
+    assert err < 1e-6
+    # dot-product identity at 256^2 -> 200^2
+    big = torch.tensor(rng.normal(size=(256, 256)), dtype=torch.float32, device="cuda",
+                       requires_grad=True)
+    yb = torch.tensor(rng.normal(size=(200, 200)), dtype=torch.float32, device="cuda")
+    ob = runtime.interp_transform(big, 1.0, 0.0, 200, 1.1, 0.0, [tf._struct()], method, 0.0)
+    (ob * yb).sum().backward()
+    lhs = float((ob.detach().double() * yb.double()).sum())
+    rhs = float((big.detach().double() * big.grad.double()).sum())
+    print(f"  <Sx,y> {lhs:.6e} <x,S^T y> {rhs:.6e}")
+    assert abs(lhs - rhs) <= 1e-5 * max(abs(lhs), 1.0)
+
+
+def test_resampling_gradients_through_the_api(dl):
+    """PSF.interpolate, ApplyInterpolation in a LayeredDetector (then a
+    downsample) and Wavefront.rotate carry gradients to their images; a
+    tracked transformation raises."""
+    print("[interp vjp] API", flush=True)
+    from dlux_amd import layers
+    rng = np.random.default_rng(9)
+    n = 16
+    data = torch.tensor(rng.uniform(0, 1, (n, n)), dtype=torch.float32, device="cuda",
+                        requires_grad=True)
+    tfd = dict(rotation=0.3, translation=[0.1, -0.05])
+    xs = orc.spec_xs(n, 0.25)
+    q = orc.coord_transform(np.array(np.meshgrid(xs, xs, indexing="xy")), **tfd)
+    S = _sampling_matrix(n, 0.25, 0.0, q[0], q[1], "linear")
+    psf = dl.PSF(data, 0.25)
+    w = rng.normal(size=(n, n))
+    (psf.interpolate(dl.CoordTransform(**tfd)).data
+     * torch.as_tensor(w, dtype=torch.float32, device="cuda")).sum().backward()
+    ref = (S.T @ w.ravel()).reshape(n, n)
+    assert np.max(np.abs(data.grad.double().cpu().numpy() - ref)) / np.max(np.abs(ref)) < 1e-6
+    data.grad = None
+    det = dl.LayeredDetector([layers.ApplyInterpolation(dl.CoordTransform(**tfd)),
+                              layers.Downsample(2)])
+    w2 = rng.normal(size=(n // 2, n // 2))
+    (det(dl.PSF(data, 0.25)) * torch.as_tensor(w2, dtype=torch.float32, device="cuda")).sum() \
+        .backward()
+    ref = (S.T @ np.kron(w2, np.ones((2, 2))).ravel()).reshape(n, n)
+    assert np.max(np.abs(data.grad.double().cpu().numpy() - ref)) / np.max(np.abs(ref)) < 1e-6
+    ph = torch.tensor(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)),
+                      dtype=torch.complex64, device="cuda", requires_grad=True)
+    wf = dl.Wavefront(1e-6, n, 1.0).set(phasor=ph)
+    wf.rotate(0.2).psf.sum().backward()
+    assert ph.grad is not None and torch.isfinite(ph.grad).all()
+    rot = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+    with pytest.raises(NotImplementedError):
+        psf.interpolate(dl.CoordTransform(rotation=rot))
+
+
+def test_intensity_gradient(dl):
+    """dlx_intensity carries gradients to the fields: dL/dE = 2 E dL/dI, summed
+    over the batch or per plane (1e-6 relative)."""
+    print("[intensity vjp]", flush=True)
+    from dlux_amd import runtime
+    rng = np.random.default_rng(10)
+    e = rng.normal(size=(3, 12, 12)) + 1j * rng.normal(size=(3, 12, 12))
+    for sum_batch in (True, False):
+        E = torch.tensor(e, dtype=torch.complex64, device="cuda", requires_grad=True)
+        g = rng.normal(size=(12, 12) if sum_batch else (3, 12, 12))
+        I = runtime.intensity(E, sum_batch=sum_batch)
+        ref_i = (np.abs(e) ** 2).sum(0) if sum_batch else np.abs(e) ** 2
+        assert np.max(np.abs(I.detach().double().cpu().numpy() - ref_i)) < 1e-5 * ref_i.max()
+        (I * torch.as_tensor(g, dtype=torch.float32, device="cuda")).sum().backward()
+        ref = 2 * e * (g[None] if sum_batch else g)
+        err = np.max(np.abs(E.grad.cpu().numpy() - ref)) / np.max(np.abs(ref))
+        assert err < 1e-6, (sum_batch, err)

@@ -670,3 +670,24 @@ def test_distorted_coords_host():
     assert np.max(np.abs(np.asarray(out) - ref)) < 1e-6
     with pytest.raises(TypeError):
         _chain_coords((object(),), 4, 1.0, 0.0, "cpu")
+
+
+def test_interp_vjp_abi_validation_without_gpu(lib):
+    """dlx_interp_transform_vjp / dlx_interp_points_vjp reject bad arguments
+    before any launch; an empty batch is a no-op."""
+    from dlux_amd import _lib
+    tf = (_lib.CoordTransformC * 1)()
+    assert lib.dlx_interp_transform_vjp(0, 1.0, 0.0, 4, 1.0, 0.0, 0, tf, 1, 1, None, None, None,
+                                        None) != 0
+    assert lib.dlx_interp_transform_vjp(4, 1.0, 0.0, 4, 1.0, 0.0, 0, tf, 2, 1, None, None, None,
+                                        None) != 0
+    assert lib.dlx_interp_transform_vjp(4, 1.0, 0.0, 4, 1.0, 0.0, 0, tf, 1, 1, None, None, None,
+                                        None) != 0
+    assert lib.dlx_interp_transform_vjp(4, 1.0, 0.0, 4, 1.0, 0.0, 0, tf, 1, 0, None, None, None,
+                                        None) == 0
+    assert lib.dlx_interp_points_vjp(4, -1.0, 0.0, 3, None, None, 1, 1, None, None, None,
+                                     None) != 0
+    assert lib.dlx_interp_points_vjp(4, 1.0, 0.0, 3, None, None, 1, 1, None, None, None,
+                                     None) != 0
+    assert lib.dlx_interp_points_vjp(4, 1.0, 0.0, 3, None, None, 1, 0, None, None, None,
+                                     None) == 0
