@@ -115,6 +115,19 @@ def _bcast(v, n: int) -> np.ndarray:
 # ---------------------------------------------------------------------------
 
 
+def _untracked_inputs(name: str, *tensors) -> None:
+    """Raise when an input of a non-differentiable entry point requires grad
+    while gradients are recorded (instead of returning a result without a
+    gradient path)."""
+    if not torch.is_grad_enabled():
+        return
+    for t in tensors:
+        if isinstance(t, torch.Tensor) and t.requires_grad:
+            raise NotImplementedError(
+                f"{name} is not differentiable; gradients flow through the fused paths "
+                "(psf_poly, soft apertures, basis evaluation, detector, resampling, intensity)")
+
+
 def transfer_matrix_T(wavelengths, n_in, pixel_scales_in, n_out, pixel_scales_out,
                       shifts=0.0, focal_length=None, focal_shift=0.0, inverse=False):
     """(B, N_out, N_in) complex64: transposed transfer matrices for B instances."""
@@ -134,6 +147,7 @@ def mft(fields: torch.Tensor, wavelengths, pixel_scales_in, n_out: int, pixel_sc
         focal_length=None, shift_x=0.0, shift_y=0.0, inverse=False, field_scale=None,
         precision: str | None = None) -> torch.Tensor:
     """Batched MFT of (B, N, N) complex64 device fields -> (B, n_out, n_out)."""
+    _untracked_inputs("mft", fields)
     f = as_complex(fields)
     squeeze = f.ndim == 2
     if squeeze:
@@ -159,6 +173,7 @@ def mft(fields: torch.Tensor, wavelengths, pixel_scales_in, n_out: int, pixel_sc
 def pupil_field(n: int, wavelengths, pixel_scale: float, center=0.0, tilt=(0.0, 0.0),
                 amp=None, opd=None, phase=None, scale=1.0, field_in=None) -> torch.Tensor:
     """(B, n, n) complex64: s * amp * exp(i(k_b (opd + tilt) + phase)) [* field_in]."""
+    _untracked_inputs("pupil_field", amp, opd, phase, field_in)
     wl = np.atleast_1d(np.asarray(wavelengths, np.float64))
     b = wl.size
     dev = device()
@@ -269,6 +284,7 @@ def eval_basis(basis, coeffs, base=None) -> torch.Tensor:
 
 def power(fields: torch.Tensor) -> torch.Tensor:
     """Per-instance sum |E|^2 of (B, ...) complex64 fields -> (B,) float32."""
+    _untracked_inputs("power", fields)
     f = as_complex(fields)
     b = f.shape[0] if f.ndim == 3 else 1
     npix = f.numel() // b
@@ -466,6 +482,7 @@ class MftBatch:
         self._keep = []
         nout_of = {}
         for i, d in enumerate(instances):
+            _untracked_inputs("mft_batched_psf", d["field"])
             f = as_complex(d["field"], self.dev)
             if f.ndim != 2 or f.shape[0] != f.shape[1]:
                 raise ValueError(f"instance {i}: field must be square, got {tuple(f.shape)}")
@@ -513,6 +530,7 @@ class MftBatch:
         self = cls.__new__(cls)
         self.dev = device()
         self.lib = _lib.load()
+        _untracked_inputs("mft_batched_psf", *fields)
         self._keep = [as_complex(f, self.dev) for f in fields]
         for f in self._keep:
             if f.ndim not in (2, 3) or f.shape[-1] != f.shape[-2]:
@@ -1013,6 +1031,7 @@ def fft2(fields, m: int, inverse: bool = False) -> torch.Tensor:
     MFT's transfer matrices are the centred DFT matrices and its nfringes
     normalisation is 1/m.
     """
+    _untracked_inputs("fft2", fields)
     f = as_complex(fields)
     squeeze = f.ndim == 2
     if squeeze:
@@ -1037,6 +1056,7 @@ def fft2(fields, m: int, inverse: bool = False) -> torch.Tensor:
 
 def asm_prop(fields, wavelengths, pixel_scale: float, distance: float, m: int) -> torch.Tensor:
     """Angular-spectrum propagation of (B, N, N) fields padded to (B, m, m)."""
+    _untracked_inputs("asm_prop", fields)
     f = as_complex(fields)
     squeeze = f.ndim == 2
     if squeeze:
@@ -1058,6 +1078,7 @@ def lct(fields, wavelengths, abcd, pixel_scale_in: float, center_in: float, n_ou
         precision: str | None = None) -> torch.Tensor:
     """Collins-integral LCT of (B, N, N) fields to (B, n_out, n_out) on the
     given output grid (dlx_lct)."""
+    _untracked_inputs("lct", fields)
     f = as_complex(fields)
     squeeze = f.ndim == 2
     if squeeze:
@@ -1082,6 +1103,7 @@ def lct_fft(fields, wavelengths, abcd, pixel_scale_in: float, center_in: float, 
             center_out: float = 0.0):
     """FFT-grid LCT of (B, N, N) fields to (B, m, m) (dlx_lct_fft); returns
     (fields, output pixel scale per field)."""
+    _untracked_inputs("lct_fft", fields)
     f = as_complex(fields)
     squeeze = f.ndim == 2
     if squeeze:

@@ -220,3 +220,25 @@ def test_intensity_gradient(dl):
         ref = 2 * e * (g[None] if sum_batch else g)
         err = np.max(np.abs(E.grad.cpu().numpy() - ref)) / np.max(np.abs(ref))
         assert err < 1e-6, (sum_batch, err)
+
+
+def test_non_differentiable_entry_points_raise_on_tracked_inputs(dl):
+    """The unfused MFT, FFT, ASM, LCT, power, pupil-field and batched-MFT
+    entry points raise on an input that requires grad (rather than return a
+    result with no gradient path); under no_grad they run."""
+    print("[untracked inputs]", flush=True)
+    from dlux_amd import runtime
+    e = torch.ones((1, 16, 16), dtype=torch.complex64, device="cuda", requires_grad=True)
+    calls = [lambda: runtime.mft(e, 1e-6, 1 / 16, 8, 1e-7),
+             lambda: runtime.fft2(e, 32),
+             lambda: runtime.asm_prop(e, 1e-6, 1 / 16, 1e-3, 32),
+             lambda: runtime.power(e),
+             lambda: runtime.pupil_field(16, 1e-6, 1 / 16, opd=e.real[0]),
+             lambda: runtime.mft_batched_psf([dict(field=e[0], wavelength=1e-6,
+                                                   pixel_scale_in=1 / 16, n_out=8,
+                                                   pixel_scale_out=1e-7)])]
+    for f in calls:
+        with pytest.raises(NotImplementedError):
+            f()
+        with torch.no_grad():
+            f()
