@@ -127,8 +127,8 @@ hipError_t launch_dft_split4(const DftChunk& c, int nb, int n_in, int n_out, int
 
 // Pupil field in split4 (rows y, k = x).  A quad of lanes produces one chunk
 // (8 pixels): lane q loads pixels 2q, 2q+1 of amplitude / OPD / phase once and
-// emits them for up to 8 wavelengths with quad-cooperative 16-byte stores.
-constexpr int kPupilWl = 8;
+// emits them for up to 16 wavelengths with quad-cooperative 16-byte stores.
+constexpr int kPupilWl = 16;
 
 __global__ void __launch_bounds__(256)
 pupil_split4_kernel(PupilChunk c, int nb, int n, double dx, double x0, double tx, double ty,
