@@ -267,6 +267,64 @@ def test_tilt_by_whole_pixels_shifts_psf():
     assert np.allclose(b[:-3 or None, 2:][0:21, 0:22], a[3:, :-2][0:21, 0:22], atol=1e-15)
 
 
+def _ps_case():
+    mask, basis = orc.circular_aperture(32, 1.0, oversample=1, zernike_nolls=range(2, 8))
+    c = np.random.default_rng(9).normal(0, 40e-9, basis.shape[0])
+    return mask, basis, c
+
+
+def test_point_sources_power_is_total_flux():
+    """PointSources.model (sources.py:392-411): weights normalised to 1, times each
+    source's flux, summed over sources.  At FFT sampling (p_out = lambda / (N dx),
+    N_out = N) the MFT is a scaled DFT, so by Parseval each unit-power pupil puts
+    exactly its weight into the image whatever its tilt: the image sums to the
+    total flux.  Unnormalised weights, a flux applied to the field instead of the
+    intensity, or a missing flux each change this sum."""
+    mask, basis, c = _ps_case()
+    n, wl = 32, 1e-6
+    p = wl / (n * (1.0 / n))
+    positions = [(0.0, 0.0), (0.37 * p, -1.2 * p), (-2.6 * p, 0.9 * p)]
+    fluxes = [1.0, 250.0, 3.5e4]
+    img = orc.point_sources_psf(mask, basis, c, [wl] * 3, [1.0, 2.0, 3.0], positions, fluxes,
+                                1.0, n, p)
+    assert np.isclose(img.sum(), sum(fluxes), rtol=1e-12)
+
+
+def test_point_sources_colocated_and_unnormalised_weights():
+    """Co-located sources add up to (sum of fluxes) x the single-source PSF with
+    normalised weights; weights that are a multiple of each other give the same
+    image (spectra.py:84-92 normalises them; tests/test_sources.py:87-90 asserts
+    weights sum to 1)."""
+    mask, basis, c = _ps_case()
+    wls = np.array([0.9e-6, 1.0e-6, 1.1e-6])
+    w = np.array([0.5, 1.5, 2.0])
+    pos = (1e-7, -2e-7)
+    single = orc.poly_psf(mask, basis, c, wls, w / w.sum(), 1.0, 20, 3e-7, offset=pos)
+    both = orc.point_sources_psf(mask, basis, c, wls, w, [pos, pos], [3.0, 7.0], 1.0, 20,
+                                 3e-7)
+    assert np.allclose(both, 10.0 * single, rtol=1e-12, atol=0)
+    scaled = orc.point_sources_psf(mask, basis, c, wls, 17.0 * w, [pos, pos], [3.0, 7.0],
+                                   1.0, 20, 3e-7)
+    assert np.allclose(scaled, both, rtol=1e-12, atol=0)
+
+
+def test_point_sources_whole_pixel_offsets_shift_each_image():
+    """A source at (i p, j p) is the on-axis source's image moved by (i, j) pixels
+    (+x to larger column index, +y to larger row index); two such sources add."""
+    mask, basis, c = _ps_case()
+    p, n_out = 2.5e-7, 24
+    wls, w = [0.95e-6, 1.05e-6], [1.0, 1.0]
+    centre = orc.point_sources_psf(mask, basis, c, wls, w, [(0.0, 0.0)], [1.0], 1.0, n_out, p)
+    img = orc.point_sources_psf(mask, basis, c, wls, w, [(2 * p, -3 * p), (-1 * p, 1 * p)],
+                                [2.0, 5.0], 1.0, n_out, p)
+    expect = np.zeros_like(centre)
+    expect[0:n_out - 3, 2:] += 2.0 * centre[3:, :n_out - 2]
+    expect[1:, 0:n_out - 1] += 5.0 * centre[:n_out - 1, 1:]
+    # compare away from the borders, where parts of the shifted images left the window
+    inner = (slice(4, n_out - 4), slice(4, n_out - 4))
+    assert np.allclose(img[inner], expect[inner], rtol=1e-9, atol=1e-12 * centre.max())
+
+
 def test_gradient_matches_central_differences():
     rng = np.random.default_rng(8)
     mask, basis = orc.circular_aperture(32, 1.0, oversample=1, zernike_nolls=range(2, 7))
@@ -284,6 +342,34 @@ def test_gradient_matches_central_differences():
         lm = orc.gaussian_loss(orc.poly_psf(mask, basis, cm, wls, w, 1.0, 16, 3e-7), data, 1e4)
         fd.append((lp - lm) / (2 * h))
     assert np.allclose(grad, fd, rtol=1e-5)
+
+
+def test_gradient_at_grad_workload_matches_central_differences():
+    """The adjoint gradient at the full phase-retrieval size (512^2 pupil, 36
+    modes ~ N(0, 50 nm), 16 wavelengths on 0.9-1.1 um, 128^2 output, 1e6-photon
+    Poisson target from separate coefficients) against central differences at
+    h = 1e-3 nm for three coefficients (tip, defocus and the last mode)."""
+    from dlux_amd import workloads
+    cfg = workloads.GRAD
+    mask = orc.circular_aperture(cfg.n_in, 1.0, oversample=1)
+    basis = orc.zernike_basis(cfg.js, orc.pixel_coords(cfg.n_in, 1.0), 1.0)
+    flux = cfg.extra["photons"]
+    truth = cfg.coefficients(seed=cfg.extra["target_seed"])
+    args = (cfg.wavelengths, cfg.weights, 1.0, cfg.n_out, cfg.p_out)
+    data = np.random.default_rng(cfg.extra["noise_seed"]).poisson(
+        flux * orc.poly_psf(mask, basis, truth, *args)).astype(float)
+    c = cfg.coefficients()
+    _, grad = orc.poly_psf_grad(mask, basis, c, *args, data, flux)
+    h = 1e-3 * 1e-9
+    for j in (0, 2, len(c) - 1):
+        cp, cm = c.copy(), c.copy()
+        cp[j] += h
+        cm[j] -= h
+        lp = orc.gaussian_loss(orc.poly_psf(mask, basis, cp, *args), data, flux)
+        lm = orc.gaussian_loss(orc.poly_psf(mask, basis, cm, *args), data, flux)
+        fd = (lp - lm) / (2 * h)
+        assert np.isclose(grad[j], fd, rtol=1e-5, atol=1e-6 * np.max(np.abs(grad))), (j, grad[j],
+                                                                                     fd)
 
 
 # -------------------------------------------------------------------- ASM
@@ -625,6 +711,17 @@ def test_interp_grid_against_scipy_and_linear_functions():
             assert np.allclose(got[inside], ref[inside])
         else:
             assert np.allclose(got, ref, atol=1e-12)
+    # exact half-pixel samples take the upper knot on both axes (interpax keeps the
+    # right neighbour unless the left one is strictly nearer); parity at ties with
+    # interpax itself is unpinned (interpax is absent)
+    n2, d2 = 8, 0.5
+    img2 = np.arange(n2 * n2, dtype=float).reshape(n2, n2)
+    kn2 = orc.spec_xs(n2, d2, 0.0)
+    half = kn2[:-1] + d2 / 2
+    got = orc.interp_grid(img2, n2, d2, 0.0, half, half[::-1], "nearest")
+    rows = np.arange(n2 - 1, 0, -1)
+    cols = np.arange(1, n2)
+    assert np.array_equal(got, img2[rows, cols])
     X, Y = np.meshgrid(knots, knots)
     lin = 2.0 * X - 3.0 * Y + 0.5
     got = orc.interp_grid(lin, n, d, c, xs, ys, "linear", fill=0.0)
