@@ -342,7 +342,7 @@ interp_kernel(TfChain ch, int n_out, double d_out, double c_out, int n_in, doubl
     return;
   }
   if (method == 0) {
-    const int c = min((int)rint(u), n_in - 1), r = min((int)rint(v), n_in - 1);
+    const int c = min((int)floor(u + 0.5), n_in - 1), r = min((int)floor(v + 0.5), n_in - 1);
     for (int b = 0; b < batch; ++b) out[(long)b * total + p] = in[(long)b * nn + (long)r * n_in + c];
     return;
   }
@@ -376,7 +376,7 @@ interp_points_kernel(long npts, const double* __restrict__ qx, const double* __r
     if (!inside) {
       r = fill;
     } else if (method == 0) {
-      r = f[(long)min((int)rint(v), n_in - 1) * n_in + min((int)rint(u), n_in - 1)];
+      r = f[(long)min((int)floor(v + 0.5), n_in - 1) * n_in + min((int)floor(u + 0.5), n_in - 1)];
     } else {
       const int c0 = min((int)floor(u), n_in > 1 ? n_in - 2 : 0);
       const int r0 = min((int)floor(v), n_in > 1 ? n_in - 2 : 0);
@@ -429,7 +429,7 @@ interp_vjp_kernel(TfChain ch, int n_out, double d_out, double c_out, const doubl
   if (!(u >= 0.0 && u <= n_in - 1 && v >= 0.0 && v <= n_in - 1)) return;
   const long nn = (long)n_in * n_in;
   if (method == 0) {
-    const long k = (long)min((int)rint(v), n_in - 1) * n_in + min((int)rint(u), n_in - 1);
+    const long k = (long)min((int)floor(v + 0.5), n_in - 1) * n_in + min((int)floor(u + 0.5), n_in - 1);
     for (int b = 0; b < batch; ++b) atomicAdd(acc + (long)b * nn + k, (double)dout[(long)b * npts + p]);
     return;
   }

@@ -904,7 +904,11 @@ def interp_grid(image, n, d, c, xs, ys, method="linear", fill=0.0):
     inside = (u >= 0) & (u <= n - 1) & (v >= 0) & (v <= n - 1)
     uc, vc = np.clip(u, 0, n - 1), np.clip(v, 0, n - 1)
     if method == "nearest":
-        out = image[np.rint(vc).astype(int), np.rint(uc).astype(int)]
+        # the nearer knot; at an exact half-pixel tie the upper one (interpax's
+        # nearest rule keeps the right neighbour unless the left is strictly nearer)
+        rn = np.minimum(np.floor(vc + 0.5).astype(int), n - 1)
+        cn = np.minimum(np.floor(uc + 0.5).astype(int), n - 1)
+        out = image[rn, cn]
     else:
         c0 = np.minimum(np.floor(uc).astype(int), max(n - 2, 0))
         r0 = np.minimum(np.floor(vc).astype(int), max(n - 2, 0))

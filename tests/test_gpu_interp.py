@@ -65,6 +65,29 @@ def test_interp_transform_and_points_vs_oracle(dl, method):
         runtime.interp_points(torch.zeros((4, 4)), 1.0, 0.0, [0.0], [0.0], "cubic")
 
 
+def test_nearest_takes_the_upper_knot_at_half_pixel_ties(dl):
+    """Samples exactly half-way between knots (a 2x upsampling by an integer
+    grid shift) pick the upper knot on both axes, in the points kernel and the
+    grid kernel alike."""
+    print("[interp] nearest ties", flush=True)
+    from dlux_amd import runtime
+    n, d = 8, 0.5
+    img = np.arange(n * n, dtype=np.float32).reshape(n, n)
+    half = orc.spec_xs(n, d, 0.0)[:-1] + d / 2
+    xs, ys = np.meshgrid(half, half[::-1])
+    got = runtime.interp_points(torch.as_tensor(img), d, 0.0, xs, ys, "nearest", fill=-1.0)
+    ref = orc.interp_grid(img, n, d, 0.0, xs, ys, "nearest", -1.0)
+    assert np.array_equal(got.cpu().numpy(), ref.astype(np.float32))
+    assert np.array_equal(ref[0], img[n - 1, 1:])
+    # grid form: n - 1 output knots centred between the input ones
+    got = runtime.interp_transform(torch.as_tensor(img[None]), d, 0.0, n - 1, d, d / 2, [],
+                                   "nearest", fill=-1.0)
+    xo = orc.spec_xs(n - 1, d, d / 2)
+    gx, gy = np.meshgrid(xo, xo)
+    ref = orc.interp_grid(img, n, d, 0.0, gx, gy, "nearest", -1.0)
+    assert np.array_equal(got[0].cpu().numpy(), ref.astype(np.float32))
+
+
 def test_resampling_api_vs_oracle(dl):
     """PSF.rotate / interpolate, Wavefront.rotate / scale_to / interpolate and
     the two layers, each against the oracle rule on the same grids."""
