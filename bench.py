@@ -11,8 +11,9 @@ two MFMA contractions and the weighted |E|^2 sum.  Inputs are resident in HBM.
 
 Both arithmetic forms are timed (complex64/FP32 MFMA and split-BF16x3 MFMA);
 ``value`` is the split-BF16x3 rate when its peak-normalised error against the
-FP64 oracle is <= 1e-5 (otherwise the FP32 rate).  The FP64 oracle run on the
-host cores is the error reference and the CPU baseline.
+FP64 oracle is <= 1e-5 (otherwise the FP32 rate).  The FP64 oracle is the
+error reference; the CPU baseline is an FP32 NumPy restatement of dLux's path
+(oracle/dlux_fp32.py) timed on the host cores at every workload.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -42,8 +43,8 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0,
-                    help="budget for the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="budget for the CPU baseline samples")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-detector", action="store_true", help="skip the detector-chain leg")
@@ -79,33 +80,108 @@ def _time_form(torch, runtime, base_sys, coeff_sets, cfg, prec, steps, warmup):
     return (t1 - t0) * 1e3 / steps, s1, s2
 
 
-def _cpu_baseline(cfg, coeffs0, budget_s):
-    """FP64 oracle on the host cores: the error reference and the CPU baseline."""
+def _fp64_reference(cfg, coeffs0):
+    """The FP64 oracle's PSF of the headline workload: the error reference."""
     from oracle import dlux_oracle as orc
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max((p.get("num_threads", 1) for p in threadpool_info()), default=1)
-    except Exception:  # pragma: no cover
-        threads = os.cpu_count() or 1
     width = cfg.spider_px * 1.0 / cfg.n_in
     mask = orc.circular_aperture(cfg.n_in, 1.0, oversample=1, spider_width=width,
                                  spider_angles=list(cfg.spider_angles))
     basis = orc.zernike_basis(cfg.js, orc.pixel_coords(cfg.n_in, 1.0), 1.0)
-    times, ref = [], None
+    return orc.poly_psf(mask, basis, coeffs0, cfg.wavelengths, cfg.weights, 1.0, cfg.n_out,
+                        cfg.p_out)
+
+
+def _median_rate(fn, reps, budget_s, per=1.0):
+    """`per` units / median seconds of `fn` over up to `reps` timed runs after one
+    warmup run (fewer runs when the budget runs out, at least 3)."""
+    fn()
+    times = []
     t_start = time.perf_counter()
-    k = 0
-    while True:
-        c = coeffs0 if k == 0 else cfg.coefficients(seed=1000 + k)
+    while len(times) < reps:
         t0 = time.perf_counter()
-        psf = orc.poly_psf(mask, basis, c, cfg.wavelengths, cfg.weights, 1.0, cfg.n_out,
-                           cfg.p_out)
+        fn()
         times.append(time.perf_counter() - t0)
-        if k == 0:
-            ref = psf
-        k += 1
-        if time.perf_counter() - t_start > budget_s or k >= 20:
+        if len(times) >= 3 and time.perf_counter() - t_start > budget_s:
             break
-    return ref, len(times) / sum(times), threads, len(times)
+    return per / float(np.median(times)), len(times)
+
+
+def _cpu_baseline(workloads, budget_s):
+    """FP32 NumPy restatement of dLux's path (oracle/dlux_fp32.py: dLux's default
+    FP32 arithmetic and operation order) timed on this host's cores, per workload:
+    mono, poly (the headline's unit, PSF/s), grad (steps/s) and the uneven batch
+    (instances/s per (N_in, N_out) group, and the 4096-instance batch they imply).
+    Median of up to 7 runs after a warmup, each workload within its share of the
+    budget."""
+    from oracle import dlux_fp32 as f32
+    from oracle import dlux_oracle as orc
+    info = {"blas": None, "blas_threads": None}
+    try:
+        from threadpoolctl import threadpool_info
+        blas = [p for p in threadpool_info() if p.get("user_api") == "blas"]
+        if blas:
+            info = {"blas": f"{blas[0].get('internal_api')} {blas[0].get('version')}",
+                    "blas_threads": int(blas[0].get("num_threads", 1))}
+    except Exception:  # pragma: no cover
+        pass
+
+    def inputs(cfg):
+        width = cfg.spider_px * 1.0 / cfg.n_in if cfg.spider_px else None
+        angles = list(cfg.spider_angles) if cfg.spider_px else None
+        mask = orc.circular_aperture(cfg.n_in, 1.0, oversample=1, spider_width=width,
+                                     spider_angles=angles).astype(np.float32)
+        basis = np.nan_to_num(orc.zernike_basis(cfg.js, orc.pixel_coords(cfg.n_in, 1.0), 1.0))
+        return mask, basis.astype(np.float32)
+
+    share = budget_s / 4.0
+    out = {}
+    for cfg in (workloads.MONO, workloads.POLY):
+        mask, basis = inputs(cfg)
+        c = cfg.coefficients()
+        rate, n = _median_rate(lambda: f32.poly_psf(mask, basis, c, cfg.wavelengths,
+                                                    cfg.weights, 1.0, cfg.n_out, cfg.p_out),
+                               7, share)
+        out[cfg.name] = {"psf_per_s": rate, "runs": n}
+    cfg = workloads.GRAD
+    mask, basis = inputs(cfg)
+    truth = cfg.coefficients(seed=cfg.extra["target_seed"])
+    flux = cfg.extra["photons"]
+    model = f32.poly_psf(mask, basis, truth, cfg.wavelengths, cfg.weights, 1.0, cfg.n_out,
+                         cfg.p_out)
+    data = np.random.default_rng(cfg.extra["noise_seed"]).poisson(
+        flux * np.clip(model, 0, None)).astype(np.float32)
+    c = cfg.coefficients()
+    rate, n = _median_rate(lambda: f32.poly_psf_grad(mask, basis, c, cfg.wavelengths,
+                                                     cfg.weights, 1.0, cfg.n_out, cfg.p_out,
+                                                     data, flux), 7, share)
+    out["grad"] = {"steps_per_s": rate, "runs": n}
+    bc = workloads.BATCH
+    sources = bc.sources()
+    groups = {}
+    rng = np.random.default_rng(bc.seed)
+    for n_in in bc.n_in_choices:
+        m = orc.circular_aperture(n_in, 1.0, oversample=1)
+        field = (m * np.exp(1j * rng.uniform(0, 0.3, (n_in, n_in)))).astype(np.complex64)
+        field /= np.sqrt(np.sum(np.abs(field) ** 2))
+        for n_out in bc.n_out_choices:
+            rate, n = _median_rate(lambda: f32.mft_intensity(field, 1e-6, 1.0 / n_in, n_out,
+                                                             bc.p_out, (0.7, -1.3)),
+                                   7, share / 9)
+            groups[f"{n_in}->{n_out}"] = {"instances_per_s": rate, "runs": n}
+    batch_s = sum(bc.n_wl / groups[f"{s['n_in']}->{s['n_out']}"]["instances_per_s"]
+                  for s in sources)
+    out["uneven"] = {"groups": groups, "batch_s": batch_s,
+                     "psf_per_s": len(sources) / batch_s,
+                     "instances_per_s": len(sources) * bc.n_wl / batch_s}
+    threads = info["blas_threads"] or (os.cpu_count() or 1)
+    cpu = {"value": out["poly"]["psf_per_s"], "unit": "PSF/s", "cores": int(threads),
+           "kind": "port",
+           "sample": (f"CPU restatement of dLux (FP32 NumPy, dLux's operation order; "
+                      f"{info['blas']} BLAS, {threads} threads): median of "
+                      f"{out['poly']['runs']} full 64-lambda PSFs of the same workload"),
+           "cpu_count": os.cpu_count(), "blas": info["blas"],
+           "blas_threads": info["blas_threads"], "workloads": out}
+    return cpu
 
 
 def _grad_leg(torch, runtime, workloads, prec, steps, warmup, with_oracle):
@@ -458,14 +534,13 @@ def main():
     cpu = None
     errs = {p: None for p in psfs}
     if not args.no_cpu:
-        print("[bench] FP64 oracle on the host (error reference + CPU baseline)",
-              file=sys.stderr, flush=True)
-        ref, cpu_rate, threads, nsamp = _cpu_baseline(cfg, coeffs0, args.cpu_seconds)
+        print("[bench] FP64 oracle on the host (error reference)", file=sys.stderr, flush=True)
+        ref = _fp64_reference(cfg, coeffs0)
         for p in psfs:
             errs[p] = float(np.max(np.abs(psfs[p] - ref)) / np.max(ref))
-        cpu = {"value": cpu_rate, "unit": "PSF/s", "cores": int(threads), "kind": "port",
-               "sample": f"{nsamp} full PSFs of the same workload, FP64 NumPy oracle "
-                         f"(numpy BLAS, {threads} threads)"}
+        print("[bench] CPU baseline: FP32 restatement on the host cores", file=sys.stderr,
+              flush=True)
+        cpu = _cpu_baseline(workloads, args.cpu_seconds)
 
     print("[bench] gradient leg", file=sys.stderr, flush=True)
     # the step is host-bound: a longer warmup lets the host side settle (the first
