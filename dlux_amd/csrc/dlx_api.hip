@@ -306,14 +306,15 @@ int dlx_mft(const dlx_mft_geom* geom, int32_t batch, const void* in, void* out,
     c[b] = mft_norm(g, wavelength[b], pixel_scale_in[b], pixel_scale_out[b]) *
            (field_scale ? field_scale[b] : 1.0);
 
-  void *Wx = nullptr, *Wy = nullptr, *hx = nullptr, *hy = nullptr;
+  void *Wx = nullptr, *Wy = nullptr;
+  CacheLease hx(s), hy(s);
   if (int st = get_matrices(g, batch, wavelength, pixel_scale_in, pixel_scale_out, shift_x,
-                            split, Wx_ws, s, &Wx, &hx))
+                            split, Wx_ws, s, &Wx, &hx.h))
     return st;
   if (same) {
     Wy = Wx;
   } else if (int st = get_matrices(g, batch, wavelength, pixel_scale_in, pixel_scale_out,
-                                   shift_y, split, Wy_ws, s, &Wy, &hy)) {
+                                   shift_y, split, Wy_ws, s, &Wy, &hy.h)) {
     return st;
   }
   int st;
@@ -328,8 +329,8 @@ int dlx_mft(const dlx_mft_geom* geom, int32_t batch, const void* in, void* out,
                   (float2*)Ut, c.data(), out, EPI_STORE, s, false);
   }
   if (st) return st;
-  if (int r = cache_release(hx, s)) return r;
-  return cache_release(hy, s);
+  if (int r = hx.release()) return r;
+  return hy.release();
 }
 
 size_t dlx_lct_workspace_size(int32_t n_in, int32_t n_out, int32_t batch, int32_t precision) {
@@ -385,9 +386,10 @@ int dlx_lct(int32_t n_in, int32_t n_out, int32_t batch, const double* abcd,
                              split ? 1.0 : 0.0, A, B, D, pixel_scale_in, center_in,
                              pixel_scale_out, center_out};
   key.insert(key.end(), wavelength, wavelength + batch);
-  void *cached = nullptr, *handle = nullptr;
+  void* cached = nullptr;
+  CacheLease lease(s);
   bool fresh = true;
-  if (int st = cache_acquire_raw(key, (size_t)batch * kbytes, s, &cached, &handle, &fresh))
+  if (int st = cache_acquire_raw(key, (size_t)batch * kbytes, s, &cached, &lease.h, &fresh))
     return st;
   uint8_t* Kp = cached ? (uint8_t*)cached : K_ws;
   if (fresh) {
@@ -408,7 +410,7 @@ int dlx_lct(int32_t n_in, int32_t n_out, int32_t batch, const double* abcd,
         DLX_HIP(launch_lct_matrix(c, nb, n_in, n_out, x0i, pixel_scale_in, x0o,
                                   pixel_scale_out, (float2*)Kp + (long)b0 * kst, kst, s));
     }
-    if (int st = cache_ready(handle, s)) return st;
+    if (int st = cache_ready(lease.h, s)) return st;
   }
   dlx_mft_geom g{};
   g.n_in = n_in;
@@ -423,7 +425,7 @@ int dlx_lct(int32_t n_in, int32_t n_out, int32_t batch, const double* abcd,
     st = mft_fp32(g, batch, (const float2*)in, (const float2*)Kp, (const float2*)Kp,
                   (float2*)Ut, alpha.data(), out, EPI_STORE, s, false);
   }
-  if (int r = cache_release(handle, s)) return r;
+  if (int r = lease.release()) return r;
   return st;
 }
 
@@ -620,9 +622,10 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
     for (int i = 0; i < nb; ++i)
       alpha[i] = mft_norm(g, wavelength[l0 + i], pixel_scale_in, pixel_scale_out) *
                  std::sqrt(weight[l0 + i]);
-    void *W = nullptr, *h = nullptr;
+    void* W = nullptr;
+    CacheLease h(s);
     if (int st = get_matrices(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr, split,
-                              W_ws, s, &W, &h))
+                              W_ws, s, &W, &h.h))
       return st;
     if (split) {
       PupilChunk pc;
@@ -641,7 +644,7 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
                             (float2*)Ut, alpha.data(), I, EPI_INTENSITY, s, true))
         return st;
     }
-    if (int st = cache_release(h, s)) return st;
+    if (int st = h.release()) return st;
     DLX_HIP(launch_sum_planes(nout2, nb, I, l0 > 0, psf, s));
   }
   return DLX_OK;
@@ -709,12 +712,13 @@ int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavele
                          std::sqrt(weight[l0 + i]));
       gc.kw[i] = (float)(2.0 * M_PI / wavelength[l0 + i]);
     }
-    void *Wf = nullptr, *hf = nullptr, *Wa = nullptr, *ha = nullptr;
+    void *Wf = nullptr, *Wa = nullptr;
+    CacheLease hf(s), ha(s);
     if (int st = get_matrices(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr, split,
-                              Wf_ws, s, &Wf, &hf))
+                              Wf_ws, s, &Wf, &hf.h))
       return st;
     if (int st = get_matrices(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr, split,
-                              Wa_ws, s, &Wa, &ha, true))
+                              Wa_ws, s, &Wa, &ha.h, true))
       return st;
     if (split) {
       const long rb_in = split4_row_bytes(g.n_in), rb_out = split4_row_bytes(g.n_out);
@@ -810,8 +814,8 @@ int dlx_psf_poly_vjp(const dlx_mft_geom* geom, int32_t nwl, const double* wavele
       g4.C2 = H2; g4.U = U;
       DLX_HIP(launch_cgemm_nt(g4, s));
     }
-    if (int st = cache_release(hf, s)) return st;
-    if (int st = cache_release(ha, s)) return st;
+    if (int st = hf.release()) return st;
+    if (int st = ha.release()) return st;
     if (grad_opd || grad_phase)
       DLX_HIP(launch_grad_reduce(gc, nb, nin * nin, H, l0 > 0, grad_opd, grad_phase, s));
     if (grad_amp)   // sum over wavelengths of Re(conj(U) Pbar) = dL/dA

@@ -629,9 +629,10 @@ int dlx_mft_batched_psf(int32_t n, const dlx_mft_instance* instances, int32_t n_
     key.insert(key.end(), {(double)j.rows, (double)j.cols, (double)j.sign_inv, j.c_scale,
                            j.c_off, j.r_scale, j.r_off});
   const size_t wbytes = Q.off_U - Q.off_W;
-  void *wcache = nullptr, *whandle = nullptr;
+  void* wcache = nullptr;
+  CacheLease whandle(s);
   bool fresh = true;
-  if (int st = cache_acquire_raw(key, wbytes, s, &wcache, &whandle, &fresh)) return st;
+  if (int st = cache_acquire_raw(key, wbytes, s, &wcache, &whandle.h, &fresh)) return st;
   if (wcache) {
     uint8_t* w0 = ws + Q.off_W;
     auto rebase = [&](const uint8_t* p) -> uint8_t* {
@@ -661,8 +662,8 @@ int dlx_mft_batched_psf(int32_t n, const dlx_mft_instance* instances, int32_t n_
   if (eager)
     if (int st = launch_plan(Q, ws, fresh, s)) return st;
   if (fresh)
-    if (int st = cache_ready(whandle, s)) return st;
-  return cache_release(whandle, s);
+    if (int st = cache_ready(whandle.h, s)) return st;
+  return whandle.release();
 }
 
 int dlx_batched_graph_stats(uint64_t* captures, uint64_t* launches, int32_t* entries) {

@@ -8,8 +8,14 @@
 // instances, contiguous, in complex64 or split4 form.
 //
 // Stream safety: an entry records an event after its generation; a user on any
-// stream waits on it (hipStreamWaitEvent) before the matrices are read, and
-// records a "last use" event that eviction synchronises on before freeing.
+// stream waits on it (hipStreamWaitEvent) before the matrices are read.  Each
+// stream that used an entry keeps its own "last use" event on the entry, and
+// eviction synchronises on every one of them before the block is reused, so a
+// block read by work still queued on one stream is never handed to another.
+// An entry is pinned from its acquire to its release: eviction skips pinned
+// entries (a call that holds two entries, such as the gradient's forward and
+// adjoint matrices, can never have the first evicted to make room for the
+// second), and an acquire that finds no room falls back to the workspace.
 // The cache is bypassed while the stream is being captured into a graph
 // (allocation and event records are not capturable); callers then generate the
 // matrices into their workspace.
@@ -23,13 +29,20 @@
 namespace dlx {
 namespace {
 
+// Streams that may use one entry before it is evicted: a user on a further
+// stream first orders itself after the oldest user (hipStreamWaitEvent) and
+// takes over its slot.
+constexpr int kMaxUsers = 16;
+
 struct MatEntry {
   std::vector<double> key;
   void* ptr = nullptr;
   size_t bytes = 0;
   hipEvent_t ready = nullptr;
-  hipEvent_t last_use = nullptr;
-  bool used = false;
+  // last use per stream (stream handle, event recorded after its last reader)
+  std::vector<std::pair<hipStream_t, hipEvent_t>> users;
+  size_t oldest = 0;         // next slot to take over when users is full
+  int pins = 0;              // acquires not yet released
   bool pending = false;      // raw entry whose contents are not enqueued yet
   uint64_t tick = 0;
 };
@@ -59,10 +72,11 @@ MatCache& cache() {
   return c;
 }
 
-// Retire an entry: wait for its last use (only that entry's work, not the
-// device), then keep its block as a spare (or free it when `release`).
+// Retire an entry: wait for its last use on every stream that read it (only
+// that entry's work, not the device), then keep its block as a spare (or free
+// it when `release`).
 void free_entry(MatCache& c, MatEntry* e, bool release) {
-  if (e->used) (void)hipEventSynchronize(e->last_use);
+  for (auto& u : e->users) (void)hipEventSynchronize(u.second);
   (void)hipEventSynchronize(e->ready);
   if (release) {
     (void)hipFree(e->ptr);
@@ -71,7 +85,7 @@ void free_entry(MatCache& c, MatEntry* e, bool release) {
     c.spare_bytes += e->bytes;
   }
   (void)hipEventDestroy(e->ready);
-  (void)hipEventDestroy(e->last_use);
+  for (auto& u : e->users) (void)hipEventDestroy(u.second);
   delete e;
 }
 
@@ -81,18 +95,37 @@ void release_spares(MatCache& c) {
   c.spare_bytes = 0;
 }
 
-// Evict least-recently-used entries until `need` more bytes fit the budget
-// (live entries plus spare blocks).
-void make_room(MatCache& c, size_t need) {
-  while (!c.entries.empty() && c.used + need > c.budget) {
-    size_t lru = 0;
-    for (size_t i = 1; i < c.entries.size(); ++i)
-      if (c.entries[i]->tick < c.entries[lru]->tick) lru = i;
+// Evict least-recently-used unpinned entries until `need` more bytes fit the
+// budget (live entries plus spare blocks).  Returns false when the pinned
+// entries alone leave no room.
+bool make_room(MatCache& c, size_t need) {
+  while (c.used + need > c.budget) {
+    size_t lru = c.entries.size();
+    for (size_t i = 0; i < c.entries.size(); ++i)
+      if (c.entries[i]->pins == 0 &&
+          (lru == c.entries.size() || c.entries[i]->tick < c.entries[lru]->tick))
+        lru = i;
+    if (lru == c.entries.size()) return false;
     MatEntry* e = c.entries[lru];
     c.used -= e->bytes;
     c.entries.erase(c.entries.begin() + lru);
     free_entry(c, e, false);
   }
+  return true;
+}
+
+MatEntry* new_entry(std::vector<double>&& key, void* mem, size_t bytes, uint64_t tick) {
+  MatEntry* e = new MatEntry();
+  e->key = std::move(key);
+  e->ptr = mem;
+  e->bytes = bytes;
+  e->tick = tick;
+  e->pins = 1;
+  if (hipEventCreateWithFlags(&e->ready, hipEventDisableTiming) != hipSuccess) {
+    delete e;
+    return nullptr;
+  }
+  return e;
 }
 
 // A block of at least `need` bytes: the smallest spare that is no more than
@@ -160,6 +193,7 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
       e->tick = ++c.tick;
       ++c.hits;
       DLX_HIP(hipStreamWaitEvent(s, e->ready, 0));
+      ++e->pins;
       *ptr = e->ptr;
       *handle = e;
       return DLX_OK;
@@ -170,19 +204,13 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
                              : mat_bytes(g.n_out, g.n_in, split);
   const size_t need = align_up(per * (size_t)nb);
   if (need > c.budget) return DLX_OK;  // larger than the whole cache: caller generates
-  make_room(c, need);
+  if (!make_room(c, need)) return DLX_OK;  // pinned entries fill it: caller generates
   size_t bytes = 0;
   void* mem = take_block(c, need, &bytes);
   if (!mem) return DLX_OK;  // no memory for the cache: caller generates into its workspace
-  MatEntry* e = new MatEntry();
-  e->key = std::move(key);
-  e->ptr = mem;
-  e->bytes = bytes;
-  e->tick = ++c.tick;
-  if (hipEventCreateWithFlags(&e->ready, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&e->last_use, hipEventDisableTiming) != hipSuccess) {
+  MatEntry* e = new_entry(std::move(key), mem, bytes, ++c.tick);
+  if (!e) {
     (void)hipFree(mem);
-    delete e;
     set_error("hipEventCreate failed in the matrix cache");
     return DLX_ERR_HIP;
   }
@@ -237,6 +265,7 @@ int cache_acquire_raw(const std::vector<double>& key_in, size_t bytes, hipStream
       ++c.hits;
       *fresh = false;
       DLX_HIP(hipStreamWaitEvent(s, e->ready, 0));
+      ++e->pins;
       *ptr = e->ptr;
       *handle = e;
       return DLX_OK;
@@ -259,23 +288,17 @@ int cache_acquire_raw(const std::vector<double>& key_in, size_t bytes, hipStream
     c.seen.push_back({h, ++c.tick});
     return DLX_OK;   // first sighting: the caller generates into its workspace
   }
-  make_room(c, need);
+  if (!make_room(c, need)) return DLX_OK;
   size_t got = 0;
   void* mem = take_block(c, need, &got);
   if (!mem) return DLX_OK;
-  MatEntry* e = new MatEntry();
-  e->key = std::move(key);
-  e->ptr = mem;
-  e->bytes = got;
-  e->tick = ++c.tick;
-  e->pending = true;
-  if (hipEventCreateWithFlags(&e->ready, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&e->last_use, hipEventDisableTiming) != hipSuccess) {
+  MatEntry* e = new_entry(std::move(key), mem, got, ++c.tick);
+  if (!e) {
     (void)hipFree(mem);
-    delete e;
     set_error("hipEventCreate failed in the matrix cache");
     return DLX_ERR_HIP;
   }
+  e->pending = true;
   c.entries.push_back(e);
   c.used += got;
   *ptr = mem;
@@ -299,8 +322,40 @@ int cache_release(void* handle, hipStream_t s) {
   MatCache& c = cache();
   std::lock_guard<std::mutex> lock(c.mu);
   MatEntry* e = (MatEntry*)handle;
-  DLX_HIP(hipEventRecord(e->last_use, s));
-  e->used = true;
+  if (e->pins > 0) --e->pins;
+  // this stream's last use: re-record its own event, or take a new slot (or the
+  // oldest user's, after ordering this stream behind that user's last use)
+  hipEvent_t ev = nullptr;
+  for (auto& u : e->users)
+    if (u.first == s) ev = u.second;
+  if (!ev && (int)e->users.size() < kMaxUsers) {
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      // no event: keep the ordering by syncing the stream before the entry can go
+      (void)hipStreamSynchronize(s);
+      ev = nullptr;
+    } else {
+      e->users.push_back({s, ev});
+    }
+  } else if (!ev) {
+    auto& u = e->users[e->oldest];
+    e->oldest = (e->oldest + 1) % e->users.size();
+    DLX_HIP(hipStreamWaitEvent(s, u.second, 0));
+    u.first = s;
+    ev = u.second;
+  }
+  if (ev) DLX_HIP(hipEventRecord(ev, s));
+  if (e->pending) {
+    // acquired for filling but never made ready (the caller failed before
+    // enqueueing its contents): the block holds nothing valid, drop the entry
+    for (size_t i = 0; i < c.entries.size(); ++i)
+      if (c.entries[i] == e) {
+        c.entries.erase(c.entries.begin() + i);
+        c.used -= e->bytes;
+        free_entry(c, e, false);
+        break;
+      }
+  }
   return DLX_OK;
 }
 
@@ -312,9 +367,17 @@ extern "C" int dlx_cache_clear(void) {
   clear_error();
   MatCache& c = cache();
   std::lock_guard<std::mutex> lock(c.mu);
-  for (MatEntry* e : c.entries) free_entry(c, e, true);
-  c.entries.clear();
-  c.used = 0;
+  // entries pinned by a call in progress (another thread) stay until released
+  std::vector<MatEntry*> keep;
+  for (MatEntry* e : c.entries) {
+    if (e->pins > 0) {
+      keep.push_back(e);
+    } else {
+      c.used -= e->bytes;
+      free_entry(c, e, true);
+    }
+  }
+  c.entries.swap(keep);
   release_spares(c);
   return DLX_OK;
 }

@@ -243,4 +243,24 @@ int cache_acquire_raw(const std::vector<double>& key, size_t bytes, hipStream_t 
                       void** ptr, void** handle, bool* fresh);
 int cache_ready(void* handle, hipStream_t s);
 
+// Holds a cache entry from its acquire (pass &lease.h) to its release, and
+// releases it on scope exit too, so an error return never leaves an entry
+// pinned.  release() records the last use on the stream after the work that
+// reads the entry has been enqueued.
+struct CacheLease {
+  void* h = nullptr;
+  hipStream_t s = nullptr;
+  explicit CacheLease(hipStream_t st) : s(st) {}
+  CacheLease(const CacheLease&) = delete;
+  CacheLease& operator=(const CacheLease&) = delete;
+  int release() {
+    void* x = h;
+    h = nullptr;
+    return cache_release(x, s);
+  }
+  ~CacheLease() {
+    if (h) (void)cache_release(h, s);
+  }
+};
+
 }  // namespace dlx
