@@ -220,3 +220,76 @@ def test_matrix_cache_eviction_reuses_blocks(rt):
     finally:
         rt.cache_set_budget(16 << 30)
         rt.cache_clear()
+
+
+def test_psf_poly_vjp_under_cache_pressure(rt):
+    """The gradient holds the forward and the adjoint matrices at once.  With a
+    budget between one and two entries, and the forward entry already resident
+    from a forward call, the adjoint acquire must not evict (and recycle) the
+    forward entry the same call still reads: the gradient equals the uncached one."""
+    print("[cache-vjp]", flush=True)
+    n, n_out, nwl = 96, 48, 6
+    mask = orc.circular_aperture(n, 1.0, oversample=1)
+    rng = np.random.default_rng(5)
+    opd = torch.as_tensor(rng.normal(0, 30e-9, (n, n)), dtype=torch.float32, device="cuda")
+    wls = np.linspace(0.9e-6, 1.1e-6, nwl)
+    w = np.ones(nwl) / nwl
+    G = torch.as_tensor(rng.normal(size=(n_out, n_out)), dtype=torch.float32, device="cuda")
+    kw = dict(amp=mask, opd=opd, scale=1.0 / n ** 2, precision="bf16x3")
+    rt.cache_clear()
+    rt.cache_set_budget(0)
+    try:
+        ref_psf = rt.psf_poly(n, n_out, wls, w, 1.0 / n, 2.5e-7, **kw).cpu().numpy()
+        ref = rt.psf_poly_vjp(n, n_out, wls, w, 1.0 / n, 2.5e-7, G, **kw)[0].cpu().numpy()
+    finally:
+        rt.cache_set_budget(16 << 30)
+    per_entry = nwl * n_out * n * 8
+    rt.cache_set_budget(int(1.5 * per_entry))
+    try:
+        for _ in range(2):
+            psf = rt.psf_poly(n, n_out, wls, w, 1.0 / n, 2.5e-7, **kw).cpu().numpy()
+            g = rt.psf_poly_vjp(n, n_out, wls, w, 1.0 / n, 2.5e-7, G, **kw)[0].cpu().numpy()
+            assert np.array_equal(psf, ref_psf)
+            assert np.array_equal(g, ref)
+        assert rt.cache_stats()["entries"] == 1
+    finally:
+        rt.cache_set_budget(16 << 30)
+        rt.cache_clear()
+
+
+def test_cache_entry_shared_by_two_streams_survives_eviction(rt):
+    """An entry read by queued work on one stream and then by another stream,
+    then evicted from the second stream: the block is recycled only after both
+    streams' reads, so every result equals the uncached one."""
+    print("[cache-streams]", flush=True)
+    n, n_out, nwl = 256, 128, 8
+    mask = orc.circular_aperture(n, 1.0, oversample=1)
+    grids = [np.linspace(0.8e-6 + 0.01e-6 * k, 1.2e-6, nwl) for k in range(3)]
+    w = np.ones(nwl) / nwl
+    kw = dict(amp=mask, scale=1.0 / n ** 2, precision="bf16x3")
+    rt.cache_clear()
+    rt.cache_set_budget(0)
+    try:
+        refs = [rt.psf_poly(n, n_out, g, w, 1.0 / n, 2.5e-7, **kw).cpu().numpy() for g in grids]
+    finally:
+        rt.cache_set_budget(16 << 30)
+    per_entry = nwl * n_out * n * 8
+    rt.cache_set_budget(int(1.5 * per_entry))
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    try:
+        torch.cuda.synchronize()
+        with torch.cuda.stream(sa):
+            outs_a = [rt.psf_poly(n, n_out, grids[0], w, 1.0 / n, 2.5e-7, **kw)
+                      for _ in range(24)]
+        with torch.cuda.stream(sb):
+            outs_b = [rt.psf_poly(n, n_out, g, w, 1.0 / n, 2.5e-7, **kw)
+                      for g in (grids[0], grids[1], grids[2], grids[0])]
+        torch.cuda.synchronize()
+        for o in outs_a:
+            assert np.array_equal(o.cpu().numpy(), refs[0])
+        for o, k in zip(outs_b, (0, 1, 2, 0)):
+            assert np.array_equal(o.cpu().numpy(), refs[k])
+    finally:
+        torch.cuda.synchronize()
+        rt.cache_set_budget(16 << 30)
+        rt.cache_clear()
