@@ -52,6 +52,10 @@ def _parse():
     ap.add_argument("--no-lct", action="store_true", help="skip the LCT leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient leg")
+    ap.add_argument("--no-asm", action="store_true", help="skip the angular-spectrum leg")
+    ap.add_argument("--forms", nargs="+", default=["fp32", "bf16x3"],
+                    choices=["fp32", "bf16x3"],
+                    help="arithmetic forms of the headline workload to time")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a streaming kernel of known size (intensity of 8 Mi "
                          "complex64: 64 MiB read, 32 MiB written) to scale PMC byte counters")
@@ -518,13 +522,13 @@ def main():
                   for i in range(16)]
 
     psfs = {}
-    for prec in ("fp32", "bf16x3"):
+    for prec in args.forms:
         with runtime.precision(prec):
             psfs[prec] = base.propagate(cfg.wavelengths, weights=cfg.weights).double().cpu().numpy()
     torch.cuda.synchronize()
 
     res = {}
-    for prec in ("fp32", "bf16x3"):
+    for prec in args.forms:
         print(f"[bench] timing {prec}: warmup {args.warmup}, steps {args.steps}",
               file=sys.stderr, flush=True)
         ms, s1, s2 = _time_form(torch, runtime, base, coeff_sets, cfg, prec, args.steps,
@@ -552,7 +556,7 @@ def main():
                                                     max(args.steps // 20, 3), 2)
 
     print("[bench] angular-spectrum leg", file=sys.stderr, flush=True)
-    asm = _asm_leg(torch, runtime, max(args.steps // 5, 10))
+    asm = None if args.no_asm else _asm_leg(torch, runtime, max(args.steps // 5, 10))
     print("[bench] detector leg", file=sys.stderr, flush=True)
     detector = None if args.no_detector else _detector_leg(torch, max(args.steps // 5, 10),
                                                            not args.no_cpu)
@@ -565,7 +569,10 @@ def main():
                                                       args.serving_seconds)
 
     ok = {p: (errs[p] is not None and errs[p] <= ERR_BOUND) for p in psfs}
-    head = "bf16x3" if ok["bf16x3"] or errs["bf16x3"] is None else "fp32"
+    if "bf16x3" in res and (ok["bf16x3"] or errs["bf16x3"] is None or "fp32" not in res):
+        head = "bf16x3"
+    else:
+        head = "fp32"
     ms = res[head]["ms"]
     n1 = cfg.n_out * cfg.n_in ** 2 * 8.0 * cfg.n_wl        # stage-1 algorithmic flops/launch
     t1_ms, c1 = res[head]["s1"]
@@ -611,7 +618,8 @@ def main():
                       "stage1_avg_ms": res[p]["s1"][0] / max(res[p]["s1"][1], 1),
                       "stage2_avg_ms": res[p]["s2"][0] / max(res[p]["s2"][1], 1)}
                   for p in res},
-        "speedup_bf16x3_over_fp32": res["fp32"]["ms"] / res["bf16x3"]["ms"],
+        "speedup_bf16x3_over_fp32": (res["fp32"]["ms"] / res["bf16x3"]["ms"]
+                                     if len(res) == 2 else None),
         "grad": grad,
         "batched": batched,
         "asm": asm,
