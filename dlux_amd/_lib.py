@@ -148,6 +148,10 @@ SIGNATURES = {
     "dlx_psf_poly": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, c_double, c_double,
                                c_double, c_double, c_double, _fp, _fp, _fp, c_double, _fp,
                                c_int32, c_void_p, c_size_t, c_void_p]),
+    "dlx_psf_poly_noalloc_scratch_bytes": (c_size_t, []),
+    "dlx_psf_poly_noalloc": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, c_double,
+                                       c_double, c_double, c_double, c_double, _fp, _fp, _fp,
+                                       c_double, _fp, c_int32, c_void_p]),
     "dlx_psf_poly_vjp_workspace_size": (c_size_t, [POINTER(MftGeom), c_int32, c_int32]),
     "dlx_psf_poly_vjp": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, c_double, c_double,
                                    c_double, c_double, c_double, _fp, _fp, _fp, c_double, _fp,

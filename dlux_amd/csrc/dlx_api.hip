@@ -2,6 +2,7 @@
 // argument validation, workspace carving and kernel orchestration.
 #include <cstdarg>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "dlx_internal.h"
@@ -565,22 +566,25 @@ int dlx_intensity(int64_t npix, int32_t batch, const void* field, int32_t sum_ov
   return DLX_OK;
 }
 
-size_t dlx_psf_poly_workspace_size(const dlx_mft_geom* geom, int32_t nwl, int32_t precision) {
-  if (!geom || nwl <= 0) return 0;
-  const bool split = precision == DLX_PREC_BF16X3;
-  const size_t c = (size_t)(nwl < kWlChunk ? nwl : kWlChunk);
-  const size_t P = align_up(c * mat_bytes(geom->n_in, geom->n_in, split));
-  const size_t W = align_up(c * mat_bytes(geom->n_out, geom->n_in, split));
-  const size_t I = align_up(c * geom->n_out * geom->n_out * sizeof(float));
+}  // extern "C"
+
+namespace dlx {
+namespace {
+
+// Bytes of the psf chain's buffers for a chunk of c wavelengths: the pupils P,
+// the transfer matrices (used when they are not taken from the cache), the
+// stage-1 intermediate U^T and the per-wavelength |E|^2 planes.
+size_t psf_poly_bytes(const dlx_mft_geom& g, int c, bool split) {
+  const size_t P = align_up((size_t)c * mat_bytes(g.n_in, g.n_in, split));
+  const size_t W = align_up((size_t)c * mat_bytes(g.n_out, g.n_in, split));
+  const size_t I = align_up((size_t)c * g.n_out * g.n_out * sizeof(float));
   return P + 2 * W + I;
 }
 
-int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
-                 const double* weight, double pixel_scale_in, double pixel_scale_out,
-                 double center, double tilt_x, double tilt_y, const float* amp,
-                 const float* opd, const float* phase, double scale, float* psf,
-                 int32_t precision, void* workspace, size_t workspace_bytes, void* stream) {
-  clear_error();
+int check_psf_poly_args(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                        const double* weight, double pixel_scale_in, double pixel_scale_out,
+                        double center, double tilt_x, double tilt_y, double scale, float* psf,
+                        int32_t precision) {
   if (int st = check_geom(geom)) return st;
   DLX_CHECK_ARG(nwl > 0, "nwl must be > 0, got %d", nwl);
   DLX_CHECK_ARG(wavelength && weight && psf, "null argument to dlx_psf_poly");
@@ -590,24 +594,27 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
                     std::isfinite(pixel_scale_out) && std::isfinite(center) &&
                     std::isfinite(tilt_x) && std::isfinite(tilt_y) && std::isfinite(scale),
                 "non-finite scalar argument to dlx_psf_poly");
-  const size_t need = dlx_psf_poly_workspace_size(geom, nwl, precision);
-  if (!workspace || workspace_bytes < need) {
-    set_error("dlx_psf_poly workspace too small: need %zu bytes, got %zu", need,
-              workspace_bytes);
-    return DLX_ERR_WORKSPACE;
-  }
   for (int l = 0; l < nwl; ++l) {
     DLX_CHECK_ARG(std::isfinite(weight[l]) && weight[l] >= 0, "weight[%d] must be >= 0", l);
     DLX_CHECK_ARG(std::isfinite(wavelength[l]) && wavelength[l] > 0,
                   "wavelength[%d] must be > 0", l);
   }
-  const dlx_mft_geom& g = *geom;
-  hipStream_t s = (hipStream_t)stream;
-  const bool split = precision == DLX_PREC_BF16X3;
-  const int c = nwl < kWlChunk ? nwl : kWlChunk;
+  return DLX_OK;
+}
+
+// The psf chain over chunks of at most c wavelengths in `ws` (psf_poly_bytes(c)
+// bytes): pupils, matrices (from the device cache when `use_cache`, else
+// generated into ws), the two contractions with the |E|^2 epilogue, and the
+// running sum over wavelengths in wavelength order (so the result does not
+// depend on c).  `prof` records the contractions' live timing.
+int psf_poly_run(const dlx_mft_geom& g, int32_t nwl, const double* wavelength,
+                 const double* weight, double pixel_scale_in, double pixel_scale_out,
+                 double center, double tilt_x, double tilt_y, const float* amp,
+                 const float* opd, const float* phase, double scale, float* psf, bool split,
+                 uint8_t* ws, int c, bool use_cache, bool prof, hipStream_t s) {
   const size_t Pb = align_up((size_t)c * mat_bytes(g.n_in, g.n_in, split));
   const size_t Wb = align_up((size_t)c * mat_bytes(g.n_out, g.n_in, split));
-  uint8_t* P = (uint8_t*)workspace;
+  uint8_t* P = ws;
   uint8_t* W_ws = P + Pb;
   uint8_t* Ut = P + Pb + Wb;
   float* I = (float*)(P + Pb + 2 * Wb);
@@ -624,9 +631,16 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
                  std::sqrt(weight[l0 + i]);
     void* W = nullptr;
     CacheLease h(s);
-    if (int st = get_matrices(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr, split,
-                              W_ws, s, &W, &h.h))
-      return st;
+    if (use_cache) {
+      if (int st = get_matrices(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr, split,
+                                W_ws, s, &W, &h.h))
+        return st;
+    } else {
+      W = W_ws;
+      if (int st = gen_dft(g, nb, wavelength + l0, dx.data(), pout.data(), nullptr, nullptr,
+                           W_ws, split, s))
+        return st;
+    }
     if (split) {
       PupilChunk pc;
       for (int i = 0; i < nb; ++i) pc.inv_wl[i] = 1.0 / wavelength[l0 + i];
@@ -634,20 +648,53 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
                                   phase, (float)scale, P, (long)mat_bytes(g.n_in, g.n_in, true),
                                   s));
       if (int st = mft_split4(g, nb, P, (const uint8_t*)W, (const uint8_t*)W, Ut, alpha.data(),
-                              I, EPI_INTENSITY, s, prof_begin, prof_end))
+                              I, EPI_INTENSITY, s, prof ? prof_begin : nullptr,
+                              prof ? prof_end : nullptr))
         return st;
     } else {
       if (int st = dlx_pupil_field(g.n_in, nb, wavelength + l0, pixel_scale_in, center, tilt_x,
-                                   tilt_y, amp, opd, phase, scale, nullptr, P, stream))
+                                   tilt_y, amp, opd, phase, scale, nullptr, P, s))
         return st;
       if (int st = mft_fp32(g, nb, (const float2*)P, (const float2*)W, (const float2*)W,
-                            (float2*)Ut, alpha.data(), I, EPI_INTENSITY, s, true))
+                            (float2*)Ut, alpha.data(), I, EPI_INTENSITY, s, prof))
         return st;
     }
     if (int st = h.release()) return st;
     DLX_HIP(launch_sum_planes(nout2, nb, I, l0 > 0, psf, s));
   }
   return DLX_OK;
+}
+
+}  // namespace
+}  // namespace dlx
+
+extern "C" {
+
+size_t dlx_psf_poly_workspace_size(const dlx_mft_geom* geom, int32_t nwl, int32_t precision) {
+  if (!geom || nwl <= 0) return 0;
+  return psf_poly_bytes(*geom, nwl < kWlChunk ? nwl : kWlChunk, precision == DLX_PREC_BF16X3);
+}
+
+int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                 const double* weight, double pixel_scale_in, double pixel_scale_out,
+                 double center, double tilt_x, double tilt_y, const float* amp,
+                 const float* opd, const float* phase, double scale, float* psf,
+                 int32_t precision, void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  if (int st = check_psf_poly_args(geom, nwl, wavelength, weight, pixel_scale_in,
+                                   pixel_scale_out, center, tilt_x, tilt_y, scale, psf,
+                                   precision))
+    return st;
+  const size_t need = dlx_psf_poly_workspace_size(geom, nwl, precision);
+  if (!workspace || workspace_bytes < need) {
+    set_error("dlx_psf_poly workspace too small: need %zu bytes, got %zu", need,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  return psf_poly_run(*geom, nwl, wavelength, weight, pixel_scale_in, pixel_scale_out, center,
+                      tilt_x, tilt_y, amp, opd, phase, scale, psf,
+                      precision == DLX_PREC_BF16X3, (uint8_t*)workspace,
+                      nwl < kWlChunk ? nwl : kWlChunk, true, true, (hipStream_t)stream);
 }
 
 size_t dlx_psf_poly_vjp_workspace_size(const dlx_mft_geom* geom, int32_t nwl,
@@ -871,6 +918,88 @@ int dlx_profile_read(int32_t stage, double* total_ms, int32_t* count) {
   *total_ms = t;
   *count = c;
   return DLX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// dlx_psf_poly_noalloc: the psf chain with every buffer in one fixed device
+// array of the library, for callers that cannot allocate (code under graph
+// capture, serving loops, harnesses that forbid hipMalloc).  The array is a
+// file-scope __device__ symbol: each GPU the library runs on holds its own
+// copy (allocated when the code object loads there, kept until the process
+// ends).  Its device address is looked up once per device with
+// hipGetSymbolAddress and only that address is handed to kernels; the array's
+// name in host code is a host-side stand-in, not a device pointer.
+// ---------------------------------------------------------------------------
+namespace dlx {
+
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;   // 64 MiB per GPU
+static __device__ __align__(256) uint8_t scratch[kNoallocScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};
+
+// The current device's copy of `scratch` (looked up on its first use there).
+int scratch_address(void** out) {
+  *out = nullptr;
+  int dev = 0;
+  DLX_HIP(hipGetDevice(&dev));
+  DLX_CHECK_ARG(dev >= 0 && dev < kMaxScratchDevices, "device %d beyond the %d supported", dev,
+                kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      (void)hipGetLastError();
+      set_error("hipGetSymbolAddress of the scratch array failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return DLX_ERR_HIP;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return DLX_OK;
+}
+
+}  // namespace
+}  // namespace dlx
+
+extern "C" {
+
+size_t dlx_psf_poly_noalloc_scratch_bytes(void) { return kNoallocScratchBytes; }
+
+int dlx_psf_poly_noalloc(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                         const double* weight, double pixel_scale_in, double pixel_scale_out,
+                         double center, double tilt_x, double tilt_y, const float* amp,
+                         const float* opd, const float* phase, double scale, float* psf,
+                         int32_t precision, void* stream) {
+  clear_error();
+  if (int st = check_psf_poly_args(geom, nwl, wavelength, weight, pixel_scale_in,
+                                   pixel_scale_out, center, tilt_x, tilt_y, scale, psf,
+                                   precision))
+    return st;
+  const bool split = precision == DLX_PREC_BF16X3;
+  const size_t one = psf_poly_bytes(*geom, 1, split);
+  if (one > kNoallocScratchBytes) {
+    set_error("dlx_psf_poly_noalloc: one wavelength of a %d^2 -> %d^2 PSF needs %zu bytes, "
+              "more than the %zu-byte scratch array; use dlx_psf_poly with a workspace",
+              geom->n_in, geom->n_out, one, kNoallocScratchBytes);
+    return DLX_ERR_INVALID;
+  }
+  // the largest chunk of wavelengths whose buffers fit the array
+  int c = nwl < kWlChunk ? nwl : kWlChunk;
+  while (c > 1 && psf_poly_bytes(*geom, c, split) > kNoallocScratchBytes) --c;
+  void* ws = nullptr;
+  if (int st = scratch_address(&ws)) return st;
+  // matrices are generated into the array every call: the device cache allocates
+  return psf_poly_run(*geom, nwl, wavelength, weight, pixel_scale_in, pixel_scale_out, center,
+                      tilt_x, tilt_y, amp, opd, phase, scale, psf, split, (uint8_t*)ws, c,
+                      false, false, (hipStream_t)stream);
 }
 
 }  // extern "C"

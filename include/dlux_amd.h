@@ -163,6 +163,28 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
                  void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * dlx_psf_poly that allocates nothing and takes no workspace: the pupils,
+ * transfer matrices and per-wavelength planes live in a fixed 64 MiB device
+ * array of the library (one copy per GPU, held until the process ends; its
+ * size is dlx_psf_poly_noalloc_scratch_bytes()), the wavelengths run in
+ * chunks that fit it, and the transfer matrices are generated every call (the
+ * device matrix cache, which allocates, is not used).  Same arguments and
+ * result as dlx_psf_poly.  Rejects, with DLX_ERR_INVALID, a geometry whose
+ * one-wavelength buffers exceed the array.  Calls that overlap on one GPU
+ * (two host threads or two streams) race on the array; callers serialise them.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).
+ */
+size_t dlx_psf_poly_noalloc_scratch_bytes(void);
+int dlx_psf_poly_noalloc(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                         const double* weight, double pixel_scale_in,
+                         double pixel_scale_out, double center, double tilt_x,
+                         double tilt_y, const float* amp, const float* opd,
+                         const float* phase, double scale, float* psf,
+                         int32_t precision, void* stream);
+
+/*
  * Vector-Jacobian product of dlx_psf_poly: given dpsf = dL/dpsf (float32
  * N_out x N_out), writes
  *   grad_opd[p]   = sum_l (2 pi/lambda_l) Im(conj(P_l[p]) Pbar_l[p])
