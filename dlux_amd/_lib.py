@@ -165,6 +165,12 @@ SIGNATURES = {
                                                c_int32]),
     "dlx_detector_apply": (c_int32, [c_int32, c_int32, _fp, POINTER(DetectorOp), c_int32, _fp,
                                      c_void_p, c_size_t, c_void_p]),
+    "dlx_psf_poly_detector_workspace_size": (c_size_t, [POINTER(MftGeom), c_int32, c_int32,
+                                                        POINTER(DetectorOp), c_int32]),
+    "dlx_psf_poly_detector": (c_int32, [POINTER(MftGeom), c_int32, _dp, _dp, c_double,
+                                        c_double, c_double, c_double, c_double, _fp, _fp, _fp,
+                                        c_double, POINTER(DetectorOp), c_int32, _fp, c_int32,
+                                        c_void_p, c_size_t, c_void_p]),
     "dlx_detector_vjp": (c_int32, [c_int32, c_int32, _fp, POINTER(DetectorOp), c_int32, _fp,
                                    _fp, c_void_p, c_size_t, c_void_p]),
     "dlx_detector_vjp_params_workspace_size": (c_size_t, [c_int32, c_int32, POINTER(DetectorOp),

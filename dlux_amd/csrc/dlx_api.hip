@@ -606,12 +606,15 @@ int check_psf_poly_args(const dlx_mft_geom* geom, int32_t nwl, const double* wav
 // bytes): pupils, matrices (from the device cache when `use_cache`, else
 // generated into ws), the two contractions with the |E|^2 epilogue, and the
 // running sum over wavelengths in wavelength order (so the result does not
-// depend on c).  `prof` records the contractions' live timing.
+// depend on c).  With `planes` set, every wavelength's w_l |E_l|^2 plane is
+// written there instead (planes + l N_out^2) and nothing is summed.  `prof`
+// records the contractions' live timing.
 int psf_poly_run(const dlx_mft_geom& g, int32_t nwl, const double* wavelength,
                  const double* weight, double pixel_scale_in, double pixel_scale_out,
                  double center, double tilt_x, double tilt_y, const float* amp,
                  const float* opd, const float* phase, double scale, float* psf, bool split,
-                 uint8_t* ws, int c, bool use_cache, bool prof, hipStream_t s) {
+                 uint8_t* ws, int c, bool use_cache, bool prof, hipStream_t s,
+                 float* planes = nullptr) {
   const size_t Pb = align_up((size_t)c * mat_bytes(g.n_in, g.n_in, split));
   const size_t Wb = align_up((size_t)c * mat_bytes(g.n_out, g.n_in, split));
   uint8_t* P = ws;
@@ -624,6 +627,7 @@ int psf_poly_run(const dlx_mft_geom& g, int32_t nwl, const double* wavelength,
   std::vector<double> dx(c, pixel_scale_in), pout(c, pixel_scale_out), alpha(c);
   for (int l0 = 0; l0 < nwl; l0 += c) {
     const int nb = nwl - l0 < c ? nwl - l0 : c;
+    float* Ic = planes ? planes + (long)l0 * nout2 : I;
     // |E_l|^2 = (c_l sqrt(w_l))^2 |Y^T P X|^2: the scale is applied in the
     // stage-2 epilogue, so the matrices depend on the geometry only.
     for (int i = 0; i < nb; ++i)
@@ -648,7 +652,7 @@ int psf_poly_run(const dlx_mft_geom& g, int32_t nwl, const double* wavelength,
                                   phase, (float)scale, P, (long)mat_bytes(g.n_in, g.n_in, true),
                                   s));
       if (int st = mft_split4(g, nb, P, (const uint8_t*)W, (const uint8_t*)W, Ut, alpha.data(),
-                              I, EPI_INTENSITY, s, prof ? prof_begin : nullptr,
+                              Ic, EPI_INTENSITY, s, prof ? prof_begin : nullptr,
                               prof ? prof_end : nullptr))
         return st;
     } else {
@@ -656,11 +660,11 @@ int psf_poly_run(const dlx_mft_geom& g, int32_t nwl, const double* wavelength,
                                    tilt_y, amp, opd, phase, scale, nullptr, P, s))
         return st;
       if (int st = mft_fp32(g, nb, (const float2*)P, (const float2*)W, (const float2*)W,
-                            (float2*)Ut, alpha.data(), I, EPI_INTENSITY, s, prof))
+                            (float2*)Ut, alpha.data(), Ic, EPI_INTENSITY, s, prof))
         return st;
     }
     if (int st = h.release()) return st;
-    DLX_HIP(launch_sum_planes(nout2, nb, I, l0 > 0, psf, s));
+    if (!planes) DLX_HIP(launch_sum_planes(nout2, nb, I, l0 > 0, psf, s));
   }
   return DLX_OK;
 }
@@ -695,6 +699,54 @@ int dlx_psf_poly(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength
                       tilt_x, tilt_y, amp, opd, phase, scale, psf,
                       precision == DLX_PREC_BF16X3, (uint8_t*)workspace,
                       nwl < kWlChunk ? nwl : kWlChunk, true, true, (hipStream_t)stream);
+}
+
+size_t dlx_psf_poly_detector_workspace_size(const dlx_mft_geom* geom, int32_t nwl,
+                                            int32_t precision, const dlx_detector_op* ops,
+                                            int32_t nops) {
+  if (!geom || nwl <= 0) return 0;
+  const int c = nwl < kWlChunk ? nwl : kWlChunk;
+  const size_t planes = align_up((size_t)nwl * geom->n_out * geom->n_out * sizeof(float));
+  return psf_poly_bytes(*geom, c, precision == DLX_PREC_BF16X3) + planes +
+         detector_ws_bytes(geom->n_out, 1, ops, nops);
+}
+
+int dlx_psf_poly_detector(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                          const double* weight, double pixel_scale_in, double pixel_scale_out,
+                          double center, double tilt_x, double tilt_y, const float* amp,
+                          const float* opd, const float* phase, double scale,
+                          const dlx_detector_op* ops, int32_t nops, float* out,
+                          int32_t precision, void* workspace, size_t workspace_bytes,
+                          void* stream) {
+  clear_error();
+  if (int st = check_psf_poly_args(geom, nwl, wavelength, weight, pixel_scale_in,
+                                   pixel_scale_out, center, tilt_x, tilt_y, scale, out,
+                                   precision))
+    return st;
+  int32_t side = 0;
+  if (int st = dlx_detector_out_side(geom->n_out, ops, nops, &side)) return st;
+  const size_t need = dlx_psf_poly_detector_workspace_size(geom, nwl, precision, ops, nops);
+  if (!workspace || workspace_bytes < need) {
+    set_error("dlx_psf_poly_detector workspace too small: need %zu bytes, got %zu", need,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  const bool split = precision == DLX_PREC_BF16X3;
+  const int c = nwl < kWlChunk ? nwl : kWlChunk;
+  uint8_t* ws = (uint8_t*)workspace;
+  const size_t chain = psf_poly_bytes(*geom, c, split);
+  float* planes = (float*)(ws + chain);
+  const size_t pbytes = align_up((size_t)nwl * geom->n_out * geom->n_out * sizeof(float));
+  hipStream_t s = (hipStream_t)stream;
+  // every wavelength's w_l |E_l|^2 plane, then the detector chain whose first
+  // pass loads the sum over the planes (in wavelength order: the PSF that
+  // dlx_psf_poly returns, never written)
+  if (int st = psf_poly_run(*geom, nwl, wavelength, weight, pixel_scale_in, pixel_scale_out,
+                            center, tilt_x, tilt_y, amp, opd, phase, scale, nullptr, split, ws,
+                            c, true, true, s, planes))
+    return st;
+  return detector_apply_sum(geom->n_out, 1, nwl, (long)geom->n_out * geom->n_out, planes, ops,
+                            nops, out, ws + chain + pbytes, workspace_bytes - chain - pbytes, s);
 }
 
 size_t dlx_psf_poly_vjp_workspace_size(const dlx_mft_geom* geom, int32_t nwl,

@@ -19,6 +19,11 @@
 //   multiply-adds per pixel, which keeps the pass HBM-bound.
 //   A convolve followed by a downsample runs as one pass (det_conv_down_kernel).
 //
+// The chain's first pass can read the sum of several images instead of one
+// (SumIn): the wavelength planes of a polychromatic PSF, summed in plane order
+// as each input element is loaded, so the |E|^2 planes go straight into the
+// detector without the PSF being written and read back.
+//
 // The vector-Jacobian product re-runs the chain keeping each pass's input and
 // its pre-store values, then walks the passes backwards: the store-side ops'
 // derivatives, then the adjoint of the pass (correlation with the unflipped
@@ -34,6 +39,21 @@ constexpr int kMaxKernel = 64;   // convolution kernel side limit (LDS tile)
 constexpr int kTileX = 64;       // correlation tile: 64 x 16 outputs,
 constexpr int kTileY = 16;       // 4 consecutive columns per thread
 constexpr int kThreads = 256;
+
+// Input of a pass: one image, or (n > 1) the sum of n images `stride` floats
+// apart, added in order (plane 0 + plane 1 + ...).
+struct SumIn {
+  int n;
+  long stride;
+};
+
+// s[idx], or the in-order sum over the planes of si.
+__device__ __forceinline__ float ld_sum(const float* __restrict__ s, long idx, const SumIn& si) {
+  float v = s[idx];
+#pragma unroll 8
+  for (int q = 1; q < si.n; ++q) v += s[idx + (long)q * si.stride];
+  return v;
+}
 
 struct EwList {
   int n;
@@ -210,8 +230,9 @@ __device__ __forceinline__ void load_window(const float* row, int K, float (&a)[
 template <int RPT>
 __device__ __forceinline__ void conv_tile(const float* __restrict__ s, int n, int ty0, int tx0,
                                           int K, int lo, const float* __restrict__ w, int flip,
-                                          int sep, const EwList& pre, float* lds, int row,
-                                          int c0, float (&acc)[RPT][4]) {
+                                          int sep, const EwList& pre, const SumIn& si,
+                                          float* lds, int row, int c0,
+                                          float (&acc)[RPT][4]) {
   const int TW = conv_tile_w(K), TH = kTileY * RPT + K - 1;
   float* wl = lds;
   float* tl = lds + conv_tile_offset(K, sep);
@@ -236,6 +257,16 @@ __device__ __forceinline__ void conv_tile(const float* __restrict__ s, int n, in
       const int gy = min(max(ty0 + lo + rb + u * kRowStep, 0), n - 1);
       v0[u] = s[(long)gy * n + cx0];
       v1[u] = s[(long)gy * n + cx1];
+    }
+    // summed input: the other planes in order, a whole batch of rows per plane
+    for (int q = 1; q < si.n; ++q) {
+      const float* sq = s + (long)q * si.stride;
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) {
+        const int gy = min(max(ty0 + lo + rb + u * kRowStep, 0), n - 1);
+        v0[u] += sq[(long)gy * n + cx0];
+        v1[u] += sq[(long)gy * n + cx1];
+      }
     }
     if (rb == r0) {
       if (sep) {
@@ -364,7 +395,7 @@ __global__ void __launch_bounds__(kThreads)
 det_corr_kernel(const float* __restrict__ src, int n, int tiles_x, int K, int lo,
                 const float* __restrict__ w, int flip, int sep, EwList pre, EwList post,
                 const float* __restrict__ aux, EwList dpre, float* __restrict__ r_save,
-                float* __restrict__ out) {
+                float* __restrict__ out, SumIn si) {
   extern __shared__ float lds[];
   int tile, image;
   xcd_tile(has_map(pre) || has_map(post) || has_map(dpre), tile, image);
@@ -380,7 +411,7 @@ det_corr_kernel(const float* __restrict__ src, int n, int tiles_x, int K, int lo
 #pragma unroll
   for (int u = 0; u < RPT; ++u) load_map4(post, pi, n, ty0 + row + kTileY * u, tx0 + c0, mv[u]);
   float acc[RPT][4];
-  conv_tile<RPT>(src + base, n, ty0, tx0, K, lo, w, flip, sep, pre, lds, row, c0, acc);
+  conv_tile<RPT>(src + base, n, ty0, tx0, K, lo, w, flip, sep, pre, si, lds, row, c0, acc);
   // four consecutive outputs go out as one 16-byte store where they can
   const bool vec = n % 4 == 0 && (((uintptr_t)out | (uintptr_t)r_save) & 15) == 0;
 #pragma unroll
@@ -424,7 +455,7 @@ template <int RPT>
 __global__ void __launch_bounds__(kThreads)
 det_conv_down_kernel(const float* __restrict__ src, int n, int th, int tw, int tiles_x, int K,
                      int lo, const float* __restrict__ w, int sep, EwList pre, EwList post,
-                     int k, int m, EwList post2, float* __restrict__ out) {
+                     int k, int m, EwList post2, float* __restrict__ out, SumIn si) {
   extern __shared__ float lds[];
   int tile, image;
   xcd_tile(has_map(pre) || has_map(post) || has_map(post2), tile, image);
@@ -438,7 +469,7 @@ det_conv_down_kernel(const float* __restrict__ src, int n, int th, int tw, int t
 #pragma unroll
   for (int u = 0; u < RPT; ++u) load_map4(post, pi, n, ty0 + row + kTileY * u, tx0 + c0, mv[u]);
   float acc[RPT][4];
-  conv_tile<RPT>(src + base, n, ty0, tx0, K, lo, w, 1, sep, pre, lds, row, c0, acc);
+  conv_tile<RPT>(src + base, n, ty0, tx0, K, lo, w, 1, sep, pre, si, lds, row, c0, acc);
   float* ot = lds;   // (16 RPT) x 64 convolve outputs (the tile is no longer read)
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
@@ -486,21 +517,21 @@ det_conv_down_kernel(const float* __restrict__ src, int n, int th, int tw, int t
 // Grid: (pixel blocks of the output image, image); in-image indices are 32-bit.
 __global__ void __launch_bounds__(256)
 det_down_kernel(const float* __restrict__ src, int n, int k, int m, EwList pre, EwList post,
-                float* __restrict__ r_save, float* __restrict__ out) {
+                float* __restrict__ r_save, float* __restrict__ out, SumIn si) {
   const int q = blockIdx.x * 256 + threadIdx.x;
   if (q >= m * m) return;
   const float* s = src + (long)blockIdx.y * n * n;
   const long o = (long)blockIdx.y * m * m + q;
   float acc;
   if (k == 1) {
-    acc = ew_apply(pre, s[q], q);
+    acc = ew_apply(pre, ld_sum(s, q, si), q);
   } else {
     const int oy = q / m, ox = q - oy * m;
     acc = 0.f;
     for (int dy = 0; dy < k; ++dy) {
       const int rp = (oy * k + dy) * n + ox * k;
       float rs = 0.f;
-      for (int dx = 0; dx < k; ++dx) rs += ew_apply(pre, s[rp + dx], rp + dx);
+      for (int dx = 0; dx < k; ++dx) rs += ew_apply(pre, ld_sum(s, rp + dx, si), rp + dx);
       acc += rs;
     }
   }
@@ -512,11 +543,19 @@ det_down_kernel(const float* __restrict__ src, int n, int k, int m, EwList pre, 
 // out = post(pre(src)), r_save = pre(src).
 __global__ void __launch_bounds__(256)
 det_ew_kernel(const float* __restrict__ src, int npix, EwList pre, EwList post,
-              float* __restrict__ r_save, float* __restrict__ out) {
+              float* __restrict__ r_save, float* __restrict__ out, SumIn si) {
   const int q = (blockIdx.x * 256 + threadIdx.x) * 4;
   if (q >= npix) return;
   const long o = (long)blockIdx.y * npix + q;
-  const float4 v = *(const float4*)(src + o);
+  float4 v = *(const float4*)(src + o);
+#pragma unroll 8
+  for (int pl = 1; pl < si.n; ++pl) {
+    const float4 a = *(const float4*)(src + o + (long)pl * si.stride);
+    v.x += a.x;
+    v.y += a.y;
+    v.z += a.z;
+    v.w += a.w;
+  }
   float4 r;
   r.x = ew_apply(pre, v.x, q);
   r.y = ew_apply(pre, v.y, q + 1);
@@ -881,7 +920,7 @@ int check_grid(const std::vector<Pass>& passes, int batch) {
 
 // Forward pass i: src -> dst (r_save may be NULL).
 int run_pass(const Pass& p, int batch, const float* src, float* r_save, float* dst,
-             hipStream_t st) {
+             hipStream_t st, SumIn si = SumIn{1, 0}) {
   if (p.conv) {
     const int rpt = conv_rpt(p.K, p.sep);
     const int tiles_x = (p.n_in + kTileX - 1) / kTileX;
@@ -892,22 +931,23 @@ int run_pass(const Pass& p, int batch, const float* src, float* r_save, float* d
     if (rpt == 2)
       hipLaunchKernelGGL(det_corr_kernel<2>, grid, dim3(kThreads), lds, st, src, p.n_in, tiles_x,
                          p.K, lo, p.w, 1, p.sep, p.pre, p.post, (const float*)nullptr, EwList{},
-                         r_save, dst);
+                         r_save, dst, si);
     else
       hipLaunchKernelGGL(det_corr_kernel<1>, grid, dim3(kThreads), lds, st, src, p.n_in, tiles_x,
                          p.K, lo, p.w, 1, p.sep, p.pre, p.post, (const float*)nullptr, EwList{},
-                         r_save, dst);
+                         r_save, dst, si);
     return launch_status("det_corr_kernel");
   }
   const int npix = p.n_out * p.n_out;
-  const bool aligned = ((uintptr_t)src | (uintptr_t)dst | (uintptr_t)r_save) % 16 == 0;
+  const bool aligned = ((uintptr_t)src | (uintptr_t)dst | (uintptr_t)r_save) % 16 == 0 &&
+                       (si.n == 1 || si.stride % 4 == 0);
   if (p.k == 1 && npix % 4 == 0 && aligned) {
     hipLaunchKernelGGL(det_ew_kernel, dim3((npix / 4 + 255) / 256, batch), dim3(256), 0, st, src,
-                       npix, p.pre, p.post, r_save, dst);
+                       npix, p.pre, p.post, r_save, dst, si);
     return launch_status("det_ew_kernel");
   }
   hipLaunchKernelGGL(det_down_kernel, dim3((npix + 255) / 256, batch), dim3(256), 0, st, src,
-                     p.n_in, p.k, p.n_out, p.pre, p.post, r_save, dst);
+                     p.n_in, p.k, p.n_out, p.pre, p.post, r_save, dst, si);
   return launch_status("det_down_kernel");
 }
 
@@ -920,7 +960,7 @@ bool can_fuse(const std::vector<Pass>& passes, size_t i) {
 }
 
 int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, float* dst,
-                  hipStream_t st) {
+                  hipStream_t st, SumIn si = SumIn{1, 0}) {
   const int rpt = conv_rpt(c.K, c.sep);
   const int k = d.k;
   const int th = (kTileY * rpt / k) * k, tw = (kTileX / k) * k;
@@ -931,14 +971,61 @@ int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, flo
   const dim3 grid(tiles_x * tiles_y, batch);
   if (rpt == 2)
     hipLaunchKernelGGL(det_conv_down_kernel<2>, grid, dim3(kThreads), lds, st, src, c.n_in, th,
-                       tw, tiles_x, c.K, lo, c.w, c.sep, c.pre, c.post, k, d.n_out, d.post, dst);
+                       tw, tiles_x, c.K, lo, c.w, c.sep, c.pre, c.post, k, d.n_out, d.post, dst,
+                       si);
   else
     hipLaunchKernelGGL(det_conv_down_kernel<1>, grid, dim3(kThreads), lds, st, src, c.n_in, th,
-                       tw, tiles_x, c.K, lo, c.w, c.sep, c.pre, c.post, k, d.n_out, d.post, dst);
+                       tw, tiles_x, c.K, lo, c.w, c.sep, c.pre, c.post, k, d.n_out, d.post, dst,
+                       si);
   return launch_status("det_conv_down_kernel");
 }
 
 }  // namespace
+
+size_t detector_ws_bytes(int n, int batch, const dlx_detector_op* ops, int nops) {
+  std::vector<Pass> passes;
+  if (batch < 0 || plan(n, ops, nops, passes)) return 0;
+  return layout(passes, batch, false).total;
+}
+
+int detector_apply_sum(int n, int batch, int nsum, long sum_stride, const float* in,
+                       const dlx_detector_op* ops, int nops, float* out, void* workspace,
+                       size_t workspace_bytes, hipStream_t s) {
+  std::vector<Pass> passes;
+  int st = plan(n, ops, nops, passes);
+  if (st) return st;
+  DLX_CHECK_ARG(batch >= 0, "negative detector batch");
+  DLX_CHECK_ARG(nsum >= 1 && (nsum == 1 || batch == 1),
+                "a summed detector input needs one image and nsum >= 1 (got batch %d, nsum %d)",
+                batch, nsum);
+  if (batch == 0) return DLX_OK;
+  DLX_CHECK_ARG(in && out, "null image pointer to the detector chain");
+  if ((st = check_grid(passes, batch))) return st;
+  const Layout L = layout(passes, batch, false);
+  if (workspace_bytes < L.total || (L.total && !workspace)) {
+    set_error("detector chain workspace too small: need %zu bytes, got %zu", L.total,
+              workspace_bytes);
+    return DLX_ERR_WORKSPACE;
+  }
+  char* ws = (char*)workspace;
+  const float* src = in;
+  const size_t np = passes.size();
+  int buf = 0;
+  for (size_t i = 0; i < np;) {
+    const bool fuse = can_fuse(passes, i);
+    const size_t next = i + (fuse ? 2 : 1);
+    float* dst = next == np ? out : (float*)(ws + L.g_off[buf]);
+    buf ^= 1;
+    const SumIn si = i == 0 ? SumIn{nsum, sum_stride} : SumIn{1, 0};
+    st = fuse ? run_conv_down(passes[i], passes[i + 1], batch, src, dst, s, si)
+              : run_pass(passes[i], batch, src, nullptr, dst, s, si);
+    if (st) return st;
+    src = dst;
+    i = next;
+  }
+  return DLX_OK;
+}
+
 }  // namespace dlx
 
 using namespace dlx;
@@ -966,36 +1053,8 @@ int dlx_detector_apply(int32_t n, int32_t batch, const float* in, const dlx_dete
                        int32_t nops, float* out, void* workspace, size_t workspace_bytes,
                        void* stream) {
   clear_error();
-  std::vector<Pass> passes;
-  int st = plan(n, ops, nops, passes);
-  if (st) return st;
-  DLX_CHECK_ARG(batch >= 0, "negative detector batch");
-  if (batch == 0) return DLX_OK;
-  DLX_CHECK_ARG(in && out, "null image pointer to dlx_detector_apply");
-  if ((st = check_grid(passes, batch))) return st;
-  const Layout L = layout(passes, batch, false);
-  if (workspace_bytes < L.total || (L.total && !workspace)) {
-    set_error("dlx_detector_apply workspace too small: need %zu bytes, got %zu", L.total,
-              workspace_bytes);
-    return DLX_ERR_WORKSPACE;
-  }
-  hipStream_t s = (hipStream_t)stream;
-  char* ws = (char*)workspace;
-  const float* src = in;
-  const size_t np = passes.size();
-  int buf = 0;
-  for (size_t i = 0; i < np;) {
-    const bool fuse = can_fuse(passes, i);
-    const size_t next = i + (fuse ? 2 : 1);
-    float* dst = next == np ? out : (float*)(ws + L.g_off[buf]);
-    buf ^= 1;
-    st = fuse ? run_conv_down(passes[i], passes[i + 1], batch, src, dst, s)
-              : run_pass(passes[i], batch, src, nullptr, dst, s);
-    if (st) return st;
-    src = dst;
-    i = next;
-  }
-  return DLX_OK;
+  return detector_apply_sum(n, batch, 1, 0, in, ops, nops, out, workspace, workspace_bytes,
+                            (hipStream_t)stream);
 }
 
 }  // extern "C"
@@ -1157,11 +1216,11 @@ int detector_vjp_impl(int32_t n, int32_t batch, const float* in, const dlx_detec
       if (rpt == 2)
         hipLaunchKernelGGL(det_corr_kernel<2>, grid, dim3(kThreads), lds, s, gr, p.n_in, tiles_x,
                            p.K, lo, p.w, 0, p.sep, EwList{}, EwList{}, x_in(ii), pre,
-                           (float*)nullptr, adst);
+                           (float*)nullptr, adst, SumIn{1, 0});
       else
         hipLaunchKernelGGL(det_corr_kernel<1>, grid, dim3(kThreads), lds, s, gr, p.n_in, tiles_x,
                            p.K, lo, p.w, 0, p.sep, EwList{}, EwList{}, x_in(ii), pre,
-                           (float*)nullptr, adst);
+                           (float*)nullptr, adst, SumIn{1, 0});
       if ((st = launch_status("det_corr_kernel"))) return st;
     } else {
       const int npix = p.n_in * p.n_in;

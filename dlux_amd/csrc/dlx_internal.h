@@ -243,6 +243,14 @@ int cache_acquire_raw(const std::vector<double>& key, size_t bytes, hipStream_t 
                       void** ptr, void** handle, bool* fresh);
 int cache_ready(void* handle, hipStream_t s);
 
+// The detector chain (dlx_detector.hip) on one image that is the in-order sum
+// of nsum planes sum_stride floats apart (nsum = 1: `batch` images), and its
+// workspace size.
+size_t detector_ws_bytes(int n, int batch, const dlx_detector_op* ops, int nops);
+int detector_apply_sum(int n, int batch, int nsum, long sum_stride, const float* in,
+                       const dlx_detector_op* ops, int nops, float* out, void* workspace,
+                       size_t workspace_bytes, hipStream_t s);
+
 // Holds a cache entry from its acquire (pass &lease.h) to its release, and
 // releases it on scope exit too, so an error return never leaves an entry
 // pinned.  release() records the last use on the stream after the work that

@@ -75,8 +75,36 @@ class Telescope(Instrument):
             return self.detector.model(psf, return_psf=return_psf)
         return psf if return_psf else psf.data
 
+    def _fused_detector_model(self):
+        """(image, pixel scale) of a point source through optics that lower to the
+        fused PSF and a LayeredDetector whose layers all have a chain form, from
+        one dlx_psf_poly_detector call (the wavelength sum fused into the
+        detector's first pass); None otherwise."""
+        from .detectors import LayeredDetector
+        from .sources import PointSource
+        if type(self.source) is not PointSource or not isinstance(self.detector,
+                                                                   LayeredDetector):
+            return None
+        ops, scale = self.detector._chain()
+        fused = getattr(self.optics, "_fused_psf", None)
+        if ops is None or fused is None:
+            return None
+        src = self.source.normalise()
+        weights = np.asarray(src.weights * src.flux, float)
+        wavelengths = np.atleast_1d(np.asarray(src.wavelengths, float))
+        out = fused(wavelengths, np.atleast_1d(weights), np.asarray(src.position, float),
+                    detector_ops=ops)
+        if out is None:
+            return None
+        img, p_out = out
+        return img, p_out * scale
+
     def model(self, return_psf: bool = False):
         """instruments.py:140-171."""
+        fused = self._fused_detector_model()
+        if fused is not None:
+            img, ps = fused
+            return PSF(img, ps) if return_psf else img
         psfs = self.optics.model(self.source, return_psf=True)
         data = psfs.data
         if psfs.ndim > 0:

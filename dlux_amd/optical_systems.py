@@ -223,8 +223,8 @@ class OpticalSystem(BaseOpticalSystem):
             return PSF(psf, np.mean(wf.pixel_scale))
         return psf
 
-    def _fused_psf(self, wavelengths, weights, offset):  # overridden where lowerable
-        return None
+    def _fused_psf(self, wavelengths, weights, offset, detector_ops=None):
+        return None   # overridden where the layers lower to one fused call
 
     def model(self, source, return_wf: bool = False, return_psf: bool = False):
         return source.model(self, return_wf, return_psf)
@@ -310,7 +310,10 @@ class LayeredOpticalSystem(OpticalSystem):
         plan.finalise(self.__dict__.setdefault("_plan_cache", {}))
         return plan, plan.tilt + offset
 
-    def _fused_psf(self, wavelengths, weights, offset):
+    def _fused_psf(self, wavelengths, weights, offset, detector_ops=None):
+        """(PSF, p_out) from one dlx_psf_poly call, or None when the layers do not
+        lower; with `detector_ops`, the detector image from one
+        dlx_psf_poly_detector call instead."""
         layers = list(self.layers.values())
         if not layers or not isinstance(layers[-1], MFTLayer):
             return None
@@ -319,10 +322,14 @@ class LayeredOpticalSystem(OpticalSystem):
         if lowered is None:
             return None
         plan, tilt = lowered
+        kw = dict(amp=plan.amp, opd=plan.opd, phase=plan.phase, scale=plan.scale, tilt=tilt,
+                  focal_length=prop.focal_length, inverse=prop.inverse)
+        if detector_ops is not None:
+            img = runtime.psf_poly_detector(plan.n, prop.npixels, wavelengths, weights, plan.dx,
+                                            float(prop.pixel_scale), detector_ops, **kw)
+            return img, float(prop.pixel_scale)
         psf = runtime.psf_poly(plan.n, prop.npixels, wavelengths, weights, plan.dx,
-                               float(prop.pixel_scale), amp=plan.amp, opd=plan.opd,
-                               phase=plan.phase, scale=plan.scale, tilt=tilt,
-                               focal_length=prop.focal_length, inverse=prop.inverse)
+                               float(prop.pixel_scale), **kw)
         return psf, float(prop.pixel_scale)
 
 
@@ -354,15 +361,19 @@ class ParametricLayeredOpticalSystem(ParametricOpticalSystem, LayeredOpticalSyst
         outputs["final_wavefront"] = wf
         return wf, outputs
 
-    def _fused_psf(self, wavelengths, weights, offset):
+    def _fused_psf(self, wavelengths, weights, offset, detector_ops=None):
         lowered = self._lower(list(self.layers.values()), offset)
         if lowered is None:
             return None
         plan, tilt = lowered
         n_out, p_out, f = self._focus_params()
-        psf = runtime.psf_poly(plan.n, n_out, wavelengths, weights, plan.dx, p_out,
-                               amp=plan.amp, opd=plan.opd, phase=plan.phase, scale=plan.scale,
-                               tilt=tilt, focal_length=f)
+        kw = dict(amp=plan.amp, opd=plan.opd, phase=plan.phase, scale=plan.scale, tilt=tilt,
+                  focal_length=f)
+        if detector_ops is not None:
+            img = runtime.psf_poly_detector(plan.n, n_out, wavelengths, weights, plan.dx, p_out,
+                                            detector_ops, **kw)
+            return img, p_out
+        psf = runtime.psf_poly(plan.n, n_out, wavelengths, weights, plan.dx, p_out, **kw)
         return psf, p_out
 
 

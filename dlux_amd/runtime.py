@@ -1013,6 +1013,47 @@ def detector(images, ops) -> torch.Tensor:
     return _detector_impl(images, ops)
 
 
+def psf_poly_detector(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: float,
+                      pixel_scale_out: float, ops, amp=None, opd=None, phase=None, scale=1.0,
+                      tilt=(0.0, 0.0), center=0.0, focal_length=None, inverse=False,
+                      precision: str | None = None) -> torch.Tensor:
+    """The fused polychromatic PSF (as psf_poly) through the detector chain
+    `ops` (as detector), with the wavelength sum fused into the chain's first
+    pass (dlx_psf_poly_detector): equal to detector(psf_poly(...), ops).
+    When gradients are recorded for the pupil inputs or the ops' parameters it
+    runs as that composition, whose two parts are differentiable."""
+    wl, w, g, prec = _poly_args(n_in, n_out, wavelengths, weights, focal_length, inverse,
+                                precision)
+    dev = device()
+    amp_t, opd_t, ph_t = _pupil_inputs(n_in, amp, opd, phase, dev)
+    if torch.is_grad_enabled():
+        tracked = any(t is not None and t.requires_grad for t in (amp_t, opd_t, ph_t))
+        for _kind, arg in ops:
+            parts = arg if isinstance(arg, (tuple, list)) else (arg,)
+            tracked = tracked or any(_tracked_param(a) for a in parts)
+        if tracked:
+            psf = psf_poly(n_in, n_out, wl, w, pixel_scale_in, pixel_scale_out, amp=amp_t,
+                           opd=opd_t, phase=ph_t, scale=scale, tilt=tilt, center=center,
+                           focal_length=focal_length, inverse=inverse, precision=precision)
+            return detector(psf, ops)
+    lib = _lib.load()
+    arr, keep = _detector_ops(ops, dev)
+    side = ctypes.c_int32(0)
+    _lib.check(lib.dlx_detector_out_side(int(n_out), arr, len(ops), ctypes.byref(side)),
+               "dlx_detector_out_side")
+    out = torch.empty((side.value, side.value), dtype=torch.float32, device=dev)
+    ws = workspace(lib.dlx_psf_poly_detector_workspace_size(g, wl.size, prec, arr, len(ops)),
+                   dev)
+    st = lib.dlx_psf_poly_detector(g, wl.size, _lib.dptr(wl), _lib.dptr(w), float(pixel_scale_in),
+                                   float(pixel_scale_out), float(center), float(tilt[0]),
+                                   float(tilt[1]), _ptr(amp_t), _ptr(opd_t), _ptr(ph_t),
+                                   float(scale), arr, len(ops), out.data_ptr(), prec,
+                                   ws.data_ptr(), ws.numel(), stream_handle())
+    _lib.check(st, "dlx_psf_poly_detector")
+    del keep
+    return out
+
+
 def detector_vjp(images, ops, dout) -> torch.Tensor:
     """(d detector(images) / d images)^T dout (dlx_detector_vjp)."""
     return _detector_vjp_impl(images, ops, dout)

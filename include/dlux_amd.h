@@ -576,6 +576,30 @@ int dlx_detector_apply(int32_t n, int32_t batch, const float* in, const dlx_dete
                        void* stream);
 
 /*
+ * The polychromatic PSF of dlx_psf_poly (same arguments) seen through the
+ * detector chain `ops` (as dlx_detector_apply on the N_out x N_out PSF), with
+ * the wavelength sum fused into the chain: the contractions' |E|^2 epilogue
+ * writes each wavelength's w_l |E_l|^2 plane, and the chain's first pass
+ * (convolve, convolve + downsample, downsample or elementwise) loads the sum
+ * over the planes in wavelength order as it reads each pixel, so the PSF is
+ * never written or read back.  The result equals dlx_psf_poly followed by
+ * dlx_detector_apply.  `out` holds out_side^2 floats (dlx_detector_out_side).
+ * Replaces: Telescope.model for a point source (instruments.py:140-171):
+ * source.model(optics) -> OpticalSystem.propagate (optical_systems.py:147-223)
+ * -> detector.model (detectors.py:103-128, layers/detector_layers.py:100-293).
+ */
+size_t dlx_psf_poly_detector_workspace_size(const dlx_mft_geom* geom, int32_t nwl,
+                                            int32_t precision, const dlx_detector_op* ops,
+                                            int32_t nops);
+int dlx_psf_poly_detector(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,
+                          const double* weight, double pixel_scale_in,
+                          double pixel_scale_out, double center, double tilt_x,
+                          double tilt_y, const float* amp, const float* opd,
+                          const float* phase, double scale, const dlx_detector_op* ops,
+                          int32_t nops, float* out, int32_t precision, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
+/*
  * Vector-Jacobian product of dlx_detector_apply with respect to its input
  * images: din = (d out / d in)^T dout.  The chain is re-run forward to keep
  * each pass's input and pre-store values (workspace sized with for_vjp = 1).

@@ -75,6 +75,26 @@ def test_abi_rejects_bad_arguments_without_gpu(lib):
     assert st == 4 and b"power-of-two" in lib.dlx_last_error()
 
 
+def test_psf_poly_detector_abi_validation_without_gpu(lib):
+    """The fused PSF + detector entry point validates the chain against the PSF
+    side and its workspace before any device work."""
+    from dlux_amd import _lib
+    g = _lib.MftGeom(n_in=64, n_out=30, inverse=0, has_focal=0, focal_length=0.0,
+                     focal_shift=0.0)
+    ops = (_lib.DetectorOp * 1)()
+    ops[0].kind, ops[0].size = _lib.DLX_DET_DOWNSAMPLE, 4
+    one = _lib.dptr([1e-6])
+    st = lib.dlx_psf_poly_detector(g, 1, one, _lib.dptr([1.0]), 1 / 64, 2.5e-7, 0.0, 0.0, 0.0,
+                                   None, None, None, 1.0, ops, 1, 4096, 1, None, 0, None)
+    assert st == 1 and b"divisible" in lib.dlx_last_error()
+    ops[0].size = 3
+    need = lib.dlx_psf_poly_detector_workspace_size(g, 1, 1, ops, 1)
+    assert need >= lib.dlx_psf_poly_workspace_size(g, 1, 1)
+    st = lib.dlx_psf_poly_detector(g, 1, one, _lib.dptr([1.0]), 1 / 64, 2.5e-7, 0.0, 0.0, 0.0,
+                                   None, None, None, 1.0, ops, 1, 4096, 1, None, 0, None)
+    assert st == 3 and b"workspace" in lib.dlx_last_error()
+
+
 def test_workspace_queries(lib):
     from dlux_amd import _lib
     g = _lib.MftGeom(n_in=512, n_out=256, inverse=0, has_focal=0, focal_length=0.0,
