@@ -356,6 +356,54 @@ def _detector_leg(torch, steps, check, warmup=3):
     return res
 
 
+def _telescope_leg(torch, runtime, workloads, base, cfg, steps, ref, warmup=3):
+    """Telescope.model of a point source through the headline optics and a
+    detector (ApplyJitter sigma 0.5 px 9 x 9, ApplyPixelResponse, Downsample 4,
+    ApplySaturation, AddConstant): the detector fused into the wavelength sum
+    (one dlx_psf_poly_detector call) against the PSF then the detector chain."""
+    import dlux_amd as dl
+    from dlux_amd import layers
+    n = cfg.n_out
+    g = torch.Generator(device="cuda").manual_seed(13)
+    resp = 0.9 + 0.2 * torch.rand((n, n), generator=g, device="cuda")
+    det = dl.LayeredDetector([layers.ApplyJitter(0.5, 9, 3), layers.ApplyPixelResponse(resp),
+                              layers.Downsample(4), layers.ApplySaturation(2e-3),
+                              layers.AddConstant(1e-6)])
+    src = dl.PointSource(cfg.wavelengths, weights=cfg.weights)
+    tel = dl.Telescope(base, src, det)
+
+    def unfused():
+        return det.model(base.model(src, return_psf=True))
+
+    out = {}
+    with runtime.precision("bf16x3"):
+        for name, fn in (("fused", tel.model), ("unfused", unfused)):
+            for _ in range(warmup):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                img = fn()
+            torch.cuda.synchronize()
+            out[name] = ((time.perf_counter() - t0) * 1e3 / steps, img)
+    err = None
+    if ref is not None:
+        from oracle import dlux_oracle as orc
+        chain = [("convolve", orc.jitter_kernel(0.5, 9, 3)),
+                 ("pixel_response", resp.double().cpu().numpy()), ("downsample", 4),
+                 ("saturate", float(np.float32(2e-3))), ("add", float(np.float32(1e-6)))]
+        img_ref = orc.detector(ref, chain)
+        err = float(np.max(np.abs(out["fused"][1].double().cpu().numpy() - img_ref)) /
+                    np.max(np.abs(img_ref)))
+    return {"workload": "telescope: point source (64 lambda, flat) through the poly optics "
+                        "and a detector (jitter 9x9 sigma 0.5 px -> pixel response -> "
+                        "downsample 4 -> saturation -> constant), split-BF16x3",
+            "fused_ms_per_image": out["fused"][0], "unfused_ms_per_image": out["unfused"][0],
+            "images_per_s": 1e3 / out["fused"][0],
+            "identical_to_unfused": bool(torch.equal(out["fused"][1], out["unfused"][1])),
+            "max_rel_err_vs_fp64": err}
+
+
 def _aperture_leg(torch, steps, warmup=3):
     """Dynamic aperture: a 512^2 pupil from an aberrated circular aperture (36
     Zernikes on its own transformed coordinates) and a 3-arm spider, whose
@@ -560,6 +608,10 @@ def main():
     print("[bench] detector leg", file=sys.stderr, flush=True)
     detector = None if args.no_detector else _detector_leg(torch, max(args.steps // 5, 10),
                                                            not args.no_cpu)
+    print("[bench] telescope leg", file=sys.stderr, flush=True)
+    telescope = None if args.no_detector or "bf16x3" not in args.forms else _telescope_leg(
+        torch, runtime, workloads, base, cfg, max(args.steps // 5, 10),
+        None if args.no_cpu else ref)
     print("[bench] dynamic-aperture leg", file=sys.stderr, flush=True)
     aperture = None if args.no_aperture else _aperture_leg(torch, max(args.steps // 5, 10))
     print("[bench] lct leg", file=sys.stderr, flush=True)
@@ -624,6 +676,7 @@ def main():
         "batched": batched,
         "asm": asm,
         "detector": detector,
+        "telescope": telescope,
         "aperture": aperture,
         "lct": lct,
         "serving": serve,

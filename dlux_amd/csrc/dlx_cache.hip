@@ -12,6 +12,10 @@
 // stream that used an entry keeps its own "last use" event on the entry, and
 // eviction synchronises on every one of them before the block is reused, so a
 // block read by work still queued on one stream is never handed to another.
+// An evicted entry's block becomes a spare without the host waiting: it keeps
+// the entry's events, and the stream that next takes the block waits on them
+// on the device (hipStreamWaitEvent) before writing it; only a release to
+// hipFree synchronises on them.
 // An entry is pinned from its acquire to its release: eviction skips pinned
 // entries (a call that holds two entries, such as the gradient's forward and
 // adjoint matrices, can never have the first evicted to make room for the
@@ -36,6 +40,7 @@ constexpr int kMaxUsers = 16;
 
 struct MatEntry {
   std::vector<double> key;
+  int dev = 0;
   void* ptr = nullptr;
   size_t bytes = 0;
   hipEvent_t ready = nullptr;
@@ -57,11 +62,21 @@ struct MatCache {
   size_t used = 0;                    // bytes of live entries
   // blocks of evicted entries, kept for reuse: hipFree synchronises the whole
   // device, which would stall every stream of a serving loop on each eviction
-  std::vector<std::pair<void*, size_t>> spare;
+  struct Spare {
+    void* ptr;
+    size_t bytes;
+    int dev;
+    std::vector<hipEvent_t> after;   // the evicted entry's ready and last-use events
+  };
+  std::vector<Spare> spare;
   size_t spare_bytes = 0;
   uint64_t tick = 0, hits = 0, misses = 0;
   bool init = false;
 };
+
+// Per-thread bypass (dlx_cache_bypass): calls from this thread generate their
+// matrices into their workspace and never allocate, evict or wait.
+thread_local bool t_bypass = false;
 
 MatCache& cache() {
   static MatCache c;
@@ -72,25 +87,32 @@ MatCache& cache() {
   return c;
 }
 
-// Retire an entry: wait for its last use on every stream that read it (only
-// that entry's work, not the device), then keep its block as a spare (or free
-// it when `release`).
+// Retire an entry.  As a spare (release false) its block keeps the entry's
+// events, which the next taker's stream waits on; released (true), the host
+// waits for its last use on every stream that read it (only that entry's
+// work, not the device) and frees it.
 void free_entry(MatCache& c, MatEntry* e, bool release) {
-  for (auto& u : e->users) (void)hipEventSynchronize(u.second);
-  (void)hipEventSynchronize(e->ready);
   if (release) {
+    for (auto& u : e->users) (void)hipEventSynchronize(u.second);
+    (void)hipEventSynchronize(e->ready);
     (void)hipFree(e->ptr);
+    (void)hipEventDestroy(e->ready);
+    for (auto& u : e->users) (void)hipEventDestroy(u.second);
   } else {
-    c.spare.push_back({e->ptr, e->bytes});
+    MatCache::Spare b{e->ptr, e->bytes, e->dev, {e->ready}};
+    for (auto& u : e->users) b.after.push_back(u.second);
+    c.spare.push_back(std::move(b));
     c.spare_bytes += e->bytes;
   }
-  (void)hipEventDestroy(e->ready);
-  for (auto& u : e->users) (void)hipEventDestroy(u.second);
   delete e;
 }
 
 void release_spares(MatCache& c) {
-  for (auto& b : c.spare) (void)hipFree(b.first);
+  for (auto& b : c.spare) {
+    for (hipEvent_t ev : b.after) (void)hipEventSynchronize(ev);
+    (void)hipFree(b.ptr);
+    for (hipEvent_t ev : b.after) (void)hipEventDestroy(ev);
+  }
   c.spare.clear();
   c.spare_bytes = 0;
 }
@@ -114,9 +136,11 @@ bool make_room(MatCache& c, size_t need) {
   return true;
 }
 
-MatEntry* new_entry(std::vector<double>&& key, void* mem, size_t bytes, uint64_t tick) {
+MatEntry* new_entry(std::vector<double>&& key, int dev, void* mem, size_t bytes,
+                    uint64_t tick) {
   MatEntry* e = new MatEntry();
   e->key = std::move(key);
+  e->dev = dev;
   e->ptr = mem;
   e->bytes = bytes;
   e->tick = tick;
@@ -128,21 +152,31 @@ MatEntry* new_entry(std::vector<double>&& key, void* mem, size_t bytes, uint64_t
   return e;
 }
 
-// A block of at least `need` bytes: the smallest spare that is no more than
-// twice as large, else a new allocation (spares are released first when the
-// budget would be exceeded).  Returns its size in *bytes; NULL on failure.
-void* take_block(MatCache& c, size_t need, size_t* bytes) {
+// A block of at least `need` bytes on device `dev` for work on stream s: the
+// smallest spare there that is no more than twice as large (s then waits, on
+// the device, for the work that last read it), else a new allocation (spares
+// are released first when the budget would be exceeded).  Returns its size in
+// *bytes; NULL on failure.
+void* take_block(MatCache& c, size_t need, int dev, hipStream_t s, size_t* bytes) {
   size_t best = c.spare.size();
   for (size_t i = 0; i < c.spare.size(); ++i)
-    if (c.spare[i].second >= need && c.spare[i].second <= 2 * need &&
-        (best == c.spare.size() || c.spare[i].second < c.spare[best].second))
+    if (c.spare[i].dev == dev && c.spare[i].bytes >= need && c.spare[i].bytes <= 2 * need &&
+        (best == c.spare.size() || c.spare[i].bytes < c.spare[best].bytes))
       best = i;
   if (best < c.spare.size()) {
-    void* p = c.spare[best].first;
-    *bytes = c.spare[best].second;
-    c.spare_bytes -= *bytes;
+    MatCache::Spare b = std::move(c.spare[best]);
     c.spare.erase(c.spare.begin() + best);
-    return p;
+    c.spare_bytes -= b.bytes;
+    bool ordered = true;
+    for (hipEvent_t ev : b.after)
+      if (hipStreamWaitEvent(s, ev, 0) != hipSuccess) ordered = false;
+    if (!ordered) {   // no device-side order: wait on the host instead
+      (void)hipGetLastError();
+      for (hipEvent_t ev : b.after) (void)hipEventSynchronize(ev);
+    }
+    for (hipEvent_t ev : b.after) (void)hipEventDestroy(ev);
+    *bytes = b.bytes;
+    return b.ptr;
   }
   if (c.used + c.spare_bytes + need > c.budget) release_spares(c);
   void* mem = nullptr;
@@ -171,7 +205,7 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
     (void)hipGetLastError();
     return DLX_OK;
   }
-  if (cs != hipStreamCaptureStatusNone || c.budget == 0) return DLX_OK;
+  if (cs != hipStreamCaptureStatusNone || c.budget == 0 || t_bypass) return DLX_OK;
   int dev = 0;
   DLX_HIP(hipGetDevice(&dev));
 
@@ -206,9 +240,9 @@ int cache_acquire(const dlx_mft_geom& g, int nb, const double* wl, const double*
   if (need > c.budget) return DLX_OK;  // larger than the whole cache: caller generates
   if (!make_room(c, need)) return DLX_OK;  // pinned entries fill it: caller generates
   size_t bytes = 0;
-  void* mem = take_block(c, need, &bytes);
+  void* mem = take_block(c, need, dev, s, &bytes);
   if (!mem) return DLX_OK;  // no memory for the cache: caller generates into its workspace
-  MatEntry* e = new_entry(std::move(key), mem, bytes, ++c.tick);
+  MatEntry* e = new_entry(std::move(key), dev, mem, bytes, ++c.tick);
   if (!e) {
     (void)hipFree(mem);
     set_error("hipEventCreate failed in the matrix cache");
@@ -247,7 +281,7 @@ int cache_acquire_raw(const std::vector<double>& key_in, size_t bytes, hipStream
     (void)hipGetLastError();
     return DLX_OK;
   }
-  if (cs != hipStreamCaptureStatusNone || c.budget == 0) return DLX_OK;
+  if (cs != hipStreamCaptureStatusNone || c.budget == 0 || t_bypass) return DLX_OK;
   int dev = 0;
   DLX_HIP(hipGetDevice(&dev));
   std::vector<double> key;
@@ -290,9 +324,9 @@ int cache_acquire_raw(const std::vector<double>& key_in, size_t bytes, hipStream
   }
   if (!make_room(c, need)) return DLX_OK;
   size_t got = 0;
-  void* mem = take_block(c, need, &got);
+  void* mem = take_block(c, need, dev, s, &got);
   if (!mem) return DLX_OK;
-  MatEntry* e = new_entry(std::move(key), mem, got, ++c.tick);
+  MatEntry* e = new_entry(std::move(key), dev, mem, got, ++c.tick);
   if (!e) {
     (void)hipFree(mem);
     set_error("hipEventCreate failed in the matrix cache");
@@ -389,6 +423,12 @@ extern "C" int dlx_cache_set_budget(size_t bytes) {
   c.budget = bytes;
   make_room(c, 0);
   if (c.used + c.spare_bytes > c.budget) release_spares(c);
+  return DLX_OK;
+}
+
+extern "C" int dlx_cache_bypass(int32_t on) {
+  clear_error();
+  t_bypass = on != 0;
   return DLX_OK;
 }
 

@@ -375,12 +375,13 @@ def _psf_poly_impl(p: dict, amp_t, opd_t, ph_t, out=None, ws=None) -> torch.Tens
 
 
 def _psf_poly_vjp_impl(p: dict, amp_t, opd_t, ph_t, dpsf, need_opd=True, need_phase=False,
-                      need_amp=False):
+                      need_amp=False, ws=None):
     lib = _lib.load()
     dev = p["dev"]
     n = p["n_in"]
     need = lib.dlx_psf_poly_vjp_workspace_size(p["g"], p["wl"].size, p["prec"])
-    ws = workspace(need, dev)
+    if ws is None or ws.numel() < need:
+        ws = workspace(need, dev)
     g_opd = torch.empty((n, n), dtype=torch.float32, device=dev) if need_opd else None
     g_ph = torch.empty((n, n), dtype=torch.float32, device=dev) if need_phase else None
     g_amp = torch.empty((n, n), dtype=torch.float32, device=dev) if need_amp else None
@@ -453,9 +454,11 @@ def psf_poly_vjp(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: fl
                  pixel_scale_out: float, dpsf, amp=None, opd=None, phase=None, scale=1.0,
                  tilt=(0.0, 0.0), center=0.0, focal_length=None, inverse=False,
                  precision: str | None = None, need_opd: bool = True,
-                 need_phase: bool = False, need_amp: bool = False):
+                 need_phase: bool = False, need_amp: bool = False,
+                 ws: torch.Tensor | None = None):
     """(dL/dOPD, dL/dphase) of the fused PSF for dL/dPSF = dpsf (either may be
-    None), and dL/d amp (at the fixed scale) third when ``need_amp``."""
+    None), and dL/d amp (at the fixed scale) third when ``need_amp``.  ``ws``:
+    an optional caller workspace (used when large enough)."""
     wl, w, g, prec = _poly_args(n_in, n_out, wavelengths, weights, focal_length, inverse,
                                 precision)
     dev = device()
@@ -463,7 +466,8 @@ def psf_poly_vjp(n_in: int, n_out: int, wavelengths, weights, pixel_scale_in: fl
     p = dict(dev=dev, g=g, wl=wl, w=w, prec=prec, n_in=int(n_in), n_out=int(n_out),
              dx=float(pixel_scale_in), pout=float(pixel_scale_out), center=float(center),
              tilt=(float(tilt[0]), float(tilt[1])), scale=float(scale))
-    return _psf_poly_vjp_impl(p, amp_t, opd_t, ph_t, dpsf, need_opd, need_phase, need_amp)
+    return _psf_poly_vjp_impl(p, amp_t, opd_t, ph_t, dpsf, need_opd, need_phase, need_amp,
+                              ws=ws)
 
 
 class MftBatch:
@@ -521,12 +525,14 @@ class MftBatch:
     @classmethod
     def from_arrays(cls, fields, field_index, wavelength, pixel_scale_in, n_out,
                     pixel_scale_out, weight, out_index, shift_x=0.0, shift_y=0.0,
-                    slice_index=0, n_outputs: int | None = None):
+                    slice_index=0, n_outputs: int | None = None,
+                    ws: torch.Tensor | None = None):
         """Vectorised construction: `fields` is a list of complex64 device
         tensors, each (N, N) or a stack (B, N, N); instance i uses slice
         slice_index[i] of fields[field_index[i]].  Every other argument is a
         scalar or one value per instance.  The descriptor array is filled from a
-        numpy structured array with the C layout, without per-instance Python."""
+        numpy structured array with the C layout, without per-instance Python.
+        ``ws``: an optional caller workspace (used when large enough)."""
         self = cls.__new__(cls)
         self.dev = device()
         self.lib = _lib.load()
@@ -568,7 +574,8 @@ class MftBatch:
             *[(0 if t is None else t.data_ptr()) for t in self.outputs])
         self.ws_bytes = int(self.lib.dlx_mft_batched_workspace_size(n, self.arr,
                                                                     self.n_outputs))
-        self.ws = workspace(self.ws_bytes, self.dev)
+        self.ws = ws if ws is not None and ws.numel() >= self.ws_bytes else \
+            workspace(self.ws_bytes, self.dev)
         return self
 
     def __call__(self) -> list:
@@ -1208,6 +1215,20 @@ def cache_clear() -> None:
 def cache_set_budget(nbytes: int) -> None:
     """Device memory the transfer-matrix cache may hold (0 disables it)."""
     _lib.check(_lib.load().dlx_cache_set_budget(int(nbytes)), "dlx_cache_set_budget")
+
+
+class cache_bypass:
+    """Context manager: calls made from this thread inside it skip the
+    transfer-matrix cache (dlx_cache_bypass), generating their matrices into
+    their workspace -- no allocation, eviction or wait."""
+
+    def __enter__(self):
+        _lib.check(_lib.load().dlx_cache_bypass(1), "dlx_cache_bypass")
+        return self
+
+    def __exit__(self, *exc):
+        _lib.check(_lib.load().dlx_cache_bypass(0), "dlx_cache_bypass")
+        return False
 
 
 def cache_stats() -> dict:

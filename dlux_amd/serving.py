@@ -14,15 +14,21 @@ runs on one host thread:
     wavelength counts); gradient requests run forward + hand-written VJP +
     basis projection; ASM requests run the padded angular-spectrum
     propagator;
-  * work is issued on a small pool of streams (default priority) and each
-    launch group records an event; the loop polls events, finalises the
+  * work is issued on a small pool of streams (default priority), each group
+    on the stream with the least work queued (estimated device time), so a
+    short request never waits behind a long one while another stream is free;
+    each launch group records an event; the loop polls events, finalises the
     requests of completed groups (latency = completion - arrival) and frees
     their memory, so later arrivals are admitted while earlier work still
-    runs and their kernels overlap on the device.
+    runs and their kernels overlap on the device;
+  * every stream has one workspace arena, grown to the largest group it ran
+    and reused (work on a stream is ordered), and the loop's thread skips the
+    library's transfer-matrix cache (the mix's wavelength grids rarely
+    repeat), so serving allocates nothing per request beyond its results
+    and never waits on the host for an eviction.
 
 Pupil masks, Zernike bases and their normalisation are built once per pupil
-size and stay resident; transfer matrices of the fused gradient path stay in
-the library's device cache.
+size and stay resident.
 
 The PSF of a request is sum_l w_l |MFT_l(P_l)|^2 with flat weights (the
 OpticalSystem.propagate chain of optical_systems.py:147-223); a gradient
@@ -126,9 +132,10 @@ def make_requests(rate: float, duration: float, cfg: MixConfig = MIX, seed: int 
 
 
 class _Group:
-    def __init__(self, reqs, event, keep, nbytes, finish):
+    def __init__(self, reqs, event, keep, nbytes, finish, stream=0, cost=0.0):
         self.reqs, self.event, self.keep, self.nbytes, self.finish = (reqs, event, keep,
                                                                       nbytes, finish)
+        self.stream, self.cost = stream, cost
 
 
 class Server:
@@ -143,7 +150,10 @@ class Server:
         self.precision = precision
         self.cfg = cfg
         self._geo = {}
-        self._next_stream = 0
+        # estimated device time (s) of the work queued on each stream and not yet done
+        self._queued = [0.0] * n_streams
+        self._arena = [None] * n_streams   # per-stream workspace, reused
+        self._cur = 0                      # stream index of the group being launched
 
     # ---- resident pupil data --------------------------------------------
     def geometry(self, n: int):
@@ -186,10 +196,28 @@ class Server:
         keep.append(h)
         return h.to(self.dev, non_blocking=True)
 
+    # Device-time estimates used to place work: the contractions at ~300 TFLOP/s
+    # (8 (N_out N_in^2 + N_out^2 N_in) FLOPs per wavelength, four contractions'
+    # worth for a gradient) and the angular spectrum's four FFT passes at ~4 TB/s.
+    def cost(self, r: Request) -> float:
+        if r.kind == "asm":
+            return 4 * 16.0 * r.n_out ** 2 / 4e12
+        f = 8.0 * (r.n_out * r.n_in ** 2 + r.n_out ** 2 * r.n_in) * len(r.wavelengths)
+        return (4.0 if r.kind == "grad" else 1.0) * f / 300e12
+
+    def _ws(self, nbytes: int) -> torch.Tensor:
+        """The current stream's workspace arena, grown (x1.5) when too small."""
+        a = self._arena[self._cur]
+        if a is None or a.numel() < nbytes:
+            size = max(int(nbytes), int(1.5 * (0 if a is None else a.numel())))
+            a = self._arena[self._cur] = runtime.workspace(size, self.dev)
+        return a
+
     def _stream(self):
-        s = self.streams[self._next_stream]
-        self._next_stream = (self._next_stream + 1) % len(self.streams)
-        return s
+        """The stream with the least queued work: a group never waits behind a
+        long one on its stream while another stream is idle."""
+        i = min(range(len(self.streams)), key=lambda k: self._queued[k])
+        return i, self.streams[i]
 
     def _launch_psf(self, reqs):
         keep, fields = [], []
@@ -213,7 +241,9 @@ class Server:
         cat = {k: np.concatenate(v) for k, v in cols.items()}
         batch = runtime.MftBatch.from_arrays(
             fields, cat["fi"], cat["wl"], cat["dx"], cat["nout"], P_OUT, cat["w"], cat["oi"],
-            slice_index=cat["si"], n_outputs=len(reqs))
+            slice_index=cat["si"], n_outputs=len(reqs), ws=self._arena[self._cur])
+        if batch.ws is not self._arena[self._cur]:   # grew: keep the new one
+            self._arena[self._cur] = batch.ws
         outs = batch()
         keep.append(batch)
 
@@ -229,14 +259,20 @@ class Server:
         opd = runtime.eval_basis(basis, c)
         w = np.full(len(r.wavelengths), 1.0 / len(r.wavelengths))
         kw = dict(amp=mask, opd=opd, scale=scale, precision=self.precision)
-        psf = runtime.psf_poly(r.n_in, r.n_out, r.wavelengths, w, D / r.n_in, P_OUT, **kw)
+        lib = runtime._lib.load()
+        g = runtime.geom(r.n_in, r.n_out)
+        prec = runtime._prec_code(self.precision)
+        ws = self._ws(max(int(lib.dlx_psf_poly_workspace_size(g, len(w), prec)),
+                          int(lib.dlx_psf_poly_vjp_workspace_size(g, len(w), prec))))
+        psf = runtime.psf_poly(r.n_in, r.n_out, r.wavelengths, w, D / r.n_in, P_OUT, ws=ws,
+                               **kw)
         d = self._upload(r.data, keep)
         var = torch.clamp(d, min=1.0)
         resid = r.flux * psf - d
         loss = 0.5 * torch.sum(resid.double() ** 2 / var)
         G = (r.flux * resid / var).contiguous()
         g_opd, _ = runtime.psf_poly_vjp(r.n_in, r.n_out, r.wavelengths, w, D / r.n_in, P_OUT,
-                                        G, **kw)
+                                        G, ws=ws, **kw)
         g_c = runtime.project_basis(basis, g_opd)
         keep += [c, opd, psf, d, var, resid, G, g_opd]
 
@@ -275,7 +311,7 @@ class Server:
         used, n_inflight, nxt, done = 0, 0, 0, 0
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
-        with runtime.precision(self.precision):
+        with runtime.precision(self.precision), runtime.cache_bypass():
             while done < len(reqs):
                 now = clock()
                 while nxt < len(reqs) and reqs[nxt].arrival <= now:
@@ -290,6 +326,7 @@ class Server:
                         for r in grp.reqs:
                             r.t_done = t
                         used -= grp.nbytes
+                        self._queued[grp.stream] -= grp.cost
                         n_inflight -= len(grp.reqs)
                         done += len(grp.reqs)
                     else:
@@ -325,12 +362,15 @@ class Server:
                         elif r.kind == "asm":
                             groups.append(([r], fp, lambda r=r: self._launch_asm(r)))
                     for rs, nbytes, launch in groups:
-                        s = self._stream()
+                        c = sum(self.cost(r) for r in rs)
+                        si, s = self._stream()
+                        self._queued[si] += c
+                        self._cur = si
                         with torch.cuda.stream(s):
                             keep, finish = launch()
                             ev = torch.cuda.Event()
                             ev.record(s)
-                        inflight.append(_Group(rs, ev, keep, nbytes, finish))
+                        inflight.append(_Group(rs, ev, keep, nbytes, finish, si, c))
                 elif not inflight and nxt < len(reqs) and realtime:
                     wait = reqs[nxt].arrival - clock()
                     if wait > 0:

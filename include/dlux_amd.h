@@ -644,10 +644,16 @@ int dlx_detector_vjp_params(int32_t n, int32_t batch, const float* in,
  * evicted beyond the budget (default 16 GiB, environment DLUX_AMD_CACHE_MB).
  * A budget of 0 disables caching.  The cache is bypassed for streams under
  * graph capture.  dlx_cache_clear synchronises on outstanding uses and frees
- * every entry.
+ * every entry.  An evicted entry's block is reused without the host waiting:
+ * the stream that takes it next waits on the device for the work that last
+ * read it.  dlx_cache_bypass(1) makes the calling thread's calls skip the cache
+ * (matrices generated into their workspace, nothing allocated, evicted or
+ * waited on) until dlx_cache_bypass(0): for request loops whose wavelength
+ * grids rarely repeat.
  */
 int dlx_cache_clear(void);
 int dlx_cache_set_budget(size_t bytes);
+int dlx_cache_bypass(int32_t on);
 int dlx_cache_stats(uint64_t* hits, uint64_t* misses, size_t* bytes, int32_t* entries);
 
 /*
