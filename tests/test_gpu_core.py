@@ -293,3 +293,27 @@ def test_cache_entry_shared_by_two_streams_survives_eviction(rt):
         torch.cuda.synchronize()
         rt.cache_set_budget(16 << 30)
         rt.cache_clear()
+
+
+def test_cache_bypass_skips_the_cache_and_keeps_results(rt):
+    """Calls inside runtime.cache_bypass() neither hit nor fill the matrix cache
+    (dlx_cache_bypass is per thread) and give the cached calls' results."""
+    print("[cache-bypass]", flush=True)
+    n, n_out = 64, 32
+    mask = orc.circular_aperture(n, 1.0, oversample=1)
+    wls = np.linspace(0.9e-6, 1.1e-6, 4)
+    w = np.ones(4) / 4
+    kw = dict(amp=mask, scale=1.0 / n ** 2, precision="bf16x3")
+    rt.cache_clear()
+    a = rt.psf_poly(n, n_out, wls, w, 1.0 / n, 2.5e-7, **kw).cpu().numpy()
+    s0 = rt.cache_stats()
+    with rt.cache_bypass():
+        b = rt.psf_poly(n, n_out, wls, w, 1.0 / n, 2.5e-7, **kw).cpu().numpy()
+        c = rt.psf_poly(n, n_out, wls * 1.01, w, 1.0 / n, 2.5e-7, **kw).cpu().numpy()
+    s1 = rt.cache_stats()
+    assert (s1["hits"], s1["misses"], s1["entries"]) == (s0["hits"], s0["misses"], s0["entries"])
+    assert np.array_equal(a, b)
+    d = rt.psf_poly(n, n_out, wls * 1.01, w, 1.0 / n, 2.5e-7, **kw).cpu().numpy()
+    assert np.array_equal(c, d)
+    assert rt.cache_stats()["misses"] == s0["misses"] + 1
+    rt.cache_clear()
