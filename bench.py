@@ -36,6 +36,9 @@ ERR_BOUND = 1e-5
 # known size (profiles/mft_poly_512to256_64wl_hbm_traffic.md).  The counters cannot
 # be read from inside this process.
 TRAFFIC_BYTES = {"bf16x3": 273.2e6}
+# MFMA busy fraction of the stage-1 kernel from rocprofv3 SQ_VALU_MFMA_BUSY_CYCLES /
+# (GRBM_GUI_ACTIVE / 8 x 1024 SIMDs) (profiles/mft_poly_512to256_64wl_mfma_counters.md).
+MFMA_BUSY_PMC = {"bf16x3": 0.489}
 
 
 def _parse():
@@ -620,6 +623,10 @@ def main():
     serve = None if args.no_serving else _serving_leg(torch, args.serving_rates,
                                                       args.serving_seconds)
 
+    # the matrix cores' sustained bf16 rate on this GPU, with operands of full
+    # entropy (what the split contractions feed them) and of low entropy
+    ceiling = {"random_operands_tflops": runtime.mfma_rate(True),
+               "low_entropy_operands_tflops": runtime.mfma_rate(False)}
     ok = {p: (errs[p] is not None and errs[p] <= ERR_BOUND) for p in psfs}
     if "bf16x3" in res and (ok["bf16x3"] or errs["bf16x3"] is None or "fp32" not in res):
         head = "bf16x3"
@@ -662,7 +669,12 @@ def main():
                                f" stage 1 (pupil x transfer matrix), {cfg.n_wl} "
                                "wavelengths per launch",
                      "avg_launch_ms": avg1, "launches": c1,
-                     "mfma_util": achieved * terms / PEAK_TFLOPS[head],
+                     "mfma_flops_frac_of_peak": achieved * terms / PEAK_TFLOPS[head],
+                     "mfma_busy_pmc": MFMA_BUSY_PMC.get(head),
+                     "mfma_ceiling_measured": ceiling,
+                     "frac_of_measured_ceiling": (achieved * terms /
+                                                  ceiling["random_operands_tflops"]
+                                                  if head == "bf16x3" else None),
                      "mft_pair_tflops": pair},
         "cpu_baseline": cpu,
         "forms": {p: {"psf_per_s": 1e3 / res[p]["ms"], "ms_per_step": res[p]["ms"],

@@ -227,6 +227,7 @@ SIGNATURES = {
     "dlx_cache_clear": (c_int32, []),
     "dlx_cache_set_budget": (c_int32, [c_size_t]),
     "dlx_cache_bypass": (c_int32, [c_int32]),
+    "dlx_mfma_rate": (c_int32, [c_int32, POINTER(c_double), c_void_p, c_size_t, c_void_p]),
     "dlx_cache_stats": (c_int32, [POINTER(ctypes.c_uint64), POINTER(ctypes.c_uint64),
                                   POINTER(c_size_t), POINTER(c_int32)]),
     "dlx_profile_enable": (c_int32, [c_int32]),

@@ -1217,6 +1217,20 @@ def cache_set_budget(nbytes: int) -> None:
     _lib.check(_lib.load().dlx_cache_set_budget(int(nbytes)), "dlx_cache_set_budget")
 
 
+def mfma_rate(random_operands: bool = True) -> float:
+    """bf16 MFMA TFLOP/s this GPU sustains from registers (dlx_mfma_rate), with
+    full-entropy operands (as the split-BF16x3 contractions see) or low-entropy
+    ones (the nominal rate)."""
+    dev = device()
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    ws = workspace(2 * cus * 256 * 4, dev)
+    out = ctypes.c_double(0.0)
+    _lib.check(_lib.load().dlx_mfma_rate(int(bool(random_operands)), ctypes.byref(out),
+                                         ws.data_ptr(), ws.numel(), stream_handle()),
+               "dlx_mfma_rate")
+    return float(out.value)
+
+
 class cache_bypass:
     """Context manager: calls made from this thread inside it skip the
     transfer-matrix cache (dlx_cache_bypass), generating their matrices into

@@ -651,6 +651,18 @@ int dlx_detector_vjp_params(int32_t n, int32_t batch, const float* in,
  * waited on) until dlx_cache_bypass(0): for request loops whose wavelength
  * grids rarely repeat.
  */
+/*
+ * Matrix-core rate probe (no counterpart in the reference): the bf16 MFMA
+ * throughput (TFLOP/s, into *tflops) this GPU sustains on
+ * v_mfma_f32_32x32x16_bf16 from registers, 8 waves per CU, with low-entropy
+ * operands (random_operands = 0: the nominal rate) or with full-entropy
+ * operands like the split-BF16x3 contractions' (1).  Runs ~40 launches of
+ * 1-2 ms on the stream (the first 30 let the clocks settle) and waits for
+ * them; workspace >= 512 floats per CU.
+ */
+int dlx_mfma_rate(int32_t random_operands, double* tflops, void* workspace,
+                  size_t workspace_bytes, void* stream);
+
 int dlx_cache_clear(void);
 int dlx_cache_set_budget(size_t bytes);
 int dlx_cache_bypass(int32_t on);

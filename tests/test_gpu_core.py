@@ -317,3 +317,15 @@ def test_cache_bypass_skips_the_cache_and_keeps_results(rt):
     assert np.array_equal(c, d)
     assert rt.cache_stats()["misses"] == s0["misses"] + 1
     rt.cache_clear()
+
+
+def test_mfma_rate_probe(rt):
+    """The matrix-core probe returns positive rates no larger than the nominal
+    2.5 PFLOP/s (plus 10 %), full-entropy operands no faster than low-entropy ones
+    (plus 10 %)."""
+    print("[mfma-rate]", flush=True)
+    hi = rt.mfma_rate(False)
+    lo = rt.mfma_rate(True)
+    print(f"  low-entropy {hi:.0f} TF/s, full-entropy {lo:.0f} TF/s")
+    assert 0 < hi < 2750 and 0 < lo < 2750
+    assert lo < 1.1 * hi
