@@ -35,10 +35,31 @@ ERR_BOUND = 1e-5
 # FETCH_SIZE / WRITE_SIZE runs scaled by the same counters on a streaming kernel of
 # known size (profiles/mft_poly_512to256_64wl_hbm_traffic.md).  The counters cannot
 # be read from inside this process.
-TRAFFIC_BYTES = {"bf16x3": 273.2e6}
+TRAFFIC_BYTES = {}
 # MFMA busy fraction of the stage-1 kernel from rocprofv3 SQ_VALU_MFMA_BUSY_CYCLES /
 # (GRBM_GUI_ACTIVE / 8 x 1024 SIMDs) (profiles/mft_poly_512to256_64wl_mfma_counters.md).
-MFMA_BUSY_PMC = {"bf16x3": 0.489}
+MFMA_BUSY_PMC = {}
+
+
+def _stage1_kernel(head, prologue):
+    if head == "fp32":
+        return "cgemm_nt_fp32 stage 1 (pupil x transfer matrix)"
+    if prologue:
+        return ("cgemm_pupil_kernel stage 1 (pupil generated in the loader x transfer "
+                "matrix)")
+    return "cgemm_split4_deep_kernel stage 1 (pupil x transfer matrix)"
+
+
+def _stage1_bytes(cfg, head, prologue):
+    """Algorithmic HBM bytes of one stage-1 launch: the split4 (8 B) or complex64
+    (8 B) transfer matrices read and U^T written, plus the pupil: read as split4
+    or complex64 (8 B per pixel per wavelength), or, with the pupil generated in
+    the loader, its wavelength-independent path (8 B) and amplitude (4 B) planes
+    read once."""
+    mats = cfg.n_wl * 8.0 * 2 * cfg.n_out * cfg.n_in
+    if prologue:
+        return mats + 12.0 * cfg.n_in * cfg.n_in
+    return mats + cfg.n_wl * 8.0 * cfg.n_in * cfg.n_in
 
 
 def _parse():
@@ -68,19 +89,27 @@ def _parse():
 
 
 def _time_form(torch, runtime, base_sys, coeff_sets, cfg, prec, steps, warmup):
-    """Time `steps` propagate calls; returns (ms_per_step, stage timings)."""
+    """Time `steps` propagate calls; returns (ms_per_step, stage timings).
+
+    The step rate is timed with no events in the stream (an event record
+    between two kernels costs the stream ~5 us); the contractions' launch
+    durations come from a second pass of the same steps with the library's
+    event timing on."""
     wls, w = cfg.wavelengths, cfg.weights
     systems = [base_sys.set("aberrations.coefficients", c) for c in coeff_sets]
     with runtime.precision(prec):
         for i in range(warmup):
             systems[i % len(systems)].propagate(wls, weights=w)
         torch.cuda.synchronize()
-        runtime.profile_enable(True)
         t0 = time.perf_counter()
         for i in range(steps):
             systems[i % len(systems)].propagate(wls, weights=w)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        runtime.profile_enable(True)
+        for i in range(steps):
+            systems[i % len(systems)].propagate(wls, weights=w)
+        torch.cuda.synchronize()
         s1 = runtime.profile_read(1)
         s2 = runtime.profile_read(2)
         runtime.profile_enable(False)
@@ -633,6 +662,7 @@ def main():
     else:
         head = "fp32"
     ms = res[head]["ms"]
+    prologue = head == "bf16x3" and os.environ.get("DLUX_AMD_PUPIL_PROLOGUE", "1") != "0"
     n1 = cfg.n_out * cfg.n_in ** 2 * 8.0 * cfg.n_wl        # stage-1 algorithmic flops/launch
     t1_ms, c1 = res[head]["s1"]
     t2_ms, c2 = res[head]["s2"]
@@ -660,17 +690,14 @@ def main():
                    "precision": head},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[head],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head],
-                     "traffic": TRAFFIC_BYTES.get(head),
+                     "traffic": TRAFFIC_BYTES.get((head, prologue)),
                      "traffic_unit": "bytes per launch (rocprofv3 PMC, calibrated)",
-                     "algorithmic_bytes": float(cfg.n_wl * 8 * (cfg.n_in * cfg.n_in +
-                                                                2 * cfg.n_out * cfg.n_in)),
-                     "kernel": ("cgemm_split4_deep_kernel" if head == "bf16x3"
-                                else "cgemm_nt_fp32") +
-                               f" stage 1 (pupil x transfer matrix), {cfg.n_wl} "
-                               "wavelengths per launch",
+                     "algorithmic_bytes": _stage1_bytes(cfg, head, prologue),
+                     "kernel": _stage1_kernel(head, prologue) +
+                               f", {cfg.n_wl} wavelengths per launch",
                      "avg_launch_ms": avg1, "launches": c1,
                      "mfma_flops_frac_of_peak": achieved * terms / PEAK_TFLOPS[head],
-                     "mfma_busy_pmc": MFMA_BUSY_PMC.get(head),
+                     "mfma_busy_pmc": MFMA_BUSY_PMC.get((head, prologue)),
                      "mfma_ceiling_measured": ceiling,
                      "frac_of_measured_ceiling": (achieved * terms /
                                                   ceiling["random_operands_tflops"]

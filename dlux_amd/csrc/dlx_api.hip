@@ -2,6 +2,7 @@
 // argument validation, workspace carving and kernel orchestration.
 #include <cstdarg>
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -28,6 +29,14 @@ namespace {
 
 // Instances per fused launch of the polychromatic PSF (bounds the workspace).
 constexpr int kWlChunk = 64;
+
+// The pupil generated in stage 1's loader (default); DLUX_AMD_PUPIL_PROLOGUE=0
+// writes it with the pupil kernel instead (read at every call, for
+// comparisons).
+bool pupil_prologue_enabled() {
+  const char* v = getenv("DLUX_AMD_PUPIL_PROLOGUE");
+  return !(v && v[0] == '0');
+}
 
 // Optional live timing of the two MFT contraction launches of dlx_psf_poly:
 // hipEvents recorded on the caller's stream around each launch.
@@ -178,24 +187,42 @@ int mft_fp32(const dlx_mft_geom& g, int batch, const float2* P, const float2* Wx
 // against split4 transfer matrices Wx / Wy (N_out rows, k = N_in):
 //   stage 1: U^T = (P Wx^T)^T written split4 (N_out rows, k = N_in)
 //   stage 2: E = alpha_b Wy U, stored complex64 (EPI_STORE) or |E|^2 (EPI_INTENSITY).
+// Stage 1's launch description for instances [b0, b0 + nb) of a batch.
+GemmSplit stage1_split(const dlx_mft_geom& g, int b0, int nb, const uint8_t* P,
+                       const uint8_t* Wx, uint8_t* U) {
+  const long pbytes = (long)mat_bytes(g.n_in, g.n_in, true);
+  const long wbytes = (long)mat_bytes(g.n_out, g.n_in, true);
+  const long rb = split4_row_bytes(g.n_in);
+  GemmSplit g1{};
+  g1.A = P ? P + (long)b0 * pbytes : nullptr; g1.a_row_bytes = rb; g1.a_batch_bytes = pbytes;
+  g1.B = Wx + (long)b0 * wbytes; g1.b_row_bytes = rb; g1.b_batch_bytes = wbytes;
+  g1.C = U + (long)b0 * wbytes; g1.ldc = rb; g1.c_batch = wbytes;
+  g1.M = g.n_in; g1.N = g.n_out; g1.K = g.n_in; g1.batch = nb; g1.epilogue = EPI_SPLIT_T;
+  for (int i = 0; i < nb; ++i) g1.alpha[i] = 1.0f;
+  return g1;
+}
+
+// With `pg` set, stage 1 generates the pupil in its loader (P unused; the
+// caller has checked cgemm_pupil_supported for every slice).
 int mft_split4(const dlx_mft_geom& g, int batch, const uint8_t* P, const uint8_t* Wx,
                const uint8_t* Wy, uint8_t* U, const double* alpha, void* out, int epi,
-               hipStream_t s, int (*pb)(int, hipStream_t, int*), int (*pe)(int, hipStream_t)) {
-  const long pbytes = (long)mat_bytes(g.n_in, g.n_in, true);
+               hipStream_t s, int (*pb)(int, hipStream_t, int*), int (*pe)(int, hipStream_t),
+               const PupilGen* pg = nullptr) {
   const long wbytes = (long)mat_bytes(g.n_out, g.n_in, true);
   const long rb = split4_row_bytes(g.n_in);
   const long nout2 = (long)g.n_out * g.n_out;
   for (int b0 = 0; b0 < batch; b0 += GemmSplit::kMaxBatch) {
     const int nb = batch - b0 < GemmSplit::kMaxBatch ? batch - b0 : GemmSplit::kMaxBatch;
-    GemmSplit g1{};
-    g1.A = P + (long)b0 * pbytes; g1.a_row_bytes = rb; g1.a_batch_bytes = pbytes;
-    g1.B = Wx + (long)b0 * wbytes; g1.b_row_bytes = rb; g1.b_batch_bytes = wbytes;
-    g1.C = U + (long)b0 * wbytes; g1.ldc = rb; g1.c_batch = wbytes;
-    g1.M = g.n_in; g1.N = g.n_out; g1.K = g.n_in; g1.batch = nb; g1.epilogue = EPI_SPLIT_T;
-    for (int i = 0; i < nb; ++i) g1.alpha[i] = 1.0f;
+    const GemmSplit g1 = stage1_split(g, b0, nb, P, Wx, U);
     int slot = -1;
     if (pb) { if (int st = pb(1, s, &slot)) return st; }
-    DLX_HIP(launch_cgemm_split4(g1, s));
+    if (pg) {
+      PupilGen p = *pg;
+      for (int i = 0; i < nb; ++i) p.inv_wl[i] = pg->inv_wl[b0 + i];
+      DLX_HIP(launch_cgemm_pupil(g1, p, s));
+    } else {
+      DLX_HIP(launch_cgemm_split4(g1, s));
+    }
     if (pe) { if (int st = pe(slot, s)) return st; }
     GemmSplit g2{};
     g2.A = Wy + (long)b0 * wbytes; g2.a_row_bytes = rb; g2.a_batch_bytes = wbytes;
@@ -625,6 +652,7 @@ int psf_poly_run(const dlx_mft_geom& g, int32_t nwl, const double* wavelength,
   const double x0 = center - (g.n_in - 1) / 2.0 * pixel_scale_in;
 
   std::vector<double> dx(c, pixel_scale_in), pout(c, pixel_scale_out), alpha(c);
+  bool prep_valid = false;   // the pupil-prologue planes hold this call's pupil
   for (int l0 = 0; l0 < nwl; l0 += c) {
     const int nb = nwl - l0 < c ? nwl - l0 : c;
     float* Ic = planes ? planes + (long)l0 * nout2 : I;
@@ -646,14 +674,36 @@ int psf_poly_run(const dlx_mft_geom& g, int32_t nwl, const double* wavelength,
         return st;
     }
     if (split) {
-      PupilChunk pc;
-      for (int i = 0; i < nb; ++i) pc.inv_wl[i] = 1.0 / wavelength[l0 + i];
-      DLX_HIP(launch_pupil_split4(pc, nb, g.n_in, pixel_scale_in, x0, tilt_x, tilt_y, amp, opd,
-                                  phase, (float)scale, P, (long)mat_bytes(g.n_in, g.n_in, true),
-                                  s));
+      // the pupil generated in stage 1's loader where the shape allows (no
+      // pupil array in HBM: the loader reads the lambda-independent path and
+      // amplitude planes, kept where the pupils would go), else written by
+      // the pupil kernel
+      const size_t n2 = (size_t)g.n_in * g.n_in;
+      PupilGen pg{};
+      pg.path = (const double*)P;
+      pg.a = (const float*)(P + align_up(n2 * sizeof(double)));
+      pg.n = g.n_in;
+      for (int i = 0; i < nb; ++i) pg.inv_wl[i] = 1.0 / wavelength[l0 + i];
+      const bool fuse = !phase && pupil_prologue_enabled() &&
+                        align_up(n2 * sizeof(double)) + n2 * sizeof(float) <= Pb &&
+                        cgemm_pupil_supported(stage1_split(g, 0, nb, nullptr,
+                                                           (const uint8_t*)W, Ut), pg);
+      if (fuse && !prep_valid) {
+        DLX_HIP(launch_pupil_prep(g.n_in, pixel_scale_in, x0, tilt_x, tilt_y, amp, opd,
+                                  (float)scale, (double*)pg.path, (float*)pg.a, s));
+        prep_valid = true;
+      }
+      if (!fuse) {
+        prep_valid = false;
+        PupilChunk pc;
+        for (int i = 0; i < nb; ++i) pc.inv_wl[i] = pg.inv_wl[i];
+        DLX_HIP(launch_pupil_split4(pc, nb, g.n_in, pixel_scale_in, x0, tilt_x, tilt_y, amp,
+                                    opd, phase, (float)scale, P,
+                                    (long)mat_bytes(g.n_in, g.n_in, true), s));
+      }
       if (int st = mft_split4(g, nb, P, (const uint8_t*)W, (const uint8_t*)W, Ut, alpha.data(),
                               Ic, EPI_INTENSITY, s, prof ? prof_begin : nullptr,
-                              prof ? prof_end : nullptr))
+                              prof ? prof_end : nullptr, fuse ? &pg : nullptr))
         return st;
     } else {
       if (int st = dlx_pupil_field(g.n_in, nb, wavelength + l0, pixel_scale_in, center, tilt_x,

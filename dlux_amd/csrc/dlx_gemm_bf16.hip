@@ -14,6 +14,7 @@
 // Workgroups are numbered so that all tiles of one instance run on the same
 // XCD and share its L2 (the hardware deals workgroups round-robin over 8 XCDs).
 #include "dlx_internal.h"
+#include "dlx_phasor.h"
 #include "dlx_split.h"
 
 #include <cstdlib>
@@ -109,21 +110,27 @@ __device__ __forceinline__ void epilogue_split(const GemmSplit& g, int b, int m0
         for (int q = 0; q < 4; ++q) {
           const int mq = m0 + wm + 32 * i + 8 * q + 4 * lk;   // first of 4 rows
           if (mq >= m_pad) continue;
-          uint32_t rh[4], rl[4], ih[4], il[4];
+          float re[4], im[4];
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const bool ok = mq + e < g.M;
             float f = alpha;
             if (EPI == EPI_VJP_SPLIT_T && ok)
               f *= 2.f * g.G[(long)b * g.g_batch + (long)n * g.ldg + mq + e];
-            split_bf16(ok ? f * accR[i][j][4 * q + e] : 0.f, rh[e], rl[e]);
-            split_bf16(ok ? f * accI[i][j][4 * q + e] : 0.f, ih[e], il[e]);
+            re[e] = ok ? f * accR[i][j][4 * q + e] : 0.f;
+            im[e] = ok ? f * accI[i][j][4 * q + e] : 0.f;
+          }
+          uint32_t rh[2], rl[2], ih[2], il[2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            split2_bf16(re[2 * h], re[2 * h + 1], rh[h], rl[h]);
+            split2_bf16(im[2 * h], im[2 * h + 1], ih[h], il[h]);
           }
           uint8_t* dst = crow + (long)(mq >> 3) * kChunkBytes + ((mq & 7) >> 2) * 8;
-          *(uint2*)(dst + 0) = make_uint2(rh[0] | (rh[1] << 16), rh[2] | (rh[3] << 16));
-          *(uint2*)(dst + 16) = make_uint2(rl[0] | (rl[1] << 16), rl[2] | (rl[3] << 16));
-          *(uint2*)(dst + 32) = make_uint2(ih[0] | (ih[1] << 16), ih[2] | (ih[3] << 16));
-          *(uint2*)(dst + 48) = make_uint2(il[0] | (il[1] << 16), il[2] | (il[3] << 16));
+          *(uint2*)(dst + 0) = make_uint2(rh[0], rh[1]);
+          *(uint2*)(dst + 16) = make_uint2(rl[0], rl[1]);
+          *(uint2*)(dst + 32) = make_uint2(ih[0], ih[1]);
+          *(uint2*)(dst + 48) = make_uint2(il[0], il[1]);
         }
       } else {
 #pragma unroll
@@ -618,6 +625,143 @@ cgemm_split4_deep_kernel(GemmSplit g) {
 }
 
 // ---------------------------------------------------------------------------
+// Stage 1 with the pupil generated in the loader: the A operand (pupil rows y,
+// k = x) is never read from HBM.  Each thread owns one row of the 128-row A
+// tile and one 8-pixel chunk of a stage's 16 k.  It loads the next stage's 8
+// optical paths (FP64) and 8 amplitudes (the lambda-independent planes of
+// launch_pupil_prep) before this stage's products are issued and turns them
+// into split4 phasors (dlx_phasor.h, the pupil kernels' arithmetic) after the
+// products are issued, so the phasor work overlaps the matrix cores; the
+// four 16-byte planes go straight into the next stage's A buffer with the
+// deep kernel's swizzle.  B (the transfer matrix) is staged direct-to-LDS as
+// in the deep kernel; the epilogue is EPI_SPLIT_T.  Interior tiles only.
+// ---------------------------------------------------------------------------
+template <int TN>
+__global__ void __launch_bounds__(kThreads, 2)
+cgemm_pupil_kernel(GemmSplit g, PupilGen pg) {
+  constexpr int TM = 2, BM = 128, BN = 64 * TN;
+  constexpr int kBuf = (BM + BN) * kRowDeep;
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kBuf];
+
+  const int tiles_n = g.N / BN, tiles_m = g.M / BM;
+  const int total = tiles_n * tiles_m * g.batch;
+  const int id = xcd_remap(blockIdx.x, total);
+  const int tn = id % tiles_n;
+  const int tm = (id / tiles_n) % tiles_m;
+  const int b = id / (tiles_n * tiles_m);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32 * TM, wn = (wave & 1) * 32 * TN;
+  const int lr = lane & 31, lk = lane >> 5;
+  const int nst = g.K / (8 * kDeepChunks);
+
+  f32x16_t accR[TM][TN], accI[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) { accR[i][j][r] = 0.f; accI[i][j][r] = 0.f; }
+
+  FastRows<BN> fb_rows;
+  fb_rows.init(g.B + (long)b * g.b_batch_bytes, g.b_row_bytes, n0);
+
+  // this thread's pupil row (tile row pr) and chunk (pc) of every stage
+  const int pr = threadIdx.x >> 1, pc = threadIdx.x & 1;
+  const double inv_wl = pg.inv_wl[b];
+  const long pix0 = (long)(m0 + pr) * pg.n + pc * 8;
+  const int sw = (pr >> 1) & 7;
+  double2 rp[4];
+  float4 ra[2];
+  auto load_raw = [&](int st) {
+    const long p = pix0 + st * (8 * kDeepChunks);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rp[i] = *(const double2*)(pg.path + p + 2 * i);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) ra[i] = *(const float4*)(pg.a + p + 4 * i);
+  };
+  auto convert = [&](int st) {
+    const double pv[8] = {rp[0].x, rp[0].y, rp[1].x, rp[1].y, rp[2].x, rp[2].y, rp[3].x, rp[3].y};
+    const float av[8] = {ra[0].x, ra[0].y, ra[0].z, ra[0].w, ra[1].x, ra[1].y, ra[1].z, ra[1].w};
+    float2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = pupil_phasor(pv[e], inv_wl, 0.0, av[e]);
+    uint4 p[4];
+    pack_chunk(v, p);
+    uint8_t* dst = lds + (st & 1) * kBuf + pr * kRowDeep;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) *(uint4*)(dst + ((((pc << 2) | q) ^ sw) << 4)) = p[q];
+  };
+
+  load_raw(0);
+  fb_rows.issue(lds + BM * kRowDeep, 0);
+  convert(0);
+
+  for (int s = 0; s < nst; ++s) {
+    __builtin_amdgcn_s_waitcnt(0);   // B of stage s landed, A of stage s written
+    __syncthreads();                 // by every wave; stage s-1 reads retired
+    const uint8_t* la = lds + (s & 1) * kBuf;
+    const uint8_t* lb = la + BM * kRowDeep;
+    u32x4_t fa[TM][4], fb[TN][4];
+    auto read_a = [&](int i) {
+      const int r = wm + 32 * i + lr;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        fa[i][p] = *(const u32x4_t*)(la + r * kRowDeep + (((lk << 2) | p) ^ ((r >> 1) & 7)) * 16);
+    };
+    read_a(0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int r = wn + 32 * j + lr;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        fb[j][p] = *(const u32x4_t*)(lb + r * kRowDeep + (((lk << 2) | p) ^ ((r >> 1) & 7)) * 16);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 1; i < TM; ++i) read_a(i);
+    const bool more = s + 1 < nst;
+    if (more) {
+      load_raw(s + 1);
+      fb_rows.issue(lds + ((s + 1) & 1) * kBuf + BM * kRowDeep, (s + 1) * kDeepChunks);
+    }
+    __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      __builtin_amdgcn_sched_barrier(0);
+      const bf16x8_t arh = as_bf(fa[i][0]), arl = as_bf(fa[i][1]);
+      const bf16x8_t aih = as_bf(fa[i][2]), ail = as_bf(fa[i][3]);
+      const bf16x8_t naih = as_bf(neg_bf(fa[i][2])), nail = as_bf(neg_bf(fa[i][3]));
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const bf16x8_t brh = as_bf(fb[j][0]), brl = as_bf(fb[j][1]);
+        const bf16x8_t bih = as_bf(fb[j][2]), bil = as_bf(fb[j][3]);
+        f32x16_t r = accR[i][j], im = accI[i][j];
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, brh, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arl, bih, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brl, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bil, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nail, bih, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ail, brh, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bil, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brl, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, brh, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(arh, bih, im, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_bf16(naih, bih, r, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aih, brh, im, 0, 0, 0);
+        accR[i][j] = r;
+        accI[i][j] = im;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_setprio(0);
+    // the next stage's pupil chunk, into the A buffer stage s-1 used
+    if (more) convert(s + 1);
+  }
+  epilogue_split<TM, TN, EPI_SPLIT_T>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
+}
+
+// ---------------------------------------------------------------------------
 // Wide variant: one 8-wave workgroup per CU on a 256 x 128 tile (waves 4 x 2,
 // each 64 x 64), three 48 KiB LDS buffers (144 KiB), so the loads of stage
 // s + 2 are in flight while stage s is computed.  The per-stage wait keeps the
@@ -854,6 +998,44 @@ hipError_t launch_cgemm_split4_group(const GroupLaunch& gl, int epilogue, hipStr
   const dim3 grid(8 * per);
   return gl.tile_n == 64 ? launch_group_tn<1>(gl, epilogue, grid, s)
                          : launch_group_tn<2>(gl, epilogue, grid, s);
+}
+
+namespace {
+// Tile width of the pupil-prologue kernel for this launch (128 or 64), or 0
+// when the shape is not covered: interior 128-row tiles, whole 16-k stages,
+// 16-byte aligned planes, at most two tile columns, and (as for the deep
+// kernel) enough tiles for two workgroups per CU.
+int pupil_tile_n(const GemmSplit& g, const PupilGen& pg) {
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0 || g.batch > GemmSplit::kMaxBatch) return 0;
+  const bool aligned =
+      ((reinterpret_cast<uintptr_t>(pg.path) | reinterpret_cast<uintptr_t>(pg.a)) & 15) == 0;
+  if (g.epilogue != EPI_SPLIT_T || g.K != g.M || pg.n != g.M || g.M % 128 != 0 ||
+      g.K % (8 * kDeepChunks) != 0 || !aligned || g.b_row_bytes != split4_row_bytes(g.K) ||
+      g.ldc != split4_row_bytes(g.M))
+    return 0;
+  // every pupil element is formed once per tile column: at most two (beyond
+  // that the phasor work costs more than the pupil array's write and read)
+  const long tiles128 = (long)(g.M / 128) * (g.N / 128) * g.batch;
+  const long tiles64 = (long)(g.M / 128) * (g.N / 64) * g.batch;
+  if (g.N % 128 == 0 && g.N <= 256 && tiles128 >= 512) return 128;
+  if (g.N % 64 == 0 && g.N <= 128 && tiles64 >= 512) return 64;
+  return 0;
+}
+}  // namespace
+
+bool cgemm_pupil_supported(const GemmSplit& g, const PupilGen& pg) {
+  return pupil_tile_n(g, pg) != 0;
+}
+
+hipError_t launch_cgemm_pupil(const GemmSplit& g, const PupilGen& pg, hipStream_t s) {
+  const int tn = pupil_tile_n(g, pg);
+  if (tn == 0) return hipErrorNotSupported;
+  const unsigned tiles = (unsigned)((g.M / 128) * (g.N / tn) * g.batch);
+  if (tn == 128)
+    hipLaunchKernelGGL(cgemm_pupil_kernel<2>, dim3(tiles), dim3(kThreads), 0, s, g, pg);
+  else
+    hipLaunchKernelGGL(cgemm_pupil_kernel<1>, dim3(tiles), dim3(kThreads), 0, s, g, pg);
+  return hipGetLastError();
 }
 
 template <int EPI>

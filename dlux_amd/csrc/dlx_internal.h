@@ -135,6 +135,21 @@ struct GemmSplit {
 
 hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s);
 
+// The pupil of instance b, generated in the stage-1 loader instead of read
+// from A:  A[b](y, x) = a(y, x) exp(2 pi i path(y, x) inv_wl[b]), from the
+// lambda-independent planes written by launch_pupil_prep (n x n, row-major).
+struct PupilGen {
+  const double* path; const float* a;
+  int n;
+  double inv_wl[GemmSplit::kMaxBatch];
+};
+// Stage 1 (EPI_SPLIT_T, K = M = pg.n) with the pupil as A.  Returns
+// hipErrorNotSupported, launching nothing, for shapes it does not cover
+// (cgemm_pupil_supported is false; the caller then writes the pupil and calls
+// launch_cgemm_split4).
+hipError_t launch_cgemm_pupil(const GemmSplit& g, const PupilGen& pg, hipStream_t s);
+bool cgemm_pupil_supported(const GemmSplit& g, const PupilGen& pg);
+
 // One problem of a grouped (uneven) split4 GEMM, in device memory.  Its
 // 128 x tile_n tiles are [tile_start, tile_start + tiles_m * tiles_n) of the
 // launch's tile list, row-major over (tm, tn).
@@ -196,6 +211,11 @@ hipError_t launch_pupil_chunk(const PupilChunk& c, int nb, int n, double dx, dou
 // split4 producers
 hipError_t launch_dft_split4(const DftChunk& c, int nb, int n_in, int n_out, int inverse,
                              uint8_t* out, long batch_bytes, hipStream_t s);
+// path = tx X + ty Y + opd (FP64) and a = scale amp (float) of every pupil
+// pixel, as the pupil kernels form them (dlx_phasor.h).
+hipError_t launch_pupil_prep(int n, double dx, double x0, double tx, double ty, const float* amp,
+                             const float* opd, float scale, double* path, float* a,
+                             hipStream_t s);
 hipError_t launch_pupil_split4(const PupilChunk& c, int nb, int n, double dx, double x0,
                                double tx, double ty, const float* amp, const float* opd,
                                const float* phase, float scale, uint8_t* out, long batch_bytes,

@@ -3,6 +3,7 @@
 // intensity reductions.  All are HBM-bound; they are written for coalesced
 // 8/16-byte accesses with one pixel (or four) per lane.
 #include "dlx_internal.h"
+#include "dlx_phasor.h"
 #include "dlx_split.h"
 
 namespace dlx {
@@ -54,16 +55,10 @@ pupil_field_kernel(PupilChunk c, int n, double dx, double x0, double tx, double 
   const int b = blockIdx.y;
   if (p >= npix) return;
   const int y = (int)(p / n), x = (int)(p - (long)y * n);
-  const double X = x0 + (double)x * dx, Y = x0 + (double)y * dx;
-  double path = tx * X + ty * Y;
-  if (opd) path += (double)opd[p];
-  double t = path * c.inv_wl[b];
-  if (phase) t += (double)phase[p] * 0.15915494309189535;  // 1 / (2 pi)
-  t -= rint(t);
-  // hardware sin/cos take turns; max abs error 1.3e-7 on [-1/2, 1/2]
-  const float s = __builtin_amdgcn_sinf((float)t), co = __builtin_amdgcn_cosf((float)t);
-  const float a = scale * (amp ? amp[p] : 1.0f);
-  float2 v = make_float2(a * co, a * s);
+  const double path = pupil_path(tx, pupil_coord(x, dx, x0), ty * pupil_coord(y, dx, x0),
+                                 opd ? (double)opd[p] : 0.0);
+  const double turns0 = phase ? (double)phase[p] * 0.15915494309189535 : 0.0;  // 1 / (2 pi)
+  float2 v = pupil_phasor(path, c.inv_wl[b], turns0, scale * (amp ? amp[p] : 1.0f));
   if (in) {
     const float2 e = in[(long)b * npix + p];
     v = make_float2(e.x * v.x - e.y * v.y, e.x * v.y + e.y * v.x);
@@ -142,7 +137,7 @@ pupil_split4_kernel(PupilChunk c, int nb, int n, double dx, double x0, double tx
   const int q = (int)(t & 3);
   const long cidx = t >> 2;
   const int y = (int)(cidx / kc), ch = (int)(cidx - (long)y * kc);
-  const double Y = x0 + (double)y * dx;
+  const double tyY = ty * pupil_coord(y, dx, x0);
   double path[2], turns0[2];
   float a[2];
 #pragma unroll
@@ -150,8 +145,7 @@ pupil_split4_kernel(PupilChunk c, int nb, int n, double dx, double x0, double tx
     const int x = ch * 8 + 2 * q + e;
     const bool in = x < n;
     const long p = (long)y * n + x;
-    const double X = x0 + (double)x * dx;
-    path[e] = tx * X + ty * Y + ((in && opd) ? (double)opd[p] : 0.0);
+    path[e] = pupil_path(tx, pupil_coord(x, dx, x0), tyY, (in && opd) ? (double)opd[p] : 0.0);
     turns0[e] = (in && phase) ? (double)phase[p] * 0.15915494309189535 : 0.0;
     a[e] = in ? scale * (amp ? amp[p] : 1.0f) : 0.f;
   }
@@ -161,15 +155,30 @@ pupil_split4_kernel(PupilChunk c, int nb, int n, double dx, double x0, double tx
   for (int b = b0; b < b1; ++b) {
     float2 v[2];
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      double tt = path[e] * c.inv_wl[b] + turns0[e];
-      tt -= rint(tt);
-      // hardware sin/cos take turns; max abs error 1.3e-7 on [-1/2, 1/2]
-      const float s = __builtin_amdgcn_sinf((float)tt), co = __builtin_amdgcn_cosf((float)tt);
-      v[e] = make_float2(a[e] * co, a[e] * s);
-    }
+    for (int e = 0; e < 2; ++e) v[e] = pupil_phasor(path[e], c.inv_wl[b], turns0[e], a[e]);
     store_chunk_quad(out + (long)b * batch_bytes, y, rb, ch, q, v[0], v[1]);
   }
+}
+
+__global__ void __launch_bounds__(256)
+pupil_prep_kernel(int n, double dx, double x0, double tx, double ty,
+                  const float* __restrict__ amp, const float* __restrict__ opd, float scale,
+                  double* __restrict__ path, float* __restrict__ a) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long)n * n) return;
+  const int y = (int)(p / n), x = (int)(p - (long)y * n);
+  path[p] = pupil_path(tx, pupil_coord(x, dx, x0), ty * pupil_coord(y, dx, x0),
+                       opd ? (double)opd[p] : 0.0);
+  a[p] = scale * (amp ? amp[p] : 1.0f);
+}
+
+hipError_t launch_pupil_prep(int n, double dx, double x0, double tx, double ty, const float* amp,
+                             const float* opd, float scale, double* path, float* a,
+                             hipStream_t s) {
+  const long npix = (long)n * n;
+  hipLaunchKernelGGL(pupil_prep_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, n,
+                     dx, x0, tx, ty, amp, opd, scale, path, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_pupil_split4(const PupilChunk& c, int nb, int n, double dx, double x0,
@@ -360,7 +369,8 @@ hipError_t launch_intensity(long npix, int batch, const float2* f, int sum_batch
 }
 
 // psf[p] (=|+=) sum_b planes[b][p] in index order (deterministic); four
-// pixels per lane and an unrolled plane loop keep many loads in flight.
+// pixels per lane and a plane loop unrolled 16 deep keep 256 bytes per lane in
+// flight (a 256^2 PSF has only 16 Ki lanes).
 __global__ void __launch_bounds__(256)
 sum_planes_kernel(long npix, int nplanes, const float* __restrict__ planes, int accumulate,
                   float* __restrict__ psf, int vec_ok) {
@@ -368,7 +378,7 @@ sum_planes_kernel(long npix, int nplanes, const float* __restrict__ planes, int 
   if (p4 >= npix) return;
   if (vec_ok) {
     float4 acc = accumulate ? *(const float4*)(psf + p4) : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll 8
+#pragma unroll 16
     for (int b = 0; b < nplanes; ++b) {
       const float4 v = *(const float4*)(planes + (long)b * npix + p4);
       acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;

@@ -20,34 +20,34 @@ constexpr int kChunkBytes = 64;    // 4 planes x 8 bf16
 __host__ __device__ inline long split4_chunks(long k) { return (k + kChunk - 1) / kChunk; }
 __host__ __device__ inline long split4_row_bytes(long k) { return split4_chunks(k) * kChunkBytes; }
 
-// Round-to-nearest-even float -> bf16 bits (finite inputs).
-__device__ __forceinline__ uint32_t bf16_rne(float x) {
-  uint32_t u = __float_as_uint(x);
-  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2_split_t;
+typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2_split_t;
+
+// Two floats -> two bf16 rounded to nearest even, packed (x0 in the low half):
+// one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t bf16x2_rne(float x0, float x1) {
+  const f32x2_split_t v = {x0, x1};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_split_t));
 }
 
-// x = hi + lo + O(2^-17 |x|)
-__device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) {
-  hi = bf16_rne(x);
-  lo = bf16_rne(x - __uint_as_float(hi << 16));
+// Split of two values: x_i = hi_i + lo_i + O(2^-17 |x_i|), each pair packed.
+__device__ __forceinline__ void split2_bf16(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+  hi = bf16x2_rne(x0, x1);
+  lo = bf16x2_rne(x0 - __uint_as_float(hi << 16), x1 - __uint_as_float(hi & 0xFFFF0000u));
 }
 
 // Pack 8 complex values (one chunk) into its four 16-byte planes.
 __device__ __forceinline__ void pack_chunk(const float2 (&v)[8], uint4 (&p)[4]) {
-  uint32_t rh[8], rl[8], ih[8], il[8];
+  uint32_t rh[4], rl[4], ih[4], il[4];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    split_bf16(v[i].x, rh[i], rl[i]);
-    split_bf16(v[i].y, ih[i], il[i]);
+  for (int i = 0; i < 4; ++i) {
+    split2_bf16(v[2 * i].x, v[2 * i + 1].x, rh[i], rl[i]);
+    split2_bf16(v[2 * i].y, v[2 * i + 1].y, ih[i], il[i]);
   }
-  p[0] = make_uint4(rh[0] | (rh[1] << 16), rh[2] | (rh[3] << 16), rh[4] | (rh[5] << 16),
-                    rh[6] | (rh[7] << 16));
-  p[1] = make_uint4(rl[0] | (rl[1] << 16), rl[2] | (rl[3] << 16), rl[4] | (rl[5] << 16),
-                    rl[6] | (rl[7] << 16));
-  p[2] = make_uint4(ih[0] | (ih[1] << 16), ih[2] | (ih[3] << 16), ih[4] | (ih[5] << 16),
-                    ih[6] | (ih[7] << 16));
-  p[3] = make_uint4(il[0] | (il[1] << 16), il[2] | (il[3] << 16), il[4] | (il[5] << 16),
-                    il[6] | (il[7] << 16));
+  p[0] = make_uint4(rh[0], rh[1], rh[2], rh[3]);
+  p[1] = make_uint4(rl[0], rl[1], rl[2], rl[3]);
+  p[2] = make_uint4(ih[0], ih[1], ih[2], ih[3]);
+  p[3] = make_uint4(il[0], il[1], il[2], il[3]);
 }
 
 // Store one chunk (8 complex values at k = 8*chunk .. 8*chunk+7 of `row`).
@@ -85,13 +85,9 @@ __device__ __forceinline__ uint32_t pick4(uint32_t a0, uint32_t a1, uint32_t a2,
 
 __device__ __forceinline__ void store_chunk_quad(uint8_t* base, long row, long row_bytes,
                                                  long chunk, int q, float2 v0, float2 v1) {
-  uint32_t h0, l0, h1, l1;
-  split_bf16(v0.x, h0, l0);
-  split_bf16(v1.x, h1, l1);
-  const uint32_t w0 = h0 | (h1 << 16), w1 = l0 | (l1 << 16);
-  split_bf16(v0.y, h0, l0);
-  split_bf16(v1.y, h1, l1);
-  const uint32_t w2 = h0 | (h1 << 16), w3 = l0 | (l1 << 16);
+  uint32_t w0, w1, w2, w3;
+  split2_bf16(v0.x, v1.x, w0, w1);
+  split2_bf16(v0.y, v1.y, w2, w3);
   // g_m = the word lane q ^ m holds for plane q
   const uint32_t g0 = pick4(w0, w1, w2, w3, q);
   const uint32_t g1 = quad_xor<1>(pick4(w0, w1, w2, w3, q ^ 1));
