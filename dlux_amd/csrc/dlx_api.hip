@@ -758,7 +758,7 @@ size_t dlx_psf_poly_detector_workspace_size(const dlx_mft_geom* geom, int32_t nw
   const int c = nwl < kWlChunk ? nwl : kWlChunk;
   const size_t planes = align_up((size_t)nwl * geom->n_out * geom->n_out * sizeof(float));
   return psf_poly_bytes(*geom, c, precision == DLX_PREC_BF16X3) + planes +
-         detector_ws_bytes(geom->n_out, 1, ops, nops);
+         detector_ws_bytes(geom->n_out, 1, ops, nops, nwl);
 }
 
 int dlx_psf_poly_detector(const dlx_mft_geom* geom, int32_t nwl, const double* wavelength,

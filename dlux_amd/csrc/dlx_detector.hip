@@ -982,10 +982,44 @@ int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, flo
 
 }  // namespace
 
-size_t detector_ws_bytes(int n, int batch, const dlx_detector_op* ops, int nops) {
+// Workgroups of the chain's first launch (pass 0, fused with pass 1 where
+// run_conv_down takes both).
+long first_launch_groups(const std::vector<Pass>& passes, int batch) {
+  const Pass& p = passes[0];
+  if (p.conv) {
+    const int rpt = conv_rpt(p.K, p.sep);
+    long tiles;
+    if (can_fuse(passes, 0)) {
+      const int k = passes[1].k;
+      const int th = (kTileY * rpt / k) * k, tw = (kTileX / k) * k;
+      tiles = (long)((p.n_in + tw - 1) / tw) * ((p.n_in + th - 1) / th);
+    } else {
+      tiles = (long)((p.n_in + kTileX - 1) / kTileX) * ((p.n_in + kTileY * rpt - 1) / (kTileY * rpt));
+    }
+    return tiles * batch;
+  }
+  const long npix = (long)p.n_out * p.n_out;
+  return (p.k == 1 ? (npix / 4 + 255) / 256 : (npix + 255) / 256) * batch;
+}
+
+// A summed input (nsum planes) is loaded by the first pass, summing as it
+// loads, when that launch fills the GPU; a smaller one is summed first by the
+// pixel-parallel plane sum (same order, same bits) into one more image of
+// workspace, because the first pass's few workgroups would each add up nsum
+// planes serially.
+constexpr long kSumInLoaderGroups = 512;
+
+bool presum(const std::vector<Pass>& passes, int n, int batch, int nsum, long sum_stride) {
+  return nsum > 1 && sum_stride == (long)n * n &&
+         first_launch_groups(passes, batch) < kSumInLoaderGroups;
+}
+
+size_t detector_ws_bytes(int n, int batch, const dlx_detector_op* ops, int nops, int nsum) {
   std::vector<Pass> passes;
   if (batch < 0 || plan(n, ops, nops, passes)) return 0;
-  return layout(passes, batch, false).total;
+  const size_t extra =
+      presum(passes, n, batch, nsum, (long)n * n) ? align_up((size_t)n * n * sizeof(float)) : 0;
+  return layout(passes, batch, false).total + extra;
 }
 
 int detector_apply_sum(int n, int batch, int nsum, long sum_stride, const float* in,
@@ -1002,13 +1036,22 @@ int detector_apply_sum(int n, int batch, int nsum, long sum_stride, const float*
   DLX_CHECK_ARG(in && out, "null image pointer to the detector chain");
   if ((st = check_grid(passes, batch))) return st;
   const Layout L = layout(passes, batch, false);
-  if (workspace_bytes < L.total || (L.total && !workspace)) {
-    set_error("detector chain workspace too small: need %zu bytes, got %zu", L.total,
+  const bool pre = presum(passes, n, batch, nsum, sum_stride);
+  const size_t need = L.total + (pre ? align_up((size_t)n * n * sizeof(float)) : 0);
+  if (workspace_bytes < need || (need && !workspace)) {
+    set_error("detector chain workspace too small: need %zu bytes, got %zu", need,
               workspace_bytes);
     return DLX_ERR_WORKSPACE;
   }
   char* ws = (char*)workspace;
   const float* src = in;
+  if (pre) {
+    float* sum = (float*)(ws + L.total);
+    DLX_HIP(launch_sum_planes((long)n * n, nsum, in, 0, sum, s));
+    src = sum;
+    nsum = 1;
+    sum_stride = 0;
+  }
   const size_t np = passes.size();
   int buf = 0;
   for (size_t i = 0; i < np;) {

@@ -265,8 +265,8 @@ int cache_ready(void* handle, hipStream_t s);
 
 // The detector chain (dlx_detector.hip) on one image that is the in-order sum
 // of nsum planes sum_stride floats apart (nsum = 1: `batch` images), and its
-// workspace size.
-size_t detector_ws_bytes(int n, int batch, const dlx_detector_op* ops, int nops);
+// workspace size for nsum planes n^2 floats apart.
+size_t detector_ws_bytes(int n, int batch, const dlx_detector_op* ops, int nops, int nsum = 1);
 int detector_apply_sum(int n, int batch, int nsum, long sum_stride, const float* in,
                        const dlx_detector_op* ops, int nops, float* out, void* workspace,
                        size_t workspace_bytes, hipStream_t s);

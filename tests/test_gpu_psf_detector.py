@@ -5,7 +5,8 @@ first pass loads their sum, so the PSF is never written.
 Checks: identical bits to dlx_psf_poly followed by dlx_detector_apply (the
 planes are summed in the same wavelength order) for every kind of first
 pass (convolve + downsample, convolve, downsample, elementwise), one and two
-wavelength chunks, both arithmetic forms; the FP64 oracle (poly_psf then the
+wavelength chunks, both arithmetic forms, small frames (planes summed before
+the chain) and large ones (summed by the first pass's loads); the FP64 oracle (poly_psf then the
 detector restatement) within 1e-5 of the peak; Telescope.model of a point
 source through a LayeredDetector taking the fused call, equal to the
 unfused model, and differentiable (it runs as the composition then).
@@ -71,11 +72,14 @@ def _device(chain):
     return out
 
 
+# 64^2 frames: every chain's first launch is small, so the planes are summed by
+# the pixel-parallel plane sum first; 1024^2 frames: the first launch has >= 512
+# workgroups and sums the planes as it loads them
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
-@pytest.mark.parametrize("nwl", [7, 70])
-def test_fused_equals_psf_then_detector_and_oracle(dl, prec, nwl):
+@pytest.mark.parametrize("n,n_out,nwl", [(96, 64, 7), (96, 64, 70), (128, 1024, 7)])
+def test_fused_equals_psf_then_detector_and_oracle(dl, prec, n, n_out, nwl):
     from dlux_amd import runtime
-    n, n_out, p_out = 96, 64, 2.5e-7
+    p_out = 2.5e-7
     mask, opd = _pupil(n, 3)
     wls = np.linspace(0.85e-6, 1.15e-6, nwl)
     w = np.random.default_rng(nwl).uniform(0.5, 1.5, nwl)
@@ -87,7 +91,7 @@ def test_fused_equals_psf_then_detector_and_oracle(dl, prec, nwl):
     ref_psf = orc.poly_psf(mask, opd[None], np.ones(1), wls, w, 1.0, n_out, p_out,
                            offset=(0.7 * p_out, -1.2 * p_out))
     for name, chain in _chains(n_out, np.random.default_rng(5)).items():
-        print(f"[psf+detector] prec={prec} nwl={nwl} chain={name}", flush=True)
+        print(f"[psf+detector] prec={prec} {n}->{n_out} nwl={nwl} chain={name}", flush=True)
         ops = _device(chain)
         fused = runtime.psf_poly_detector(n, n_out, wls, w, 1.0 / n, p_out, ops, **kw)
         psf = runtime.psf_poly(n, n_out, wls, w, 1.0 / n, p_out, **kw)
