@@ -512,6 +512,149 @@ det_conv_down_kernel(const float* __restrict__ src, int n, int th, int tw, int t
   }
 }
 
+// ---------------------------------------------------------------------------
+// Column-walking separable convolution (odd K <= 13), optionally followed by a
+// KD x KD block sum (KD = 2 or 4; KD = 1: the convolution alone).  Each thread
+// owns 4 adjacent columns of one image and walks down a segment of rows with
+// no LDS and no barriers: per input row it loads the 4 + K - 1 inputs of its
+// horizontal window as aligned 16-byte pieces (neighbouring threads share
+// them through the L1 cache), forms the 4 horizontal sums and shifts them
+// into a register window of the last K rows; each output row is the vertical
+// sum over that window, and block sums of KD rows accumulate in registers as
+// the rows go by.  Loads run two rows ahead.  Needs n % 4 == 0.  The
+// arithmetic, order included, is that of conv_tile with det_corr_kernel /
+// det_conv_down_kernel.
+// ---------------------------------------------------------------------------
+constexpr int kColThreads = 256;   // 1024 columns per workgroup
+constexpr int kColAhead = 2;       // rows of loads in flight per thread (more
+                                   // registers cost more occupancy than they gain)
+
+template <int K>
+struct ColPieces {
+  static constexpr int LO = (K - 1) / 2 - (K - 1);       // first tap's offset
+  static constexpr int LO4 = -((-LO + 3) / 4) * 4;       // LO rounded down to x4
+  static constexpr int OFF = LO - LO4;                   // window start in the pieces
+  static constexpr int NV = (OFF + 4 + K - 1 + 3) / 4;   // 16-byte pieces per row
+};
+
+template <int KD, int K>
+__global__ void __launch_bounds__(kColThreads)
+det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
+                   const float* __restrict__ w, EwList pre, EwList post, int m, EwList post2,
+                   float* __restrict__ out) {
+  constexpr int LO = ColPieces<K>::LO, LO4 = ColPieces<K>::LO4;
+  constexpr int OFF = ColPieces<K>::OFF, NV = ColPieces<K>::NV;
+  constexpr int NB = KD == 1 ? 1 : 4 / KD;
+  int tile, image;
+  xcd_tile(has_map(pre) || has_map(post), tile, image);
+  const int band = tile % bands, seg = tile / bands;
+  const int c0 = band * (4 * kColThreads) + 4 * threadIdx.x;
+  if (c0 >= n) return;   // no barriers below
+  const float* s = src + (long)image * n * n;
+  const int y0 = seg * seg_h, y1 = min(y0 + seg_h, n);
+  const int total = (y1 - y0) + K - 1;   // H rows y0 + LO + i, i < total
+  // the convolution as a correlation with the reversed factors (w: K column
+  // then K row weights)
+  float wy[K], wx[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    wy[t] = w[K - 1 - t];
+    wx[t] = w[2 * K - 1 - t];
+  }
+  // this thread's pieces: columns xs + 4 v, clamped into the image; the
+  // window is inside the image for all but the edge threads
+  const int xs = c0 + LO4;
+  const bool inside = xs >= 0 && xs + 4 * NV <= n;
+  int xoff[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) xoff[v] = min(max(xs + 4 * v, 0), n - 4);
+  const int pi = first_map(post);
+  float4 q0[NV], q1[NV];
+  float m0[4], m1[4];
+  float hw[K][4];
+  float blk[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) blk[b] = 0.f;
+  // loads of H row i (input row y0 + LO + i) and of the map values of the
+  // output row it completes (y0 + i - (K - 1))
+#define DLX_COL_LOAD(i, q, mv)                                                     \
+  do {                                                                             \
+    const float* rowp_ = s + (long)min(max(y0 + LO + (i), 0), n - 1) * n;          \
+    _Pragma("unroll") for (int v = 0; v < NV; ++v) q[v] = *(const float4*)(rowp_ + xoff[v]); \
+    load_map4(post, pi, n, min(max(y0 + (i) - (K - 1), 0), n - 1), c0, mv);        \
+  } while (0)
+  // one row: H row i from the buffer (q, mq) loaded kColAhead rows earlier,
+  // which then takes the loads of row i + kColAhead
+  auto row_step = [&](int i, float4 (&q)[NV], float (&mq)[4]) {
+    float a[4 * NV], mv[4];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      a[4 * v] = q[v].x; a[4 * v + 1] = q[v].y; a[4 * v + 2] = q[v].z; a[4 * v + 3] = q[v].w;
+    }
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) mv[jj] = mq[jj];
+    if (i + kColAhead < total) DLX_COL_LOAD(i + kColAhead, q, mq);
+    const int r = y0 + LO + i;
+    const bool okr = r >= 0 && r < n;
+    if (!okr || !inside || pre.n) {
+#pragma unroll
+      for (int e = 0; e < 4 * NV; ++e) {
+        const int x = xs + e;
+        const bool ok = okr && x >= 0 && x < n;
+        a[e] = ok ? ew_apply(pre, a[e], (long)r * n + x) : 0.f;
+      }
+    }
+    // shift the window and add H row i
+#pragma unroll
+    for (int t = 0; t + 1 < K; ++t)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) hw[t][jj] = hw[t + 1][jj];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      float h = 0.f;
+#pragma unroll
+      for (int t = 0; t < K; ++t) h += a[OFF + jj + t] * wx[t];
+      hw[K - 1][jj] = h;
+    }
+    if (i < K - 1) return;
+    const int Y = y0 + i - (K - 1);
+    float o[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      float acc = 0.f;
+#pragma unroll
+      for (int t = 0; t < K; ++t) acc += hw[t][jj] * wy[t];
+      o[jj] = ew_apply_pf(post, acc, (long)Y * n + c0 + jj, pi, mv[jj]);
+    }
+    if (KD == 1) {
+      *(float4*)(out + (long)image * n * n + (long)Y * n + c0) = make_float4(o[0], o[1], o[2], o[3]);
+      return;
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (KD == 4) blk[b] += ((o[0] + o[1]) + o[2]) + o[3];
+      else blk[b] += o[2 * b] + o[2 * b + 1];
+    }
+    if (Y % KD == KD - 1) {
+      const int gy = Y / KD;
+      float* dst = out + (long)image * m * m + (long)gy * m;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int gx = c0 / KD + b;
+        dst[gx] = ew_apply(post2, blk[b], (long)gy * m + gx);
+        blk[b] = 0.f;
+      }
+    }
+  };
+  DLX_COL_LOAD(0, q0, m0);
+  if (total > 1) DLX_COL_LOAD(1, q1, m1);
+  for (int i = 0; i < total; i += kColAhead) {
+    row_step(i, q0, m0);
+    if (i + 1 < total) row_step(i + 1, q1, m1);
+  }
+#undef DLX_COL_LOAD
+}
+
 // out[b][Y][X] = post( sum_{dy,dx < k} pre(src[b][Y k + dy][X k + dx]) ), m = n / k.
 // Row sums first, then the sum of the rows (the order of utils/array_ops.py:151-161).
 // Grid: (pixel blocks of the output image, image); in-image indices are 32-bit.
@@ -919,8 +1062,69 @@ int check_grid(const std::vector<Pass>& passes, int batch) {
 }
 
 // Forward pass i: src -> dst (r_save may be NULL).
+// The column-walking kernel takes a separable forward convolution of odd K <=
+// 13 on a single (not summed) input whose side is a multiple of 4, alone or
+// with a 2 x 2 / 4 x 4 block sum.
+bool streams(const Pass& c, int k, SumIn si) {
+  return c.conv && c.sep && c.K <= 13 && c.K % 2 == 1 && c.n_in % 4 == 0 &&
+         (k == 1 || k == 2 || k == 4) && si.n == 1;
+}
+
+// Bands of 1024 columns; segments of at least 64 rows (a multiple of 4), about
+// three workgroups per CU in all (one resident round at the kernel's
+// occupancy of 3 waves per SIMD).
+void stream_grid(int n, int batch, int& bands, int& seg_h, int& nseg) {
+  bands = (n + 4 * kColThreads - 1) / (4 * kColThreads);
+  const long per = (long)bands * batch;
+  long want = (768 + per - 1) / per;
+  want = std::min<long>(want, std::max(1, n / 64));
+  seg_h = (int)(((n + want - 1) / want + 3) / 4 * 4);
+  nseg = (n + seg_h - 1) / seg_h;
+}
+
+template <int KD, int K>
+void launch_col(const dim3& grid, int threads, hipStream_t st, const Pass& c, int bands,
+                int seg_h, int m, const EwList& post2, const float* src, float* dst) {
+  hipLaunchKernelGGL((det_sep_col_kernel<KD, K>), grid, dim3(threads), 0, st, src, c.n_in, bands,
+                     seg_h, c.w, c.pre, c.post, m, post2, dst);
+}
+
+template <int KD>
+void launch_col_k(const dim3& grid, int threads, hipStream_t st, const Pass& c, int bands,
+                  int seg_h, int m, const EwList& post2, const float* src, float* dst) {
+  switch (c.K) {
+    case 1: launch_col<KD, 1>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 3: launch_col<KD, 3>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 5: launch_col<KD, 5>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 7: launch_col<KD, 7>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 9: launch_col<KD, 9>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 11: launch_col<KD, 11>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
+    default: launch_col<KD, 13>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
+  }
+}
+
+int run_stream(const Pass& c, int k, int m, const EwList& post2, int batch, const float* src,
+               float* dst, hipStream_t st) {
+  int bands, seg_h, nseg;
+  stream_grid(c.n_in, batch, bands, seg_h, nseg);
+  // one thread per 4 columns, whole waves, at most kColThreads
+  const int cols = std::min(c.n_in, 4 * kColThreads);
+  const int threads = std::min(kColThreads, (cols / 4 + 63) / 64 * 64);
+  const dim3 grid(bands * nseg, batch);
+  if (k == 4) launch_col_k<4>(grid, threads, st, c, bands, seg_h, m, post2, src, dst);
+  else if (k == 2) launch_col_k<2>(grid, threads, st, c, bands, seg_h, m, post2, src, dst);
+  else launch_col_k<1>(grid, threads, st, c, bands, seg_h, c.n_in, EwList{}, src, dst);
+  return launch_status("det_sep_col_kernel");
+}
+
+bool aligned16(const void* a, const void* b) {
+  return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
+}
+
 int run_pass(const Pass& p, int batch, const float* src, float* r_save, float* dst,
              hipStream_t st, SumIn si = SumIn{1, 0}) {
+  if (p.conv && !r_save && streams(p, 1, si) && aligned16(src, dst))
+    return run_stream(p, 1, p.n_in, EwList{}, batch, src, dst, st);
   if (p.conv) {
     const int rpt = conv_rpt(p.K, p.sep);
     const int tiles_x = (p.n_in + kTileX - 1) / kTileX;
@@ -961,6 +1165,8 @@ bool can_fuse(const std::vector<Pass>& passes, size_t i) {
 
 int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, float* dst,
                   hipStream_t st, SumIn si = SumIn{1, 0}) {
+  if (streams(c, d.k, si) && aligned16(src, dst))
+    return run_stream(c, d.k, d.n_out, d.post, batch, src, dst, st);
   const int rpt = conv_rpt(c.K, c.sep);
   const int k = d.k;
   const int th = (kTileY * rpt / k) * k, tw = (kTileX / k) * k;
@@ -986,6 +1192,12 @@ int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, flo
 // run_conv_down takes both).
 long first_launch_groups(const std::vector<Pass>& passes, int batch) {
   const Pass& p = passes[0];
+  const int k = can_fuse(passes, 0) ? passes[1].k : 1;
+  if (streams(p, k, SumIn{1, 0})) {
+    int bands, seg_h, nseg;
+    stream_grid(p.n_in, batch, bands, seg_h, nseg);
+    return (long)bands * nseg * batch;
+  }
   if (p.conv) {
     const int rpt = conv_rpt(p.K, p.sep);
     long tiles;
@@ -1003,15 +1215,17 @@ long first_launch_groups(const std::vector<Pass>& passes, int batch) {
 }
 
 // A summed input (nsum planes) is loaded by the first pass, summing as it
-// loads, when that launch fills the GPU; a smaller one is summed first by the
-// pixel-parallel plane sum (same order, same bits) into one more image of
-// workspace, because the first pass's few workgroups would each add up nsum
-// planes serially.
+// loads, when that launch fills the GPU and is a tile kernel; otherwise it is
+// summed first by the pixel-parallel plane sum (same order, same bits) into
+// one more image of workspace: a small first launch's few workgroups would
+// each add up nsum planes serially, and the streaming kernel loads one image.
 constexpr long kSumInLoaderGroups = 512;
 
 bool presum(const std::vector<Pass>& passes, int n, int batch, int nsum, long sum_stride) {
+  const int k = can_fuse(passes, 0) ? passes[1].k : 1;
   return nsum > 1 && sum_stride == (long)n * n &&
-         first_launch_groups(passes, batch) < kSumInLoaderGroups;
+         (first_launch_groups(passes, batch) < kSumInLoaderGroups ||
+          streams(passes[0], k, SumIn{1, 0}));
 }
 
 size_t detector_ws_bytes(int n, int batch, const dlx_detector_op* ops, int nops, int nsum) {

@@ -88,6 +88,27 @@ def _chains(rng):
                                                              rng.normal(size=13))),
                                      ("convolve", rng.normal(size=(14, 14))),
                                      ("convolve", rng.normal(size=(13, 13)))]))
+    # the streaming separable kernel (<= 13 taps; block sums of 1, 2, 4): the
+    # jitter -> response -> downsample 4 -> saturation -> constant chain over
+    # several segments per band; ragged bands and steps; an even kernel under a
+    # load-side map; the tap limit with 2 x 2 blocks (their own generator, so
+    # the draws of the cases above are unchanged)
+    rng = np.random.default_rng(7)
+    f9 = orc.jitter_kernel(0.5, 9, 3).sum(0)   # the kernel is outer(f9, f9)
+    cases.append(("stream-jitter-down4", 1024, 2,
+                  [("convolve_separable", (f9, f9)),
+                   ("pixel_response", rng.uniform(0.9, 1.1, (1024, 1024))), ("downsample", 4),
+                   ("saturate", None), ("add", 0.01)]))
+    cases.append(("stream-down2-ragged", 98, 3,
+                  [("convolve_separable", (rng.normal(size=7), rng.normal(size=7))),
+                   ("downsample", 2), ("add", 0.2)]))
+    cases.append(("stream-even-pre-map", 100, 2,
+                  [("pixel_response", rng.uniform(0.5, 1.5, (100, 100))), ("add", 0.1),
+                   ("convolve_separable", (rng.uniform(0, 1, 6), rng.uniform(0, 1, 6))),
+                   ("downsample", 4), ("pixel_response", rng.uniform(0.5, 1.5, (25, 25)))]))
+    cases.append(("stream-k13-down2", 130, 1,
+                  [("convolve_separable", (rng.normal(size=13), rng.normal(size=13))),
+                   ("saturate", None), ("downsample", 2)]))
     return cases
 
 
@@ -116,7 +137,7 @@ def _oracle(x, chain):
     return np.stack([orc.detector(xi, chain) for xi in x])
 
 
-@pytest.mark.parametrize("case", range(16))
+@pytest.mark.parametrize("case", range(20))
 def test_detector_chain_and_vjp_vs_oracle(dl, case):
     rng = np.random.default_rng(40 + case)
     name, n, batch, chain = _chains(rng)[case]

@@ -46,10 +46,10 @@ def _pupil(n, seed):
 
 def _chains(n_out, rng):
     jit = orc.jitter_kernel(0.7, 7, 3)
-    g = orc.gaussian(0.0, 0.7, 7)
+    f = jit.sum(0)   # jit = outer(f, f)
     return {
         "jitter-response-down-sat-add": [
-            ("convolve_separable", (g / g.sum(), g / g.sum())),
+            ("convolve_separable", (f, f)),
             ("pixel_response", rng.uniform(0.8, 1.2, (n_out, n_out))),
             ("downsample", 4), ("saturate", 3e-3), ("add", 1e-5)],
         "response-convolve": [("pixel_response", rng.uniform(0.8, 1.2, (n_out, n_out))),
