@@ -242,9 +242,10 @@ def _grad_leg(torch, runtime, workloads, prec, steps, warmup, with_oracle):
     def step():
         psf = system.propagate(cfg.wavelengths, weights=cfg.weights)
         loss = 0.5 * torch.sum((flux * psf - data) ** 2 / var)
-        coeffs.grad = None
+        if coeffs.grad is not None:
+            coeffs.grad.zero_()
         loss.backward()
-        return loss
+        return loss.detach()
 
     with runtime.precision(prec):
         for _ in range(warmup):
@@ -255,8 +256,19 @@ def _grad_leg(torch, runtime, workloads, prec, steps, warmup, with_oracle):
             step()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
-        loss = float(step().detach())
-    g = coeffs.grad.double().cpu().numpy()
+        loss = float(step())
+        g = coeffs.grad.double().cpu().numpy()
+        # the same step captured once in a HIP graph and replayed
+        graphed = runtime.graph_step(step)
+        for _ in range(warmup):
+            graphed()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            graphed()
+        torch.cuda.synchronize()
+        ms_graph = (time.perf_counter() - t0) * 1e3 / steps
+        g_graph = coeffs.grad.double().cpu().numpy()
     err = None
     if with_oracle:
         from oracle import dlux_oracle as orc
@@ -269,9 +281,13 @@ def _grad_leg(torch, runtime, workloads, prec, steps, warmup, with_oracle):
     return {"workload": "grad: 512^2 pupil, 36 Zernikes, 16 lambda 0.9-1.1 um, 128^2 output, "
                         "Gaussian loss vs Poisson(1e6 photons) target",
             "precision": prec, "steps_per_s": 1e3 / ms, "ms_per_step": ms, "loss": loss,
+            "graphed_steps_per_s": 1e3 / ms_graph, "graphed_ms_per_step": ms_graph,
+            "graphed_equals_eager": bool(np.array_equal(g, g_graph)),
             "grad_max_rel_err_vs_fp64": err,
             "mft_tflops_per_step": 3 * fwd / 1e12,
-            "note": "forward + VJP (field recomputed: 3 MFTs' worth of contractions per step)"}
+            "note": "forward + VJP (field recomputed: 3 MFTs' worth of contractions per step); "
+                    "steps_per_s: the Python loop; graphed: the step replayed as one HIP graph "
+                    "(runtime.graph_step)"}
 
 
 def _batch_leg(torch, runtime, workloads, steps, warmup):

@@ -1253,3 +1253,34 @@ def cache_stats() -> dict:
     _lib.check(_lib.load().dlx_cache_stats(ctypes.byref(h), ctypes.byref(m), ctypes.byref(b),
                                            ctypes.byref(e)), "dlx_cache_stats")
     return {"hits": h.value, "misses": m.value, "bytes": b.value, "entries": e.value}
+
+
+class graph_step:
+    """A fixed-shape step -- forward model, loss and backward, as fn() -- captured
+    once in a HIP graph and replayed, so a training or fitting loop costs one
+    graph launch of host time per step instead of the Python and dispatch work
+    of every call.  The role jax.jit plays for dLux's gradient loops.
+
+    fn reads its inputs from tensors that the caller updates in place
+    (``t.copy_(new)``, ``with torch.no_grad(): p.copy_(...)``) and zeroes
+    gradients in place (``p.grad.zero_()`` when ``p.grad`` exists), never
+    ``p.grad = None``.  Calling the object replays the graph and returns fn's
+    outputs from the capture: static tensors overwritten by every replay.
+    fn is run ``warmup`` times on a side stream first (the transfer matrices
+    are cached then, and parameter gradients allocated)."""
+
+    def __init__(self, fn, warmup: int = 3):
+        self.graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(max(int(warmup), 1)):
+                fn()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(self.graph):
+            self.outputs = fn()
+
+    def __call__(self):
+        self.graph.replay()
+        return self.outputs

@@ -72,3 +72,55 @@ def test_psf_poly_and_vjp_graph_replays(rt, prec):
             assert e_psf <= 1e-6 and e_g <= 1e-6, (rep, i, e_psf, e_g)
             eager(opds[(i + 1) % len(opds)])    # eager work between replays
             torch.cuda.synchronize()
+
+
+def test_graphed_gradient_step_matches_eager(rt):
+    """runtime.graph_step over forward PSF + Gaussian loss + backward to the
+    Zernike coefficients: every replay, with new coefficients copied in and
+    eager work between replays, gives the eager loss and gradient."""
+    print("[graph] gradient step", flush=True)
+    import dlux_amd as dl
+    n, n_out, nwl = 128, 64, 6
+    mask, basis = orc.circular_aperture(n, 1.0, oversample=1, zernike_nolls=range(2, 12))
+    wls = np.linspace(0.9e-6, 1.1e-6, nwl)
+    rng = np.random.default_rng(8)
+    sets = [rng.normal(0, 30e-9, basis.shape[0]) for _ in range(4)]
+    coeffs = torch.tensor(sets[0], dtype=torch.float32, device="cuda", requires_grad=True)
+    optics = dl.AngularOpticalSystem(
+        wf_npixels=n, diameter=1.0,
+        layers=[("aperture", dl.TransmissiveLayer(mask, normalise=True)),
+                ("aberrations", dl.BasisLayer(basis, coeffs))],
+        psf_npixels=n_out, psf_pixel_scale=dl.utils.rad2arcsec(2.5e-7))
+    data = torch.as_tensor(rng.poisson(1e4 * orc.poly_psf(mask, basis, sets[1], wls, None, 1.0,
+                                                          n_out, 2.5e-7)),
+                           dtype=torch.float32, device="cuda")
+    var = torch.clamp(data, min=1.0)
+
+    def step():
+        psf = optics.propagate(wls)
+        loss = 0.5 * torch.sum((1e4 * psf - data) ** 2 / var)
+        if coeffs.grad is not None:
+            coeffs.grad.zero_()
+        loss.backward()
+        return loss.detach()
+
+    def eager(c):
+        with torch.no_grad():
+            coeffs.copy_(torch.as_tensor(c, dtype=torch.float32))
+        loss = step()
+        torch.cuda.synchronize()
+        return float(loss), coeffs.grad.clone()
+
+    with rt.precision("bf16x3"):
+        refs = [eager(c) for c in sets]
+        g = rt.graph_step(step)
+        for rep in range(2):
+            for i, c in enumerate(sets):
+                with torch.no_grad():
+                    coeffs.copy_(torch.as_tensor(c, dtype=torch.float32))
+                loss = g()
+                torch.cuda.synchronize()
+                assert abs(float(loss) - refs[i][0]) <= 1e-6 * abs(refs[i][0]), (rep, i)
+                err = _peak_err(coeffs.grad, refs[i][1])
+                assert err <= 1e-6, (rep, i, err)
+                eager(sets[(i + 1) % len(sets)])    # eager work between replays
