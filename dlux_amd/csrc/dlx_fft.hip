@@ -26,6 +26,11 @@ namespace {
 constexpr int kThreads = 512;
 constexpr int kLdsElems = 16384;
 constexpr int kLdsBudget = 20480;   // complex64 values in 160 KiB
+// LDS row stride beyond M (complex64).  0: padded rows would spread the
+// transposed store's row reads over the banks, but push a 4-row 2048-point
+// block past half of the CU's LDS and halve the workgroups per CU (measured
+// slower).
+constexpr int kRowPad = 0;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -83,7 +88,7 @@ __device__ __forceinline__ void dft_reg<8>(float2 (&x)[8], float sgn) {
 // y_q to row[(j - k) RAD + k + q p].  p is the product of earlier radices.
 template <int RAD, int LOG2RAD>
 __device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, int total,
-                                               int log2M, int log2p, float sgn) {
+                                               int log2M, int rs, int log2p, float sgn) {
   constexpr int kItems = kLdsElems / RAD / kThreads;
   const int log2_per_row = log2M - LOG2RAD;
   const int per_row = 1 << log2_per_row;
@@ -94,7 +99,7 @@ __device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, in
     const int t = threadIdx.x + it * kThreads;
     if (t < total) {
       const int rr = t >> log2_per_row, j = t & (per_row - 1);
-      const float2* row = buf + (rr << log2M);
+      const float2* row = buf + rr * rs;
 #pragma unroll
       for (int r = 0; r < RAD; ++r) v[it][r] = row[j + r * per_row];
     }
@@ -112,7 +117,7 @@ __device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, in
         for (int r = 1; r < RAD; ++r) v[it][r] = cmul(v[it][r], tw[(r * k) << sh]);
       }
       dft_reg<RAD>(v[it], sgn);
-      float2* row = buf + (rr << log2M);
+      float2* row = buf + rr * rs;
       const int i0 = ((j - k) << LOG2RAD) + k;
 #pragma unroll
       for (int q = 0; q < RAD; ++q) row[i0 + (q << log2p)] = v[it][q];
@@ -136,9 +141,16 @@ __global__ void __launch_bounds__(kThreads)
 fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftParams P) {
   extern __shared__ float2 lds[];
   const int M = P.M, R = P.R, half_m = M >> 1;
-  float2* tw = lds + (long)R * M;  // M twiddles exp(sign 2 pi i q / M)
-  const int b = blockIdx.y;
-  const int r0 = blockIdx.x * R;
+  const int RS = M + kRowPad;      // LDS row stride
+  float2* tw = lds + (long)R * RS;  // M twiddles exp(sign 2 pi i q / M)
+  // consecutive row groups on one XCD (workgroups are dealt to the 8 XCDs
+  // round-robin), so the short runs of the transposed stores that neighbouring
+  // groups write into the same cache lines meet in that XCD's L2
+  const int groups = gridDim.x, total_wg = gridDim.x * gridDim.y;
+  int id = blockIdx.y * groups + blockIdx.x;
+  if (total_wg % 8 == 0) id = (id % 8) * (total_wg / 8) + id / 8;
+  const int b = id / groups;
+  const int r0 = (id - b * groups) * R;
   const float sgn = P.inverse ? 1.f : -1.f;
   const float2* src = in + (long)b * P.in_bstride;
 
@@ -147,39 +159,52 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
     sincospif(2.0f * (float)q / (float)M, &s, &c);
     tw[q] = make_float2(c, sgn * s);
   }
-  // load in natural order (M and R are powers of two: shifts and masks)
+  // load in natural order (M and R are powers of two: shifts and masks), 16
+  // values per thread in flight before any is written to LDS
   const int total = R * M;
   const int log2M = P.log2M;
-  for (int t = threadIdx.x; t < total; t += kThreads) {
-    const int rr = t >> log2M, nc = t & (M - 1);
-    const int row = r0 + rr;
-    float2 v = make_float2(0.f, 0.f);
-    if (P.in_mode == 0) {
-      const int sr = ((row + half_m) & (M - 1)) - P.p;
-      const int sc = ((nc + half_m) & (M - 1)) - P.p;
-      if (sr >= 0 && sr < P.N && sc >= 0 && sc < P.N) v = src[(long)sr * P.N + sc];
-    } else {
-      v = src[(long)row * M + nc];
+  constexpr int kLoadBatch = 16;
+  for (int t0 = 0; t0 < total; t0 += kLoadBatch * kThreads) {
+    float2 v[kLoadBatch];
+#pragma unroll
+    for (int i = 0; i < kLoadBatch; ++i) {
+      const int t = t0 + threadIdx.x + i * kThreads;
+      const int rr = t >> log2M, nc = t & (M - 1);
+      const int row = r0 + rr;
+      v[i] = make_float2(0.f, 0.f);
+      if (t < total) {
+        if (P.in_mode == 0) {
+          const int sr = ((row + half_m) & (M - 1)) - P.p;
+          const int sc = ((nc + half_m) & (M - 1)) - P.p;
+          if (sr >= 0 && sr < P.N && sc >= 0 && sc < P.N) v[i] = src[(long)sr * P.N + sc];
+        } else {
+          v[i] = src[(long)row * M + nc];
+        }
+      }
     }
-    lds[rr * M + nc] = v;
+#pragma unroll
+    for (int i = 0; i < kLoadBatch; ++i) {
+      const int t = t0 + threadIdx.x + i * kThreads;
+      if (t < total) lds[(t >> log2M) * RS + (t & (M - 1))] = v[i];
+    }
   }
   __syncthreads();
 
   // Stockham stages: radix 8 while it divides, then one radix-4 or radix-2 stage
   int log2p = 0;
   while (log2M - log2p >= 3) {
-    stockham_stage<8, 3>(lds, tw, total >> 3, log2M, log2p, sgn);
+    stockham_stage<8, 3>(lds, tw, total >> 3, log2M, RS, log2p, sgn);
     log2p += 3;
   }
-  if (log2M - log2p == 2) stockham_stage<4, 2>(lds, tw, total >> 2, log2M, log2p, sgn);
-  else if (log2M - log2p == 1) stockham_stage<2, 1>(lds, tw, total >> 1, log2M, log2p, sgn);
+  if (log2M - log2p == 2) stockham_stage<4, 2>(lds, tw, total >> 2, log2M, RS, log2p, sgn);
+  else if (log2M - log2p == 1) stockham_stage<2, 1>(lds, tw, total >> 1, log2M, RS, log2p, sgn);
 
   // transposed store: out[k][row] (consecutive lanes -> consecutive rows)
   float2* dst = out + (long)b * P.out_bstride;
   for (int t = threadIdx.x; t < total; t += kThreads) {
     const int k = t >> P.log2R, rr = t & (R - 1);
     const int row = r0 + rr;
-    float2 v = lds[rr * M + k];
+    float2 v = lds[rr * RS + k];
     if (P.out_mode == 0) {
       dst[(long)k * M + row] = v;
     } else {
@@ -214,7 +239,7 @@ hipError_t launch_rows(const float2* in, float2* out, int batch, FftParams P, hi
   P.log2M = ilog2(P.M);
   // rows per workgroup: about two workgroups per CU (512) when the rows fit,
   // within kLdsElems and the LDS left beside the M-entry twiddle table
-  int cap = (kLdsBudget - P.M) / P.M;
+  int cap = (kLdsBudget - P.M) / (P.M + kRowPad);
   if (cap > kLdsElems / P.M) cap = kLdsElems / P.M;
   if (cap < 1) cap = 1;
   // (at least 4 rows -- 32-byte runs in the transposed store -- while that
@@ -227,7 +252,7 @@ hipError_t launch_rows(const float2* in, float2* out, int batch, FftParams P, hi
   P.R = 1 << (ilog2(r + 1) - 1);   // largest power of two <= r
   if (P.R > P.M) P.R = P.M;
   P.log2R = ilog2(P.R);
-  const size_t lds = ((size_t)P.R * P.M + P.M) * sizeof(float2);
+  const size_t lds = ((size_t)P.R * (P.M + kRowPad) + P.M) * sizeof(float2);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   if (lds > 64 * 1024) {
     // Opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
