@@ -35,10 +35,11 @@ ERR_BOUND = 1e-5
 # FETCH_SIZE / WRITE_SIZE runs scaled by the same counters on a streaming kernel of
 # known size (profiles/mft_poly_512to256_64wl_hbm_traffic.md).  The counters cannot
 # be read from inside this process.
-TRAFFIC_BYTES = {}
+# Keyed by (form, pupil generated in stage 1's loader).
+TRAFFIC_BYTES = {("bf16x3", True): 161.4e6, ("bf16x3", False): 273.2e6}
 # MFMA busy fraction of the stage-1 kernel from rocprofv3 SQ_VALU_MFMA_BUSY_CYCLES /
 # (GRBM_GUI_ACTIVE / 8 x 1024 SIMDs) (profiles/mft_poly_512to256_64wl_mfma_counters.md).
-MFMA_BUSY_PMC = {}
+MFMA_BUSY_PMC = {("bf16x3", True): 0.442, ("bf16x3", False): 0.489}
 
 
 def _stage1_kernel(head, prologue):
