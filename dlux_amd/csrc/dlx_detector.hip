@@ -1064,10 +1064,15 @@ int check_grid(const std::vector<Pass>& passes, int batch) {
 // Forward pass i: src -> dst (r_save may be NULL).
 // The column-walking kernel takes a separable forward convolution of odd K <=
 // 13 on a single (not summed) input whose side is a multiple of 4, alone or
-// with a 2 x 2 / 4 x 4 block sum.
-bool streams(const Pass& c, int k, SumIn si) {
+// with a 2 x 2 / 4 x 4 block sum, when the batch is large enough: at least 256
+// (band, 64-row segment, image) triples.  A thread walks its rows one after
+// the other, so on a small batch (one 256^2 frame: 4 workgroups, ~100 us) the
+// tile kernels, which cover the image with many short-lived workgroups, win.
+bool streams(const Pass& c, int k, SumIn si, int batch) {
+  const long bands = (c.n_in + 4 * kColThreads - 1) / (4 * kColThreads);
+  const long segs = std::max(1, c.n_in / 64);
   return c.conv && c.sep && c.K <= 13 && c.K % 2 == 1 && c.n_in % 4 == 0 &&
-         (k == 1 || k == 2 || k == 4) && si.n == 1;
+         (k == 1 || k == 2 || k == 4) && si.n == 1 && bands * segs * batch >= 256;
 }
 
 // Bands of 1024 columns; segments of at least 64 rows (a multiple of 4), about
@@ -1123,7 +1128,7 @@ bool aligned16(const void* a, const void* b) {
 
 int run_pass(const Pass& p, int batch, const float* src, float* r_save, float* dst,
              hipStream_t st, SumIn si = SumIn{1, 0}) {
-  if (p.conv && !r_save && streams(p, 1, si) && aligned16(src, dst))
+  if (p.conv && !r_save && streams(p, 1, si, batch) && aligned16(src, dst))
     return run_stream(p, 1, p.n_in, EwList{}, batch, src, dst, st);
   if (p.conv) {
     const int rpt = conv_rpt(p.K, p.sep);
@@ -1165,7 +1170,7 @@ bool can_fuse(const std::vector<Pass>& passes, size_t i) {
 
 int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, float* dst,
                   hipStream_t st, SumIn si = SumIn{1, 0}) {
-  if (streams(c, d.k, si) && aligned16(src, dst))
+  if (streams(c, d.k, si, batch) && aligned16(src, dst))
     return run_stream(c, d.k, d.n_out, d.post, batch, src, dst, st);
   const int rpt = conv_rpt(c.K, c.sep);
   const int k = d.k;
@@ -1193,7 +1198,7 @@ int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, flo
 long first_launch_groups(const std::vector<Pass>& passes, int batch) {
   const Pass& p = passes[0];
   const int k = can_fuse(passes, 0) ? passes[1].k : 1;
-  if (streams(p, k, SumIn{1, 0})) {
+  if (streams(p, k, SumIn{1, 0}, batch)) {
     int bands, seg_h, nseg;
     stream_grid(p.n_in, batch, bands, seg_h, nseg);
     return (long)bands * nseg * batch;
@@ -1225,7 +1230,7 @@ bool presum(const std::vector<Pass>& passes, int n, int batch, int nsum, long su
   const int k = can_fuse(passes, 0) ? passes[1].k : 1;
   return nsum > 1 && sum_stride == (long)n * n &&
          (first_launch_groups(passes, batch) < kSumInLoaderGroups ||
-          streams(passes[0], k, SumIn{1, 0}));
+          streams(passes[0], k, SumIn{1, 0}, batch));
 }
 
 size_t detector_ws_bytes(int n, int batch, const dlx_detector_op* ops, int nops, int nsum) {

@@ -109,14 +109,19 @@ def _chains(rng):
     cases.append(("stream-k13-down2", 130, 1,
                   [("convolve_separable", (rng.normal(size=13), rng.normal(size=13))),
                    ("saturate", None), ("downsample", 2)]))
-    # the column-walking kernel at sides that are multiples of 4 but not of its
-    # 1024-column bands: a 100^2 image (25 of 64 lanes busy) under a load-side
-    # map, and 1028^2 (a second band of one column group)
-    cases.append(("col-100-pre-map", 100, 2,
-                  [("pixel_response", rng.uniform(0.5, 1.5, (100, 100))),
+    # the column-walking kernel (batches of >= 256 band-segment-image triples):
+    # the jitter chain over 32 frames of 512^2; sides that are multiples of 4 but
+    # not of its 1024-column bands: 128^2 frames (32 of 64 lanes busy) under a
+    # load-side map, and 1028^2 (a second band of one column group)
+    cases.append(("col-512-jitter-down4", 512, 32,
+                  [("convolve_separable", (f9, f9)),
+                   ("pixel_response", rng.uniform(0.9, 1.1, (512, 512))), ("downsample", 4),
+                   ("saturate", None), ("add", 0.01)]))
+    cases.append(("col-128-pre-map", 128, 128,
+                  [("pixel_response", rng.uniform(0.5, 1.5, (128, 128))),
                    ("convolve_separable", (rng.uniform(0, 1, 7), rng.uniform(0, 1, 7))),
                    ("downsample", 2), ("add", 0.05)]))
-    cases.append(("col-1028-down4", 1028, 1,
+    cases.append(("col-1028-down4", 1028, 8,
                   [("convolve_separable", (rng.normal(size=5), rng.normal(size=5))),
                    ("downsample", 4), ("saturate", None)]))
     return cases
@@ -147,7 +152,7 @@ def _oracle(x, chain):
     return np.stack([orc.detector(xi, chain) for xi in x])
 
 
-@pytest.mark.parametrize("case", range(22))
+@pytest.mark.parametrize("case", range(23))
 def test_detector_chain_and_vjp_vs_oracle(dl, case):
     rng = np.random.default_rng(40 + case)
     name, n, batch, chain = _chains(rng)[case]
