@@ -148,7 +148,11 @@ int dlx_intensity(int64_t npix, int32_t batch, const void* field,
  *   E_l = sqrt(w_l) * c_l * Y_l^T P_l X_l
  * and psf = sum_l |E_l|^2 (float32 N_out x N_out).  amp/opd/phase as in
  * dlx_pupil_field (NULL allowed); `scale` is the lambda-independent pupil
- * scale s (initial 1/N^2 times any power normalisation).
+ * scale s (initial 1/N^2 times any power normalisation).  In the split-BF16x3
+ * form with no phase, the pupil is generated inside the first contraction
+ * where its tiles fill the GPU (no pupil array is written); the result is the
+ * same bits either way (environment DLUX_AMD_PUPIL_PROLOGUE=0 forces the
+ * two-kernel form).
  * Replaces the reference chain OpticalSystem.propagate -> propagate_mono ->
  * layers -> to_focus -> MFT -> psf.sum(0): optical_systems.py:147-223,
  * :391-425, :560-594, :659-680.
