@@ -167,6 +167,66 @@ __device__ __forceinline__ void epilogue_split(const GemmSplit& g, int b, int m0
     }
 }
 
+// EPI_SPLIT_T for an interior tile, staged through LDS: the lanes of the
+// direct epilogue each write 8-byte pieces of 32 rows 4 KiB apart; here the
+// tile's rows are assembled in LDS, 32 at a time (rows padded by 16 bytes, so
+// the 32 rows of a lane group fall on different banks), and the workgroup
+// writes each row's BM * 8 contiguous bytes with 16-byte stores.  Same bytes
+// as epilogue_split.  `lds`: the main loop's buffers (>= 32 * (BM * 8 + 16)
+// bytes), free once every wave has passed the first barrier below.
+template <int TM, int TN>
+__device__ __forceinline__ void epilogue_split_t_lds(const GemmSplit& g, int b, int m0, int n0,
+                                                     int wm, int wn, int lr, int lk,
+                                                     f32x16_t (&accR)[TM][TN],
+                                                     f32x16_t (&accI)[TM][TN], uint8_t* lds) {
+  constexpr int BM = 64 * TM;
+  constexpr int kSeg = BM * 8;          // one Uᵀ row of the tile: BM / 8 chunks of 64 B
+  constexpr int kRS = kSeg + 16;        // LDS row stride
+  const float alpha = g.alpha[b];
+  const int wave = threadIdx.x >> 6;
+  uint8_t* crow0 = (uint8_t*)g.C + (long)b * g.c_batch + (long)(m0 >> 3) * kChunkBytes;
+#pragma unroll
+  for (int pw = 0; pw < 2; ++pw)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      __syncthreads();   // the previous pass's rows (or the main loop's reads) are done
+      if ((wave & 1) == pw) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int mq = wm + 32 * i + 8 * q + 4 * lk;   // tile-local, first of 4
+            float re[4], im[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              re[e] = alpha * accR[i][j][4 * q + e];
+              im[e] = alpha * accI[i][j][4 * q + e];
+            }
+            uint32_t rh[2], rl[2], ih[2], il[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              split2_bf16(re[2 * h], re[2 * h + 1], rh[h], rl[h]);
+              split2_bf16(im[2 * h], im[2 * h + 1], ih[h], il[h]);
+            }
+            uint8_t* dst = lds + lr * kRS + (mq >> 3) * kChunkBytes + ((mq & 7) >> 2) * 8;
+            *(uint2*)(dst + 0) = make_uint2(rh[0], rh[1]);
+            *(uint2*)(dst + 16) = make_uint2(rl[0], rl[1]);
+            *(uint2*)(dst + 32) = make_uint2(ih[0], ih[1]);
+            *(uint2*)(dst + 48) = make_uint2(il[0], il[1]);
+          }
+      }
+      __syncthreads();
+      // rows n0 + pw * 32 TN + 32 j + (0..31): 16-byte pieces, kSeg / 16 per row
+      const int nbase = n0 + pw * 32 * TN + 32 * j;
+#pragma unroll
+      for (int t = threadIdx.x; t < 32 * (kSeg / 16); t += kThreads) {
+        const int r = t / (kSeg / 16), piece = t - r * (kSeg / 16);
+        const uint4 v = *(const uint4*)(lds + r * kRS + piece * 16);
+        *(uint4*)(crow0 + (long)(nbase + r) * g.ldc + piece * 16) = v;
+      }
+    }
+}
+
 template <int TM, int TN, int EPI>
 __global__ void __launch_bounds__(kThreads)
 cgemm_split4_kernel(GemmSplit g) {
@@ -621,7 +681,10 @@ cgemm_split4_deep_kernel(GemmSplit g) {
   deep_mainloop<TM, TN>(lds, g.A + (long)b * g.a_batch_bytes, g.a_row_bytes, g.M,
                         g.B + (long)b * g.b_batch_bytes, g.b_row_bytes, g.N, g.K, m0, n0,
                         accR, accI);
-  epilogue_split<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
+  if (EPI == EPI_SPLIT_T && m0 + BM <= g.M && n0 + BN <= g.N)
+    epilogue_split_t_lds<TM, TN>(g, b, m0, n0, wm, wn, lr, lk, accR, accI, lds);
+  else
+    epilogue_split<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
 }
 
 // ---------------------------------------------------------------------------
@@ -758,7 +821,7 @@ cgemm_pupil_kernel(GemmSplit g, PupilGen pg) {
     // the next stage's pupil chunk, into the A buffer stage s-1 used
     if (more) convert(s + 1);
   }
-  epilogue_split<TM, TN, EPI_SPLIT_T>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
+  epilogue_split_t_lds<TM, TN>(g, b, m0, n0, wm, wn, lr, lk, accR, accI, lds);
 }
 
 // ---------------------------------------------------------------------------
