@@ -174,11 +174,12 @@ __device__ __forceinline__ void epilogue_split(const GemmSplit& g, int b, int m0
 // writes each row's BM * 8 contiguous bytes with 16-byte stores.  Same bytes
 // as epilogue_split.  `lds`: the main loop's buffers (>= 32 * (BM * 8 + 16)
 // bytes), free once every wave has passed the first barrier below.
-template <int TM, int TN>
+template <int TM, int TN, int EPI = EPI_SPLIT_T>
 __device__ __forceinline__ void epilogue_split_t_lds(const GemmSplit& g, int b, int m0, int n0,
                                                      int wm, int wn, int lr, int lk,
                                                      f32x16_t (&accR)[TM][TN],
                                                      f32x16_t (&accI)[TM][TN], uint8_t* lds) {
+  static_assert(EPI == EPI_SPLIT_T || EPI == EPI_VJP_SPLIT_T, "split4 transposed epilogues");
   constexpr int BM = 64 * TM;
   constexpr int kSeg = BM * 8;          // one Uᵀ row of the tile: BM / 8 chunks of 64 B
   constexpr int kRS = kSeg + 16;        // LDS row stride
@@ -199,8 +200,12 @@ __device__ __forceinline__ void epilogue_split_t_lds(const GemmSplit& g, int b, 
             float re[4], im[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-              re[e] = alpha * accR[i][j][4 * q + e];
-              im[e] = alpha * accI[i][j][4 * q + e];
+              float f = alpha;
+              if (EPI == EPI_VJP_SPLIT_T)   // element (m, n) scaled by 2 G(n, m)
+                f *= 2.f * g.G[(long)b * g.g_batch + (long)(n0 + wn + 32 * j + lr) * g.ldg +
+                               m0 + mq + e];
+              re[e] = f * accR[i][j][4 * q + e];
+              im[e] = f * accI[i][j][4 * q + e];
             }
             uint32_t rh[2], rl[2], ih[2], il[2];
 #pragma unroll
@@ -681,10 +686,13 @@ cgemm_split4_deep_kernel(GemmSplit g) {
   deep_mainloop<TM, TN>(lds, g.A + (long)b * g.a_batch_bytes, g.a_row_bytes, g.M,
                         g.B + (long)b * g.b_batch_bytes, g.b_row_bytes, g.N, g.K, m0, n0,
                         accR, accI);
-  if (EPI == EPI_SPLIT_T && m0 + BM <= g.M && n0 + BN <= g.N)
-    epilogue_split_t_lds<TM, TN>(g, b, m0, n0, wm, wn, lr, lk, accR, accI, lds);
-  else
-    epilogue_split<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
+  if constexpr (EPI == EPI_SPLIT_T || EPI == EPI_VJP_SPLIT_T) {
+    if (m0 + BM <= g.M && n0 + BN <= g.N) {
+      epilogue_split_t_lds<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI, lds);
+      return;
+    }
+  }
+  epilogue_split<TM, TN, EPI>(g, b, m0, n0, wm, wn, lr, lk, accR, accI);
 }
 
 // ---------------------------------------------------------------------------
@@ -990,7 +998,10 @@ cgemm_split4_group_kernel(GroupLaunch gl) {
     v.C = pr.C; v.ldc = pr.ldc; v.c_batch = 0;
     v.M = pr.M; v.N = pr.N; v.K = pr.K; v.batch = 1; v.epilogue = EPI;
     v.alpha[0] = pr.alpha;
-    epilogue_split<TM, TN, EPI>(v, 0, m0, n0, wm, wn, lr, lk, accR, accI);
+    if (EPI == EPI_SPLIT_T && m0 + BM <= pr.M && n0 + BN <= pr.N)
+      epilogue_split_t_lds<TM, TN>(v, 0, m0, n0, wm, wn, lr, lk, accR, accI, lds);
+    else
+      epilogue_split<TM, TN, EPI>(v, 0, m0, n0, wm, wn, lr, lk, accR, accI);
   }
 }
 
