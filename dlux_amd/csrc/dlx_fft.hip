@@ -215,10 +215,12 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
         const double fy = (double)(oy - half_m) * P.asm_df, fx = (double)(ox - half_m) * P.asm_df;
         const double arg = P.asm_inv_wl2 - fx * fx - fy * fy;
         if (arg > 0.0) {
+          // the phase z sqrt(1/lambda^2 - f^2) in FP64 turns (tens of thousands of
+          // turns at millimetre distances), reduced, then the hardware sin / cos
+          // (turns; max abs error 1.3e-7 on [-1/2, 1/2])
           double t = P.asm_z * sqrt(arg);
           t -= rint(t);
-          float s, c;
-          sincospif(2.0f * (float)t, &s, &c);
+          const float s = __builtin_amdgcn_sinf((float)t), c = __builtin_amdgcn_cosf((float)t);
           v = make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
         } else {
           v = make_float2(0.f, 0.f);
