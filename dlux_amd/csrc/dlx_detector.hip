@@ -108,6 +108,30 @@ __device__ __forceinline__ float ew_apply_pf(const EwList& l, float v, long p, i
   return v;
 }
 
+// ew_apply on N values at once, for a list with at most one pixel-response
+// op, whose map values are mv (unused where the list has none): no memory
+// reads but the op list's own, once per op for all N values, so a caller
+// streaming its inputs ahead is not made to wait for them here.  Each value
+// sees the ops in list order, as in ew_apply.
+template <int N>
+__device__ __forceinline__ void ew_apply_mv(const EwList& l, float (&v)[N],
+                                            const float (&mv)[N]) {
+  for (int i = 0; i < l.n; ++i) {
+    const int k = l.kind[i];
+    const float c = l.value[i];
+    if (k == DLX_DET_PIXEL_RESPONSE) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) v[j] *= mv[j];
+    } else if (k == DLX_DET_SATURATE) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) v[j] = v[j] > c ? c : v[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) v[j] += c;
+    }
+  }
+}
+
 // The four map values of op `pi` at (y, x0..x0+3), clamped into the n x n map;
 // one 16-byte load when the four are in the image and aligned.
 __device__ __forceinline__ void load_map4(const EwList& l, int pi, int n, int y, int x0,
@@ -514,20 +538,35 @@ det_conv_down_kernel(const float* __restrict__ src, int n, int th, int tw, int t
 
 // ---------------------------------------------------------------------------
 // Column-walking separable convolution (odd K <= 13), optionally followed by a
-// KD x KD block sum (KD = 2 or 4; KD = 1: the convolution alone).  Each thread
-// owns 4 adjacent columns of one image and walks down a segment of rows with
-// no LDS and no barriers: per input row it loads the 4 + K - 1 inputs of its
-// horizontal window as aligned 16-byte pieces (neighbouring threads share
-// them through the L1 cache), forms the 4 horizontal sums and shifts them
-// into a register window of the last K rows; each output row is the vertical
-// sum over that window, and block sums of KD rows accumulate in registers as
-// the rows go by.  Loads run two rows ahead.  Needs n % 4 == 0.  The
-// arithmetic, order included, is that of conv_tile with det_corr_kernel /
-// det_conv_down_kernel.
+// KD x KD block sum (KD = 2 or 4; KD = 1: the convolution alone).  A workgroup
+// is one wave; its 64 lanes own 4 adjacent columns each (a 256-column band of
+// one image) and walk down a segment of rows with no barriers.  Input rows
+// (the band plus its K - 1 halo columns) and the rows of the map that the ops
+// after the convolution multiply by are streamed global -> LDS by the wave's
+// own direct loads (global_load_lds) into a ring of kColRing slots, kColRing - 1
+// rows ahead, without holding registers.  Per row a lane reads its 4 + K - 1
+// inputs from the slot and forms the 4 horizontal sums; each output row is the
+// vertical sum over the last K of those.  Rows are processed in unrolled
+// blocks of kColU, the horizontal sums of a block and of the K - 1 rows before
+// it held in one register window that moves by kColU rows per block.  Block
+// sums of KD rows accumulate in registers as the rows go by.  Pieces of rows
+// or columns outside the image are loaded from a zero buffer, so no lane
+// masks anything.  Needs n % 4 == 0, 16-byte aligned images and map, and no
+// op before the convolution.  The arithmetic, order included, is that of
+// conv_tile with det_corr_kernel / det_conv_down_kernel.
 // ---------------------------------------------------------------------------
-constexpr int kColThreads = 256;   // 1024 columns per workgroup
-constexpr int kColAhead = 2;       // rows of loads in flight per thread (more
-                                   // registers cost more occupancy than they gain)
+constexpr int kColBand = 256;              // columns per wave
+constexpr int kColRing = 4;                // LDS row slots per wave
+constexpr int kColU = 8;                   // rows per unrolled block (a multiple of kColRing)
+constexpr int kSlotIn = kColBand + 16;     // a slot's input row: the band + 16 halo columns
+constexpr int kSlot = kSlotIn + kColBand;  // + the map row
+static_assert(kColU % kColRing == 0, "slot indices are constant in a block");
+
+// 16 bytes of zeros: the source of every piece outside the image
+__device__ __attribute__((aligned(16))) const float kColZeros[4] = {0.f, 0.f, 0.f, 0.f};
+
+typedef const __attribute__((address_space(1))) void col_gbl_t;
+typedef __attribute__((address_space(3))) void col_lds_t;
 
 template <int K>
 struct ColPieces {
@@ -537,19 +576,32 @@ struct ColPieces {
   static constexpr int NV = (OFF + 4 + K - 1 + 3) / 4;   // 16-byte pieces per row
 };
 
-template <int KD, int K>
-__global__ void __launch_bounds__(kColThreads)
+// s_waitcnt immediate: at most n vector-memory operations outstanding
+// (gfx9: vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt left unconstrained).
+constexpr int vmcnt_imm(int n) { return (n & 15) | (7 << 4) | (15 << 8) | (((n >> 4) & 3) << 14); }
+
+// MAP: the ops after the convolution hold one pixel-response op, whose map
+// rows are streamed with the input rows.
+template <int KD, int K, bool MAP>
+__global__ void __launch_bounds__(64)
 det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
-                   const float* __restrict__ w, EwList pre, EwList post, int m, EwList post2,
+                   const float* __restrict__ w, EwList post, int m, EwList post2,
                    float* __restrict__ out) {
   constexpr int LO = ColPieces<K>::LO, LO4 = ColPieces<K>::LO4;
   constexpr int OFF = ColPieces<K>::OFF, NV = ColPieces<K>::NV;
   constexpr int NB = KD == 1 ? 1 : 4 / KD;
+  constexpr int kLoads = MAP ? 3 : 2;   // direct loads per row
+  static_assert(4 * 63 + 4 * NV <= kSlotIn, "window inside the slot");
+  // one LDS object per slot, so every slot index below is a constant and the
+  // compiler orders each slot's reads after that slot's own loads only
+  __shared__ __attribute__((aligned(16))) float sl0[kSlot], sl1[kSlot], sl2[kSlot], sl3[kSlot];
+  float* const ring[kColRing] = {sl0, sl1, sl2, sl3};
   int tile, image;
-  xcd_tile(has_map(pre) || has_map(post), tile, image);
+  xcd_tile(has_map(post), tile, image);
   const int band = tile % bands, seg = tile / bands;
-  const int c0 = band * (4 * kColThreads) + 4 * threadIdx.x;
-  if (c0 >= n) return;   // no barriers below
+  const int lane = threadIdx.x;
+  const int cb = band * kColBand;          // the wave's first column
+  const int c0 = cb + 4 * lane;            // this lane's
   const float* s = src + (long)image * n * n;
   const int y0 = seg * seg_h, y1 = min(y0 + seg_h, n);
   const int total = (y1 - y0) + K - 1;   // H rows y0 + LO + i, i < total
@@ -561,98 +613,117 @@ det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
     wy[t] = w[K - 1 - t];
     wx[t] = w[2 * K - 1 - t];
   }
-  // this thread's pieces: columns xs + 4 v, clamped into the image; the
-  // window is inside the image for all but the edge threads
-  const int xs = c0 + LO4;
-  const bool inside = xs >= 0 && xs + 4 * NV <= n;
-  int xoff[NV];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) xoff[v] = min(max(xs + 4 * v, 0), n - 4);
-  const int pi = first_map(post);
-  float4 q0[NV], q1[NV];
-  float m0[4], m1[4];
-  float hw[K][4];
+  // direct loads move 16 bytes a lane to the slot at 16 bytes x lane: lane l
+  // brings columns cb + LO4 + 4 l, lanes 0..3 also the 16 past the band.  A
+  // piece is wholly inside or wholly outside the image (n and every offset
+  // are multiples of 4).
+  const int xl = cb + LO4 + 4 * lane;
+  const int xh = cb + LO4 + kColBand + 4 * (lane & 3);
+  const bool okl = xl >= 0 && xl < n, okh = xh >= 0 && xh < n;
+  const float* zeros = kColZeros;
+  const float* mapp = MAP ? post.map[first_map(post)] : src;
+  const int xm = min(c0, n - 4);   // lanes past the image store nothing
+  // H row i (input row y0 + LO + i) and the map row of the output row it
+  // completes (y0 + i - (K - 1)) into a slot
+  auto issue = [&](int i, float* slot) {
+    const int r = y0 + LO + i;
+    const bool okr = r >= 0 && r < n;
+    const float* row = s + (long)r * n;
+    __builtin_amdgcn_global_load_lds((col_gbl_t*)(okr && okl ? row + xl : zeros),
+                                     (col_lds_t*)slot, 16, 0, 0);
+    if (lane < 4)
+      __builtin_amdgcn_global_load_lds((col_gbl_t*)(okr && okh ? row + xh : zeros),
+                                       (col_lds_t*)(slot + kColBand), 16, 0, 0);
+    if (MAP) {
+      const long ry = min(max(y0 + i - (K - 1), 0), n - 1);
+      __builtin_amdgcn_global_load_lds((col_gbl_t*)(mapp + ry * n + xm),
+                                       (col_lds_t*)(slot + kSlotIn), 16, 0, 0);
+    }
+  };
+  // hw[t]: the horizontal sums of H row i0 - (K - 1) + t in the block at i0
+  float hw[K - 1 + kColU][4];
   float blk[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) blk[b] = 0.f;
-  // loads of H row i (input row y0 + LO + i) and of the map values of the
-  // output row it completes (y0 + i - (K - 1))
-#define DLX_COL_LOAD(i, q, mv)                                                     \
-  do {                                                                             \
-    const float* rowp_ = s + (long)min(max(y0 + LO + (i), 0), n - 1) * n;          \
-    _Pragma("unroll") for (int v = 0; v < NV; ++v) q[v] = *(const float4*)(rowp_ + xoff[v]); \
-    load_map4(post, pi, n, min(max(y0 + (i) - (K - 1), 0), n - 1), c0, mv);        \
-  } while (0)
-  // one row: H row i from the buffer (q, mq) loaded kColAhead rows earlier,
-  // which then takes the loads of row i + kColAhead
-  auto row_step = [&](int i, float4 (&q)[NV], float (&mq)[4]) {
-    float a[4 * NV], mv[4];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      a[4 * v] = q[v].x; a[4 * v + 1] = q[v].y; a[4 * v + 2] = q[v].z; a[4 * v + 3] = q[v].w;
-    }
+  for (int u = 0; u + 1 < kColRing; ++u)
+    if (u < total) issue(u, ring[u]);
+
+  for (int i0 = 0; i0 < total; i0 += kColU) {
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) mv[jj] = mq[jj];
-    if (i + kColAhead < total) DLX_COL_LOAD(i + kColAhead, q, mq);
-    const int r = y0 + LO + i;
-    const bool okr = r >= 0 && r < n;
-    if (!okr || !inside || pre.n) {
+    for (int u = 0; u < kColU; ++u) {
+      const int i = i0 + u;
+      if (i >= total) break;
+      // row i's loads have landed once at most those of the kColRing - 2 rows
+      // issued after it remain outstanding (loads complete in order; stores
+      // issued since only make the wait stricter); near the end, fewer rows
+      // were issued: wait for all
+      if (i + kColRing - 2 < total) __builtin_amdgcn_s_waitcnt(vmcnt_imm(kLoads * (kColRing - 2)));
+      else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+      const float* slot = ring[u % kColRing];
+      float a[4 * NV], mv[4];
 #pragma unroll
-      for (int e = 0; e < 4 * NV; ++e) {
-        const int x = xs + e;
-        const bool ok = okr && x >= 0 && x < n;
-        a[e] = ok ? ew_apply(pre, a[e], (long)r * n + x) : 0.f;
+      for (int v = 0; v < NV; ++v) {
+        const float4 t = *(const float4*)(slot + 4 * lane + 4 * v);
+        a[4 * v] = t.x; a[4 * v + 1] = t.y; a[4 * v + 2] = t.z; a[4 * v + 3] = t.w;
       }
-    }
-    // shift the window and add H row i
+      if (MAP) {
+        const float4 t = *(const float4*)(slot + kSlotIn + 4 * lane);
+        mv[0] = t.x; mv[1] = t.y; mv[2] = t.z; mv[3] = t.w;
+      } else {
+        mv[0] = mv[1] = mv[2] = mv[3] = 1.f;
+      }
+      // refill the slot row i - 1 used with row i + kColRing - 1
+      if (i + kColRing - 1 < total) issue(i + kColRing - 1, ring[(u + kColRing - 1) % kColRing]);
 #pragma unroll
-    for (int t = 0; t + 1 < K; ++t)
+      for (int jj = 0; jj < 4; ++jj) {
+        float h = 0.f;
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj) hw[t][jj] = hw[t + 1][jj];
+        for (int t = 0; t < K; ++t) h += a[OFF + jj + t] * wx[t];
+        hw[K - 1 + u][jj] = h;
+      }
+      if (i < K - 1) continue;
+      const int Y = y0 + i - (K - 1);
+      float o[4];
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-      float h = 0.f;
+      for (int jj = 0; jj < 4; ++jj) {
+        float acc = 0.f;
 #pragma unroll
-      for (int t = 0; t < K; ++t) h += a[OFF + jj + t] * wx[t];
-      hw[K - 1][jj] = h;
-    }
-    if (i < K - 1) return;
-    const int Y = y0 + i - (K - 1);
-    float o[4];
-#pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-      float acc = 0.f;
-#pragma unroll
-      for (int t = 0; t < K; ++t) acc += hw[t][jj] * wy[t];
-      o[jj] = ew_apply_pf(post, acc, (long)Y * n + c0 + jj, pi, mv[jj]);
-    }
-    if (KD == 1) {
-      *(float4*)(out + (long)image * n * n + (long)Y * n + c0) = make_float4(o[0], o[1], o[2], o[3]);
-      return;
-    }
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      if (KD == 4) blk[b] += ((o[0] + o[1]) + o[2]) + o[3];
-      else blk[b] += o[2 * b] + o[2 * b + 1];
-    }
-    if (Y % KD == KD - 1) {
-      const int gy = Y / KD;
-      float* dst = out + (long)image * m * m + (long)gy * m;
+        for (int t = 0; t < K; ++t) acc += hw[u + t][jj] * wy[t];
+        o[jj] = acc;
+      }
+      ew_apply_mv(post, o, mv);
+      if (KD == 1) {
+        if (c0 < n)
+          *(float4*)(out + (long)image * n * n + (long)Y * n + c0) =
+              make_float4(o[0], o[1], o[2], o[3]);
+        continue;
+      }
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        const int gx = c0 / KD + b;
-        dst[gx] = ew_apply(post2, blk[b], (long)gy * m + gx);
-        blk[b] = 0.f;
+        if (KD == 4) blk[b] += ((o[0] + o[1]) + o[2]) + o[3];
+        else blk[b] += o[2 * b] + o[2 * b + 1];
+      }
+      if (Y % KD == KD - 1) {
+        float one[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) one[b] = 1.f;
+        ew_apply_mv(post2, blk, one);
+        if (c0 < n) {
+          float* dst = out + (long)image * m * m + (long)(Y / KD) * m + c0 / KD;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) dst[b] = blk[b];
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) blk[b] = 0.f;
       }
     }
-  };
-  DLX_COL_LOAD(0, q0, m0);
-  if (total > 1) DLX_COL_LOAD(1, q1, m1);
-  for (int i = 0; i < total; i += kColAhead) {
-    row_step(i, q0, m0);
-    if (i + 1 < total) row_step(i + 1, q1, m1);
+    // move the window to the next block
+#pragma unroll
+    for (int t = 0; t < K - 1; ++t)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) hw[t][jj] = hw[t + kColU][jj];
   }
-#undef DLX_COL_LOAD
 }
 
 // out[b][Y][X] = post( sum_{dy,dx < k} pre(src[b][Y k + dy][X k + dx]) ), m = n / k.
@@ -1064,47 +1135,68 @@ int check_grid(const std::vector<Pass>& passes, int batch) {
 // Forward pass i: src -> dst (r_save may be NULL).
 // The column-walking kernel takes a separable forward convolution of odd K <=
 // 13 on a single (not summed) input whose side is a multiple of 4, alone or
-// with a 2 x 2 / 4 x 4 block sum, when the batch is large enough: at least 256
-// (band, 64-row segment, image) triples.  A thread walks its rows one after
-// the other, so on a small batch (one 256^2 frame: 4 workgroups, ~100 us) the
-// tile kernels, which cover the image with many short-lived workgroups, win.
-bool streams(const Pass& c, int k, SumIn si, int batch) {
-  const long bands = (c.n_in + 4 * kColThreads - 1) / (4 * kColThreads);
-  const long segs = std::max(1, c.n_in / 64);
+// with a 2 x 2 / 4 x 4 block sum, when the batch is large enough: at least 512
+// (256-column band, segment of rows, image) waves.  A wave walks its rows one
+// after the other, so on a small batch (one 256^2 frame: 4 waves) the tile
+// kernels, which cover the image with many short-lived workgroups, win.
+bool streams(const Pass& c, int k, const EwList& post2, SumIn si, int batch) {
+  const long bands = (c.n_in + kColBand - 1) / kColBand;
+  const long segs = std::max(1, c.n_in / 32);
+  // the kernel reads no map in its loop but the one it streams: no op before
+  // the convolution, no map after the block sum, at most one map after the
+  // convolution (16-byte aligned: it is read 16 bytes a lane)
+  int maps = 0;
+  bool map16 = true;
+  for (int i = 0; i < c.post.n; ++i)
+    if (c.post.kind[i] == DLX_DET_PIXEL_RESPONSE) {
+      ++maps;
+      map16 = (reinterpret_cast<uintptr_t>(c.post.map[i]) & 15) == 0;
+    }
+  bool others = c.pre.n > 0;
+  for (int i = 0; i < post2.n; ++i) others = others || post2.kind[i] == DLX_DET_PIXEL_RESPONSE;
   return c.conv && c.sep && c.K <= 13 && c.K % 2 == 1 && c.n_in % 4 == 0 &&
-         (k == 1 || k == 2 || k == 4) && si.n == 1 && bands * segs * batch >= 256;
+         (k == 1 || k == 2 || k == 4) && si.n == 1 && maps <= 1 && map16 && !others &&
+         bands * segs * batch >= 512;
 }
 
-// Bands of 1024 columns; segments of at least 64 rows (a multiple of 4), about
-// three workgroups per CU in all (one resident round at the kernel's
-// occupancy of 3 waves per SIMD).
+// Bands of kColBand columns; segments of at least 32 rows (a multiple of 4),
+// about kColWaves waves in all: one resident round at 16 one-wave workgroups
+// per CU.
+constexpr long kColWaves = 4096;
+
 void stream_grid(int n, int batch, int& bands, int& seg_h, int& nseg) {
-  bands = (n + 4 * kColThreads - 1) / (4 * kColThreads);
+  bands = (n + kColBand - 1) / kColBand;
   const long per = (long)bands * batch;
-  long want = (768 + per - 1) / per;
-  want = std::min<long>(want, std::max(1, n / 64));
+  long want = (kColWaves + per - 1) / per;
+  want = std::min<long>(want, std::max(1, n / 32));
   seg_h = (int)(((n + want - 1) / want + 3) / 4 * 4);
   nseg = (n + seg_h - 1) / seg_h;
 }
 
 template <int KD, int K>
-void launch_col(const dim3& grid, int threads, hipStream_t st, const Pass& c, int bands,
-                int seg_h, int m, const EwList& post2, const float* src, float* dst) {
-  hipLaunchKernelGGL((det_sep_col_kernel<KD, K>), grid, dim3(threads), 0, st, src, c.n_in, bands,
-                     seg_h, c.w, c.pre, c.post, m, post2, dst);
+void launch_col(const dim3& grid, hipStream_t st, const Pass& c, int bands, int seg_h, int m,
+                const EwList& post2, const float* src, float* dst) {
+  bool map = false;
+  for (int i = 0; i < c.post.n; ++i) map = map || c.post.kind[i] == DLX_DET_PIXEL_RESPONSE;
+  if (map)
+    hipLaunchKernelGGL((det_sep_col_kernel<KD, K, true>), grid, dim3(64), 0, st, src, c.n_in,
+                       bands, seg_h, c.w, c.post, m, post2, dst);
+  else
+    hipLaunchKernelGGL((det_sep_col_kernel<KD, K, false>), grid, dim3(64), 0, st, src, c.n_in,
+                       bands, seg_h, c.w, c.post, m, post2, dst);
 }
 
 template <int KD>
-void launch_col_k(const dim3& grid, int threads, hipStream_t st, const Pass& c, int bands,
-                  int seg_h, int m, const EwList& post2, const float* src, float* dst) {
+void launch_col_k(const dim3& grid, hipStream_t st, const Pass& c, int bands, int seg_h, int m,
+                  const EwList& post2, const float* src, float* dst) {
   switch (c.K) {
-    case 1: launch_col<KD, 1>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
-    case 3: launch_col<KD, 3>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
-    case 5: launch_col<KD, 5>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
-    case 7: launch_col<KD, 7>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
-    case 9: launch_col<KD, 9>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
-    case 11: launch_col<KD, 11>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
-    default: launch_col<KD, 13>(grid, threads, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 1: launch_col<KD, 1>(grid, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 3: launch_col<KD, 3>(grid, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 5: launch_col<KD, 5>(grid, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 7: launch_col<KD, 7>(grid, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 9: launch_col<KD, 9>(grid, st, c, bands, seg_h, m, post2, src, dst); break;
+    case 11: launch_col<KD, 11>(grid, st, c, bands, seg_h, m, post2, src, dst); break;
+    default: launch_col<KD, 13>(grid, st, c, bands, seg_h, m, post2, src, dst); break;
   }
 }
 
@@ -1112,13 +1204,10 @@ int run_stream(const Pass& c, int k, int m, const EwList& post2, int batch, cons
                float* dst, hipStream_t st) {
   int bands, seg_h, nseg;
   stream_grid(c.n_in, batch, bands, seg_h, nseg);
-  // one thread per 4 columns, whole waves, at most kColThreads
-  const int cols = std::min(c.n_in, 4 * kColThreads);
-  const int threads = std::min(kColThreads, (cols / 4 + 63) / 64 * 64);
   const dim3 grid(bands * nseg, batch);
-  if (k == 4) launch_col_k<4>(grid, threads, st, c, bands, seg_h, m, post2, src, dst);
-  else if (k == 2) launch_col_k<2>(grid, threads, st, c, bands, seg_h, m, post2, src, dst);
-  else launch_col_k<1>(grid, threads, st, c, bands, seg_h, c.n_in, EwList{}, src, dst);
+  if (k == 4) launch_col_k<4>(grid, st, c, bands, seg_h, m, post2, src, dst);
+  else if (k == 2) launch_col_k<2>(grid, st, c, bands, seg_h, m, post2, src, dst);
+  else launch_col_k<1>(grid, st, c, bands, seg_h, c.n_in, EwList{}, src, dst);
   return launch_status("det_sep_col_kernel");
 }
 
@@ -1128,7 +1217,7 @@ bool aligned16(const void* a, const void* b) {
 
 int run_pass(const Pass& p, int batch, const float* src, float* r_save, float* dst,
              hipStream_t st, SumIn si = SumIn{1, 0}) {
-  if (p.conv && !r_save && streams(p, 1, si, batch) && aligned16(src, dst))
+  if (p.conv && !r_save && streams(p, 1, EwList{}, si, batch) && aligned16(src, dst))
     return run_stream(p, 1, p.n_in, EwList{}, batch, src, dst, st);
   if (p.conv) {
     const int rpt = conv_rpt(p.K, p.sep);
@@ -1170,7 +1259,7 @@ bool can_fuse(const std::vector<Pass>& passes, size_t i) {
 
 int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, float* dst,
                   hipStream_t st, SumIn si = SumIn{1, 0}) {
-  if (streams(c, d.k, si, batch) && aligned16(src, dst))
+  if (streams(c, d.k, d.post, si, batch) && aligned16(src, dst))
     return run_stream(c, d.k, d.n_out, d.post, batch, src, dst, st);
   const int rpt = conv_rpt(c.K, c.sep);
   const int k = d.k;
@@ -1193,12 +1282,18 @@ int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, flo
 
 }  // namespace
 
-// Workgroups of the chain's first launch (pass 0, fused with pass 1 where
-// run_conv_down takes both).
+// Whether the chain's first launch (pass 0, fused with pass 1 where
+// run_conv_down takes both) is the column-walking kernel, given aligned images.
+bool first_streams(const std::vector<Pass>& passes, int batch) {
+  const bool fuse = can_fuse(passes, 0);
+  return streams(passes[0], fuse ? passes[1].k : 1, fuse ? passes[1].post : EwList{},
+                 SumIn{1, 0}, batch);
+}
+
+// Workgroups of the chain's first launch.
 long first_launch_groups(const std::vector<Pass>& passes, int batch) {
   const Pass& p = passes[0];
-  const int k = can_fuse(passes, 0) ? passes[1].k : 1;
-  if (streams(p, k, SumIn{1, 0}, batch)) {
+  if (first_streams(passes, batch)) {
     int bands, seg_h, nseg;
     stream_grid(p.n_in, batch, bands, seg_h, nseg);
     return (long)bands * nseg * batch;
@@ -1227,10 +1322,9 @@ long first_launch_groups(const std::vector<Pass>& passes, int batch) {
 constexpr long kSumInLoaderGroups = 512;
 
 bool presum(const std::vector<Pass>& passes, int n, int batch, int nsum, long sum_stride) {
-  const int k = can_fuse(passes, 0) ? passes[1].k : 1;
   return nsum > 1 && sum_stride == (long)n * n &&
          (first_launch_groups(passes, batch) < kSumInLoaderGroups ||
-          streams(passes[0], k, SumIn{1, 0}, batch));
+          first_streams(passes, batch));
 }
 
 size_t detector_ws_bytes(int n, int batch, const dlx_detector_op* ops, int nops, int nsum) {

@@ -156,8 +156,9 @@ class ApplyJitter(DetectorLayer):
             f = self._factor_tracked()
             return ("convolve_separable", (f, f))
         key = (float(self.sigma), self.kernel_size, self.oversample)
-        f = self._device(key, lambda: runtime.as_float(self._factor()))
-        return ("convolve_separable", (f, f))
+        # both factors in one device buffer, so the call passes it as is
+        ff = self._device(key, lambda: runtime.as_float(np.tile(self._factor(), 2)))
+        return ("convolve_separable", (ff[:self.kernel_size], ff[self.kernel_size:]))
 
     def __call__(self, psf: PSF) -> PSF:
         return self._run(psf)

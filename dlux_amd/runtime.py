@@ -866,10 +866,15 @@ def _detector_ops(ops, dev):
             wy, wx = (as_float(a, dev).reshape(-1) for a in arg)
             if wy.numel() != wx.numel():
                 raise ValueError("convolve_separable factors must have the same length")
-            t = torch.cat([wy, wx]).contiguous()
-            keep.append(t)
+            if wy.is_contiguous() and wx.is_contiguous() and \
+                    wx.data_ptr() == wy.data_ptr() + 4 * wy.numel():
+                keep += [wy, wx]   # already one buffer (column factor, then row factor)
+                op.data = wy.data_ptr()
+            else:
+                t = torch.cat([wy, wx]).contiguous()
+                keep.append(t)
+                op.data = t.data_ptr()
             op.size = int(wy.numel())
-            op.data = t.data_ptr()
         elif kind == "downsample":
             op.size = int(arg)
         else:

@@ -109,10 +109,10 @@ def _chains(rng):
     cases.append(("stream-k13-down2", 130, 1,
                   [("convolve_separable", (rng.normal(size=13), rng.normal(size=13))),
                    ("saturate", None), ("downsample", 2)]))
-    # the column-walking kernel (batches of >= 256 band-segment-image triples):
-    # the jitter chain over 32 frames of 512^2; sides that are multiples of 4 but
-    # not of its 1024-column bands: 128^2 frames (32 of 64 lanes busy) under a
-    # load-side map, and 1028^2 (a second band of one column group)
+    # the column-walking kernel (batches of >= 512 band-segment-image waves):
+    # the jitter chain over 32 frames of 512^2; a load-side map, which sends
+    # 128^2 frames to the tile kernels; 1028^2 (a fifth 256-column band of one
+    # column group)
     cases.append(("col-512-jitter-down4", 512, 32,
                   [("convolve_separable", (f9, f9)),
                    ("pixel_response", rng.uniform(0.9, 1.1, (512, 512))), ("downsample", 4),
@@ -171,6 +171,56 @@ def test_detector_chain_and_vjp_vs_oracle(dl, case):
     vref = np.stack([orc.detector_vjp(xi, chain, gi) for xi, gi in zip(x, g)])
     err = _err(vjp, vref)
     print(f"  vjp err {err:.2e}")
+    assert err < CHAIN_TOL
+
+
+# Chains the column-walking kernel takes (separable, odd K <= 13, n % 4 == 0,
+# no op before the convolution, at most one map after it and none after the
+# block sum, at least 512 waves of 256 columns x 32 rows): block sums of 4, 2
+# and 1 (the convolution alone), with and without the streamed map, the 1- and
+# 13-tap limits, bands that end inside the image (128, 300, 1028) and halos
+# past both edges, ragged segments (300).  One image at a time the same chains
+# run on the tile kernels, whose arithmetic the column kernel repeats in the
+# same order: the results must be bit-identical, and match the oracle.
+COLUMN_CASES = [
+    ("jitter-map-down4", 512, 32, 9, [("pixel_response", "map"), ("downsample", 4),
+                                      ("saturate", None), ("add", 0.01)]),
+    ("k5-down4-ragged-band", 1028, 8, 5, [("downsample", 4), ("saturate", None)]),
+    ("k7-down2-half-band", 128, 128, 7, [("downsample", 2), ("add", 0.05)]),
+    ("k13-map-alone", 256, 64, 13, [("pixel_response", "map"), ("add", -0.2)]),
+    ("k1-down2", 64, 256, 1, [("downsample", 2)]),
+    ("k3-map-down4-ragged-seg", 300, 40, 3, [("add", 0.3), ("pixel_response", "map"),
+                                             ("downsample", 4), ("add", 1.0)]),
+    ("k11-alone-saturate", 200, 100, 11, [("saturate", None), ("add", 0.5)]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(COLUMN_CASES)))
+def test_column_kernel_bit_identical_to_tile_kernels(dl, case):
+    name, n, batch, K, tail = COLUMN_CASES[case]
+    print(f"[detector column] {name} n={n} batch={batch} K={K}", flush=True)
+    rng = np.random.default_rng(90 + case)
+    x = rng.uniform(0.0, 1.0, (batch, n, n)).astype(np.float32)
+    chain = [("convolve_separable", (rng.uniform(0, 1, K), rng.uniform(0, 1, K)))]
+    side = n
+    for kind, arg in tail:
+        if kind == "pixel_response":
+            arg = rng.uniform(0.8, 1.2, (side, side))
+        elif kind == "downsample":
+            side //= arg
+        chain.append((kind, arg))
+    chain = _resolve(chain, x[:2].astype(np.float64))
+    dev = [(k, tuple(torch.as_tensor(np.asarray(b, np.float32)).cuda() for b in a)
+            if isinstance(a, tuple) else torch.as_tensor(np.asarray(a, np.float32)).cuda()
+            if isinstance(a, np.ndarray) else a) for k, a in chain]
+    xt = torch.as_tensor(x).cuda()
+    got = dl.runtime.detector(xt, dev)
+    one = torch.stack([dl.runtime.detector(xt[b], dev) for b in range(batch)])
+    torch.cuda.synchronize()
+    assert torch.equal(got, one), f"{(got != one).sum().item()} pixels differ"
+    ref = _oracle(x[:2].astype(np.float64), chain)
+    err = _err(got[:2], ref)
+    print(f"  err vs oracle {err:.2e}")
     assert err < CHAIN_TOL
 
 
