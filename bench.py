@@ -347,7 +347,7 @@ def _asm_leg(torch, runtime, steps, warmup=3):
             "gflops": flops / (ms * 1e-3) / 1e9}
 
 
-def _detector_leg(torch, steps, check, warmup=3):
+def _detector_leg(torch, steps, check, warmup=20):
     """Detector chain over a stack of oversampled PSFs: 64 frames of 1024^2
     (4x-oversampled 256^2 detector images) through ApplyJitter(sigma 0.5 px,
     9 x 9) -> ApplyPixelResponse -> Downsample(4) -> ApplySaturation ->
@@ -655,7 +655,7 @@ def main():
     print("[bench] angular-spectrum leg", file=sys.stderr, flush=True)
     asm = None if args.no_asm else _asm_leg(torch, runtime, max(args.steps // 5, 10))
     print("[bench] detector leg", file=sys.stderr, flush=True)
-    detector = None if args.no_detector else _detector_leg(torch, max(args.steps // 5, 10),
+    detector = None if args.no_detector else _detector_leg(torch, max(args.steps // 2, 40),
                                                            not args.no_cpu)
     print("[bench] telescope leg", file=sys.stderr, flush=True)
     telescope = None if args.no_detector or "bf16x3" not in args.forms else _telescope_leg(

@@ -543,15 +543,15 @@ det_conv_down_kernel(const float* __restrict__ src, int n, int th, int tw, int t
 // one image) and walk down a segment of rows with no barriers.  Input rows
 // (the band plus its K - 1 halo columns) and the rows of the map that the ops
 // after the convolution multiply by are streamed global -> LDS by the wave's
-// own direct loads (global_load_lds) into a ring of kColRing slots, kColRing - 1
+// own direct loads (buffer loads into LDS) into a ring of kColRing slots, kColRing - 1
 // rows ahead, without holding registers.  Per row a lane reads its 4 + K - 1
 // inputs from the slot and forms the 4 horizontal sums; each output row is the
 // vertical sum over the last K of those.  Rows are processed in unrolled
 // blocks of kColU, the horizontal sums of a block and of the K - 1 rows before
 // it held in one register window that moves by kColU rows per block.  Block
 // sums of KD rows accumulate in registers as the rows go by.  Pieces of rows
-// or columns outside the image are loaded from a zero buffer, so no lane
-// masks anything.  Needs n % 4 == 0, 16-byte aligned images and map, and no
+// or columns outside the image load as zeros (buffer range checks), so no
+// lane masks anything.  Needs n % 4 == 0, 16-byte aligned images and map, and no
 // op before the convolution.  The arithmetic, order included, is that of
 // conv_tile with det_corr_kernel / det_conv_down_kernel.
 // ---------------------------------------------------------------------------
@@ -559,13 +559,14 @@ constexpr int kColBand = 256;              // columns per wave
 constexpr int kColRing = 4;                // LDS row slots per wave
 constexpr int kColU = 8;                   // rows per unrolled block (a multiple of kColRing)
 constexpr int kSlotIn = kColBand + 16;     // a slot's input row: the band + 16 halo columns
-constexpr int kSlot = kSlotIn + kColBand;  // + the map row
+constexpr int kSlotMap = kSlotIn + kColBand;   // + the map row
 static_assert(kColU % kColRing == 0, "slot indices are constant in a block");
 
-// 16 bytes of zeros: the source of every piece outside the image
-__device__ __attribute__((aligned(16))) const float kColZeros[4] = {0.f, 0.f, 0.f, 0.f};
+// Word 3 of a raw buffer descriptor (32-bit data format, no swizzle, no
+// stride): loads range-checked against the descriptor's size in bytes.
+constexpr int kRawBuffer = 0x00020000;
+constexpr int kColOutside = 0x40000000;   // a byte offset past any row
 
-typedef const __attribute__((address_space(1))) void col_gbl_t;
 typedef __attribute__((address_space(3))) void col_lds_t;
 
 template <int K>
@@ -580,101 +581,86 @@ struct ColPieces {
 // (gfx9: vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt left unconstrained).
 constexpr int vmcnt_imm(int n) { return (n & 15) | (7 << 4) | (15 << 8) | (((n >> 4) & 3) << 14); }
 
-// MAP: the ops after the convolution hold one pixel-response op, whose map
-// rows are streamed with the input rows.
+typedef __attribute__((address_space(3))) float lds_float;
+
+typedef float lds_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 lds_ld4(const lds_float* p) {
+  const lds_f4 v = *(const __attribute__((address_space(3))) lds_f4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Per-wave state of det_sep_col_kernel: where its rows come from and go to.
 template <int KD, int K, bool MAP>
-__global__ void __launch_bounds__(64)
-det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
-                   const float* __restrict__ w, EwList post, int m, EwList post2,
-                   float* __restrict__ out) {
-  constexpr int LO = ColPieces<K>::LO, LO4 = ColPieces<K>::LO4;
-  constexpr int OFF = ColPieces<K>::OFF, NV = ColPieces<K>::NV;
-  constexpr int NB = KD == 1 ? 1 : 4 / KD;
-  constexpr int kLoads = MAP ? 3 : 2;   // direct loads per row
-  static_assert(4 * 63 + 4 * NV <= kSlotIn, "window inside the slot");
-  // one LDS object per slot, so every slot index below is a constant and the
-  // compiler orders each slot's reads after that slot's own loads only
-  __shared__ __attribute__((aligned(16))) float sl0[kSlot], sl1[kSlot], sl2[kSlot], sl3[kSlot];
-  float* const ring[kColRing] = {sl0, sl1, sl2, sl3};
-  int tile, image;
-  xcd_tile(has_map(post), tile, image);
-  const int band = tile % bands, seg = tile / bands;
-  const int lane = threadIdx.x;
-  const int cb = band * kColBand;          // the wave's first column
-  const int c0 = cb + 4 * lane;            // this lane's
-  const float* s = src + (long)image * n * n;
-  const int y0 = seg * seg_h, y1 = min(y0 + seg_h, n);
-  const int total = (y1 - y0) + K - 1;   // H rows y0 + LO + i, i < total
-  // the convolution as a correlation with the reversed factors (w: K column
-  // then K row weights)
+struct ColWave {
+  static constexpr int LO = ColPieces<K>::LO, OFF = ColPieces<K>::OFF, NV = ColPieces<K>::NV;
+  static constexpr int NB = KD == 1 ? 1 : 4 / KD;
+  static constexpr int kLoads = MAP ? 3 : 2;   // direct loads per row
+  lds_float* ring[kColRing];
+  const float* s;      // this image
+  const float* mapp;   // the streamed map (MAP)
+  float* out;          // this image's output
+  int n, m, y0, total, lane, c0, vl, vh, vm;
   float wy[K], wx[K];
-#pragma unroll
-  for (int t = 0; t < K; ++t) {
-    wy[t] = w[K - 1 - t];
-    wx[t] = w[2 * K - 1 - t];
-  }
-  // direct loads move 16 bytes a lane to the slot at 16 bytes x lane: lane l
-  // brings columns cb + LO4 + 4 l, lanes 0..3 also the 16 past the band.  A
-  // piece is wholly inside or wholly outside the image (n and every offset
-  // are multiples of 4).
-  const int xl = cb + LO4 + 4 * lane;
-  const int xh = cb + LO4 + kColBand + 4 * (lane & 3);
-  const bool okl = xl >= 0 && xl < n, okh = xh >= 0 && xh < n;
-  const float* zeros = kColZeros;
-  const float* mapp = MAP ? post.map[first_map(post)] : src;
-  const int xm = min(c0, n - 4);   // lanes past the image store nothing
+  const EwList* post;
+  const EwList* post2;
+
   // H row i (input row y0 + LO + i) and the map row of the output row it
   // completes (y0 + i - (K - 1)) into a slot
-  auto issue = [&](int i, float* slot) {
+  __device__ __forceinline__ void issue(int i, lds_float* slot) const {
     const int r = y0 + LO + i;
     const bool okr = r >= 0 && r < n;
-    const float* row = s + (long)r * n;
-    __builtin_amdgcn_global_load_lds((col_gbl_t*)(okr && okl ? row + xl : zeros),
-                                     (col_lds_t*)slot, 16, 0, 0);
+    const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(s + (long)(okr ? r : 0) * n), (short)0, okr ? 4 * n : 0, kRawBuffer);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(row, (col_lds_t*)slot, 16, vl, 0, 0, 0);
     if (lane < 4)
-      __builtin_amdgcn_global_load_lds((col_gbl_t*)(okr && okh ? row + xh : zeros),
-                                       (col_lds_t*)(slot + kColBand), 16, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(row, (col_lds_t*)(slot + kColBand), 16, vh, 0, 0,
+                                               0);
     if (MAP) {
-      const long ry = min(max(y0 + i - (K - 1), 0), n - 1);
-      __builtin_amdgcn_global_load_lds((col_gbl_t*)(mapp + ry * n + xm),
-                                       (col_lds_t*)(slot + kSlotIn), 16, 0, 0);
+      const int ry = min(max(y0 + i - (K - 1), 0), n - 1);
+      const __amdgpu_buffer_rsrc_t mrow = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(mapp + (long)ry * n), (short)0, 4 * n, kRawBuffer);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(mrow, (col_lds_t*)(slot + kSlotIn), 16, vm, 0, 0,
+                                               0);
     }
-  };
-  // hw[t]: the horizontal sums of H row i0 - (K - 1) + t in the block at i0
-  float hw[K - 1 + kColU][4];
-  float blk[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) blk[b] = 0.f;
-#pragma unroll
-  for (int u = 0; u + 1 < kColRing; ++u)
-    if (u < total) issue(u, ring[u]);
+  }
 
-  for (int i0 = 0; i0 < total; i0 += kColU) {
+  // One block of kColU rows from row i0.  CHECK: the block may run past the
+  // last row or be one of the first K - 1 rows (checked per row); otherwise
+  // every row of the block, and every row it issues, exists, and every row
+  // completes an output row.  hw[t]: the horizontal sums of H row
+  // i0 - (K - 1) + t.
+  template <bool CHECK>
+  __device__ __forceinline__ void block(int i0, float (&hw)[K - 1 + kColU][4],
+                                        float (&blk)[NB]) const {
 #pragma unroll
     for (int u = 0; u < kColU; ++u) {
       const int i = i0 + u;
-      if (i >= total) break;
+      if (CHECK && i >= total) break;
       // row i's loads have landed once at most those of the kColRing - 2 rows
       // issued after it remain outstanding (loads complete in order; stores
       // issued since only make the wait stricter); near the end, fewer rows
       // were issued: wait for all
-      if (i + kColRing - 2 < total) __builtin_amdgcn_s_waitcnt(vmcnt_imm(kLoads * (kColRing - 2)));
-      else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-      const float* slot = ring[u % kColRing];
+      if (!CHECK || i + kColRing - 2 < total)
+        __builtin_amdgcn_s_waitcnt(vmcnt_imm(kLoads * (kColRing - 2)));
+      else
+        __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+      const lds_float* slot = ring[u % kColRing];
       float a[4 * NV], mv[4];
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        const float4 t = *(const float4*)(slot + 4 * lane + 4 * v);
+        const float4 t = lds_ld4(slot + 4 * lane + 4 * v);
         a[4 * v] = t.x; a[4 * v + 1] = t.y; a[4 * v + 2] = t.z; a[4 * v + 3] = t.w;
       }
       if (MAP) {
-        const float4 t = *(const float4*)(slot + kSlotIn + 4 * lane);
+        const float4 t = lds_ld4(slot + kSlotIn + 4 * lane);
         mv[0] = t.x; mv[1] = t.y; mv[2] = t.z; mv[3] = t.w;
       } else {
         mv[0] = mv[1] = mv[2] = mv[3] = 1.f;
       }
       // refill the slot row i - 1 used with row i + kColRing - 1
-      if (i + kColRing - 1 < total) issue(i + kColRing - 1, ring[(u + kColRing - 1) % kColRing]);
+      if (!CHECK || i + kColRing - 1 < total)
+        issue(i + kColRing - 1, ring[(u + kColRing - 1) % kColRing]);
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) {
         float h = 0.f;
@@ -682,7 +668,7 @@ det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
         for (int t = 0; t < K; ++t) h += a[OFF + jj + t] * wx[t];
         hw[K - 1 + u][jj] = h;
       }
-      if (i < K - 1) continue;
+      if (CHECK && i < K - 1) continue;
       const int Y = y0 + i - (K - 1);
       float o[4];
 #pragma unroll
@@ -692,11 +678,9 @@ det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
         for (int t = 0; t < K; ++t) acc += hw[u + t][jj] * wy[t];
         o[jj] = acc;
       }
-      ew_apply_mv(post, o, mv);
+      ew_apply_mv(*post, o, mv);
       if (KD == 1) {
-        if (c0 < n)
-          *(float4*)(out + (long)image * n * n + (long)Y * n + c0) =
-              make_float4(o[0], o[1], o[2], o[3]);
+        if (c0 < n) *(float4*)(out + (long)Y * n + c0) = make_float4(o[0], o[1], o[2], o[3]);
         continue;
       }
 #pragma unroll
@@ -708,9 +692,9 @@ det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
         float one[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) one[b] = 1.f;
-        ew_apply_mv(post2, blk, one);
+        ew_apply_mv(*post2, blk, one);
         if (c0 < n) {
-          float* dst = out + (long)image * m * m + (long)(Y / KD) * m + c0 / KD;
+          float* dst = out + (long)(Y / KD) * m + c0 / KD;
 #pragma unroll
           for (int b = 0; b < NB; ++b) dst[b] = blk[b];
         }
@@ -724,6 +708,75 @@ det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) hw[t][jj] = hw[t + kColU][jj];
   }
+};
+
+// MAP: the ops after the convolution hold one pixel-response op, whose map
+// rows are streamed with the input rows.
+template <int KD, int K, bool MAP>
+__global__ void __launch_bounds__(64)
+det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
+                   const float* __restrict__ w, EwList post, int m, EwList post2,
+                   float* __restrict__ out) {
+  using W = ColWave<KD, K, MAP>;
+  constexpr int LO4 = ColPieces<K>::LO4;
+  static_assert(4 * 63 + 4 * W::NV <= kSlotIn, "window inside the slot");
+  // one LDS object per slot, so every slot index below is a constant and the
+  // compiler orders each slot's reads after that slot's own loads only
+  constexpr int SL = MAP ? kSlotMap : kSlotIn;   // floats per slot
+  __shared__ __attribute__((aligned(16))) float sl0[SL], sl1[SL], sl2[SL], sl3[SL];
+  float* const sl[kColRing] = {sl0, sl1, sl2, sl3};
+  static_assert(kColRing == 4, "one LDS object per slot");
+  W wv;
+#pragma unroll
+  for (int q = 0; q < kColRing; ++q) wv.ring[q] = (lds_float*)sl[q];
+  int tile, image;
+  xcd_tile(has_map(post), tile, image);
+  const int band = tile % bands, seg = tile / bands;
+  wv.lane = threadIdx.x;
+  const int cb = band * kColBand;          // the wave's first column
+  wv.c0 = cb + 4 * wv.lane;                // this lane's
+  wv.n = n;
+  wv.m = m;
+  wv.s = src + (long)image * n * n;
+  wv.out = out + (long)image * (KD == 1 ? n : m) * (KD == 1 ? n : m);
+  wv.y0 = seg * seg_h;
+  wv.total = (min(wv.y0 + seg_h, n) - wv.y0) + K - 1;   // H rows y0 + LO + i, i < total
+  // the convolution as a correlation with the reversed factors (w: K column
+  // then K row weights)
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    wv.wy[t] = w[K - 1 - t];
+    wv.wx[t] = w[2 * K - 1 - t];
+  }
+  // The loads are raw buffer loads with one descriptor per row, 16 bytes a
+  // lane, written to the slot at 16 bytes x lane: lane l brings columns
+  // cb + LO4 + 4 l, lanes 0..3 also the 16 past the band.  A piece is wholly
+  // inside or wholly outside the image (n and every offset are multiples of
+  // 4); a lane whose byte offset is at or past the descriptor's size reads
+  // zeros, which is how pieces outside the image (columns, or a whole row)
+  // come in.
+  const int xl = cb + LO4 + 4 * wv.lane;
+  const int xh = cb + LO4 + kColBand + 4 * (wv.lane & 3);
+  wv.vl = xl >= 0 && xl < n ? 4 * xl : kColOutside;
+  wv.vh = xh >= 0 && xh < n ? 4 * xh : kColOutside;
+  wv.vm = 4 * min(wv.c0, n - 4);   // lanes past the image store nothing
+  wv.mapp = MAP ? post.map[first_map(post)] : src;
+  wv.post = &post;
+  wv.post2 = &post2;
+  float hw[K - 1 + kColU][4];
+  float blk[W::NB];
+#pragma unroll
+  for (int b = 0; b < W::NB; ++b) blk[b] = 0.f;
+#pragma unroll
+  for (int u = 0; u + 1 < kColRing; ++u)
+    if (u < wv.total) wv.issue(u, wv.ring[u]);
+  // the first blocks hold the K - 1 rows that complete no output; blocks
+  // whose rows and refills all exist run unchecked
+  constexpr int kFirst = (K - 1 + kColU - 1) / kColU * kColU;
+  int i0 = 0;
+  for (; i0 < kFirst && i0 < wv.total; i0 += kColU) wv.template block<true>(i0, hw, blk);
+  for (; i0 + kColU + kColRing - 1 <= wv.total; i0 += kColU) wv.template block<false>(i0, hw, blk);
+  for (; i0 < wv.total; i0 += kColU) wv.template block<true>(i0, hw, blk);
 }
 
 // out[b][Y][X] = post( sum_{dy,dx < k} pre(src[b][Y k + dy][X k + dx]) ), m = n / k.
