@@ -4,16 +4,19 @@
 // dLux: out = fftshift(fft2(ifftshift(pad(x)))) / M      (utils/propagation.py:52-64)
 //       inverse: fftshift(ifft2(ifftshift(pad(x)))) * M  (= conj-sign DFT / M)
 //
-// A 2-D transform is two launches of one kernel.  Each workgroup loads R
-// rows of length M (power of two) into LDS -- applying the zero pad and
-// ifftshift as an index map on the fly -- runs a Stockham auto-sort FFT on
-// every row (radix-8 stages, then one radix-4 or radix-2 stage; each stage
-// reads its butterfly inputs into registers, synchronises, and writes the
-// outputs in place), and writes the rows out TRANSPOSED.  The second launch therefore
-// reads the first's columns as rows and its transposed write restores the
-// orientation; that write also applies fftshift, the 1/M scale and, for the
-// angular-spectrum method, the transfer function H(fx, fy).  HBM traffic is
-// one read and one write of the M x M complex64 array per launch.
+// A 2-D transform is two launches of one kernel, templated on log2 M so the
+// index arithmetic folds.  Each workgroup loads R rows of length M (power of
+// two) into LDS -- applying the zero pad and ifftshift as an index map on the
+// fly -- runs a Stockham auto-sort FFT on every row (radix-8 stages, then one
+// radix-4 or radix-2 stage; each stage reads its butterfly inputs into
+// registers, synchronises, and writes the outputs in place), and writes the
+// rows out TRANSPOSED.  The second launch therefore reads the first's columns
+// as rows and its transposed write restores the orientation; that write also
+// applies fftshift and the 1/M scale.  HBM traffic is one read and one write of
+// the M x M complex64 array per launch.  The angular-spectrum method takes
+// three launches instead of four (launch_asm): the y transforms of its forward
+// and inverse 2-D transforms, with the transfer function between them, run
+// back to back in LDS.
 #include <cmath>
 
 #include "dlx_internal.h"
@@ -26,11 +29,14 @@ namespace {
 constexpr int kThreads = 512;
 constexpr int kLdsElems = 16384;
 constexpr int kLdsBudget = 20480;   // complex64 values in 160 KiB
-// LDS row stride beyond M (complex64).  0: padded rows would spread the
-// transposed store's row reads over the banks, but push a 4-row 2048-point
-// block past half of the CU's LDS and halve the workgroups per CU (measured
-// slower).
-constexpr int kRowPad = 0;
+
+// One 8-byte store of a complex64 (the compiler may otherwise split a float2
+// store into two 4-byte stores, doubling the write requests of the transposed
+// store's short runs).
+typedef float cplx_v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_c64(float2* p, float2 v) {
+  *(cplx_v*)p = cplx_v{v.x, v.y};
+}
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -82,24 +88,26 @@ __device__ __forceinline__ void dft_reg<8>(float2 (&x)[8], float sgn) {
   }
 }
 
-// One Stockham stage of radix RAD on `rows` rows of length M held in LDS:
-// item j (of M/RAD per row) reads x_r = row[j + r M/RAD], twiddles them by
-// exp(sgn 2 pi i r k / (p RAD)) with k = j mod p, transforms, and writes
-// y_q to row[(j - k) RAD + k + q p].  p is the product of earlier radices.
-template <int RAD, int LOG2RAD>
+// One Stockham stage of radix RAD on rows of length M = 2^LOG2M held in LDS
+// (row stride M): item j (of M/RAD per row) reads x_r = row[j + r M/RAD],
+// twiddles them by exp(sgn 2 pi i r k / (p RAD)) with k = j mod p, transforms,
+// and writes y_q to row[(j - k) RAD + k + q p].  p = 2^LOG2P is the product of
+// earlier radices.  The sizes are template arguments, so the index arithmetic
+// folds into shifts, masks and the LDS instructions' immediate offsets.
+template <int RAD, int LOG2RAD, int LOG2M, int LOG2P>
 __device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, int total,
-                                               int log2M, int rs, int log2p, float sgn) {
+                                               float sgn) {
   constexpr int kItems = kLdsElems / RAD / kThreads;
-  const int log2_per_row = log2M - LOG2RAD;
-  const int per_row = 1 << log2_per_row;
-  const int p = 1 << log2p;
+  constexpr int log2_per_row = LOG2M - LOG2RAD;
+  constexpr int per_row = 1 << log2_per_row;
+  constexpr int p = 1 << LOG2P;
   float2 v[kItems][RAD];
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const int t = threadIdx.x + it * kThreads;
     if (t < total) {
       const int rr = t >> log2_per_row, j = t & (per_row - 1);
-      const float2* row = buf + rr * rs;
+      const float2* row = buf + (rr << LOG2M);
 #pragma unroll
       for (int r = 0; r < RAD; ++r) v[it][r] = row[j + r * per_row];
     }
@@ -112,40 +120,84 @@ __device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, in
       const int rr = t >> log2_per_row, j = t & (per_row - 1);
       const int k = j & (p - 1);
       if (p > 1) {
-        const int sh = log2M - log2p - LOG2RAD;   // twiddle index r k M / (p RAD)
+        constexpr int sh = LOG2M - LOG2P - LOG2RAD;   // twiddle index r k M / (p RAD)
 #pragma unroll
-        for (int r = 1; r < RAD; ++r) v[it][r] = cmul(v[it][r], tw[(r * k) << sh]);
+        for (int r = 1; r < RAD; ++r) {
+          const float2 w = tw[(r * k) << sh];
+          v[it][r] = cmul(v[it][r], make_float2(w.x, sgn * w.y));
+        }
       }
       dft_reg<RAD>(v[it], sgn);
-      float2* row = buf + rr * rs;
+      float2* row = buf + (rr << LOG2M);
       const int i0 = ((j - k) << LOG2RAD) + k;
 #pragma unroll
-      for (int q = 0; q < RAD; ++q) row[i0 + (q << log2p)] = v[it][q];
+      for (int q = 0; q < RAD; ++q) row[i0 + (q << LOG2P)] = v[it][q];
     }
   }
   __syncthreads();
 }
 
+// All stages from p = 2^LOG2P: radix 8 while it divides, then one radix-4 or
+// radix-2 stage.
+template <int LOG2M, int LOG2P>
+__device__ __forceinline__ void stockham_from(float2* buf, const float2* tw, int total,
+                                              float sgn) {
+  if constexpr (LOG2M - LOG2P >= 3) {
+    stockham_stage<8, 3, LOG2M, LOG2P>(buf, tw, total >> 3, sgn);
+    stockham_from<LOG2M, LOG2P + 3>(buf, tw, total, sgn);
+  } else if constexpr (LOG2M - LOG2P == 2) {
+    stockham_stage<4, 2, LOG2M, LOG2P>(buf, tw, total >> 2, sgn);
+  } else if constexpr (LOG2M - LOG2P == 1) {
+    stockham_stage<2, 1, LOG2M, LOG2P>(buf, tw, total >> 1, sgn);
+  }
+}
+
 struct FftParams {
   int M, log2M, R, log2R, inverse;
-  int in_mode;      // 0: centred + zero-padded from an N x N source; 1: plain M x M rows
+  // 0: centred + zero-padded from an N x N source (ifftshift as an index map);
+  // 1: plain M x M rows; 2: rows gathered from the columns of an M x M array
+  int in_mode;
   int N, p;         // source side and pad offset (in_mode 0)
   long in_bstride, out_bstride;
-  int out_mode;     // 0: transposed; 1: transposed + fftshift + scale (+ H)
+  // 0: transposed; 1: transposed + fftshift + scale (+ H); 2: plain rows;
+  // 3: plain rows + fftshift + scale
+  int out_mode;
   float scale;
-  int asm_on;
+  // asm_on with out_mode 1: H(fx, fy) of the centred frequencies, fused into
+  // the store.  asm_mid: the rows are y-lines of a spectrum transformed along
+  // x (row index kx); each is transformed along y, multiplied by H at the
+  // natural-order frequencies (kx, ky), and transformed back along y.
+  int asm_on, asm_mid;
   double asm_z, asm_inv_wl2, asm_df;
 };
 
+// The angular-spectrum transfer function at frequency indices (iy, ix) of the
+// centred spectrum (-M/2 .. M/2 - 1 times df), applied to v:
+// v exp(2 pi i z sqrt(1/lambda^2 - fx^2 - fy^2)) where the root is real, 0
+// elsewhere.  The phase is formed in FP64 turns (tens of thousands of turns at
+// millimetre distances), reduced, then taken by the hardware sin / cos
+// (turns; max abs error 1.3e-7 on [-1/2, 1/2]).
+__device__ __forceinline__ float2 apply_h(float2 v, int iy, int ix, const FftParams& P) {
+  const double fy = (double)iy * P.asm_df, fx = (double)ix * P.asm_df;
+  const double arg = P.asm_inv_wl2 - fx * fx - fy * fy;
+  if (!(arg > 0.0)) return make_float2(0.f, 0.f);
+  double t = P.asm_z * sqrt(arg);
+  t -= rint(t);
+  const float s = __builtin_amdgcn_sinf((float)t), c = __builtin_amdgcn_cosf((float)t);
+  return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+}
+
+template <int LOG2M>
 __global__ void __launch_bounds__(kThreads)
 fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftParams P) {
   extern __shared__ float2 lds[];
-  const int M = P.M, R = P.R, half_m = M >> 1;
-  const int RS = M + kRowPad;      // LDS row stride
-  float2* tw = lds + (long)R * RS;  // M twiddles exp(sign 2 pi i q / M)
+  constexpr int M = 1 << LOG2M, half_m = M >> 1;
+  constexpr int RS = M;            // LDS row stride
+  const int R = P.R;
+  float2* tw = lds + (long)R * RS;  // M twiddles (cos, sin)(2 pi q / M)
   // consecutive row groups on one XCD (workgroups are dealt to the 8 XCDs
-  // round-robin), so the short runs of the transposed stores that neighbouring
-  // groups write into the same cache lines meet in that XCD's L2
+  // round-robin), so the short runs of the transposed stores (and loads) that
+  // neighbouring groups access in the same cache lines meet in that XCD's L2
   const int groups = gridDim.x, total_wg = gridDim.x * gridDim.y;
   int id = blockIdx.y * groups + blockIdx.x;
   if (total_wg % 8 == 0) id = (id % 8) * (total_wg / 8) + id / 8;
@@ -157,76 +209,89 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
   for (int q = threadIdx.x; q < M; q += kThreads) {
     float s, c;
     sincospif(2.0f * (float)q / (float)M, &s, &c);
-    tw[q] = make_float2(c, sgn * s);
+    tw[q] = make_float2(c, s);
   }
-  // load in natural order (M and R are powers of two: shifts and masks), 16
-  // values per thread in flight before any is written to LDS
+  // load (M and R are powers of two: shifts and masks), 16 values per thread
+  // in flight before any is written to LDS
   const int total = R * M;
-  const int log2M = P.log2M;
   constexpr int kLoadBatch = 16;
   for (int t0 = 0; t0 < total; t0 += kLoadBatch * kThreads) {
     float2 v[kLoadBatch];
 #pragma unroll
     for (int i = 0; i < kLoadBatch; ++i) {
       const int t = t0 + threadIdx.x + i * kThreads;
-      const int rr = t >> log2M, nc = t & (M - 1);
-      const int row = r0 + rr;
       v[i] = make_float2(0.f, 0.f);
       if (t < total) {
         if (P.in_mode == 0) {
-          const int sr = ((row + half_m) & (M - 1)) - P.p;
+          const int rr = t >> LOG2M, nc = t & (M - 1);
+          const int sr = ((r0 + rr + half_m) & (M - 1)) - P.p;
           const int sc = ((nc + half_m) & (M - 1)) - P.p;
           if (sr >= 0 && sr < P.N && sc >= 0 && sc < P.N) v[i] = src[(long)sr * P.N + sc];
+        } else if (P.in_mode == 1) {
+          v[i] = src[(long)r0 * M + t];
         } else {
-          v[i] = src[(long)row * M + nc];
+          // element (rr, x) of the group is column r0 + rr of row x
+          const int x = t >> P.log2R, rr = t & (R - 1);
+          v[i] = src[(long)x * M + r0 + rr];
         }
       }
     }
 #pragma unroll
     for (int i = 0; i < kLoadBatch; ++i) {
       const int t = t0 + threadIdx.x + i * kThreads;
-      if (t < total) lds[(t >> log2M) * RS + (t & (M - 1))] = v[i];
+      if (t < total) {
+        if (P.in_mode == 2) lds[(t & (R - 1)) * RS + (t >> P.log2R)] = v[i];
+        else lds[t] = v[i];
+      }
     }
   }
   __syncthreads();
 
-  // Stockham stages: radix 8 while it divides, then one radix-4 or radix-2 stage
-  int log2p = 0;
-  while (log2M - log2p >= 3) {
-    stockham_stage<8, 3>(lds, tw, total >> 3, log2M, RS, log2p, sgn);
-    log2p += 3;
-  }
-  if (log2M - log2p == 2) stockham_stage<4, 2>(lds, tw, total >> 2, log2M, RS, log2p, sgn);
-  else if (log2M - log2p == 1) stockham_stage<2, 1>(lds, tw, total >> 1, log2M, RS, log2p, sgn);
+  stockham_from<LOG2M, 0>(lds, tw, total, sgn);
 
-  // transposed store: out[k][row] (consecutive lanes -> consecutive rows)
+  if (P.asm_mid) {
+    // y-line kx = r0 + rr at natural-order frequency ky: multiply by H, then
+    // transform back along y
+    for (int t = threadIdx.x; t < total; t += kThreads) {
+      const int rr = t >> LOG2M, ky = t & (M - 1);
+      const int kx = r0 + rr;
+      lds[t] = apply_h(lds[t], ((ky + half_m) & (M - 1)) - half_m,
+                       ((kx + half_m) & (M - 1)) - half_m, P);
+    }
+    __syncthreads();
+    stockham_from<LOG2M, 0>(lds, tw, total, -sgn);
+  }
+
   float2* dst = out + (long)b * P.out_bstride;
+  if (P.out_mode >= 2) {
+    // plain rows (out_mode 3: fftshift on both axes and the scale)
+    for (int t = threadIdx.x; t < total; t += kThreads) {
+      const int rr = t >> LOG2M, k = t & (M - 1);
+      float2 v = lds[t];
+      if (P.out_mode == 2) {
+        store_c64(dst + (long)(r0 + rr) * M + k, v);
+      } else {
+        v.x *= P.scale;
+        v.y *= P.scale;
+        const int oy = (r0 + rr + half_m) & (M - 1), ox = (k + half_m) & (M - 1);
+        store_c64(dst + (long)oy * M + ox, v);
+      }
+    }
+    return;
+  }
+  // transposed store: out[k][row] (consecutive lanes -> consecutive rows)
   for (int t = threadIdx.x; t < total; t += kThreads) {
     const int k = t >> P.log2R, rr = t & (R - 1);
     const int row = r0 + rr;
     float2 v = lds[rr * RS + k];
     if (P.out_mode == 0) {
-      dst[(long)k * M + row] = v;
+      store_c64(dst + (long)k * M + row, v);
     } else {
       const int oy = (k + half_m) & (M - 1), ox = (row + half_m) & (M - 1);
       v.x *= P.scale;
       v.y *= P.scale;
-      if (P.asm_on) {
-        const double fy = (double)(oy - half_m) * P.asm_df, fx = (double)(ox - half_m) * P.asm_df;
-        const double arg = P.asm_inv_wl2 - fx * fx - fy * fy;
-        if (arg > 0.0) {
-          // the phase z sqrt(1/lambda^2 - f^2) in FP64 turns (tens of thousands of
-          // turns at millimetre distances), reduced, then the hardware sin / cos
-          // (turns; max abs error 1.3e-7 on [-1/2, 1/2])
-          double t = P.asm_z * sqrt(arg);
-          t -= rint(t);
-          const float s = __builtin_amdgcn_sinf((float)t), c = __builtin_amdgcn_cosf((float)t);
-          v = make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
-        } else {
-          v = make_float2(0.f, 0.f);
-        }
-      }
-      dst[(long)oy * M + ox] = v;
+      if (P.asm_on) v = apply_h(v, oy - half_m, ox - half_m, P);
+      store_c64(dst + (long)oy * M + ox, v);
     }
   }
 }
@@ -237,11 +302,25 @@ int ilog2(int m) {
   return l;
 }
 
+template <int LOG2M>
+hipError_t launch_rows_m(const float2* in, float2* out, int batch, const FftParams& P,
+                         size_t lds, hipStream_t s) {
+  if (lds > 64 * 1024) {
+    // Opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
+    (void)hipFuncSetAttribute((const void*)fft_rows_kernel<LOG2M>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipGetLastError();
+  }
+  hipLaunchKernelGGL(fft_rows_kernel<LOG2M>, dim3(P.M / P.R, batch), dim3(kThreads), lds, s, in,
+                     out, P);
+  return hipGetLastError();
+}
+
 hipError_t launch_rows(const float2* in, float2* out, int batch, FftParams P, hipStream_t s) {
   P.log2M = ilog2(P.M);
   // rows per workgroup: about two workgroups per CU (512) when the rows fit,
   // within kLdsElems and the LDS left beside the M-entry twiddle table
-  int cap = (kLdsBudget - P.M) / (P.M + kRowPad);
+  int cap = (kLdsBudget - P.M) / P.M;
   if (cap > kLdsElems / P.M) cap = kLdsElems / P.M;
   if (cap < 1) cap = 1;
   // (at least 4 rows -- 32-byte runs in the transposed store -- while that
@@ -254,17 +333,24 @@ hipError_t launch_rows(const float2* in, float2* out, int batch, FftParams P, hi
   P.R = 1 << (ilog2(r + 1) - 1);   // largest power of two <= r
   if (P.R > P.M) P.R = P.M;
   P.log2R = ilog2(P.R);
-  const size_t lds = ((size_t)P.R * (P.M + kRowPad) + P.M) * sizeof(float2);
+  const size_t lds = ((size_t)P.R * P.M + P.M) * sizeof(float2);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  if (lds > 64 * 1024) {
-    // Opt in to more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
-    (void)hipFuncSetAttribute((const void*)fft_rows_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipGetLastError();
+  switch (P.log2M) {
+    case 1: return launch_rows_m<1>(in, out, batch, P, lds, s);
+    case 2: return launch_rows_m<2>(in, out, batch, P, lds, s);
+    case 3: return launch_rows_m<3>(in, out, batch, P, lds, s);
+    case 4: return launch_rows_m<4>(in, out, batch, P, lds, s);
+    case 5: return launch_rows_m<5>(in, out, batch, P, lds, s);
+    case 6: return launch_rows_m<6>(in, out, batch, P, lds, s);
+    case 7: return launch_rows_m<7>(in, out, batch, P, lds, s);
+    case 8: return launch_rows_m<8>(in, out, batch, P, lds, s);
+    case 9: return launch_rows_m<9>(in, out, batch, P, lds, s);
+    case 10: return launch_rows_m<10>(in, out, batch, P, lds, s);
+    case 11: return launch_rows_m<11>(in, out, batch, P, lds, s);
+    case 12: return launch_rows_m<12>(in, out, batch, P, lds, s);
+    case 13: return launch_rows_m<13>(in, out, batch, P, lds, s);
+    default: return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(fft_rows_kernel, dim3(P.M / P.R, batch), dim3(kThreads), lds, s, in, out,
-                     P);
-  return hipGetLastError();
 }
 
 }  // namespace
@@ -302,6 +388,45 @@ hipError_t launch_fft2_centred(const float2* in, int N, int M, int batch, int in
     B.asm_df = 1.0 / ((double)M * dx);
   }
   return launch_rows(tmp, out, batch, B, s);
+}
+
+// Angular-spectrum propagation of one N x N field padded to M x M:
+// out = fftshift(ifft2(fft2(ifftshift(pad x)) H')) with H' the transfer
+// function at the natural-order frequencies -- the centred forward and
+// inverse transforms with their fftshift / ifftshift pairs cancelled between
+// them.  Three launches: x transforms of the padded field (transposed store);
+// per y-line (fixed kx) the forward y transform, H', and the inverse y
+// transform, in LDS; the inverse x transforms of rows gathered from columns,
+// stored with fftshift and the 1/M^2 scale.
+hipError_t launch_asm(const float2* in, int N, int M, double z, double wavelength, double dx,
+                      float2* tmp, float2* tmp2, float2* out, hipStream_t s) {
+  FftParams A{};
+  A.M = M;
+  A.in_mode = 0;
+  A.N = N;
+  A.p = (M - N) / 2;
+  A.in_bstride = (long)N * N;
+  A.out_bstride = (long)M * M;
+  A.out_mode = 0;
+  hipError_t e = launch_rows(in, tmp, 1, A, s);
+  if (e != hipSuccess) return e;
+  FftParams B = A;
+  B.in_mode = 1;
+  B.in_bstride = (long)M * M;
+  B.out_mode = 2;
+  B.asm_mid = 1;
+  B.asm_z = z;
+  B.asm_inv_wl2 = 1.0 / (wavelength * wavelength);
+  B.asm_df = 1.0 / ((double)M * dx);
+  e = launch_rows(tmp, tmp2, 1, B, s);
+  if (e != hipSuccess) return e;
+  FftParams C = A;
+  C.inverse = 1;
+  C.in_mode = 2;
+  C.in_bstride = (long)M * M;
+  C.out_mode = 3;
+  C.scale = (float)(1.0 / ((double)M * M));
+  return launch_rows(tmp2, out, 1, C, s);
 }
 
 }  // namespace dlx
@@ -368,18 +493,12 @@ extern "C" int dlx_asm_prop(int32_t n, int32_t m, int32_t batch, const double* w
   const size_t slab = dlx_fft2_workspace_size(n, m, batch);
   float2* tmp = (float2*)workspace;
   float2* spec = (float2*)((char*)workspace + slab);
-  const float inv_m = 1.0f / (float)m;
   for (int b = 0; b < batch; ++b) {
     DLX_CHECK_ARG(std::isfinite(wavelength[b]) && wavelength[b] > 0,
                   "wavelength[%d] must be > 0", b);
-    const float2* src = (const float2*)in + (long)b * n * n;
-    float2* sp = spec + (long)b * m * m;
-    // forward centred transform with H fused into its final store
-    DLX_HIP(launch_fft2_centred(src, n, m, 1, 0, inv_m, tmp, sp, 1, distance, wavelength[b],
-                                pixel_scale, s));
-    // inverse centred transform back to the (padded) spatial grid: total 1/M^2
-    DLX_HIP(launch_fft2_centred(sp, m, m, 1, 1, inv_m, tmp, (float2*)out + (long)b * m * m, 0,
-                                0, 0, 0, s));
+    DLX_HIP(launch_asm((const float2*)in + (long)b * n * n, n, m, distance, wavelength[b],
+                       pixel_scale, tmp + (long)b * m * m, spec + (long)b * m * m,
+                       (float2*)out + (long)b * m * m, s));
   }
   return DLX_OK;
 }
