@@ -38,6 +38,16 @@ __device__ __forceinline__ void store_c64(float2* p, float2 v) {
   *(cplx_v*)p = cplx_v{v.x, v.y};
 }
 
+// LDS position of element x of row rr (row length 2^LOG2M): x with its bits
+// 3..8 XORed into bits 0..5 and a row term XORed in.  A bijection on
+// [0, M) for every row, so each row keeps its own M slots; it spreads the
+// Stockham stages' strided writes (stride RAD p) and the transposed
+// accesses (one column of R rows, M apart) over the LDS banks.
+template <int LOG2M>
+__device__ __forceinline__ int lds_pos(int x, int rr) {
+  return x ^ ((x >> 3) & 63) ^ ((rr * 20) & ((1 << LOG2M) - 1));
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -109,7 +119,7 @@ __device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, in
       const int rr = t >> log2_per_row, j = t & (per_row - 1);
       const float2* row = buf + (rr << LOG2M);
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) v[it][r] = row[j + r * per_row];
+      for (int r = 0; r < RAD; ++r) v[it][r] = row[lds_pos<LOG2M>(j + r * per_row, rr)];
     }
   }
   __syncthreads();
@@ -131,7 +141,7 @@ __device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, in
       float2* row = buf + (rr << LOG2M);
       const int i0 = ((j - k) << LOG2RAD) + k;
 #pragma unroll
-      for (int q = 0; q < RAD; ++q) row[i0 + (q << LOG2P)] = v[it][q];
+      for (int q = 0; q < RAD; ++q) row[lds_pos<LOG2M>(i0 + (q << LOG2P), rr)] = v[it][q];
     }
   }
   __syncthreads();
@@ -240,8 +250,9 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
     for (int i = 0; i < kLoadBatch; ++i) {
       const int t = t0 + threadIdx.x + i * kThreads;
       if (t < total) {
-        if (P.in_mode == 2) lds[(t & (R - 1)) * RS + (t >> P.log2R)] = v[i];
-        else lds[t] = v[i];
+        const int rr = P.in_mode == 2 ? t & (R - 1) : t >> LOG2M;
+        const int x = P.in_mode == 2 ? t >> P.log2R : t & (M - 1);
+        lds[rr * RS + lds_pos<LOG2M>(x, rr)] = v[i];
       }
     }
   }
@@ -255,8 +266,8 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
     for (int t = threadIdx.x; t < total; t += kThreads) {
       const int rr = t >> LOG2M, ky = t & (M - 1);
       const int kx = r0 + rr;
-      lds[t] = apply_h(lds[t], ((ky + half_m) & (M - 1)) - half_m,
-                       ((kx + half_m) & (M - 1)) - half_m, P);
+      float2* e = lds + rr * RS + lds_pos<LOG2M>(ky, rr);
+      *e = apply_h(*e, ((ky + half_m) & (M - 1)) - half_m, ((kx + half_m) & (M - 1)) - half_m, P);
     }
     __syncthreads();
     stockham_from<LOG2M, 0>(lds, tw, total, -sgn);
@@ -267,7 +278,7 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
     // plain rows (out_mode 3: fftshift on both axes and the scale)
     for (int t = threadIdx.x; t < total; t += kThreads) {
       const int rr = t >> LOG2M, k = t & (M - 1);
-      float2 v = lds[t];
+      float2 v = lds[rr * RS + lds_pos<LOG2M>(k, rr)];
       if (P.out_mode == 2) {
         store_c64(dst + (long)(r0 + rr) * M + k, v);
       } else {
@@ -283,7 +294,7 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
   for (int t = threadIdx.x; t < total; t += kThreads) {
     const int k = t >> P.log2R, rr = t & (R - 1);
     const int row = r0 + rr;
-    float2 v = lds[rr * RS + k];
+    float2 v = lds[rr * RS + lds_pos<LOG2M>(k, rr)];
     if (P.out_mode == 0) {
       store_c64(dst + (long)k * M + row, v);
     } else {
