@@ -779,6 +779,259 @@ det_sep_col_kernel(const float* __restrict__ src, int n, int bands, int seg_h,
   for (; i0 < wv.total; i0 += kColU) wv.template block<true>(i0, hw, blk);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the column-walking chain (image cotangent only): for
+//   out = post2( blocksum_KD( post( conv_K(x) ) ) )
+// with post holding only pixel-response (at most one) and add ops and post2
+// no map,
+//   din = corr_K( d1 * up_KD( dout * post2'(b) ) )
+// where b are the block sums entering post2 (saved by a forward recompute with
+// post2 left out), post2' is ew_deriv of post2 at b, d1 = post's derivative
+// (the map value, or 1), up_KD replicates a block's value over its KD x KD
+// pixels and corr_K is the correlation with the unflipped factors (the
+// convolution's adjoint; zero outside the image).  Same walk as
+// det_sep_col_kernel: one wave per 256-column band and segment of rows, the
+// rows the cotangent image h = d1 * up(...) needs (map row, dout and b block
+// rows) streamed into a 4-slot LDS ring by buffer loads three rows ahead; each
+// lane forms h on its own 16-byte piece of the band (the lanes 0..3 also on
+// the 16 columns past it) into a wave-private LDS row, then reads its window
+// of that row as the forward reads its input.  The products are those of
+// det_ew_grad_kernel / det_up_kernel / det_corr_kernel (flip 0), in the same
+// order, so the result equals the multi-pass adjoint to the bit.  Needs
+// n % (4 KD) == 0 (block pieces wholly inside or outside the image).
+// ---------------------------------------------------------------------------
+template <int KD>
+struct ColBlk {
+  // block columns a slot holds, from a multiple of 4 at or below the first
+  // block column of the band's window; lanes of 16 bytes that load them
+  static constexpr int kCols = KD == 1 ? 272 : KD == 2 ? 144 : 72;
+  static constexpr int kLanes = kCols / 4;
+};
+
+// d(ops of l)(v)/dv at N values, for a list with no map (each value as
+// ew_deriv; the op list read once per op for all values).
+template <int N>
+__device__ __forceinline__ void ew_deriv_nomap(const EwList& l, float (&v)[N], float (&d)[N]) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) d[j] = 1.f;
+  for (int i = 0; i < l.n; ++i) {
+    const int k = l.kind[i];
+    const float t = l.value[i];
+    if (k == DLX_DET_SATURATE) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        d[j] *= v[j] < t ? 1.f : (v[j] == t ? 0.5f : 0.f);
+        v[j] = v[j] > t ? t : v[j];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) v[j] += t;
+    }
+  }
+}
+
+template <int KD, int K, bool MAP>
+struct ColVjpWave {
+  static constexpr int LO = ColPieces<K>::LO, LO4 = ColPieces<K>::LO4;
+  static constexpr int OFF = ColPieces<K>::OFF, NV = ColPieces<K>::NV;
+  static constexpr int BC = ColBlk<KD>::kCols, BL = ColBlk<KD>::kLanes;
+  // slot layout (floats): [map row: kSlotIn] [dout block row: BC] [b block row: BC]
+  static constexpr int kMapOff = 0;
+  static constexpr int kGOff = MAP ? kSlotIn : 0;
+  static constexpr int kBOff = kGOff + BC;
+  static constexpr int kSlotF = KD == 1 ? kBOff : kBOff + BC;
+  // direct loads per row: map (band + halo), dout block row (KD = 1: band +
+  // halo), b block row (KD > 1)
+  static constexpr int kLoads = (MAP ? 2 : 0) + (KD == 1 ? 2 : 2);
+  lds_float* ring[kColRing];
+  lds_float* hrow;          // the cotangent row h (kSlotIn floats)
+  const float* g;           // dout of this image (m x m)
+  const float* bs;          // saved block sums of this image (m x m; KD > 1)
+  const float* mapp;        // the map (MAP)
+  float* out;               // din of this image
+  int n, m, y0, total, lane, c0, cb, bc0, vl, vh, vbl, vbh;
+  float wy[K], wx[K];
+  const EwList* post2;
+
+  __device__ __forceinline__ void issue(int i, lds_float* slot) const {
+    const int r = y0 + LO + i;
+    const bool okr = r >= 0 && r < n;
+    if (MAP) {
+      const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(mapp + (long)(okr ? r : 0) * n), (short)0, okr ? 4 * n : 0, kRawBuffer);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(row, (col_lds_t*)(slot + kMapOff), 16, vl, 0, 0,
+                                               0);
+      if (lane < 4)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(row, (col_lds_t*)(slot + kMapOff + kColBand),
+                                                 16, vh, 0, 0, 0);
+    }
+    const int br = okr ? r / KD : 0;
+    const __amdgpu_buffer_rsrc_t grow = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(g + (long)br * m), (short)0, okr ? 4 * m : 0, kRawBuffer);
+    if (KD == 1) {
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(grow, (col_lds_t*)(slot + kGOff), 16, vbl, 0, 0,
+                                               0);
+      if (lane < 4)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(grow, (col_lds_t*)(slot + kGOff + kColBand), 16,
+                                                 vbh, 0, 0, 0);
+    } else {
+      const __amdgpu_buffer_rsrc_t brow = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(bs + (long)br * m), (short)0, okr ? 4 * m : 0, kRawBuffer);
+      if (lane < BL) {
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(grow, (col_lds_t*)(slot + kGOff), 16, vbl, 0,
+                                                 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(brow, (col_lds_t*)(slot + kBOff), 16, vbl, 0,
+                                                 0, 0);
+      }
+    }
+  }
+
+  // h on the 4 columns x0 .. x0 + 3 (x0 = cb + LO4 + 4 q for piece q of the
+  // window) from the slot: dout * post2'(b) of each column's block, times the
+  // map value where post holds one.  Block column floor(x / KD) - bc0 >= 0 for
+  // every column of the window; the slot holds zeros for block columns (and
+  // the map for columns) outside the image, so h is 0 there.
+  __device__ __forceinline__ float4 h_piece(const lds_float* slot, int q) const {
+    const int x0 = cb + LO4 + 4 * q;
+    float v[4];
+    if (KD == 1) {
+      const float4 t = lds_ld4(slot + kGOff + 4 * q);
+      v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else if (KD == 2) {
+      const int bc = (x0 >> 1) - bc0;
+      float b[2] = {slot[kBOff + bc], slot[kBOff + bc + 1]}, d[2];
+      ew_deriv_nomap(*post2, b, d);
+      const float h0 = slot[kGOff + bc] * d[0], h1 = slot[kGOff + bc + 1] * d[1];
+      v[0] = h0; v[1] = h0; v[2] = h1; v[3] = h1;
+    } else {
+      const int bc = (x0 >> 2) - bc0;
+      float b[1] = {slot[kBOff + bc]}, d[1];
+      ew_deriv_nomap(*post2, b, d);
+      v[0] = v[1] = v[2] = v[3] = slot[kGOff + bc] * d[0];
+    }
+    if (MAP) {
+      const float4 mv = lds_ld4(slot + kMapOff + 4 * q);
+      v[0] *= mv.x; v[1] *= mv.y; v[2] *= mv.z; v[3] *= mv.w;
+    }
+    return make_float4(v[0], v[1], v[2], v[3]);
+  }
+
+  template <bool CHECK>
+  __device__ __forceinline__ void block(int i0, float (&hw)[K - 1 + kColU][4]) const {
+#pragma unroll
+    for (int u = 0; u < kColU; ++u) {
+      const int i = i0 + u;
+      if (CHECK && i >= total) break;
+      if (!CHECK || i + kColRing - 2 < total)
+        __builtin_amdgcn_s_waitcnt(vmcnt_imm(kLoads * (kColRing - 2)));
+      else
+        __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+      const lds_float* slot = ring[u % kColRing];
+      // h on this lane's piece (and the halo lanes' second piece) into the row
+      const float4 hp = h_piece(slot, lane);
+      const float4 hh = h_piece(slot, 64 + (lane & 3));   // stored by lanes 0..3
+      if (!CHECK || i + kColRing - 1 < total)
+        issue(i + kColRing - 1, ring[(u + kColRing - 1) % kColRing]);
+      *(__attribute__((address_space(3))) lds_f4*)(hrow + 4 * lane) =
+          lds_f4{hp.x, hp.y, hp.z, hp.w};
+      if (lane < 4)
+        *(__attribute__((address_space(3))) lds_f4*)(hrow + kColBand + 4 * lane) =
+            lds_f4{hh.x, hh.y, hh.z, hh.w};
+      // (a wave's LDS accesses complete in order: the window reads below see
+      // the writes above, and the next row's writes follow these reads)
+      float a[4 * NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const float4 t = lds_ld4(hrow + 4 * lane + 4 * v);
+        a[4 * v] = t.x; a[4 * v + 1] = t.y; a[4 * v + 2] = t.z; a[4 * v + 3] = t.w;
+      }
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        float h = 0.f;
+#pragma unroll
+        for (int t = 0; t < K; ++t) h += a[OFF + jj + t] * wx[t];
+        hw[K - 1 + u][jj] = h;
+      }
+      if (CHECK && i < K - 1) continue;
+      const int Y = y0 + i - (K - 1);
+      float o[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        float acc = 0.f;
+#pragma unroll
+        for (int t = 0; t < K; ++t) acc += hw[u + t][jj] * wy[t];
+        o[jj] = acc;
+      }
+      if (c0 < n) *(float4*)(out + (long)Y * n + c0) = make_float4(o[0], o[1], o[2], o[3]);
+    }
+#pragma unroll
+    for (int t = 0; t < K - 1; ++t)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) hw[t][jj] = hw[t + kColU][jj];
+  }
+};
+
+template <int KD, int K, bool MAP>
+__global__ void __launch_bounds__(64)
+det_sep_col_vjp_kernel(const float* __restrict__ dout, const float* __restrict__ bsave, int n,
+                       int bands, int seg_h, const float* __restrict__ w, EwList post, int m,
+                       EwList post2, float* __restrict__ din) {
+  using W = ColVjpWave<KD, K, MAP>;
+  constexpr int LO4 = ColPieces<K>::LO4;
+  constexpr int SL = W::kSlotF;
+  static_assert(4 * 63 + 4 * W::NV <= kSlotIn, "window inside the row");
+  __shared__ __attribute__((aligned(16))) float sl0[SL], sl1[SL], sl2[SL], sl3[SL];
+  __shared__ __attribute__((aligned(16))) float hr[kSlotIn];
+  float* const sl[kColRing] = {sl0, sl1, sl2, sl3};
+  static_assert(kColRing == 4, "one LDS object per slot");
+  W wv;
+#pragma unroll
+  for (int q = 0; q < kColRing; ++q) wv.ring[q] = (lds_float*)sl[q];
+  wv.hrow = (lds_float*)hr;
+  int tile, image;
+  xcd_tile(has_map(post), tile, image);
+  const int band = tile % bands, seg = tile / bands;
+  wv.lane = threadIdx.x;
+  wv.cb = band * kColBand;
+  wv.c0 = wv.cb + 4 * wv.lane;
+  wv.n = n;
+  wv.m = m;
+  wv.g = dout + (long)image * m * m;
+  wv.bs = bsave + (long)image * m * m;
+  wv.out = din + (long)image * n * n;
+  wv.y0 = seg * seg_h;
+  wv.total = (min(wv.y0 + seg_h, n) - wv.y0) + K - 1;
+  // the adjoint correlates with the unflipped factors
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    wv.wy[t] = w[t];
+    wv.wx[t] = w[K + t];
+  }
+  const int xl = wv.cb + LO4 + 4 * wv.lane;
+  const int xh = wv.cb + LO4 + kColBand + 4 * (wv.lane & 3);
+  wv.vl = xl >= 0 && xl < n ? 4 * xl : kColOutside;
+  wv.vh = xh >= 0 && xh < n ? 4 * xh : kColOutside;
+  // block columns from bc0 (a multiple of 4; m % 4 == 0, so a piece is wholly
+  // inside or outside the image)
+  constexpr int kLog2KD = KD == 4 ? 2 : KD == 2 ? 1 : 0;
+  wv.bc0 = ((wv.cb + LO4) >> kLog2KD) & ~3;   // floor, then down to a multiple of 4
+  const int bl = wv.bc0 + 4 * wv.lane;
+  wv.vbl = bl >= 0 && bl < m ? 4 * bl : kColOutside;
+  const int bh = wv.bc0 + kColBand + 4 * (wv.lane & 3);   // KD = 1 only
+  wv.vbh = bh >= 0 && bh < m ? 4 * bh : kColOutside;
+  wv.mapp = MAP ? post.map[first_map(post)] : dout;
+  wv.post2 = &post2;
+  float hw[K - 1 + kColU][4];
+#pragma unroll
+  for (int u = 0; u + 1 < kColRing; ++u)
+    if (u < wv.total) wv.issue(u, wv.ring[u]);
+  constexpr int kFirst = (K - 1 + kColU - 1) / kColU * kColU;
+  int i0 = 0;
+  for (; i0 < kFirst && i0 < wv.total; i0 += kColU) wv.template block<true>(i0, hw);
+  for (; i0 + kColU + kColRing - 1 <= wv.total; i0 += kColU) wv.template block<false>(i0, hw);
+  for (; i0 < wv.total; i0 += kColU) wv.template block<true>(i0, hw);
+}
+
 // out[b][Y][X] = post( sum_{dy,dx < k} pre(src[b][Y k + dy][X k + dx]) ), m = n / k.
 // Row sums first, then the sum of the rows (the order of utils/array_ops.py:151-161).
 // Grid: (pixel blocks of the output image, image); in-image indices are 32-bit.
@@ -1264,6 +1517,34 @@ int run_stream(const Pass& c, int k, int m, const EwList& post2, int batch, cons
   return launch_status("det_sep_col_kernel");
 }
 
+template <int KD, int K>
+void launch_col_vjp(const dim3& grid, hipStream_t st, const Pass& c, int bands, int seg_h, int m,
+                    const EwList& post2, const float* dout, const float* bsave, float* din) {
+  bool map = false;
+  for (int i = 0; i < c.post.n; ++i) map = map || c.post.kind[i] == DLX_DET_PIXEL_RESPONSE;
+  if (map)
+    hipLaunchKernelGGL((det_sep_col_vjp_kernel<KD, K, true>), grid, dim3(64), 0, st, dout, bsave,
+                       c.n_in, bands, seg_h, c.w, c.post, m, post2, din);
+  else
+    hipLaunchKernelGGL((det_sep_col_vjp_kernel<KD, K, false>), grid, dim3(64), 0, st, dout,
+                       bsave, c.n_in, bands, seg_h, c.w, c.post, m, post2, din);
+}
+
+template <int KD>
+void launch_col_vjp_k(const dim3& grid, hipStream_t st, const Pass& c, int bands, int seg_h,
+                      int m, const EwList& post2, const float* dout, const float* bsave,
+                      float* din) {
+  switch (c.K) {
+    case 1: launch_col_vjp<KD, 1>(grid, st, c, bands, seg_h, m, post2, dout, bsave, din); break;
+    case 3: launch_col_vjp<KD, 3>(grid, st, c, bands, seg_h, m, post2, dout, bsave, din); break;
+    case 5: launch_col_vjp<KD, 5>(grid, st, c, bands, seg_h, m, post2, dout, bsave, din); break;
+    case 7: launch_col_vjp<KD, 7>(grid, st, c, bands, seg_h, m, post2, dout, bsave, din); break;
+    case 9: launch_col_vjp<KD, 9>(grid, st, c, bands, seg_h, m, post2, dout, bsave, din); break;
+    case 11: launch_col_vjp<KD, 11>(grid, st, c, bands, seg_h, m, post2, dout, bsave, din); break;
+    default: launch_col_vjp<KD, 13>(grid, st, c, bands, seg_h, m, post2, dout, bsave, din); break;
+  }
+}
+
 bool aligned16(const void* a, const void* b) {
   return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
 }
@@ -1544,6 +1825,44 @@ int param_grads(const EwList& l, float* const* grads, const float* v0, const flo
   return launch_status("det_param_final_kernel");
 }
 
+// The image cotangent of a chain the column-walking kernel runs forward, by
+// its adjoint kernel: a convolve pass (no op before it; after it only adds
+// and at most one pixel response, whose derivatives need no values) alone or
+// fused with a block sum of 2 or 4 (whose ops hold no map), on sides that are
+// multiples of 4 KD.
+bool vjp_streams(const std::vector<Pass>& passes, int batch) {
+  const Pass& c = passes[0];
+  const bool fuse = passes.size() == 2 && can_fuse(passes, 0);
+  if (passes.size() != 1 && !fuse) return false;
+  const int k = fuse ? passes[1].k : 1;
+  if (k != 1 && k != 2 && k != 4) return false;
+  for (int i = 0; i < c.post.n; ++i)
+    if (c.post.kind[i] != DLX_DET_PIXEL_RESPONSE && c.post.kind[i] != DLX_DET_ADD_CONSTANT)
+      return false;
+  return c.n_in % (4 * k) == 0 &&
+         streams(c, k, fuse ? passes[1].post : EwList{}, SumIn{1, 0}, batch);
+}
+
+// Forward recompute of the block sums entering the last ops (KD > 1) into bs,
+// then the adjoint walk into din.
+int run_stream_vjp(const std::vector<Pass>& passes, int batch, const float* in,
+                   const float* dout, float* bs, float* din, hipStream_t st) {
+  const Pass& c = passes[0];
+  const bool fuse = passes.size() == 2;
+  const int k = fuse ? passes[1].k : 1;
+  const int m = c.n_in / k;
+  const EwList post2 = fuse ? passes[1].post : EwList{};
+  int res;
+  if (k > 1 && (res = run_stream(c, k, m, EwList{}, batch, in, bs, st))) return res;
+  int bands, seg_h, nseg;
+  stream_grid(c.n_in, batch, bands, seg_h, nseg);
+  const dim3 grid(bands * nseg, batch);
+  if (k == 4) launch_col_vjp_k<4>(grid, st, c, bands, seg_h, m, post2, dout, bs, din);
+  else if (k == 2) launch_col_vjp_k<2>(grid, st, c, bands, seg_h, m, post2, dout, bs, din);
+  else launch_col_vjp_k<1>(grid, st, c, bands, seg_h, m, EwList{}, dout, dout, din);
+  return launch_status("det_sep_col_vjp_kernel");
+}
+
 int detector_vjp_impl(int32_t n, int32_t batch, const float* in, const dlx_detector_op* ops,
                       int32_t nops, const float* dout, float* din, float* const* grads,
                       void* workspace, size_t workspace_bytes, void* stream, const char* name) {
@@ -1577,6 +1896,8 @@ int detector_vjp_impl(int32_t n, int32_t batch, const float* in, const dlx_detec
     const size_t nblk = ((size_t)side_max * side_max + 255) / 256;
     kws = (uint8_t*)part + align_up(nblk * kMaxEw * sizeof(double));
   }
+  if (!grads && vjp_streams(passes, batch) && aligned16(in, din) && aligned16(dout, ws))
+    return run_stream_vjp(passes, batch, in, dout, (float*)ws, din, s);
   auto x_in = [&](size_t i) -> const float* {
     return i == 0 ? in : (const float*)(ws + L.x_off[i]);
   };

@@ -192,6 +192,8 @@ COLUMN_CASES = [
     ("k3-map-down4-ragged-seg", 300, 40, 3, [("add", 0.3), ("pixel_response", "map"),
                                              ("downsample", 4), ("add", 1.0)]),
     ("k11-alone-saturate", 200, 100, 11, [("saturate", None), ("add", 0.5)]),
+    ("k5-map-down4-partial-band", 320, 48, 5, [("pixel_response", "map"), ("downsample", 4),
+                                               ("saturate", None)]),
 ]
 
 
@@ -219,6 +221,43 @@ def test_column_kernel_bit_identical_to_tile_kernels(dl, case):
     torch.cuda.synchronize()
     assert torch.equal(got, one), f"{(got != one).sum().item()} pixels differ"
     ref = _oracle(x[:2].astype(np.float64), chain)
+    err = _err(got[:2], ref)
+    print(f"  err vs oracle {err:.2e}")
+    assert err < CHAIN_TOL
+
+
+# The image cotangent of those chains: where the convolution's ops need no
+# values for their derivative (adds, one map) and the side divides by 4 KD, one
+# forward recompute of the block sums and the adjoint column walk
+# (det_sep_col_vjp_kernel); one image at a time, the multi-pass adjoint (saved
+# pass inputs, elementwise derivatives, block replication, correlation).  The
+# products are the same in the same order: bit-identical, and the oracle's.
+@pytest.mark.parametrize("case", range(len(COLUMN_CASES)))
+def test_column_vjp_bit_identical_to_multipass(dl, case):
+    name, n, batch, K, tail = COLUMN_CASES[case]
+    print(f"[detector column vjp] {name} n={n} batch={batch} K={K}", flush=True)
+    rng = np.random.default_rng(120 + case)
+    x = rng.uniform(0.0, 1.0, (batch, n, n)).astype(np.float32)
+    chain = [("convolve_separable", (rng.uniform(0, 1, K), rng.uniform(0, 1, K)))]
+    side = n
+    for kind, arg in tail:
+        if kind == "pixel_response":
+            arg = rng.uniform(0.8, 1.2, (side, side))
+        elif kind == "downsample":
+            side //= arg
+        chain.append((kind, arg))
+    chain = _resolve(chain, x[:2].astype(np.float64))
+    dev = [(k, tuple(torch.as_tensor(np.asarray(b, np.float32)).cuda() for b in a)
+            if isinstance(a, tuple) else torch.as_tensor(np.asarray(a, np.float32)).cuda()
+            if isinstance(a, np.ndarray) else a) for k, a in chain]
+    g = rng.normal(size=(batch, side, side)).astype(np.float32)
+    xt, gt = torch.as_tensor(x).cuda(), torch.as_tensor(g).cuda()
+    got = dl.runtime.detector_vjp(xt, dev, gt)
+    one = torch.stack([dl.runtime.detector_vjp(xt[b], dev, gt[b]) for b in range(batch)])
+    torch.cuda.synchronize()
+    assert torch.equal(got, one), f"{(got != one).sum().item()} pixels differ"
+    ref = np.stack([orc.detector_vjp(x[b].astype(np.float64), chain, g[b].astype(np.float64))
+                    for b in range(2)])
     err = _err(got[:2], ref)
     print(f"  err vs oracle {err:.2e}")
     assert err < CHAIN_TOL
