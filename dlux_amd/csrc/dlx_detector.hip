@@ -886,28 +886,38 @@ struct ColVjpWave {
     }
   }
 
-  // h on the 4 columns x0 .. x0 + 3 (x0 = cb + LO4 + 4 q for piece q of the
-  // window) from the slot: dout * post2'(b) of each column's block, times the
-  // map value where post holds one.  Block column floor(x / KD) - bc0 >= 0 for
-  // every column of the window; the slot holds zeros for block columns (and
-  // the map for columns) outside the image, so h is 0 there.
-  __device__ __forceinline__ float4 h_piece(const lds_float* slot, int q) const {
+  // The block values of piece q of the window (columns x0 .. x0 + 3, x0 =
+  // cb + LO4 + 4 q): dout * post2'(b) of each column's block (KD = 4: one
+  // block, KD = 2: two).  Block column floor(x / KD) - bc0 >= 0 for every
+  // column of the window; the slot holds zeros for block columns outside the
+  // image (and the map zeros for columns outside it), so h is 0 there.
+  __device__ __forceinline__ void block_g(const lds_float* slot, int q, float (&gd)[2]) const {
     const int x0 = cb + LO4 + 4 * q;
+    if (KD == 2) {
+      const int bc = (x0 >> 1) - bc0;
+      float b[2] = {slot[kBOff + bc], slot[kBOff + bc + 1]}, d[2];
+      ew_deriv_nomap(*post2, b, d);
+      gd[0] = slot[kGOff + bc] * d[0];
+      gd[1] = slot[kGOff + bc + 1] * d[1];
+    } else if (KD == 4) {
+      const int bc = (x0 >> 2) - bc0;
+      float b[1] = {slot[kBOff + bc]}, d[1];
+      ew_deriv_nomap(*post2, b, d);
+      gd[0] = gd[1] = slot[kGOff + bc] * d[0];
+    }
+  }
+
+  // h on piece q from its block values (KD > 1) or the dout row (KD = 1),
+  // times the map value where post holds one
+  __device__ __forceinline__ float4 h_piece(const lds_float* slot, int q,
+                                            const float (&gd)[2]) const {
     float v[4];
     if (KD == 1) {
       const float4 t = lds_ld4(slot + kGOff + 4 * q);
       v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-    } else if (KD == 2) {
-      const int bc = (x0 >> 1) - bc0;
-      float b[2] = {slot[kBOff + bc], slot[kBOff + bc + 1]}, d[2];
-      ew_deriv_nomap(*post2, b, d);
-      const float h0 = slot[kGOff + bc] * d[0], h1 = slot[kGOff + bc + 1] * d[1];
-      v[0] = h0; v[1] = h0; v[2] = h1; v[3] = h1;
     } else {
-      const int bc = (x0 >> 2) - bc0;
-      float b[1] = {slot[kBOff + bc]}, d[1];
-      ew_deriv_nomap(*post2, b, d);
-      v[0] = v[1] = v[2] = v[3] = slot[kGOff + bc] * d[0];
+      v[0] = v[1] = gd[0];
+      v[2] = v[3] = gd[1];
     }
     if (MAP) {
       const float4 mv = lds_ld4(slot + kMapOff + 4 * q);
@@ -916,8 +926,13 @@ struct ColVjpWave {
     return make_float4(v[0], v[1], v[2], v[3]);
   }
 
+  // gm, gh: the block values of this lane's piece and of its halo piece,
+  // kept while the rows stay in one block row (a new block row starts where
+  // y0 + LO + i is a multiple of KD: y0 and i0 are multiples of KD, so at fixed
+  // u in every block).
   template <bool CHECK>
-  __device__ __forceinline__ void block(int i0, float (&hw)[K - 1 + kColU][4]) const {
+  __device__ __forceinline__ void block(int i0, float (&hw)[K - 1 + kColU][4], float (&gm)[2],
+                                        float (&gh)[2]) const {
 #pragma unroll
     for (int u = 0; u < kColU; ++u) {
       const int i = i0 + u;
@@ -927,9 +942,14 @@ struct ColVjpWave {
       else
         __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
       const lds_float* slot = ring[u % kColRing];
+      const bool new_block = ((LO % KD) + KD + u) % KD == 0;   // constant once unrolled
+      if (KD > 1 && (new_block || (CHECK && i == 0))) {
+        block_g(slot, lane, gm);
+        block_g(slot, 64 + (lane & 3), gh);
+      }
       // h on this lane's piece (and the halo lanes' second piece) into the row
-      const float4 hp = h_piece(slot, lane);
-      const float4 hh = h_piece(slot, 64 + (lane & 3));   // stored by lanes 0..3
+      const float4 hp = h_piece(slot, lane, gm);
+      const float4 hh = h_piece(slot, 64 + (lane & 3), gh);   // stored by lanes 0..3
       if (!CHECK || i + kColRing - 1 < total)
         issue(i + kColRing - 1, ring[(u + kColRing - 1) % kColRing]);
       *(__attribute__((address_space(3))) lds_f4*)(hrow + 4 * lane) =
@@ -1022,14 +1042,16 @@ det_sep_col_vjp_kernel(const float* __restrict__ dout, const float* __restrict__
   wv.mapp = MAP ? post.map[first_map(post)] : dout;
   wv.post2 = &post2;
   float hw[K - 1 + kColU][4];
+  float gm[2] = {0.f, 0.f}, gh[2] = {0.f, 0.f};
 #pragma unroll
   for (int u = 0; u + 1 < kColRing; ++u)
     if (u < wv.total) wv.issue(u, wv.ring[u]);
   constexpr int kFirst = (K - 1 + kColU - 1) / kColU * kColU;
   int i0 = 0;
-  for (; i0 < kFirst && i0 < wv.total; i0 += kColU) wv.template block<true>(i0, hw);
-  for (; i0 + kColU + kColRing - 1 <= wv.total; i0 += kColU) wv.template block<false>(i0, hw);
-  for (; i0 < wv.total; i0 += kColU) wv.template block<true>(i0, hw);
+  for (; i0 < kFirst && i0 < wv.total; i0 += kColU) wv.template block<true>(i0, hw, gm, gh);
+  for (; i0 + kColU + kColRing - 1 <= wv.total; i0 += kColU)
+    wv.template block<false>(i0, hw, gm, gh);
+  for (; i0 < wv.total; i0 += kColU) wv.template block<true>(i0, hw, gm, gh);
 }
 
 // out[b][Y][X] = post( sum_{dy,dx < k} pre(src[b][Y k + dy][X k + dx]) ), m = n / k.
