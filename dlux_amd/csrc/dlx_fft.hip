@@ -104,9 +104,9 @@ __device__ __forceinline__ void dft_reg<8>(float2 (&x)[8], float sgn) {
 // and writes y_q to row[(j - k) RAD + k + q p].  p = 2^LOG2P is the product of
 // earlier radices.  The sizes are template arguments, so the index arithmetic
 // folds into shifts, masks and the LDS instructions' immediate offsets.
-template <int RAD, int LOG2RAD, int LOG2M, int LOG2P>
-__device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, int total,
-                                               float sgn) {
+template <int RAD, int LOG2RAD, int LOG2M, int LOG2P, bool INV>
+__device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, int total) {
+  constexpr float sgn = INV ? 1.f : -1.f;   // the transform's sign, folded into the code
   constexpr int kItems = kLdsElems / RAD / kThreads;
   constexpr int log2_per_row = LOG2M - LOG2RAD;
   constexpr int per_row = 1 << log2_per_row;
@@ -149,16 +149,15 @@ __device__ __forceinline__ void stockham_stage(float2* buf, const float2* tw, in
 
 // All stages from p = 2^LOG2P: radix 8 while it divides, then one radix-4 or
 // radix-2 stage.
-template <int LOG2M, int LOG2P>
-__device__ __forceinline__ void stockham_from(float2* buf, const float2* tw, int total,
-                                              float sgn) {
+template <int LOG2M, int LOG2P, bool INV>
+__device__ __forceinline__ void stockham_from(float2* buf, const float2* tw, int total) {
   if constexpr (LOG2M - LOG2P >= 3) {
-    stockham_stage<8, 3, LOG2M, LOG2P>(buf, tw, total >> 3, sgn);
-    stockham_from<LOG2M, LOG2P + 3>(buf, tw, total, sgn);
+    stockham_stage<8, 3, LOG2M, LOG2P, INV>(buf, tw, total >> 3);
+    stockham_from<LOG2M, LOG2P + 3, INV>(buf, tw, total);
   } else if constexpr (LOG2M - LOG2P == 2) {
-    stockham_stage<4, 2, LOG2M, LOG2P>(buf, tw, total >> 2, sgn);
+    stockham_stage<4, 2, LOG2M, LOG2P, INV>(buf, tw, total >> 2);
   } else if constexpr (LOG2M - LOG2P == 1) {
-    stockham_stage<2, 1, LOG2M, LOG2P>(buf, tw, total >> 1, sgn);
+    stockham_stage<2, 1, LOG2M, LOG2P, INV>(buf, tw, total >> 1);
   }
 }
 
@@ -213,13 +212,24 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
   if (total_wg % 8 == 0) id = (id % 8) * (total_wg / 8) + id / 8;
   const int b = id / groups;
   const int r0 = (id - b * groups) * R;
-  const float sgn = P.inverse ? 1.f : -1.f;
   const float2* src = in + (long)b * P.in_bstride;
 
-  for (int q = threadIdx.x; q < M; q += kThreads) {
-    float s, c;
-    sincospif(2.0f * (float)q / (float)M, &s, &c);
-    tw[q] = make_float2(c, s);
+  if constexpr (M >= 4) {
+    // one quadrant by sincospif, the other three by its exact symmetries
+    for (int q = threadIdx.x; q < M / 4; q += kThreads) {
+      float s, c;
+      sincospif(2.0f * (float)q / (float)M, &s, &c);
+      tw[q] = make_float2(c, s);
+      tw[q + M / 4] = make_float2(-s, c);
+      tw[q + M / 2] = make_float2(-c, -s);
+      tw[q + 3 * M / 4] = make_float2(s, -c);
+    }
+  } else {
+    for (int q = threadIdx.x; q < M; q += kThreads) {
+      float s, c;
+      sincospif(2.0f * (float)q / (float)M, &s, &c);
+      tw[q] = make_float2(c, s);
+    }
   }
   // load (M and R are powers of two: shifts and masks), 16 values per thread
   // in flight before any is written to LDS
@@ -258,7 +268,8 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
   }
   __syncthreads();
 
-  stockham_from<LOG2M, 0>(lds, tw, total, sgn);
+  if (P.inverse) stockham_from<LOG2M, 0, true>(lds, tw, total);
+  else stockham_from<LOG2M, 0, false>(lds, tw, total);
 
   if (P.asm_mid) {
     // y-line kx = r0 + rr at natural-order frequency ky: multiply by H, then
@@ -270,7 +281,8 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
       *e = apply_h(*e, ((ky + half_m) & (M - 1)) - half_m, ((kx + half_m) & (M - 1)) - half_m, P);
     }
     __syncthreads();
-    stockham_from<LOG2M, 0>(lds, tw, total, -sgn);
+    if (P.inverse) stockham_from<LOG2M, 0, false>(lds, tw, total);
+    else stockham_from<LOG2M, 0, true>(lds, tw, total);
   }
 
   float2* dst = out + (long)b * P.out_bstride;
