@@ -1465,7 +1465,7 @@ int check_grid(const std::vector<Pass>& passes, int batch) {
 // 13 on a single (not summed) input whose side is a multiple of 4, alone or
 // with a 2 x 2 / 4 x 4 block sum, when the batch is large enough: at least 512
 // (256-column band, segment of rows, image) waves.  A wave walks its rows one
-// after the other, so on a small batch (one 256^2 frame: 4 waves) the tile
+// after the other, so on a small batch (one 256^2 frame: 8 waves) the tile
 // kernels, which cover the image with many short-lived workgroups, win.
 bool streams(const Pass& c, int k, const EwList& post2, SumIn si, int batch) {
   const long bands = (c.n_in + kColBand - 1) / kColBand;
