@@ -271,33 +271,37 @@ class Wavefront(Base):
                     "Output spec cannot specify n; FFT output n is determined by the ",
                     "pad parameter.",
                 )
-        outs, pss, cs = [], [], []
-        for i in range(self._nb()):
-            wl = float(self._wl_list()[i])
-            ph = self.phasor[i] if self.phasor.ndim == 3 else self.phasor
-            d_fft, c_fft = fourier.fft_spec(n_out, d_in, wl, focal_length)
-            if spec_out is not None:
-                in_ramp = fourier.fft_phase_ramp(
-                    CoordSpec(n_in, d_in, self.center).xs, wl, c_fft - spec_out.c,
-                    focal_length, inverse)
+        # every wavelength's transform in one batched call; the ramps (when
+        # re-centring) differ per wavelength and multiply by broadcasting
+        wls = np.asarray(self._wl_list(), float)
+        vec = self.phasor.ndim == 3
+        specs = [fourier.fft_spec(n_out, d_in, float(wl), focal_length) for wl in wls]
+        ph = self.phasor
+        out_ramp = None
+        if spec_out is not None:
+            xs_in = CoordSpec(n_in, d_in, self.center).xs
+            rin, rout = [], []
+            for wl, (d_fft, c_fft) in zip(wls, specs):
+                rin.append(fourier.fft_phase_ramp_1d(xs_in, float(wl), c_fft - spec_out.c,
+                                                     focal_length, inverse))
                 so = spec_out.set(n=n_out, d=d_fft)
-                shift = fourier.fft_spec(so.n, so.d, wl, focal_length)[1]
-                out_ramp = fourier.fft_phase_ramp(so.xs, wl, shift, focal_length, inverse)
-                c_out = spec_out.c
-                ph = ph * in_ramp
-            else:
-                out_ramp = None
-                c_out = np.asarray(c_fft, float)
-            out, ps = _FFT(ph, wl, d_in, focal_length=focal_length, pad=pad, inverse=inverse)
-            if out_ramp is not None:
-                out = out * out_ramp
-            outs.append(out)
-            pss.append(ps)
-            cs.append(np.atleast_1d(c_out).astype(float))
-        if self.phasor.ndim == 3:
-            return self.set(phasor=torch.stack(outs), pixel_scale=np.asarray(pss, float),
-                            center=cs[0])
-        return self.set(phasor=outs[0], pixel_scale=np.asarray(pss[0], float), center=cs[0])
+                shift = fourier.fft_spec(so.n, so.d, float(wl), focal_length)[1]
+                rout.append(fourier.fft_phase_ramp_1d(so.xs, float(wl), shift, focal_length,
+                                                      inverse))
+            rin, rout = torch.stack(rin), torch.stack(rout)   # (B, n), (B, n_out)
+            in_ramp = rin[:, None, :] * rin[:, :, None]
+            out_ramp = rout[:, None, :] * rout[:, :, None]
+            if not vec:
+                in_ramp, out_ramp = in_ramp[0], out_ramp[0]
+            ph = ph * in_ramp
+            c_out = np.atleast_1d(spec_out.c).astype(float)
+        else:
+            c_out = np.atleast_1d(np.asarray(specs[0][1], float))
+        out, ps = _FFT(ph, wls if vec else float(wls[0]), d_in, focal_length=focal_length,
+                       pad=pad, inverse=inverse)
+        if out_ramp is not None:
+            out = out * out_ramp
+        return self.set(phasor=out, pixel_scale=np.asarray(ps, float), center=c_out)
 
     # --- resampling (wavefronts.py:442-566) ---------------------------------
     def _fields(self, complex: bool):

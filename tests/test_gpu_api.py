@@ -191,6 +191,30 @@ def test_fft_layer_centered(dl):
     assert np.isclose(float(out.pixel_scale), ps)
 
 
+@pytest.mark.parametrize("centre", [False, True])
+def test_propagate_fft_vectorised_equals_per_wavelength(dl, centre):
+    """A vectorised wavefront (3 wavelengths) runs one batched padded FFT, with the
+    re-centring ramps per wavelength: each slice equals the single-wavelength
+    propagation to the bit, with its own pixel scale (the wavefront keeps one
+    centre, the first wavelength's, as before)."""
+    print(f"[fft-vectorised] centre={centre}", flush=True)
+    from dlux_amd.coordinates import CoordSpec
+    n, wls = 32, np.array([0.9e-6, 1.0e-6, 1.2e-6])
+    rng = np.random.default_rng(5)
+    ph = (rng.normal(size=(3, n, n)) + 1j * rng.normal(size=(3, n, n))).astype(np.complex64)
+    spec = CoordSpec(c=0.0) if centre else None
+    wf = dl.Wavefront.from_phasor(torch.as_tensor(ph).cuda(), wls, diameter=1.0)
+    out = wf.propagate_FFT(pad=2, spec_out=spec)
+    assert tuple(out.phasor.shape) == (3, 2 * n, 2 * n)
+    for i, wl in enumerate(wls):
+        one = dl.Wavefront.from_phasor(torch.as_tensor(ph[i]).cuda(), wl, diameter=1.0)
+        ref = one.propagate_FFT(pad=2, spec_out=spec)
+        assert torch.equal(out.phasor[i], ref.phasor), i
+        assert np.isclose(np.ravel(out.pixel_scale)[i], float(ref.pixel_scale), rtol=0, atol=0)
+        if centre or i == 0:   # one centre per wavefront: the first wavelength's
+            assert np.allclose(out.center, ref.center)
+
+
 def test_point_sources_sum(dl):
     print("[point-sources]", flush=True)
     from dlux_amd import workloads
