@@ -198,10 +198,11 @@ class Server:
 
     # Device-time estimates used to place work: the contractions at ~300 TFLOP/s
     # (8 (N_out N_in^2 + N_out^2 N_in) FLOPs per wavelength, four contractions'
-    # worth for a gradient) and the angular spectrum's four FFT passes at ~4 TB/s.
+    # worth for a gradient) and the angular spectrum's three FFT launches (each
+    # reading and writing the padded array) at ~2.3 TB/s.
     def cost(self, r: Request) -> float:
         if r.kind == "asm":
-            return 4 * 16.0 * r.n_out ** 2 / 4e12
+            return 3 * 16.0 * r.n_out ** 2 / 2.3e12
         f = 8.0 * (r.n_out * r.n_in ** 2 + r.n_out ** 2 * r.n_in) * len(r.wavelengths)
         return (4.0 if r.kind == "grad" else 1.0) * f / 300e12
 
