@@ -178,6 +178,12 @@ struct FftParams {
   // natural-order frequencies (kx, ky), and transformed back along y.
   int asm_on, asm_mid;
   double asm_z, asm_inv_wl2, asm_df;
+  // rows_lo > 0 (in_mode 0): only the rows that hold the padded field, [0,
+  // rows_lo) and [M - rows_lo, M), are transformed (the others are zero rows
+  // whose transforms are zero); the launch has 2 rows_lo / R groups.
+  // zlo < zhi (in_mode 1): positions [zlo, zhi) of every row load as zeros
+  // (the columns the previous launch left unwritten).
+  int rows_lo, zlo, zhi;
 };
 
 // The angular-spectrum transfer function at frequency indices (iy, ix) of the
@@ -211,7 +217,8 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
   int id = blockIdx.y * groups + blockIdx.x;
   if (total_wg % 8 == 0) id = (id % 8) * (total_wg / 8) + id / 8;
   const int b = id / groups;
-  const int r0 = (id - b * groups) * R;
+  int r0 = (id - b * groups) * R;
+  if (P.rows_lo > 0 && r0 >= P.rows_lo) r0 += M - 2 * P.rows_lo;
   const float2* src = in + (long)b * P.in_bstride;
 
   if constexpr (M >= 4) {
@@ -248,7 +255,8 @@ fft_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftPara
           const int sc = ((nc + half_m) & (M - 1)) - P.p;
           if (sr >= 0 && sr < P.N && sc >= 0 && sc < P.N) v[i] = src[(long)sr * P.N + sc];
         } else if (P.in_mode == 1) {
-          v[i] = src[(long)r0 * M + t];
+          const int y = t & (M - 1);
+          if (y < P.zlo || y >= P.zhi) v[i] = src[(long)r0 * M + t];
         } else {
           // element (rr, x) of the group is column r0 + rr of row x
           const int x = t >> P.log2R, rr = t & (R - 1);
@@ -334,27 +342,32 @@ hipError_t launch_rows_m(const float2* in, float2* out, int batch, const FftPara
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
   }
-  hipLaunchKernelGGL(fft_rows_kernel<LOG2M>, dim3(P.M / P.R, batch), dim3(kThreads), lds, s, in,
+  const int rows = P.rows_lo > 0 ? 2 * P.rows_lo : P.M;
+  hipLaunchKernelGGL(fft_rows_kernel<LOG2M>, dim3(rows / P.R, batch), dim3(kThreads), lds, s, in,
                      out, P);
   return hipGetLastError();
 }
 
-hipError_t launch_rows(const float2* in, float2* out, int batch, FftParams P, hipStream_t s) {
-  P.log2M = ilog2(P.M);
-  // rows per workgroup: about two workgroups per CU (512) when the rows fit,
-  // within kLdsElems and the LDS left beside the M-entry twiddle table
-  int cap = (kLdsBudget - P.M) / P.M;
-  if (cap > kLdsElems / P.M) cap = kLdsElems / P.M;
+// Rows per workgroup for length-M rows: about two workgroups per CU (512)
+// when the rows fit, within kLdsElems and the LDS left beside the M-entry
+// twiddle table, at least 4 rows (32-byte runs in the transposed store) while
+// that still leaves one workgroup per CU; a power of two.
+int rows_per_group(int M) {
+  int cap = (kLdsBudget - M) / M;
+  if (cap > kLdsElems / M) cap = kLdsElems / M;
   if (cap < 1) cap = 1;
-  // (at least 4 rows -- 32-byte runs in the transposed store -- while that
-  // still leaves one workgroup per CU)
-  int r = P.M / 512;
-  const int r_min = P.M / 256 < 4 ? P.M / 256 : 4;
+  int r = M / 512;
+  const int r_min = M / 256 < 4 ? M / 256 : 4;
   if (r < r_min) r = r_min;
   if (r < 1) r = 1;
   if (r > cap) r = cap;
-  P.R = 1 << (ilog2(r + 1) - 1);   // largest power of two <= r
-  if (P.R > P.M) P.R = P.M;
+  int R = 1 << (ilog2(r + 1) - 1);   // largest power of two <= r
+  return R > M ? M : R;
+}
+
+hipError_t launch_rows(const float2* in, float2* out, int batch, FftParams P, hipStream_t s) {
+  P.log2M = ilog2(P.M);
+  P.R = rows_per_group(P.M);
   P.log2R = ilog2(P.R);
   const size_t lds = ((size_t)P.R * P.M + P.M) * sizeof(float2);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
@@ -385,6 +398,21 @@ bool fft_supported(int m) { return m >= 2 && m <= kMaxFftSide && (m & (m - 1)) =
 
 // Full centred 2-D transform of `batch` N x N fields, padded to M x M.
 // tmp: batch * M * M complex64 scratch.
+// A launch over the rows of an N x N field padded to M x M (ifftshifted):
+// the rows that hold the field are [0, N/2) and [M - N/2, M); transform only
+// those when N/2 is a multiple of the rows per workgroup.
+static void set_padded_rows(int N, int M, FftParams& A) {
+  const int half = N / 2;
+  A.rows_lo = N < M && half > 0 && half % rows_per_group(M) == 0 ? half : 0;
+}
+
+// The launch after one that skipped zero rows reads, in each of its rows, the
+// positions those rows would have written as zeros.
+static void set_zero_columns(const FftParams& A, FftParams& B) {
+  B.zlo = A.rows_lo > 0 ? A.rows_lo : 0;
+  B.zhi = A.rows_lo > 0 ? A.M - A.rows_lo : 0;
+}
+
 hipError_t launch_fft2_centred(const float2* in, int N, int M, int batch, int inverse,
                                float scale, float2* tmp, float2* out, int asm_on, double z,
                                double wavelength, double dx, hipStream_t s) {
@@ -397,13 +425,16 @@ hipError_t launch_fft2_centred(const float2* in, int N, int M, int batch, int in
   A.in_bstride = (long)N * N;
   A.out_bstride = (long)M * M;
   A.out_mode = 0;
+  set_padded_rows(N, M, A);
   hipError_t e = launch_rows(in, tmp, batch, A, s);
   if (e != hipSuccess) return e;
   FftParams B = A;
+  B.rows_lo = 0;
   B.in_mode = 1;
   B.in_bstride = (long)M * M;
   B.out_mode = 1;
   B.scale = scale;
+  set_zero_columns(A, B);
   B.asm_on = asm_on;
   if (asm_on) {
     B.asm_z = z;
@@ -431,12 +462,15 @@ hipError_t launch_asm(const float2* in, int N, int M, double z, double wavelengt
   A.in_bstride = (long)N * N;
   A.out_bstride = (long)M * M;
   A.out_mode = 0;
+  set_padded_rows(N, M, A);
   hipError_t e = launch_rows(in, tmp, 1, A, s);
   if (e != hipSuccess) return e;
   FftParams B = A;
+  B.rows_lo = 0;
   B.in_mode = 1;
   B.in_bstride = (long)M * M;
   B.out_mode = 2;
+  set_zero_columns(A, B);
   B.asm_mid = 1;
   B.asm_z = z;
   B.asm_inv_wl2 = 1.0 / (wavelength * wavelength);
@@ -444,6 +478,7 @@ hipError_t launch_asm(const float2* in, int N, int M, double z, double wavelengt
   e = launch_rows(tmp, tmp2, 1, B, s);
   if (e != hipSuccess) return e;
   FftParams C = A;
+  C.rows_lo = 0;
   C.inverse = 1;
   C.in_mode = 2;
   C.in_bstride = (long)M * M;
