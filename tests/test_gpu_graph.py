@@ -124,3 +124,45 @@ def test_graphed_gradient_step_matches_eager(rt):
                 err = _peak_err(coeffs.grad, refs[i][1])
                 assert err <= 1e-6, (rep, i, err)
                 eager(sets[(i + 1) % len(sets)])    # eager work between replays
+
+
+def test_graphed_telescope_model_matches_eager(rt):
+    """runtime.graph_step over Telescope.model (point source, 8 wavelengths,
+    the detector fused into the wavelength sum): every replay, with a new
+    response map copied in and eager calls between replays, equals the eager
+    image to the bit."""
+    print("[graph] telescope model", flush=True)
+    import dlux_amd as dl
+    from dlux_amd import layers
+    n, n_out, nwl = 128, 64, 8
+    mask, basis = orc.circular_aperture(n, 1.0, oversample=1, zernike_nolls=range(2, 8))
+    rng = np.random.default_rng(21)
+    coeffs = torch.tensor(rng.normal(0, 30e-9, basis.shape[0]), dtype=torch.float32,
+                          device="cuda")
+    optics = dl.AngularOpticalSystem(
+        wf_npixels=n, diameter=1.0,
+        layers=[("aperture", dl.TransmissiveLayer(mask, normalise=True)),
+                ("aberrations", dl.BasisLayer(basis, coeffs))],
+        psf_npixels=n_out, psf_pixel_scale=dl.utils.rad2arcsec(2.5e-7))
+    resps = [torch.as_tensor(rng.uniform(0.9, 1.1, (n_out, n_out)), dtype=torch.float32,
+                             device="cuda") for _ in range(3)]
+    resp = resps[0].clone()
+    det = dl.LayeredDetector([layers.ApplyJitter(0.5, 5, 3), layers.ApplyPixelResponse(resp),
+                              layers.Downsample(2), layers.AddConstant(1e-6)])
+    wls = np.linspace(0.9e-6, 1.1e-6, nwl)
+    tel = dl.Telescope(optics, dl.PointSource(wls), det)
+    with rt.precision("bf16x3"):
+        refs = []
+        for r in resps:
+            resp.copy_(r)
+            refs.append(tel.model().clone())
+        torch.cuda.synchronize()
+        g = rt.graph_step(tel.model)
+        for rep in range(2):
+            for k, r in enumerate(resps):
+                resp.copy_(r)
+                out = g()
+                torch.cuda.synchronize()
+                assert torch.equal(out, refs[k]), (rep, k)
+                tel.model()   # eager work between replays
+                torch.cuda.synchronize()
