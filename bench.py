@@ -322,8 +322,9 @@ def _batch_leg(torch, runtime, workloads, steps, warmup):
 def _asm_leg(torch, runtime, steps, warmup=10):
     """Angular-spectrum propagation of a 1024^2 complex64 field padded to 2048^2
     (the serving mix's ASM request) in three launches: the x transforms of the
-    padded field; per y-line the forward y transform, the transfer function and
-    the inverse y transform; the inverse x transforms with fftshift and scale."""
+    padded field's rows that hold the field; per y-line the forward y transform,
+    the transfer function and the inverse y transform; the inverse x transforms
+    with fftshift and scale."""
     n, m = 1024, 2048
     g = torch.Generator(device="cuda").manual_seed(7)
     f = torch.polar(torch.ones((n, n), device="cuda"),
@@ -338,9 +339,9 @@ def _asm_leg(torch, runtime, steps, warmup=10):
     end.record()
     torch.cuda.synchronize()
     ms = beg.elapsed_time(end) / steps
-    # the three launches' reads and writes of complex64 values: the n^2 field and
-    # the padded m^2 array out; m^2 in and out twice more
-    nbytes = 8 * (n * n + m * m) + 2 * 2 * 8 * m * m
+    # the three launches' reads and writes of complex64 values: the n^2 field in
+    # and its n rows' transforms (n x m) out; those in and m^2 out; m^2 in and out
+    nbytes = 8 * (n * n + 2 * n * m + 3 * m * m)
     flops = 2 * 5 * m * m * np.log2(m * m)
     return {"workload": "asm: 1024^2 complex64 padded to 2048^2, z = 50 mm, lambda = 1 um",
             "ms_per_call": ms, "launches": 3, "algorithmic_bytes": nbytes,
