@@ -1571,12 +1571,23 @@ bool aligned16(const void* a, const void* b) {
   return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
 }
 
+// Rows per thread of a tile launch: two where the LDS allows, one when two
+// would leave fewer than 256 workgroups (one small frame: 64 x 16 tiles halve
+// the latency of one 64 x 32 tile each) and the block height k still fits.
+int tile_rpt(const Pass& c, int k, int batch) {
+  const int rpt = conv_rpt(c.K, c.sep);
+  if (rpt == 1 || k > kTileY) return rpt;
+  const int th = k > 1 ? (kTileY * 2 / k) * k : kTileY * 2, tw = k > 1 ? (kTileX / k) * k : kTileX;
+  const long tiles = (long)((c.n_in + tw - 1) / tw) * ((c.n_in + th - 1) / th);
+  return tiles * batch < 256 ? 1 : 2;
+}
+
 int run_pass(const Pass& p, int batch, const float* src, float* r_save, float* dst,
              hipStream_t st, SumIn si = SumIn{1, 0}) {
   if (p.conv && !r_save && streams(p, 1, EwList{}, si, batch) && aligned16(src, dst))
     return run_stream(p, 1, p.n_in, EwList{}, batch, src, dst, st);
   if (p.conv) {
-    const int rpt = conv_rpt(p.K, p.sep);
+    const int rpt = tile_rpt(p, 1, batch);
     const int tiles_x = (p.n_in + kTileX - 1) / kTileX;
     const int tiles_y = (p.n_in + kTileY * rpt - 1) / (kTileY * rpt);
     const int lo = (p.K - 1) / 2 - (p.K - 1);
@@ -1617,7 +1628,7 @@ int run_conv_down(const Pass& c, const Pass& d, int batch, const float* src, flo
                   hipStream_t st, SumIn si = SumIn{1, 0}) {
   if (streams(c, d.k, d.post, si, batch) && aligned16(src, dst))
     return run_stream(c, d.k, d.n_out, d.post, batch, src, dst, st);
-  const int rpt = conv_rpt(c.K, c.sep);
+  const int rpt = tile_rpt(c, d.k, batch);
   const int k = d.k;
   const int th = (kTileY * rpt / k) * k, tw = (kTileX / k) * k;
   const int tiles_x = (c.n_in + tw - 1) / tw;
@@ -1655,7 +1666,7 @@ long first_launch_groups(const std::vector<Pass>& passes, int batch) {
     return (long)bands * nseg * batch;
   }
   if (p.conv) {
-    const int rpt = conv_rpt(p.K, p.sep);
+    const int rpt = tile_rpt(p, can_fuse(passes, 0) ? passes[1].k : 1, batch);
     long tiles;
     if (can_fuse(passes, 0)) {
       const int k = passes[1].k;
