@@ -652,7 +652,6 @@ int psf_poly_run(const dlx_mft_geom& g, int32_t nwl, const double* wavelength,
   const double x0 = center - (g.n_in - 1) / 2.0 * pixel_scale_in;
 
   std::vector<double> dx(c, pixel_scale_in), pout(c, pixel_scale_out), alpha(c);
-  bool prep_valid = false;   // the pupil-prologue planes hold this call's pupil
   for (int l0 = 0; l0 < nwl; l0 += c) {
     const int nb = nwl - l0 < c ? nwl - l0 : c;
     float* Ic = planes ? planes + (long)l0 * nout2 : I;
@@ -674,27 +673,23 @@ int psf_poly_run(const dlx_mft_geom& g, int32_t nwl, const double* wavelength,
         return st;
     }
     if (split) {
-      // the pupil generated in stage 1's loader where the shape allows (no
-      // pupil array in HBM: the loader reads the lambda-independent path and
-      // amplitude planes, kept where the pupils would go), else written by
-      // the pupil kernel
-      const size_t n2 = (size_t)g.n_in * g.n_in;
+      // the pupil generated in stage 1's loader from the amplitude and OPD
+      // planes where the shape allows (no pupil array in HBM), else written
+      // by the pupil kernel
       PupilGen pg{};
-      pg.path = (const double*)P;
-      pg.a = (const float*)(P + align_up(n2 * sizeof(double)));
+      pg.opd = opd;
+      pg.amp = amp;
+      pg.dx = pixel_scale_in;
+      pg.x0 = x0;
+      pg.tx = tilt_x;
+      pg.ty = tilt_y;
+      pg.scale = (float)scale;
       pg.n = g.n_in;
       for (int i = 0; i < nb; ++i) pg.inv_wl[i] = 1.0 / wavelength[l0 + i];
       const bool fuse = !phase && pupil_prologue_enabled() &&
-                        align_up(n2 * sizeof(double)) + n2 * sizeof(float) <= Pb &&
                         cgemm_pupil_supported(stage1_split(g, 0, nb, nullptr,
                                                            (const uint8_t*)W, Ut), pg);
-      if (fuse && !prep_valid) {
-        DLX_HIP(launch_pupil_prep(g.n_in, pixel_scale_in, x0, tilt_x, tilt_y, amp, opd,
-                                  (float)scale, (double*)pg.path, (float*)pg.a, s));
-        prep_valid = true;
-      }
       if (!fuse) {
-        prep_valid = false;
         PupilChunk pc;
         for (int i = 0; i < nb; ++i) pc.inv_wl[i] = pg.inv_wl[i];
         DLX_HIP(launch_pupil_split4(pc, nb, g.n_in, pixel_scale_in, x0, tilt_x, tilt_y, amp,

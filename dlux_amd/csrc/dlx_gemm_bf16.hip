@@ -699,10 +699,12 @@ cgemm_split4_deep_kernel(GemmSplit g) {
 // Stage 1 with the pupil generated in the loader: the A operand (pupil rows y,
 // k = x) is never read from HBM.  Each thread owns one row of the 128-row A
 // tile and one 8-pixel chunk of a stage's 16 k.  It loads the next stage's 8
-// optical paths (FP64) and 8 amplitudes (the lambda-independent planes of
-// launch_pupil_prep) before this stage's products are issued and turns them
-// into split4 phasors (dlx_phasor.h, the pupil kernels' arithmetic) after the
-// products are issued, so the phasor work overlaps the matrix cores; the
+// OPDs and 8 amplitudes (the caller's planes) before this stage's products are
+// issued and turns them into split4 phasors (path, amplitude and phasor as the
+// pupil kernels form them, dlx_phasor.h) after the products are issued, so the
+// phasor work overlaps the matrix cores; with no tilt the path is the OPD
+// itself (tx X + ty Y is +-0 and adds nothing to the phase), the loader skips
+// the coordinate arithmetic; the
 // four 16-byte planes go straight into the next stage's A buffer with the
 // deep kernel's swizzle.  B (the transfer matrix) is staged direct-to-LDS as
 // in the deep kernel; the epilogue is EPI_SPLIT_T.  Interior tiles only.
@@ -742,21 +744,34 @@ cgemm_pupil_kernel(GemmSplit g, PupilGen pg) {
   const double inv_wl = pg.inv_wl[b];
   const long pix0 = (long)(m0 + pr) * pg.n + pc * 8;
   const int sw = (pr >> 1) & 7;
-  double2 rp[4];
-  float4 ra[2];
+  const bool tilted = pg.tx != 0.0 || pg.ty != 0.0;
+  const double tyY = pg.ty * pupil_coord(m0 + pr, pg.dx, pg.x0);
+  float4 ro[2], ra[2];
   auto load_raw = [&](int st) {
     const long p = pix0 + st * (8 * kDeepChunks);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) rp[i] = *(const double2*)(pg.path + p + 2 * i);
+    for (int i = 0; i < 2; ++i)
+      ro[i] = pg.opd ? *(const float4*)(pg.opd + p + 4 * i) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) ra[i] = *(const float4*)(pg.a + p + 4 * i);
+    for (int i = 0; i < 2; ++i)
+      ra[i] = pg.amp ? *(const float4*)(pg.amp + p + 4 * i) : make_float4(1.f, 1.f, 1.f, 1.f);
   };
   auto convert = [&](int st) {
-    const double pv[8] = {rp[0].x, rp[0].y, rp[1].x, rp[1].y, rp[2].x, rp[2].y, rp[3].x, rp[3].y};
+    const float ov[8] = {ro[0].x, ro[0].y, ro[0].z, ro[0].w, ro[1].x, ro[1].y, ro[1].z, ro[1].w};
     const float av[8] = {ra[0].x, ra[0].y, ra[0].z, ra[0].w, ra[1].x, ra[1].y, ra[1].z, ra[1].w};
+    const int xc = st * (8 * kDeepChunks) + pc * 8;
     float2 v[8];
+    if (tilted) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = pupil_phasor(pv[e], inv_wl, 0.0, av[e]);
+      for (int e = 0; e < 8; ++e) {
+        const double path = pupil_path(pg.tx, pupil_coord(xc + e, pg.dx, pg.x0), tyY,
+                                       (double)ov[e]);
+        v[e] = pupil_phasor(path, inv_wl, 0.0, pg.scale * av[e]);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = pupil_phasor((double)ov[e], inv_wl, 0.0, pg.scale * av[e]);
+    }
     uint4 p[4];
     pack_chunk(v, p);
     uint8_t* dst = lds + (st & 1) * kBuf + pr * kRowDeep;
@@ -1082,7 +1097,7 @@ namespace {
 int pupil_tile_n(const GemmSplit& g, const PupilGen& pg) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0 || g.batch > GemmSplit::kMaxBatch) return 0;
   const bool aligned =
-      ((reinterpret_cast<uintptr_t>(pg.path) | reinterpret_cast<uintptr_t>(pg.a)) & 15) == 0;
+      ((reinterpret_cast<uintptr_t>(pg.opd) | reinterpret_cast<uintptr_t>(pg.amp)) & 15) == 0;
   if (g.epilogue != EPI_SPLIT_T || g.K != g.M || pg.n != g.M || g.M % 128 != 0 ||
       g.K % (8 * kDeepChunks) != 0 || !aligned || g.b_row_bytes != split4_row_bytes(g.K) ||
       g.ldc != split4_row_bytes(g.M))

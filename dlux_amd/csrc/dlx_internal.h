@@ -136,10 +136,14 @@ struct GemmSplit {
 hipError_t launch_cgemm_split4(const GemmSplit& g, hipStream_t s);
 
 // The pupil of instance b, generated in the stage-1 loader instead of read
-// from A:  A[b](y, x) = a(y, x) exp(2 pi i path(y, x) inv_wl[b]), from the
-// lambda-independent planes written by launch_pupil_prep (n x n, row-major).
+// from A:  A[b](y, x) = a(y, x) exp(2 pi i path(y, x) inv_wl[b]) with
+// path = tx X + ty Y + opd (FP64) and a = scale amp, formed from the caller's
+// amplitude and OPD planes (n x n, row-major, float; either may be null) as
+// the pupil kernels form them (dlx_phasor.h).
 struct PupilGen {
-  const double* path; const float* a;
+  const float* opd; const float* amp;
+  double dx, x0, tx, ty;
+  float scale;
   int n;
   double inv_wl[GemmSplit::kMaxBatch];
 };
@@ -211,11 +215,6 @@ hipError_t launch_pupil_chunk(const PupilChunk& c, int nb, int n, double dx, dou
 // split4 producers
 hipError_t launch_dft_split4(const DftChunk& c, int nb, int n_in, int n_out, int inverse,
                              uint8_t* out, long batch_bytes, hipStream_t s);
-// path = tx X + ty Y + opd (FP64) and a = scale amp (float) of every pupil
-// pixel, as the pupil kernels form them (dlx_phasor.h).
-hipError_t launch_pupil_prep(int n, double dx, double x0, double tx, double ty, const float* amp,
-                             const float* opd, float scale, double* path, float* a,
-                             hipStream_t s);
 hipError_t launch_pupil_split4(const PupilChunk& c, int nb, int n, double dx, double x0,
                                double tx, double ty, const float* amp, const float* opd,
                                const float* phase, float scale, uint8_t* out, long batch_bytes,

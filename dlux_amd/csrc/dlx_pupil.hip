@@ -160,27 +160,6 @@ pupil_split4_kernel(PupilChunk c, int nb, int n, double dx, double x0, double tx
   }
 }
 
-__global__ void __launch_bounds__(256)
-pupil_prep_kernel(int n, double dx, double x0, double tx, double ty,
-                  const float* __restrict__ amp, const float* __restrict__ opd, float scale,
-                  double* __restrict__ path, float* __restrict__ a) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (long)n * n) return;
-  const int y = (int)(p / n), x = (int)(p - (long)y * n);
-  path[p] = pupil_path(tx, pupil_coord(x, dx, x0), ty * pupil_coord(y, dx, x0),
-                       opd ? (double)opd[p] : 0.0);
-  a[p] = scale * (amp ? amp[p] : 1.0f);
-}
-
-hipError_t launch_pupil_prep(int n, double dx, double x0, double tx, double ty, const float* amp,
-                             const float* opd, float scale, double* path, float* a,
-                             hipStream_t s) {
-  const long npix = (long)n * n;
-  hipLaunchKernelGGL(pupil_prep_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s, n,
-                     dx, x0, tx, ty, amp, opd, scale, path, a);
-  return hipGetLastError();
-}
-
 hipError_t launch_pupil_split4(const PupilChunk& c, int nb, int n, double dx, double x0,
                                double tx, double ty, const float* amp, const float* opd,
                                const float* phase, float scale, uint8_t* out, long batch_bytes,

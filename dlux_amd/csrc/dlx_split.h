@@ -31,7 +31,11 @@ __device__ __forceinline__ uint32_t bf16x2_rne(float x0, float x1) {
 }
 
 // Split of two values: x_i = hi_i + lo_i + O(2^-17 |x_i|), each pair packed.
+// The residual is formed from the rounded x_i: contraction is off, so a
+// product feeding x_i is never fused into the subtraction, and every producer
+// of the same values writes the same bits whatever the code around it.
 __device__ __forceinline__ void split2_bf16(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+#pragma clang fp contract(off)
   hi = bf16x2_rne(x0, x1);
   lo = bf16x2_rne(x0 - __uint_as_float(hi << 16), x1 - __uint_as_float(hi & 0xFFFF0000u));
 }
