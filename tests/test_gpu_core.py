@@ -129,6 +129,24 @@ def test_eval_basis_power_intensity(rt):
     assert np.allclose(Is, np.sum(np.abs(F) ** 2, 0), rtol=1e-5)
 
 
+# (modes, side): 36 modes at 512^2 (the Zernike OPD of the polychromatic PSF),
+# pixel counts that are and are not multiples of 4 (16-byte loads or the
+# scalar tail), a partial last workgroup (100^2), and more modes than the
+# loop's unroll factor
+@pytest.mark.parametrize("nb,side", [(36, 512), (37, 100), (9, 100), (8, 64), (7, 64), (40, 37)])
+def test_eval_basis_vs_fp64(rt, nb, side):
+    print(f"[basis] {nb} modes {side}^2", flush=True)
+    rng = np.random.default_rng(nb * 1000 + side)
+    basis = rng.standard_normal((nb, side, side)).astype(np.float32)
+    c = rng.standard_normal(nb).astype(np.float32)
+    base = rng.standard_normal((side, side)).astype(np.float32)
+    ref = np.tensordot(basis.astype(np.float64), c.astype(np.float64), axes=(0, 0))
+    out = rt.eval_basis(basis, c).cpu().numpy()
+    assert np.max(np.abs(out - ref)) < 1e-5 * np.max(np.abs(ref))
+    out_b = rt.eval_basis(basis, c, base=base).cpu().numpy()
+    assert np.max(np.abs(out_b - (ref + base))) < 1e-5 * np.max(np.abs(ref + base))
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_psf_poly_small(rt, prec):
     """Fused chain vs oracle.poly_psf: aperture + spiders + Zernikes, 8 wavelengths."""
