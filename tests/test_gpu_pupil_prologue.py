@@ -2,7 +2,7 @@
 (cgemm_pupil_kernel): no pupil array is written, the loader forms the same
 split4 phasors as the pupil kernel (one shared definition), so the PSF must
 be bit-identical to the two-kernel path (DLUX_AMD_PUPIL_PROLOGUE=0) for both
-tile widths, with and without amplitude / OPD, with tilts, and equal to the
+tile widths, with and without amplitude / OPD, with and without tilt, and equal to the
 FP64 oracle within 1e-5 of the peak.
 """
 
@@ -40,9 +40,12 @@ SHAPES = [(512, 256, 64), (1024, 256, 32), (512, 128, 64), (1024, 256, 16), (256
           (256, 128, 16)]
 
 
+# with a tilt the loader forms tx X + ty Y + OPD in FP64; with none it takes the
+# OPD as the path, which must give the same bits as the pupil kernel's sum
+@pytest.mark.parametrize("tilted", [True, False])
 @pytest.mark.parametrize("n_in,n_out,nwl", SHAPES)
-def test_prologue_bit_identical_to_pupil_kernel(rt, monkeypatch, n_in, n_out, nwl):
-    print(f"[pupil prologue] {n_in}->{n_out} x{nwl}", flush=True)
+def test_prologue_bit_identical_to_pupil_kernel(rt, monkeypatch, n_in, n_out, nwl, tilted):
+    print(f"[pupil prologue] {n_in}->{n_out} x{nwl} tilted={tilted}", flush=True)
     mask, opd = _pupil(n_in, n_in + nwl)
     p_out = 2.5e-7 * 512 / n_in
     wls = np.linspace(0.8e-6, 1.2e-6, nwl)
@@ -53,7 +56,8 @@ def test_prologue_bit_identical_to_pupil_kernel(rt, monkeypatch, n_in, n_out, nw
              dict(amp=mask.astype(np.float32), opd=None)]
     for i, kw in enumerate(cases):
         args = (n_in, n_out, wls, w, 1.0 / n_in, p_out)
-        common = dict(scale=scale, tilt=(1.3 * p_out, -0.6 * p_out), precision="bf16x3", **kw)
+        tilt = (1.3 * p_out, -0.6 * p_out) if tilted else (0.0, 0.0)
+        common = dict(scale=scale, tilt=tilt, precision="bf16x3", **kw)
         monkeypatch.delenv("DLUX_AMD_PUPIL_PROLOGUE", raising=False)
         fused = rt.psf_poly(*args, **common)
         monkeypatch.setenv("DLUX_AMD_PUPIL_PROLOGUE", "0")
