@@ -36,10 +36,10 @@ ERR_BOUND = 1e-5
 # known size (profiles/mft_poly_512to256_64wl_hbm_traffic.md).  The counters cannot
 # be read from inside this process.
 # Keyed by (form, pupil generated in stage 1's loader).
-TRAFFIC_BYTES = {("bf16x3", True): 159.6e6}
+TRAFFIC_BYTES = {("bf16x3", True): 151.2e6}
 # MFMA busy fraction of the stage-1 kernel from rocprofv3 SQ_VALU_MFMA_BUSY_CYCLES /
 # (GRBM_GUI_ACTIVE / 8 x 1024 SIMDs) (profiles/mft_poly_512to256_64wl_mfma_counters.md).
-MFMA_BUSY_PMC = {("bf16x3", True): 0.501}
+MFMA_BUSY_PMC = {("bf16x3", True): 0.515}
 
 
 def _stage1_kernel(head, prologue):
