@@ -150,7 +150,8 @@ int dlx_intensity(int64_t npix, int32_t batch, const void* field,
  * dlx_pupil_field (NULL allowed); `scale` is the lambda-independent pupil
  * scale s (initial 1/N^2 times any power normalisation).  In the split-BF16x3
  * form with no phase, the pupil is generated inside the first contraction
- * where its tiles fill the GPU (no pupil array is written); the result is the
+ * from amp and opd where its tiles fill the GPU and amp / opd are 16-byte
+ * aligned (no pupil array is written); the result is the
  * same bits either way (environment DLUX_AMD_PUPIL_PROLOGUE=0 forces the
  * two-kernel form).
  * Replaces the reference chain OpticalSystem.propagate -> propagate_mono ->
