@@ -427,17 +427,22 @@ def _telescope_leg(torch, runtime, workloads, base, cfg, steps, ref, warmup=3):
     def unfused():
         return det.model(base.model(src, return_psf=True))
 
+    # two rounds, the forms alternating, the faster round of each kept: the
+    # form timed first would otherwise also pay the leg's own warm-up
     out = {}
     with runtime.precision("bf16x3"):
-        for name, fn in (("fused", tel.model), ("unfused", unfused)):
-            for _ in range(warmup):
-                fn()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                img = fn()
-            torch.cuda.synchronize()
-            out[name] = ((time.perf_counter() - t0) * 1e3 / steps, img)
+        for _ in range(2):
+            for name, fn in (("fused", tel.model), ("unfused", unfused)):
+                for _ in range(warmup):
+                    fn()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    img = fn()
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3 / steps
+                if name not in out or ms < out[name][0]:
+                    out[name] = (ms, img)
     err = None
     if ref is not None:
         from oracle import dlux_oracle as orc
